@@ -1,0 +1,192 @@
+/*
+ * lie_svd_amd.h -- C ABI of the MI355X (gfx950) dense SVD library.
+ *
+ * This is the drop-in boundary for the reference crate's dense SVD path
+ * (skovnats/lie_cliffalg_analog_svd v0.44.0).  Each entry point names the
+ * Rust function it replaces.  Conventions, fixed for every entry point:
+ *
+ *  - Matrices are row-major (C order) f64, the layout of ndarray's
+ *    Array2<f64>.  Leading dimensions equal the column count.
+ *  - Outputs follow the reference exactly: sigma is descending, Vt is
+ *    returned (not V), tall inputs give thin U (m x k, k = min(m, n)),
+ *    and LieSvdMicro's sign lives in U.
+ *  - Pointer mode (lsvd_set_pointer_mode):
+ *      LSVD_PTR_HOST   (default) all array arguments are host memory; the
+ *                      call is synchronous, like the reference.
+ *      LSVD_PTR_DEVICE array arguments are device memory; the call is
+ *                      asynchronous on the handle's stream.
+ *    Shape arrays (m[], n[], offsets) are always host memory.
+ *  - The caller owns every buffer.  The library never frees caller memory.
+ *  - Errors are return codes.  The reference asserts (panics) on shape
+ *    misuse; a Rust wrapper maps LSVD_E_SHAPE back to a panic.  Numerical
+ *    non-convergence is silent in the reference (fallbacks) and is reported
+ *    here through lsvd_info_t, not as an error.
+ *  - A handle is not thread-safe; separate handles are independent.
+ */
+#ifndef LIE_SVD_AMD_H
+#define LIE_SVD_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lsvd_handle_s* lsvd_handle_t;
+typedef struct lsvd_plan_s* lsvd_plan_t;
+
+enum lsvd_status {
+    LSVD_OK = 0,
+    LSVD_E_SHAPE = 1,     /* shape misuse: the reference's assert! panics */
+    LSVD_E_HIP = 2,       /* a HIP runtime call failed */
+    LSVD_E_NONFINITE = 3, /* non-finite input where the path cannot proceed */
+    LSVD_E_NOCONV = 4,    /* reserved: the path reports non-convergence in info */
+    LSVD_E_ARG = 5,       /* bad argument (null pointer, unknown mode) */
+    LSVD_E_NOMEM = 6,     /* device allocation failed */
+    LSVD_E_LIMIT = 7      /* size beyond a documented maximum */
+};
+
+enum lsvd_pointer_mode { LSVD_PTR_HOST = 0, LSVD_PTR_DEVICE = 1 };
+
+/* Arithmetic of the Newton-Schulz polar iterations.  Every mode ends in
+ * FP64: FP64 refinement of the polar factor and FP64 Jacobi cleanup. */
+enum lsvd_precision {
+    LSVD_PREC_F64 = 0,   /* plain FP64 throughout (the reference algorithm) */
+    LSVD_PREC_F32 = 1,   /* FP32 NS iterations, FP64 refinement + cleanup */
+    LSVD_PREC_BF16 = 2   /* BF16 MFMA NS iterations (FP32 accumulate), FP64 refinement */
+};
+
+/* Per-matrix diagnostics (fills the reference's solver traces). */
+typedef struct {
+    int32_t polar_ok;       /* newton_schulz_polar's converged flag */
+    int32_t ns_iters;       /* NS iterations (all precisions) */
+    int32_t jacobi_sweeps;  /* Jacobi sweeps that applied rotations */
+    int32_t rotations;      /* rotations applied */
+    int32_t repaired;       /* left basis repaired from the right one */
+    int32_t micro_fallback; /* LieSvdMicro escalated to LieSvdSmall */
+    int32_t path;           /* 0 small, 1 micro, 2 micro->small fallback */
+    int32_t refine_iters;   /* FP64 refinement iterations (mixed precision) */
+} lsvd_info_t;
+
+/* JointDiagonalizationParams (src/lie_svd_joint.rs:18-35). */
+typedef struct {
+    int32_t max_sweeps;
+    double tol;
+    double min_pair_scale;
+    int32_t line_search_steps;
+} lsvd_joint_params_t;
+
+/* JointDiagonalizationTrace (src/lie_svd_joint.rs:43-50). */
+typedef struct {
+    double initial_offdiag;
+    double final_offdiag;
+    int32_t sweeps;
+    int32_t rotations;
+    int32_t rejected_rotations;
+} lsvd_joint_trace_t;
+
+/* Polar diagnostics. */
+typedef struct {
+    int32_t polar_ok;
+    int32_t lowp_iters;    /* iterations in the chosen low precision */
+    int32_t f64_iters;     /* FP64 iterations (refinement, or all for F64) */
+    int32_t reserved;
+    double orth_err;       /* ||Q^T Q - I||_F of the returned factor */
+    double last_delta;     /* ||X_{k+1} - X_k||_F at the last iteration */
+} lsvd_polar_info_t;
+
+/* ---- handle ---------------------------------------------------------- */
+
+/* Creates a handle on `device` with its own HIP stream and workspace. */
+int lsvd_create(int device, lsvd_handle_t* out);
+int lsvd_destroy(lsvd_handle_t h);
+/* Uses a caller-owned hipStream_t (NULL restores the handle's own). */
+int lsvd_set_stream(lsvd_handle_t h, void* hip_stream);
+void* lsvd_get_stream(lsvd_handle_t h);
+int lsvd_set_pointer_mode(lsvd_handle_t h, int mode);
+int lsvd_synchronize(lsvd_handle_t h);
+const char* lsvd_status_string(int status);
+/* Text of the last error on this handle ("" if none). */
+const char* lsvd_last_error(lsvd_handle_t h);
+/* Library version string. */
+const char* lsvd_version(void);
+
+/* ---- LieSvdSmall ----------------------------------------------------- */
+
+/* Replaces LieSvdSmall::solve (src/lie_svd_small.rs:296-323).
+ * a: n x n.  u: n x n, s: n, vt: n x n.  info may be NULL. */
+int lsvd_small_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
+                         lsvd_info_t* info);
+
+/* Replaces LieSvdSmall::solve_rectangular (src/lie_svd_small.rs:340-360).
+ * a: m x n; k = min(m, n); u: m x k, s: k, vt: k x n. */
+int lsvd_small_solve_rectangular_f64(lsvd_handle_t h, const double* a, int m, int n, double* u, double* s,
+                                     double* vt, lsvd_info_t* info);
+
+/* Replaces eigh_jacobi_full (src/lie_svd_small.rs:212-285): symmetric p
+ * (n x n) -> v (n x n, columns are eigenvectors), eig (n, descending). */
+int lsvd_eigh_jacobi_f64(lsvd_handle_t h, const double* p, int n, double* v, double* eig, lsvd_info_t* info);
+
+/* ---- LieSvdMicro ----------------------------------------------------- */
+
+/* Replaces LieSvdMicro::solve (src/lie_svd_micro.rs:21-52), n <= 4. */
+int lsvd_micro_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
+                         lsvd_info_t* info);
+
+/* ---- Ragged batch (no reference counterpart: the reference does one
+ *      matrix per call; this runs many LieSvd::solve / solve_rectangular
+ *      calls, src/lie_svd.rs:49-51 + src/lie_svd_small.rs:340, at once) -- */
+
+/* Builds a reusable plan for `count` matrices of shapes m[b] x n[b] (host
+ * arrays).  Pools are packed: matrix b lives at a_off[b] in the input pool,
+ * and its outputs at u_off[b] (m x k), s_off[b] (k) and vt_off[b] (k x n).
+ * Square matrices with n <= 4 use the LieSvdMicro schedule, as the
+ * reference's router does (src/lie_svd_adaptive.rs:93); the rest use
+ * LieSvdSmall::solve_rectangular.  The plan sorts the work, picks tile
+ * shapes from the autotune cache and captures one hipGraph. */
+int lsvd_batch_plan_create(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n,
+                           const int64_t* a_off, const int64_t* u_off, const int64_t* s_off,
+                           const int64_t* vt_off, int precision, lsvd_plan_t* out);
+/* Runs the plan: one hipGraph launch.  Pool pointers follow the pointer
+ * mode; info (count entries) may be NULL. */
+int lsvd_batch_plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
+                            double* vt_pool, lsvd_info_t* info);
+int lsvd_batch_plan_destroy(lsvd_plan_t p);
+/* One-shot convenience: create + execute + destroy. */
+int lsvd_svd_batched_f64(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                         const double* a_pool, const int64_t* u_off, double* u_pool, const int64_t* s_off,
+                         double* s_pool, const int64_t* vt_off, double* vt_pool, int precision,
+                         lsvd_info_t* info);
+
+/* Runs the autotuner for the shape buckets of a batch and stores the
+ * winners in the handle's cache (used by later plans). */
+int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, int precision);
+
+/* ---- Plan diagnostics (no reference counterpart: profiling aids) ------ */
+
+/* One persistent launch of a plan.  variant -1 is the LieSvdMicro launch. */
+typedef struct {
+    int32_t first;      /* first queue entry of the bucket */
+    int32_t count;      /* queue entries in the bucket */
+    int32_t max_n;      /* largest min(m, n) in the bucket */
+    int32_t variant;    /* kernel variant id (-1: micro) */
+    int32_t grid;       /* workgroups launched */
+    int32_t threads;    /* threads per workgroup */
+    int64_t lds_bytes;  /* dynamic LDS per workgroup */
+} lsvd_bucket_desc_t;
+
+/* Writes up to `max` descriptors; *nbuckets gets the total.  order (may be
+ * NULL) receives, for every queue entry, the caller's matrix index. */
+int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, int* nbuckets, int32_t* order);
+
+/* Launches every bucket on its own with HIP events around it on the
+ * handle's stream and writes each launch's time in ms (same order as
+ * describe).  Pool pointers follow the pointer mode of the plan. */
+int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
+                                 double* vt_pool, float* ms, int max);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LIE_SVD_AMD_H */

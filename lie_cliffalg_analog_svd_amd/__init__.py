@@ -1,0 +1,16 @@
+"""MI355X-native (gfx950) dense SVD path of skovnats/lie_cliffalg_analog_svd.
+
+Hand-written HIP kernels behind a C ABI (``include/lie_svd_amd.h``,
+``liblsvd_amd.so``), with a Python mirror of the reference's Rust API:
+``LieSvdSmall``, ``LieSvdMicro``, ``LieSvd``, ``eigh_jacobi_full`` and the
+batch plans.  ``profiles`` and ``metrics`` restate the reference's input
+generators and residual metrics (src/lib.rs:133-388).
+"""
+from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PTR_DEVICE, LSVD_PTR_HOST, LsvdError,
+                   ShapeError)
+from .handle import BatchPlan, Handle, default_handle, pack_offsets
+from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall, eigh_jacobi_full, run_small,
+                      svd_batched)
+from . import metrics, profiles
+
+__version__ = "0.1.0"

@@ -1,0 +1,173 @@
+"""ctypes binding of ``liblsvd_amd.so`` (the C ABI in include/lie_svd_amd.h).
+
+The library is the product: there is no CPU fallback.  Loading fails with a
+clear error when the shared library has not been built, and every call fails
+when no GPU is present.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblsvd_amd.so")
+
+LSVD_OK = 0
+LSVD_E_SHAPE = 1
+LSVD_E_HIP = 2
+LSVD_E_NONFINITE = 3
+LSVD_E_NOCONV = 4
+LSVD_E_ARG = 5
+LSVD_E_NOMEM = 6
+LSVD_E_LIMIT = 7
+
+LSVD_PTR_HOST = 0
+LSVD_PTR_DEVICE = 1
+
+LSVD_PREC_F64 = 0
+LSVD_PREC_F32 = 1
+LSVD_PREC_BF16 = 2
+
+
+class LsvdInfo(C.Structure):
+    _fields_ = [
+        ("polar_ok", C.c_int32),
+        ("ns_iters", C.c_int32),
+        ("jacobi_sweeps", C.c_int32),
+        ("rotations", C.c_int32),
+        ("repaired", C.c_int32),
+        ("micro_fallback", C.c_int32),
+        ("path", C.c_int32),
+        ("refine_iters", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class LsvdJointParams(C.Structure):
+    _fields_ = [
+        ("max_sweeps", C.c_int32),
+        ("tol", C.c_double),
+        ("min_pair_scale", C.c_double),
+        ("line_search_steps", C.c_int32),
+    ]
+
+
+class LsvdJointTrace(C.Structure):
+    _fields_ = [
+        ("initial_offdiag", C.c_double),
+        ("final_offdiag", C.c_double),
+        ("sweeps", C.c_int32),
+        ("rotations", C.c_int32),
+        ("rejected_rotations", C.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LsvdPolarInfo(C.Structure):
+    _fields_ = [
+        ("polar_ok", C.c_int32),
+        ("lowp_iters", C.c_int32),
+        ("f64_iters", C.c_int32),
+        ("reserved", C.c_int32),
+        ("orth_err", C.c_double),
+        ("last_delta", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LsvdBucketDesc(C.Structure):
+    _fields_ = [
+        ("first", C.c_int32),
+        ("count", C.c_int32),
+        ("max_n", C.c_int32),
+        ("variant", C.c_int32),
+        ("grid", C.c_int32),
+        ("threads", C.c_int32),
+        ("lds_bytes", C.c_int64),
+    ]
+
+    def as_dict(self):
+        return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+class LsvdError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"lsvd error {status} ({status_string(status)}): {msg}")
+        self.status = status
+
+
+class ShapeError(LsvdError, AssertionError):
+    """The reference panics (assert!) on shape misuse; this mirrors it."""
+
+
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_i32p = C.POINTER(C.c_int32)
+_i64p = C.POINTER(C.c_int64)
+_vp = C.c_void_p
+
+# name -> (restype, argtypes); symbols declared in include/lie_svd_amd.h
+SIGNATURES = {
+    "lsvd_create": (C.c_int, [C.c_int, C.POINTER(C.c_void_p)]),
+    "lsvd_destroy": (C.c_int, [_vp]),
+    "lsvd_set_stream": (C.c_int, [_vp, _vp]),
+    "lsvd_get_stream": (C.c_void_p, [_vp]),
+    "lsvd_set_pointer_mode": (C.c_int, [_vp, C.c_int]),
+    "lsvd_synchronize": (C.c_int, [_vp]),
+    "lsvd_status_string": (C.c_char_p, [C.c_int]),
+    "lsvd_last_error": (C.c_char_p, [_vp]),
+    "lsvd_version": (C.c_char_p, []),
+    "lsvd_small_solve_f64": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "lsvd_small_solve_rectangular_f64": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "lsvd_eigh_jacobi_f64": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp]),
+    "lsvd_micro_solve_f64": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
+    "lsvd_batch_plan_create": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _i64p, _i64p, _i64p, C.c_int,
+                                         C.POINTER(C.c_void_p)]),
+    "lsvd_batch_plan_execute": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "lsvd_batch_plan_destroy": (C.c_int, [_vp]),
+    "lsvd_svd_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _vp, _i64p, _vp, _i64p, _vp, _i64p,
+                                       _vp, C.c_int, _vp]),
+    "lsvd_autotune_batch": (C.c_int, [_vp, C.c_int, _i32p, _i32p, C.c_int]),
+    "lsvd_batch_plan_describe": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int), _vp]),
+    "lsvd_batch_plan_time_buckets": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+}
+
+
+def lib():
+    """Loads the HIP library; raises if it is missing (no fallback path)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(
+                f"{LIB_PATH} is missing: build it with `python -m lie_cliffalg_analog_svd_amd.build` "
+                "(hipcc, gfx950). There is no CPU fallback.")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def status_string(status: int) -> str:
+    try:
+        return lib().lsvd_status_string(status).decode()
+    except Exception:  # pragma: no cover - only while formatting another error
+        return str(status)
+
+
+def check(h, status: int) -> None:
+    if status == LSVD_OK:
+        return
+    msg = lib().lsvd_last_error(h).decode() if h else ""
+    if status == LSVD_E_SHAPE:
+        raise ShapeError(status, msg)
+    raise LsvdError(status, msg)

@@ -1,0 +1,725 @@
+// lsvd_api.cpp -- host side of the C ABI declared in include/lie_svd_amd.h.
+//
+// A handle owns a HIP stream and an autotune cache.  Every solve is
+// expressed as a batch plan: a list of Items sorted longest-first and split
+// into buckets by the tall-ified shape (M x N, M >= N).  Each bucket is one
+// persistent launch of a small_svd variant sized for that bucket; square
+// n <= 4 items go to the micro kernel, whose escalations are drained by a
+// last small_svd launch.  A plan executes as one hipGraph launch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lsvd_api_internal.hpp"
+
+using namespace lsvd;
+
+#define LSVD_VERSION "0.1.0"
+
+namespace lsvd {
+
+int set_error(lsvd_handle_t h, int code, const std::string& msg) {
+    if (h) h->last_error = msg;
+    return code;
+}
+
+int hip_fail(lsvd_handle_t h, hipError_t e, const char* what) {
+    std::string m = std::string(what) + ": " + hipGetErrorString(e);
+    return set_error(h, LSVD_E_HIP, m);
+}
+
+// ---- bucket selection --------------------------------------------------
+
+static size_t lds_budget_bytes() { return 160 * 1024 - 8 * 1024; }  // minus static LDS headroom
+
+// N class boundaries; each class gets its own persistent launch.
+static int n_class(int N) {
+    static const int bounds[] = {8, 16, 24, 32, 48, 64, 96, 128, 256, 512, 1024};
+    for (int i = 0; i < (int)(sizeof(bounds) / sizeof(bounds[0])); ++i)
+        if (N <= bounds[i]) return i;
+    return -1;
+}
+
+// Default variant for a bucket: the smallest MAXN that holds N, with the
+// arena in LDS when the bucket's largest arena fits there.
+static int default_variant(int maxN, size_t arena_bytes) {
+    const bool fits = arena_bytes <= lds_budget_bytes();
+    if (maxN <= 16 && fits) return 0;
+    if (maxN <= 32 && fits) return 1;
+    if (maxN <= 64 && fits) return 2;
+    if (maxN <= 128) return 3;
+    if (maxN <= 1024) return 4;
+    return -1;
+}
+
+static bool variant_accepts(int vid, int maxN, size_t arena_bytes) {
+    int nv = 0;
+    const SmallVariant* vs = small_variants(&nv);
+    for (int i = 0; i < nv; ++i)
+        if (vs[i].id == vid) return maxN <= vs[i].maxn && (!vs[i].lds || arena_bytes <= lds_budget_bytes());
+    return false;
+}
+
+static const SmallVariant* find_variant(int vid) {
+    int nv = 0;
+    const SmallVariant* vs = small_variants(&nv);
+    for (int i = 0; i < nv; ++i)
+        if (vs[i].id == vid) return &vs[i];
+    return nullptr;
+}
+
+// Work estimate used for longest-first ordering (flops up to a constant):
+// QR + compose ~ 4 M N^2, NS ~ 4 k N^3 with k ~ 30, Jacobi ~ 6 S N^3, S ~ 8.
+static double item_cost(int M, int N) {
+    const double n3 = (double)N * N * N;
+    return 4.0 * M * (double)N * N * (M > N) + (120.0 + 48.0 + 4.0) * n3;
+}
+
+// Sets the launch shape of a bucket for `variant`; returns the global arena
+// doubles it needs (0 for LDS variants).
+static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
+    const SmallVariant* v = find_variant(variant);
+    bk.variant = variant;
+    const size_t lds = v->lds ? bk.arena_bytes : 0;
+    const int per_cu = small_variant_blocks_per_cu(variant, lds);
+    bk.grid = std::max(1, std::min(bk.count, per_cu * h->num_cus));
+    bk.lds_bytes = lds;
+    bk.gws_stride = v->lds ? 0 : (int64_t)((bk.arena_bytes / 8 + 31) / 32 * 32);
+    return v->lds ? 0 : (size_t)bk.gws_stride * bk.grid;
+}
+
+}  // namespace lsvd
+
+// ---- plan -----------------------------------------------------------------
+
+static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                      const int64_t* u_off, const int64_t* s_off, const int64_t* vt_off, const int32_t* modes,
+                      int precision, lsvd_plan_t* out) {
+    if (!h || !out) return LSVD_E_ARG;
+    if (count < 0) return set_error(h, LSVD_E_ARG, "negative count");
+    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16)
+        return set_error(h, LSVD_E_ARG, "unknown precision mode");
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+
+    auto* p = new lsvd_plan_s();
+    p->h = h;
+    p->count = count;
+    p->precision = precision;
+
+    std::vector<Item> micro;
+    struct Tmp {
+        Item it;
+        int M, N;
+        double cost;
+    };
+    std::map<int, std::vector<Tmp>> by_class;
+    int64_t a_end = 0, u_end = 0, s_end = 0, vt_end = 0;
+    for (int b = 0; b < count; ++b) {
+        const int mb = m[b], nb = n[b];
+        const int mode = modes ? modes[b] : kModeSvd;
+        if (mb < 0 || nb < 0 || (mode != kModeSvd && mb != nb)) {
+            delete p;
+            return set_error(h, LSVD_E_SHAPE, "matrix " + std::to_string(b) + ": bad shape");
+        }
+        const int k = std::min(mb, nb);
+        Item it{};
+        it.m = mb;
+        it.n = nb;
+        it.idx = b;
+        it.mode = mode;
+        it.a_off = a_off[b];
+        it.u_off = u_off[b];
+        it.s_off = s_off[b];
+        it.vt_off = vt_off ? vt_off[b] : 0;
+        a_end = std::max<int64_t>(a_end, it.a_off + (int64_t)mb * nb);
+        u_end = std::max<int64_t>(u_end, it.u_off + (int64_t)(mode == kModeEigh ? nb * nb : mb * k));
+        s_end = std::max<int64_t>(s_end, it.s_off + k);
+        if (mode != kModeEigh) vt_end = std::max<int64_t>(vt_end, it.vt_off + (int64_t)k * nb);
+        if (mb == 0 || nb == 0) {
+            if (mode == kModeSvd) {  // LieSvdSmall::solve_rectangular asserts non-empty (:343-346)
+                delete p;
+                return set_error(h, LSVD_E_SHAPE, "LieSvdSmall::solve_rectangular: empty input");
+            }
+            continue;  // LieSvdMicro / eigh on 0x0: nothing to do
+        }
+        if (mode == kModeMicro) {
+            if (nb > 4) {
+                delete p;
+                return set_error(h, LSVD_E_SHAPE, "LieSvdMicro only supports N <= 4");
+            }
+            micro.push_back(it);
+            continue;
+        }
+        const int M = std::max(mb, nb), N = std::min(mb, nb);
+        const int cls = n_class(N);
+        if (cls < 0) {
+            delete p;
+            return set_error(h, LSVD_E_LIMIT, "matrix " + std::to_string(b) + ": min(m, n) > 1024");
+        }
+        // split classes further by whether the arena is large (tall M)
+        const size_t arena = small_arena_doubles(M, N) * 8;
+        const int key = cls * 2 + (arena > lds_budget_bytes() ? 1 : 0);
+        by_class[key].push_back({it, M, N, item_cost(M, N)});
+    }
+    p->a_len = a_end;
+    p->u_len = u_end;
+    p->s_len = s_end;
+    p->vt_len = vt_end;
+
+    // Device layout: [micro items][bucket items...][fallback items]
+    std::vector<Item> all;
+    all.insert(all.end(), micro.begin(), micro.end());
+    p->n_micro = (int)micro.size();
+    for (auto& kv : by_class) {
+        auto& v = kv.second;
+        std::stable_sort(v.begin(), v.end(), [](const Tmp& x, const Tmp& y) { return x.cost > y.cost; });
+        Bucket bk{};
+        bk.first = (int)all.size();
+        bk.count = (int)v.size();
+        int maxN = 0;
+        size_t arena = 0;
+        for (auto& t : v) {
+            all.push_back(t.it);
+            maxN = std::max(maxN, t.N);
+            arena = std::max(arena, small_arena_doubles(t.M, t.N) * 8);
+        }
+        bk.maxN = maxN;
+        bk.arena_bytes = arena;
+        int vid = default_variant(maxN, arena);
+        auto tk = h->tune.find(tune_key(maxN, arena, precision));
+        if (tk != h->tune.end() && variant_accepts(tk->second, maxN, arena)) vid = tk->second;
+        if (vid < 0) {
+            delete p;
+            return set_error(h, LSVD_E_LIMIT, "no kernel variant for N=" + std::to_string(maxN));
+        }
+        bk.variant = vid;
+        p->buckets.push_back(bk);
+    }
+    p->fallback_first = (int)all.size();
+    p->n_items = (int)all.size() + p->n_micro;  // room for fallbacks
+    all.resize(p->n_items);
+
+    // counters: [bucket counters][fallback counter][fallback count]
+    const int nb = (int)p->buckets.size();
+    p->n_counters = nb + 2;
+    std::vector<int> consts(nb + 1, 0);
+    for (int i = 0; i < nb; ++i) consts[i] = p->buckets[i].count;
+
+    // global arena slots for non-LDS variants, and launch grids
+    size_t gws_doubles = 0;
+    for (auto& bk : p->buckets) {
+        bk.gws_offset = (int64_t)gws_doubles;
+        gws_doubles += configure_bucket(h, bk, bk.variant);
+    }
+    // fallback launch: micro escalations, n <= 4 square -> variant 0
+    {
+        const size_t arena = small_arena_doubles(4, 4) * 8;
+        const int per_cu = small_variant_blocks_per_cu(0, arena);
+        p->fallback_grid = std::max(1, std::min(std::max(p->n_micro, 1), per_cu * h->num_cus));
+        p->fallback_lds = arena;
+    }
+
+#define PCHECK(expr)                                        \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) {                             \
+            int rc_ = hip_fail(h, e_, #expr);               \
+            lsvd_batch_plan_destroy(p);                     \
+            return rc_;                                     \
+        }                                                   \
+    } while (0)
+
+    PCHECK(hipMalloc(&p->d_items, sizeof(Item) * std::max(1, p->n_items)));
+    if (!all.empty())
+        PCHECK(hipMemcpy(p->d_items, all.data(), sizeof(Item) * all.size(), hipMemcpyHostToDevice));
+    PCHECK(hipMalloc(&p->d_counters, sizeof(int) * p->n_counters));
+    PCHECK(hipMalloc(&p->d_consts, sizeof(int) * (nb + 1)));
+    PCHECK(hipMemcpy(p->d_consts, consts.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+    if (gws_doubles) PCHECK(hipMalloc(&p->d_gws, sizeof(double) * gws_doubles));
+    if (h->ptr_mode == LSVD_PTR_HOST) {
+        PCHECK(hipMalloc(&p->d_a, sizeof(double) * std::max<int64_t>(1, p->a_len)));
+        PCHECK(hipMalloc(&p->d_u, sizeof(double) * std::max<int64_t>(1, p->u_len)));
+        PCHECK(hipMalloc(&p->d_s, sizeof(double) * std::max<int64_t>(1, p->s_len)));
+        PCHECK(hipMalloc(&p->d_vt, sizeof(double) * std::max<int64_t>(1, p->vt_len)));
+        PCHECK(hipMalloc(&p->d_info, sizeof(lsvd_info_t) * std::max(1, count)));
+    }
+#undef PCHECK
+    p->ptr_mode = h->ptr_mode;
+    *out = p;
+    return LSVD_OK;
+}
+
+namespace lsvd {
+
+// Enqueues the whole plan on `st` with the given device pools.
+hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
+    hipError_t e = launch_reset_counters(p->d_counters, p->n_counters, st);
+    if (e != hipSuccess) return e;
+    const int nb = (int)p->buckets.size();
+    int* fb_count = p->d_counters + nb + 1;
+    if (p->n_micro > 0) {
+        MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, fb_count};
+        e = launch_micro(ml, st);
+        if (e != hipSuccess) return e;
+    }
+    for (int i = 0; i < nb; ++i) {
+        const Bucket& bk = p->buckets[i];
+        SmallLaunch L{};
+        L.grid = bk.grid;
+        L.lds_bytes = bk.lds_bytes;
+        L.items = p->d_items + bk.first;
+        L.count_ptr = p->d_consts + i;
+        L.counter = p->d_counters + i;
+        L.pools = pools;
+        L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
+        L.gws_stride = bk.gws_stride;
+        e = launch_small_variant(bk.variant, L, st);
+        if (e != hipSuccess) return e;
+    }
+    if (p->n_micro > 0) {
+        SmallLaunch L{};
+        L.grid = p->fallback_grid;
+        L.lds_bytes = p->fallback_lds;
+        L.items = p->d_items + p->fallback_first;
+        L.count_ptr = fb_count;
+        L.counter = p->d_counters + nb;
+        L.pools = pools;
+        e = launch_small_variant(0, L, st);
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+}  // namespace lsvd
+
+static int plan_run_device(lsvd_plan_t p, const Pools& pools, bool use_graph) {
+    lsvd_handle_t h = p->h;
+    hipStream_t st = h->stream;
+    hipError_t e;
+    if (!use_graph) {
+        e = plan_enqueue(p, pools, st);
+        return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "kernel launch");
+    }
+    const bool same = p->graph_exec && p->graph_pools.a == pools.a && p->graph_pools.u == pools.u &&
+                      p->graph_pools.s == pools.s && p->graph_pools.vt == pools.vt &&
+                      p->graph_pools.info == pools.info;
+    if (!same) {
+        if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+        if (p->graph) hipGraphDestroy(p->graph);
+        p->graph_exec = nullptr;
+        p->graph = nullptr;
+        e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) return hip_fail(h, e, "hipStreamBeginCapture");
+        hipError_t le = plan_enqueue(p, pools, st);
+        e = hipStreamEndCapture(st, &p->graph);
+        if (le != hipSuccess) return hip_fail(h, le, "kernel launch (capture)");
+        if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
+        e = hipGraphInstantiate(&p->graph_exec, p->graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) return hip_fail(h, e, "hipGraphInstantiate");
+        p->graph_pools = pools;
+    }
+    e = hipGraphLaunch(p->graph_exec, st);
+    return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "hipGraphLaunch");
+}
+
+static int plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool, double* vt_pool,
+                        lsvd_info_t* info, bool use_graph) {
+    if (!p) return LSVD_E_ARG;
+    lsvd_handle_t h = p->h;
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    if (p->count > 0 && (!a_pool || !u_pool || !s_pool || (!vt_pool && p->vt_len > 0)))
+        return set_error(h, LSVD_E_ARG, "null pool pointer");
+    if (p->ptr_mode == LSVD_PTR_DEVICE) {
+        Pools pools{a_pool, u_pool, s_pool, vt_pool, info};
+        return plan_run_device(p, pools, use_graph);
+    }
+    hipStream_t st = h->stream;
+#define ECHECK(expr)                                       \
+    do {                                                   \
+        hipError_t e_ = (expr);                            \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+    if (p->a_len) ECHECK(hipMemcpyAsync(p->d_a, a_pool, sizeof(double) * p->a_len, hipMemcpyHostToDevice, st));
+    Pools pools{p->d_a, p->d_u, p->d_s, p->d_vt, info ? p->d_info : nullptr};
+    int rc = plan_run_device(p, pools, use_graph);
+    if (rc != LSVD_OK) return rc;
+    if (p->u_len) ECHECK(hipMemcpyAsync(u_pool, p->d_u, sizeof(double) * p->u_len, hipMemcpyDeviceToHost, st));
+    if (p->s_len) ECHECK(hipMemcpyAsync(s_pool, p->d_s, sizeof(double) * p->s_len, hipMemcpyDeviceToHost, st));
+    if (p->vt_len) ECHECK(hipMemcpyAsync(vt_pool, p->d_vt, sizeof(double) * p->vt_len, hipMemcpyDeviceToHost, st));
+    if (info && p->count)
+        ECHECK(hipMemcpyAsync(info, p->d_info, sizeof(lsvd_info_t) * p->count, hipMemcpyDeviceToHost, st));
+    ECHECK(hipStreamSynchronize(st));
+#undef ECHECK
+    return LSVD_OK;
+}
+
+// ---- C ABI ------------------------------------------------------------------
+
+extern "C" {
+
+const char* lsvd_version(void) { return LSVD_VERSION; }
+
+const char* lsvd_status_string(int status) {
+    switch (status) {
+        case LSVD_OK: return "ok";
+        case LSVD_E_SHAPE: return "shape error";
+        case LSVD_E_HIP: return "HIP runtime error";
+        case LSVD_E_NONFINITE: return "non-finite input";
+        case LSVD_E_NOCONV: return "no convergence";
+        case LSVD_E_ARG: return "invalid argument";
+        case LSVD_E_NOMEM: return "out of device memory";
+        case LSVD_E_LIMIT: return "size limit exceeded";
+    }
+    return "unknown status";
+}
+
+int lsvd_create(int device, lsvd_handle_t* out) {
+    if (!out) return LSVD_E_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return LSVD_E_HIP;
+    if (device < 0 || device >= ndev) return LSVD_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return LSVD_E_HIP;
+    auto* h = new lsvd_handle_s();
+    h->device = device;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) {
+        delete h;
+        return LSVD_E_HIP;
+    }
+    h->num_cus = prop.multiProcessorCount;
+    h->arch = prop.gcnArchName;
+    if (hipStreamCreateWithFlags(&h->own_stream, hipStreamNonBlocking) != hipSuccess) {
+        delete h;
+        return LSVD_E_HIP;
+    }
+    h->stream = h->own_stream;
+    *out = h;
+    return LSVD_OK;
+}
+
+int lsvd_destroy(lsvd_handle_t h) {
+    if (!h) return LSVD_E_ARG;
+    hipSetDevice(h->device);
+    for (auto& kv : h->plan_cache) lsvd_batch_plan_destroy(kv.second);
+    h->plan_cache.clear();
+    lsvd_release_extras(h);
+    if (h->own_stream) hipStreamDestroy(h->own_stream);
+    delete h;
+    return LSVD_OK;
+}
+
+int lsvd_set_stream(lsvd_handle_t h, void* stream) {
+    if (!h) return LSVD_E_ARG;
+    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    return LSVD_OK;
+}
+
+void* lsvd_get_stream(lsvd_handle_t h) { return h ? (void*)h->stream : nullptr; }
+
+int lsvd_set_pointer_mode(lsvd_handle_t h, int mode) {
+    if (!h) return LSVD_E_ARG;
+    if (mode != LSVD_PTR_HOST && mode != LSVD_PTR_DEVICE) return set_error(h, LSVD_E_ARG, "unknown pointer mode");
+    if (mode != h->ptr_mode) {
+        for (auto& kv : h->plan_cache) lsvd_batch_plan_destroy(kv.second);
+        h->plan_cache.clear();
+    }
+    h->ptr_mode = mode;
+    return LSVD_OK;
+}
+
+int lsvd_synchronize(lsvd_handle_t h) {
+    if (!h) return LSVD_E_ARG;
+    hipError_t e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "hipStreamSynchronize");
+}
+
+const char* lsvd_last_error(lsvd_handle_t h) { return h ? h->last_error.c_str() : ""; }
+
+int lsvd_batch_plan_create(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                           const int64_t* u_off, const int64_t* s_off, const int64_t* vt_off, int precision,
+                           lsvd_plan_t* out) {
+    if (!h || !out || (count > 0 && (!m || !n || !a_off || !u_off || !s_off || !vt_off))) return LSVD_E_ARG;
+    std::vector<int32_t> modes(count);
+    for (int b = 0; b < count; ++b) modes[b] = (m[b] == n[b] && n[b] <= 4) ? kModeMicro : kModeSvd;
+    return plan_build(h, count, m, n, a_off, u_off, s_off, vt_off, modes.data(), precision, out);
+}
+
+int lsvd_batch_plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool, double* vt_pool,
+                            lsvd_info_t* info) {
+    return plan_execute(p, a_pool, u_pool, s_pool, vt_pool, info, true);
+}
+
+int lsvd_batch_plan_destroy(lsvd_plan_t p) {
+    if (!p) return LSVD_E_ARG;
+    hipSetDevice(p->h->device);
+    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) hipGraphDestroy(p->graph);
+    void* ptrs[] = {p->d_items, p->d_counters, p->d_consts, p->d_gws, p->d_a, p->d_u, p->d_s, p->d_vt, p->d_info};
+    for (void* q : ptrs)
+        if (q) hipFree(q);
+    delete p;
+    return LSVD_OK;
+}
+
+int lsvd_svd_batched_f64(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                         const double* a_pool, const int64_t* u_off, double* u_pool, const int64_t* s_off,
+                         double* s_pool, const int64_t* vt_off, double* vt_pool, int precision,
+                         lsvd_info_t* info) {
+    lsvd_plan_t p = nullptr;
+    int rc = lsvd_batch_plan_create(h, count, m, n, a_off, u_off, s_off, vt_off, precision, &p);
+    if (rc != LSVD_OK) return rc;
+    rc = plan_execute(p, a_pool, u_pool, s_pool, vt_pool, info, false);
+    if (rc == LSVD_OK && h->ptr_mode == LSVD_PTR_DEVICE) {
+        hipError_t e = hipStreamSynchronize(h->stream);  // the plan's buffers are freed below
+        if (e != hipSuccess) rc = hip_fail(h, e, "hipStreamSynchronize");
+    }
+    lsvd_batch_plan_destroy(p);
+    return rc;
+}
+
+// Autotuner: for every bucket of the batch, times each compatible kernel
+// variant on synthetic inputs of the bucket's shapes (best of 3 runs) and
+// records the fastest in the handle's cache, keyed by (maxN, arena, precision).
+int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, int precision) {
+    if (!h || count < 0 || (count > 0 && (!m || !n))) return LSVD_E_ARG;
+    std::vector<int64_t> a_off(count), u_off(count), s_off(count), vt_off(count);
+    int64_t a = 0, u = 0, sv = 0, vt = 0;
+    for (int b = 0; b < count; ++b) {
+        const int k = std::min(m[b], n[b]);
+        a_off[b] = a; u_off[b] = u; s_off[b] = sv; vt_off[b] = vt;
+        a += (int64_t)m[b] * n[b]; u += (int64_t)m[b] * k; sv += k; vt += (int64_t)k * n[b];
+    }
+    const int saved_mode = h->ptr_mode;
+    h->ptr_mode = LSVD_PTR_DEVICE;
+    lsvd_plan_t p = nullptr;
+    std::vector<int32_t> modes(count, kModeSvd);
+    int rc = plan_build(h, count, m, n, a_off.data(), u_off.data(), s_off.data(), vt_off.data(), modes.data(),
+                        precision, &p);
+    h->ptr_mode = saved_mode;
+    if (rc != LSVD_OK) return rc;
+    std::vector<double> host(std::max<int64_t>(a, 1));
+    uint64_t x = 0x5EED7E57ull;
+    for (auto& v : host) {  // uniform(-1, 1) from a SplitMix64 stream
+        x += 0x9E3779B97F4A7C15ull;
+        uint64_t z = x;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        v = (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    }
+    double *da = nullptr, *du = nullptr, *ds = nullptr, *dvt = nullptr, *gws = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        for (double* q : {da, du, ds, dvt, gws})
+            if (q) hipFree(q);
+        if (e0) hipEventDestroy(e0);
+        if (e1) hipEventDestroy(e1);
+        lsvd_batch_plan_destroy(p);
+    };
+#define TCHECK(expr)                                  \
+    do {                                              \
+        hipError_t e_ = (expr);                       \
+        if (e_ != hipSuccess) {                       \
+            int r_ = hip_fail(h, e_, #expr);          \
+            cleanup();                                \
+            return r_;                                \
+        }                                             \
+    } while (0)
+    TCHECK(hipMalloc(&da, sizeof(double) * host.size()));
+    TCHECK(hipMalloc(&du, sizeof(double) * std::max<int64_t>(u, 1)));
+    TCHECK(hipMalloc(&ds, sizeof(double) * std::max<int64_t>(sv, 1)));
+    TCHECK(hipMalloc(&dvt, sizeof(double) * std::max<int64_t>(vt, 1)));
+    TCHECK(hipMemcpy(da, host.data(), sizeof(double) * host.size(), hipMemcpyHostToDevice));
+    TCHECK(hipEventCreate(&e0));
+    TCHECK(hipEventCreate(&e1));
+    Pools pools{da, du, ds, dvt, nullptr};
+    int nv = 0;
+    const SmallVariant* vs = small_variants(&nv);
+    for (size_t bi = 0; bi < p->buckets.size(); ++bi) {
+        Bucket base = p->buckets[bi];
+        int best = base.variant;
+        float best_ms = 1e30f;
+        for (int vi = 0; vi < nv; ++vi) {
+            if (!variant_accepts(vs[vi].id, base.maxN, base.arena_bytes)) continue;
+            Bucket bk = base;
+            const size_t need = configure_bucket(h, bk, vs[vi].id);
+            if (gws) {
+                hipFree(gws);
+                gws = nullptr;
+            }
+            if (need) TCHECK(hipMalloc(&gws, sizeof(double) * need));
+            float t_min = 1e30f;
+            for (int rep = 0; rep < 3; ++rep) {
+                TCHECK(hipEventRecord(e0, h->stream));
+                TCHECK(launch_reset_counters(p->d_counters, p->n_counters, h->stream));
+                SmallLaunch L{};
+                L.grid = bk.grid;
+                L.lds_bytes = bk.lds_bytes;
+                L.items = p->d_items + bk.first;
+                L.count_ptr = p->d_consts + bi;
+                L.counter = p->d_counters + bi;
+                L.pools = pools;
+                L.gws = gws;
+                L.gws_stride = bk.gws_stride;
+                TCHECK(launch_small_variant(bk.variant, L, h->stream));
+                TCHECK(hipEventRecord(e1, h->stream));
+                TCHECK(hipEventSynchronize(e1));
+                float ms = 0.f;
+                TCHECK(hipEventElapsedTime(&ms, e0, e1));
+                t_min = std::min(t_min, ms);
+            }
+            if (t_min < best_ms) {
+                best_ms = t_min;
+                best = vs[vi].id;
+            }
+        }
+        h->tune[tune_key(base.maxN, base.arena_bytes, precision)] = best;
+    }
+#undef TCHECK
+    cleanup();
+    return LSVD_OK;
+}
+
+int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, int* nbuckets, int32_t* order) {
+    if (!p || !nbuckets) return LSVD_E_ARG;
+    std::vector<lsvd_bucket_desc_t> d;
+    if (p->n_micro > 0) d.push_back({0, p->n_micro, 4, -1, (p->n_micro + 255) / 256, 256, 0});
+    for (auto& bk : p->buckets) {
+        const SmallVariant* v = find_variant(bk.variant);
+        d.push_back({bk.first, bk.count, bk.maxN, bk.variant, bk.grid, v ? v->nt : 0, (int64_t)bk.lds_bytes});
+    }
+    *nbuckets = (int)d.size();
+    for (int i = 0; i < (int)d.size() && i < max && out; ++i) out[i] = d[i];
+    if (order) {
+        std::vector<Item> items(p->fallback_first);
+        if (p->fallback_first) {
+            hipError_t e = hipMemcpy(items.data(), p->d_items, sizeof(Item) * items.size(), hipMemcpyDeviceToHost);
+            if (e != hipSuccess) return hip_fail(p->h, e, "hipMemcpy");
+        }
+        for (int i = 0; i < p->fallback_first; ++i) order[i] = items[i].idx;
+    }
+    return LSVD_OK;
+}
+
+int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
+                                 double* vt_pool, float* ms, int max) {
+    if (!p || !ms) return LSVD_E_ARG;
+    lsvd_handle_t h = p->h;
+    if (p->ptr_mode != LSVD_PTR_DEVICE) return set_error(h, LSVD_E_ARG, "time_buckets needs device pointers");
+    hipStream_t st = h->stream;
+    Pools pools{a_pool, u_pool, s_pool, vt_pool, nullptr};
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return set_error(h, LSVD_E_HIP, "hipEventCreate");
+    const int nb = (int)p->buckets.size();
+    int slot = 0;
+    int rc = LSVD_OK;
+    auto timed = [&](auto&& launch) {
+        hipError_t e = launch_reset_counters(p->d_counters, p->n_counters, st);
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        if (e == hipSuccess) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e != hipSuccess) rc = hip_fail(h, e, "bucket timing");
+        if (slot < max) ms[slot] = t;
+        ++slot;
+    };
+    if (p->n_micro > 0) {
+        timed([&]() {
+            MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, p->d_counters + nb + 1};
+            return launch_micro(ml, st);
+        });
+    }
+    for (int i = 0; i < nb && rc == LSVD_OK; ++i) {
+        const Bucket& bk = p->buckets[i];
+        timed([&]() {
+            SmallLaunch L{};
+            L.grid = bk.grid;
+            L.lds_bytes = bk.lds_bytes;
+            L.items = p->d_items + bk.first;
+            L.count_ptr = p->d_consts + i;
+            L.counter = p->d_counters + i;
+            L.pools = pools;
+            L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
+            L.gws_stride = bk.gws_stride;
+            return launch_small_variant(bk.variant, L, st);
+        });
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+// Single-matrix entry points: a cached one-item plan per (mode, m, n).
+static int single(lsvd_handle_t h, int mode, const double* a, int m, int n, double* u, double* s, double* vt,
+                  lsvd_info_t* info) {
+    if (!h) return LSVD_E_ARG;
+    if (m < 0 || n < 0) return set_error(h, LSVD_E_SHAPE, "negative dimension");
+    const int k = std::min(m, n);
+    if (mode == kModeSvd && (m == 0 || n == 0))
+        return set_error(h, LSVD_E_SHAPE, "LieSvdSmall::solve_rectangular: empty input");
+    if (k == 0 && mode != kModeSvd) {
+        if (info) *info = lsvd_info_t{};
+        return LSVD_OK;
+    }
+    if (!a || !u || !s || (mode != kModeEigh && !vt)) return set_error(h, LSVD_E_ARG, "null pointer");
+    const uint64_t key = ((uint64_t)mode << 60) | ((uint64_t)(uint32_t)m << 30) | (uint64_t)(uint32_t)n;
+    lsvd_plan_t p = nullptr;
+    auto it = h->plan_cache.find(key);
+    if (it != h->plan_cache.end()) {
+        p = it->second;
+    } else {
+        int32_t mm = m, nn = n, md = mode;
+        int64_t zero = 0;
+        int rc = plan_build(h, 1, &mm, &nn, &zero, &zero, &zero, &zero, &md, LSVD_PREC_F64, &p);
+        if (rc != LSVD_OK) return rc;
+        if (h->plan_cache.size() > 64) {
+            for (auto& kv : h->plan_cache) lsvd_batch_plan_destroy(kv.second);
+            h->plan_cache.clear();
+        }
+        h->plan_cache[key] = p;
+    }
+    lsvd_info_t tmp_info;
+    int rc = plan_execute(p, a, u, s, vt, info ? (h->ptr_mode == LSVD_PTR_HOST ? &tmp_info : info) : nullptr, false);
+    if (rc == LSVD_OK && info && h->ptr_mode == LSVD_PTR_HOST) *info = tmp_info;
+    return rc;
+}
+
+int lsvd_small_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
+                         lsvd_info_t* info) {
+    if (!h) return LSVD_E_ARG;
+    if (n == 0) {  // the reference's solve accepts 0 x 0 and returns empty factors
+        if (info) *info = lsvd_info_t{};
+        return LSVD_OK;
+    }
+    return single(h, kModeSvd, a, n, n, u, s, vt, info);
+}
+
+int lsvd_small_solve_rectangular_f64(lsvd_handle_t h, const double* a, int m, int n, double* u, double* s,
+                                     double* vt, lsvd_info_t* info) {
+    return single(h, kModeSvd, a, m, n, u, s, vt, info);
+}
+
+int lsvd_eigh_jacobi_f64(lsvd_handle_t h, const double* p, int n, double* v, double* eig, lsvd_info_t* info) {
+    return single(h, kModeEigh, p, n, n, v, eig, nullptr, info);
+}
+
+int lsvd_micro_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
+                         lsvd_info_t* info) {
+    if (h && n > 4) return set_error(h, LSVD_E_SHAPE, "LieSvdMicro only supports N <= 4");
+    return single(h, kModeMicro, a, n, n, u, s, vt, info);
+}
+
+}  // extern "C"

@@ -1,0 +1,79 @@
+// lsvd_api_internal.hpp -- handle and plan structures of the host library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <string>
+#include <vector>
+
+#include "lsvd_kernels.hpp"
+
+namespace lsvd {
+
+struct Bucket {
+    int first;           // index of the first item in the device item array
+    int count;
+    int maxN;
+    size_t arena_bytes;  // largest per-matrix arena in the bucket
+    int variant;         // small_svd variant id
+    int grid;
+    size_t lds_bytes;
+    int64_t gws_offset;  // doubles into the plan's global arena
+    int64_t gws_stride;
+};
+
+inline uint64_t tune_key(int maxN, size_t arena_bytes, int precision) {
+    return ((uint64_t)precision << 56) | ((uint64_t)maxN << 32) | (uint64_t)(arena_bytes & 0xffffffffu);
+}
+
+struct Extras;  // lazily created state of the polar / joint / vjp paths
+
+int set_error(struct lsvd_handle_s* h, int code, const std::string& msg);
+int hip_fail(struct lsvd_handle_s* h, hipError_t e, const char* what);
+
+}  // namespace lsvd
+
+struct lsvd_handle_s {
+    int device = 0;
+    int num_cus = 1;
+    std::string arch;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    int ptr_mode = LSVD_PTR_HOST;
+    std::string last_error;
+    std::map<uint64_t, int> tune;                 // autotune winners: bucket key -> variant
+    std::map<uint64_t, lsvd_plan_s*> plan_cache;  // one-item plans of the single-matrix API
+    lsvd::Extras* extras = nullptr;
+};
+
+struct lsvd_plan_s {
+    lsvd_handle_t h = nullptr;
+    int count = 0;
+    int precision = 0;
+    int ptr_mode = 0;
+    int n_micro = 0;
+    int n_items = 0;
+    int fallback_first = 0;
+    int fallback_grid = 1;
+    size_t fallback_lds = 0;
+    int n_counters = 0;
+    std::vector<lsvd::Bucket> buckets;
+    int64_t a_len = 0, u_len = 0, s_len = 0, vt_len = 0;
+    lsvd::Item* d_items = nullptr;
+    int* d_counters = nullptr;
+    int* d_consts = nullptr;
+    double* d_gws = nullptr;
+    double *d_a = nullptr, *d_u = nullptr, *d_s = nullptr, *d_vt = nullptr;
+    lsvd_info_t* d_info = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    lsvd::Pools graph_pools = {};
+};
+
+namespace lsvd {
+hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st);
+}
+
+// Frees the handle's polar / joint / vjp state (defined with those paths).
+void lsvd_release_extras(lsvd_handle_t h);

@@ -1,0 +1,53 @@
+// lsvd_kernels.hpp -- host-visible launch descriptors of the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "lsvd_types.hpp"
+
+namespace lsvd {
+
+// ---- batched LieSvdSmall (small_svd.hip) --------------------------------
+
+struct SmallVariant {
+    int id;
+    int nt;     // threads per workgroup
+    int maxn;   // largest N (after transposing wide inputs) it accepts
+    bool lds;   // arena in LDS (true) or in a global per-workgroup slot
+};
+
+struct SmallLaunch {
+    int grid;
+    size_t lds_bytes;      // dynamic LDS (arena) per workgroup, LDS variants
+    const Item* items;     // device, sorted work queue
+    const int* count_ptr;  // device, number of queue entries
+    int* counter;          // device, zeroed before the launch
+    Pools pools;
+    double* gws;           // global arena slots (global variants)
+    int64_t gws_stride;    // doubles per slot
+};
+
+// Arena rows for a tall-ified problem M x N (M >= N); ld = N | 1.
+inline int small_arena_rows(int M, int N) { return (3 * N > M + N) ? 3 * N : M + N; }
+inline size_t small_arena_doubles(int M, int N) { return (size_t)small_arena_rows(M, N) * (size_t)(N | 1); }
+
+const SmallVariant* small_variants(int* count);
+hipError_t launch_small_variant(int id, const SmallLaunch& L, hipStream_t st);
+int small_variant_blocks_per_cu(int id, size_t lds_bytes);
+hipError_t launch_reset_counters(int* counters, int n, hipStream_t st);
+
+// ---- LieSvdMicro (micro_svd.hip) ----------------------------------------
+
+struct MicroLaunch {
+    const Item* items;     // device, count entries, all n <= 4 square
+    int count;
+    Pools pools;
+    Item* fallback;        // device, items escalated to LieSvdSmall
+    int* fallback_count;   // device, zeroed before the launch
+};
+
+hipError_t launch_micro(const MicroLaunch& L, hipStream_t st);
+
+}  // namespace lsvd

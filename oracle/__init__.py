@@ -1,0 +1,281 @@
+"""CPU FP64 oracle for the dense SVD path -- TEST INFRASTRUCTURE ONLY.
+
+This package wraps ``liblsvd_oracle.so`` (built from ``lsvd_oracle.c`` by
+``oracle/Makefile``), a C restatement of the reference Rust crate
+skovnats/lie_cliffalg_analog_svd:
+
+- ``src/lie_svd_small.rs:67-418``  (LieSvdSmall: NS polar, Jacobi, QR)
+- ``src/lie_svd_micro.rs:15-214``  (LieSvdMicro: n <= 4)
+- ``src/lie_svd_joint.rs:18-539``  (LieSvdJoint: Phase-JADE)
+- ``src/lib.rs:149-194``           (metrics::compute)
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this package, and only as the checker or as
+the timed CPU baseline.  The product package never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liblsvd_oracle.so")
+_lib = None
+
+
+class Info(C.Structure):
+    _fields_ = [
+        ("polar_ok", C.c_int),
+        ("ns_iters", C.c_int),
+        ("jacobi_sweeps", C.c_int),
+        ("repaired", C.c_int),
+        ("micro_fallback", C.c_int),
+        ("rotations", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class JointParams(C.Structure):
+    _fields_ = [
+        ("max_sweeps", C.c_int),
+        ("tol", C.c_double),
+        ("min_pair_scale", C.c_double),
+        ("line_search_steps", C.c_int),
+    ]
+
+
+class JointTrace(C.Structure):
+    _fields_ = [
+        ("initial_offdiag", C.c_double),
+        ("final_offdiag", C.c_double),
+        ("sweeps", C.c_int),
+        ("rotations", C.c_int),
+        ("rejected_rotations", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+def build() -> None:
+    """Compile the oracle with its Makefile (gcc, no GPU needed)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        dp = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        i64p = C.POINTER(C.c_int64)
+        L.lso_small_solve.argtypes = [dp, C.c_int, dp, dp, dp, C.POINTER(Info)]
+        L.lso_small_solve_rectangular.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp, C.POINTER(Info)]
+        L.lso_eigh_jacobi_full.argtypes = [dp, C.c_int, dp, dp, C.POINTER(Info)]
+        L.lso_newton_schulz_polar.argtypes = [dp, C.c_int, C.c_int, dp, ip]
+        L.lso_newton_schulz_polar.restype = C.c_int
+        L.lso_gram_schmidt_orthonormal.argtypes = [dp, C.c_int, dp]
+        L.lso_orthogonality_err.argtypes = [dp, C.c_int, C.c_int]
+        L.lso_orthogonality_err.restype = C.c_double
+        L.lso_qr_reduce.argtypes = [dp, C.c_int, C.c_int, dp, dp]
+        L.lso_micro_solve.argtypes = [dp, C.c_int, dp, dp, dp, C.POINTER(Info)]
+        L.lso_micro_solve.restype = C.c_int
+        L.lso_joint_params_for_n.argtypes = [C.c_int]
+        L.lso_joint_params_for_n.restype = JointParams
+        L.lso_joint_diag_sym.argtypes = [dp, C.c_int, C.c_int, JointParams, dp, dp, C.POINTER(JointTrace)]
+        L.lso_layer_count.argtypes = [C.c_int]
+        L.lso_layer_count.restype = C.c_int
+        L.lso_layer_pairs.argtypes = [C.c_int, C.c_int, ip]
+        L.lso_layer_pairs.restype = C.c_int
+        L.lso_metrics.argtypes = [dp, C.c_int, dp, dp, dp, dp, dp]
+        L.lso_svd_batch.argtypes = [C.c_int, ip, ip, i64p, dp, i64p, dp, i64p, dp, i64p, dp, C.POINTER(Info),
+                                    C.c_int]
+        L.lso_joint_batch.argtypes = [C.c_int, C.c_int, C.c_int, dp, JointParams, dp, dp,
+                                      C.POINTER(JointTrace), C.c_int]
+        L.lso_ns_iterations.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _dp(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _c(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def small_solve(a):
+    """LieSvdSmall::solve (src/lie_svd_small.rs:296-323) -> (U, s, Vt, info)."""
+    a = _c(a)
+    n = a.shape[0]
+    assert a.shape == (n, n), "LieSvdSmall::solve: matrix must be square N x N"
+    u = np.zeros((n, n)); s = np.zeros(n); vt = np.zeros((n, n))
+    info = Info()
+    lib().lso_small_solve(_dp(a), n, _dp(u), _dp(s), _dp(vt), C.byref(info))
+    return u, s, vt, info.as_dict()
+
+
+def small_solve_rectangular(a):
+    """LieSvdSmall::solve_rectangular (src/lie_svd_small.rs:340-360)."""
+    a = _c(a)
+    m, n = a.shape
+    assert m > 0 and n > 0, "LieSvdSmall::solve_rectangular: empty input"
+    k = min(m, n)
+    u = np.zeros((m, k)); s = np.zeros(k); vt = np.zeros((k, n))
+    info = Info()
+    lib().lso_small_solve_rectangular(_dp(a), m, n, _dp(u), _dp(s), _dp(vt), C.byref(info))
+    return u, s, vt, info.as_dict()
+
+
+def eigh_jacobi_full(p):
+    """eigh_jacobi_full (src/lie_svd_small.rs:212-285) -> (V, eig, info)."""
+    p = _c(p)
+    n = p.shape[0]
+    v = np.zeros((n, n)); e = np.zeros(n)
+    info = Info()
+    lib().lso_eigh_jacobi_full(_dp(p), n, _dp(v), _dp(e), C.byref(info))
+    return v, e, info.as_dict()
+
+
+def newton_schulz_polar(a):
+    """newton_schulz_polar (src/lie_svd_small.rs:158-183) -> (Q, ok, iters)."""
+    a = _c(a)
+    m, n = a.shape
+    q = np.zeros((m, n))
+    it = C.c_int(0)
+    ok = lib().lso_newton_schulz_polar(_dp(a), m, n, _dp(q), C.byref(it))
+    return q, bool(ok), it.value
+
+
+def gram_schmidt_orthonormal(a):
+    a = _c(a)
+    n = a.shape[0]
+    out = np.zeros((n, n))
+    lib().lso_gram_schmidt_orthonormal(_dp(a), n, _dp(out))
+    return out
+
+
+def qr_reduce(a):
+    """qr_reduce (src/lie_svd_small.rs:373-418) -> (Q, R)."""
+    a = _c(a)
+    m, d = a.shape
+    q = np.zeros((m, d)); r = np.zeros((d, d))
+    lib().lso_qr_reduce(_dp(a), m, d, _dp(q), _dp(r))
+    return q, r
+
+
+def micro_solve(a):
+    """LieSvdMicro::solve (src/lie_svd_micro.rs:21-52) -> (U, s, Vt, info)."""
+    a = _c(a)
+    n = a.shape[0]
+    assert a.shape == (n, n) and n <= 4
+    u = np.zeros((n, n)); s = np.zeros(n); vt = np.zeros((n, n))
+    info = Info()
+    rc = lib().lso_micro_solve(_dp(a), n, _dp(u), _dp(s), _dp(vt), C.byref(info))
+    assert rc == 0
+    return u, s, vt, info.as_dict()
+
+
+def joint_params_for_n(n: int) -> JointParams:
+    return lib().lso_joint_params_for_n(n)
+
+
+def joint_diag_sym(mats, params: JointParams | None = None):
+    """LieSvdJoint::diagonalize_symmetric_with_params (src/lie_svd_joint.rs:71-158).
+
+    mats: (K, n, n).  Returns (V, diagonals (K, n), trace dict)."""
+    mats = _c(mats)
+    k, n, n2 = mats.shape
+    assert n == n2
+    if params is None:
+        params = joint_params_for_n(n)
+    v = np.zeros((n, n)); d = np.zeros((k, n))
+    tr = JointTrace()
+    lib().lso_joint_diag_sym(_dp(mats), k, n, params, _dp(v), _dp(d), C.byref(tr))
+    return v, d, tr.as_dict()
+
+
+def layer_pairs(n: int, layer: int):
+    """layer_pairs (src/lie_svd_joint.rs:524-539)."""
+    buf = (C.c_int * (n + 2))()
+    cnt = lib().lso_layer_pairs(n, layer, buf)
+    return [(buf[2 * i], buf[2 * i + 1]) for i in range(cnt)]
+
+
+def layer_count(n: int) -> int:
+    return lib().lso_layer_count(n)
+
+
+def metrics(a, u, s, vt, sigma_ref=None):
+    """metrics::compute (src/lib.rs:149-194) -> dict."""
+    a, u, s, vt = _c(a), _c(u), _c(s), _c(vt)
+    n = a.shape[0]
+    out = np.zeros(5)
+    sr = _dp(_c(sigma_ref)) if sigma_ref is not None else None
+    lib().lso_metrics(_dp(a), n, _dp(u), _dp(s), _dp(vt), sr, _dp(out))
+    res = {"rel_recon": out[0], "orth_u": out[1], "orth_v": out[2]}
+    if sigma_ref is not None:
+        res["sigma_max_rel"] = out[3]
+        res["sigma_tail_rel"] = out[4]
+    return res
+
+
+def svd_batch(mats, threads: int = 0, with_info: bool = False):
+    """Ragged batch through micro (square n <= 4) or solve_rectangular, one
+    matrix per OpenMP thread.  Returns a list of (U, s, Vt), plus the
+    per-matrix info dicts when with_info."""
+    count = len(mats)
+    ms = np.array([a.shape[0] for a in mats], dtype=np.int32)
+    ns = np.array([a.shape[1] for a in mats], dtype=np.int32)
+    ks = np.minimum(ms, ns)
+    a_sz = ms.astype(np.int64) * ns
+    u_sz = ms.astype(np.int64) * ks
+    vt_sz = ks.astype(np.int64) * ns
+    offs = lambda sz: np.concatenate([[0], np.cumsum(sz)[:-1]]).astype(np.int64)
+    a_off, u_off, s_off, vt_off = offs(a_sz), offs(u_sz), offs(ks.astype(np.int64)), offs(vt_sz)
+    a_pool = np.concatenate([_c(a).ravel() for a in mats]) if count else np.zeros(1)
+    u_pool = np.zeros(max(1, int(u_sz.sum()))); s_pool = np.zeros(max(1, int(ks.sum())))
+    vt_pool = np.zeros(max(1, int(vt_sz.sum())))
+    ip = lambda x: x.ctypes.data_as(C.POINTER(C.c_int))
+    lp = lambda x: x.ctypes.data_as(C.POINTER(C.c_int64))
+    infos = (Info * max(1, count))()
+    lib().lso_svd_batch(count, ip(ms), ip(ns), lp(a_off), _dp(a_pool), lp(u_off), _dp(u_pool),
+                        lp(s_off), _dp(s_pool), lp(vt_off), _dp(vt_pool), infos, threads)
+    out = []
+    for b in range(count):
+        m, n, k = int(ms[b]), int(ns[b]), int(ks[b])
+        out.append((u_pool[u_off[b]:u_off[b] + m * k].reshape(m, k),
+                    s_pool[s_off[b]:s_off[b] + k].copy(),
+                    vt_pool[vt_off[b]:vt_off[b] + k * n].reshape(k, n)))
+    if with_info:
+        return out, [infos[b].as_dict() for b in range(count)]
+    return out
+
+
+def joint_batch(families, params: JointParams | None = None, threads: int = 0):
+    """families: (F, K, n, n) -> (V (F,n,n), diag (F,K,n), traces)."""
+    families = _c(families)
+    f, k, n, _ = families.shape
+    if params is None:
+        params = joint_params_for_n(n)
+    v = np.zeros((f, n, n)); d = np.zeros((f, k, n))
+    traces = (JointTrace * f)()
+    lib().lso_joint_batch(f, k, n, _dp(families), params, _dp(v), _dp(d), traces, threads)
+    return v, d, [t.as_dict() for t in traces]
+
+
+def ns_iterations(x, iters: int, threads: int = 0):
+    """Runs exactly `iters` NS iterations in place (CPU baseline sample)."""
+    assert x.dtype == np.float64 and x.flags.c_contiguous
+    m, n = x.shape
+    lib().lso_ns_iterations(_dp(x), m, n, iters, threads)
+    return x

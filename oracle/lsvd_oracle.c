@@ -1,0 +1,863 @@
+/*
+ * lsvd_oracle.c -- CPU FP64 restatement of the reference dense SVD path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the parity oracle: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it, and
+ * only as the checker or as the timed CPU baseline.  The product library
+ * (lie_cliffalg_analog_svd_amd) never links, loads or calls it.
+ *
+ * The reference is the Rust crate skovnats/lie_cliffalg_analog_svd v0.44.0.
+ * Every function cites the file:line it restates.  Loop orders, constants,
+ * stopping tests, fallbacks and sort/sign conventions follow the Rust text.
+ * Two deliberate, documented differences:
+ *   - GEMMs (ndarray::dot -> matrixmultiply 0.3.11 in the reference) are
+ *     plain i-k-j loops here, so the summation order differs and results
+ *     agree to rounding, not bit for bit.
+ *   - Random inputs come from the caller (SplitMix64/Box-Muller in the
+ *     Python test helpers), not from rand 0.8's ChaCha12 StdRng.
+ * Compile with -ffp-contract=off (Rust never contracts a*b+c into an FMA).
+ *
+ * Storage: row-major (C order) double arrays, like ndarray's default
+ * Array2<f64> layout.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define LSO_POLAR_MAX_ITERS 120 /* src/lie_svd_small.rs:67 */
+#define LSO_POLAR_TOL 1e-13     /* src/lie_svd_small.rs:68 */
+#define LSO_JACOBI_TOL 1e-13    /* src/lie_svd_small.rs:69 */
+#define LSO_MICRO_TOL 1e-13     /* src/lie_svd_micro.rs:15 */
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct {
+    int polar_ok;      /* newton_schulz_polar's converged flag */
+    int ns_iters;      /* Newton-Schulz iterations taken */
+    int jacobi_sweeps; /* eigh_jacobi_full sweeps that ran rotations */
+    int repaired;      /* repair_left_basis_from_right was used */
+    int micro_fallback;/* LieSvdMicro escalated to LieSvdSmall */
+    int rotations;     /* rotations applied (Jacobi / micro) */
+} lso_info;
+
+/* ------------------------------------------------------------------ */
+/* dense helpers                                                       */
+/* ------------------------------------------------------------------ */
+
+static double* dalloc(size_t n) { return (double*)calloc(n ? n : 1, sizeof(double)); }
+
+/* src/lie_svd_small.rs:71-73 */
+static double frob(const double* a, size_t len) {
+    double s = 0.0;
+    for (size_t i = 0; i < len; ++i) s += a[i] * a[i];
+    return sqrt(s);
+}
+
+/* C[m x n] = op(A) * op(B); ta/tb select transposes.  A is stored as
+ * (ta ? k x m : m x k), B as (tb ? n x k : k x n), all row-major. */
+static void gemm(int ta, int tb, int m, int n, int k, const double* a, const double* b, double* c) {
+    for (int i = 0; i < m * n; ++i) c[i] = 0.0;
+    for (int i = 0; i < m; ++i) {
+        for (int p = 0; p < k; ++p) {
+            double aip = ta ? a[(size_t)p * m + i] : a[(size_t)i * k + p];
+            if (tb) {
+                for (int j = 0; j < n; ++j) c[(size_t)i * n + j] += aip * b[(size_t)j * k + p];
+            } else {
+                const double* brow = b + (size_t)p * n;
+                double* crow = c + (size_t)i * n;
+                for (int j = 0; j < n; ++j) crow[j] += aip * brow[j];
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* LieSvdSmall                                                         */
+/* ------------------------------------------------------------------ */
+
+/* gram_schmidt_orthonormal, src/lie_svd_small.rs:82-144.  a, out: n x n. */
+void lso_gram_schmidt_orthonormal(const double* a, int n, double* out) {
+    double* v = dalloc(n);
+    double* cand = dalloc(n);
+    double* best = dalloc(n);
+    memset(out, 0, sizeof(double) * (size_t)n * n);
+    for (int j = 0; j < n; ++j) {
+        for (int i = 0; i < n; ++i) v[i] = a[(size_t)i * n + j];
+        for (int k = 0; k < j; ++k) {
+            double d = 0.0;
+            for (int i = 0; i < n; ++i) d += out[(size_t)i * n + k] * v[i];
+            for (int i = 0; i < n; ++i) v[i] -= d * out[(size_t)i * n + k];
+        }
+        double norm = frob(v, n);
+        if (norm >= 1e-10) {
+            for (int i = 0; i < n; ++i) v[i] /= norm;
+        } else {
+            double best_norm = -1.0;
+            for (int k = 0; k < n; ++k) {
+                for (int i = 0; i < n; ++i) cand[i] = 0.0;
+                cand[k] = 1.0;
+                for (int col = 0; col < j; ++col) {
+                    double d = 0.0;
+                    for (int i = 0; i < n; ++i) d += out[(size_t)i * n + col] * cand[i];
+                    for (int i = 0; i < n; ++i) cand[i] -= d * out[(size_t)i * n + col];
+                }
+                double en = frob(cand, n);
+                if (en > best_norm) {
+                    best_norm = en;
+                    memcpy(best, cand, sizeof(double) * n);
+                }
+            }
+            if (best_norm > 0.0 && isfinite(best_norm)) {
+                for (int i = 0; i < n; ++i) v[i] = best[i] / best_norm;
+            } else {
+                for (int i = 0; i < n; ++i) v[i] = 0.0;
+                v[j] = 1.0;
+            }
+        }
+        for (int i = 0; i < n; ++i) out[(size_t)i * n + j] = v[i];
+    }
+    free(v);
+    free(cand);
+    free(best);
+}
+
+/* orthogonality_err = ||Q^T Q - I||_F, src/lie_svd_small.rs:146-150.
+ * q: rows x cols; returns the cols x cols residual. */
+double lso_orthogonality_err(const double* q, int rows, int cols) {
+    double* g = dalloc((size_t)cols * cols);
+    gemm(1, 0, cols, cols, rows, q, q, g);
+    for (int i = 0; i < cols; ++i) g[(size_t)i * cols + i] -= 1.0;
+    double e = frob(g, (size_t)cols * cols);
+    free(g);
+    return e;
+}
+
+/* newton_schulz_polar, src/lie_svd_small.rs:158-183.  Square n x n in the
+ * reference; rows >= cols is accepted here (the same iteration converges to
+ * the orthonormal polar factor of a tall matrix; the Gram-Schmidt fallback
+ * is square-only as in the reference).  Returns polar_ok. */
+int lso_newton_schulz_polar(const double* a, int rows, int cols, double* q, int* iters_out) {
+    const int n = cols;
+    size_t len = (size_t)rows * cols;
+    double scale = frob(a, len);
+    if (scale < 1e-300) scale = 1e-300;
+    double* x = dalloc(len);
+    double* xn = dalloc(len);
+    double* m3 = dalloc((size_t)n * n);
+    for (size_t i = 0; i < len; ++i) x[i] = a[i] / scale;
+    double ref_norm = sqrt((double)n);
+    int ok = 0, done = 0, it;
+    for (it = 0; it < LSO_POLAR_MAX_ITERS; ++it) {
+        /* xtx = x^T x ; x_next = x (3I - xtx) * 0.5   (:166-167) */
+        gemm(1, 0, n, n, rows, x, x, m3);
+        for (int i = 0; i < n; ++i)
+            for (int j = 0; j < n; ++j) m3[(size_t)i * n + j] = (i == j ? 3.0 : 0.0) - m3[(size_t)i * n + j];
+        gemm(0, 0, rows, n, n, x, m3, xn);
+        int finite = 1;
+        double dsq = 0.0;
+        for (size_t i = 0; i < len; ++i) {
+            xn[i] *= 0.5;
+            if (!isfinite(xn[i])) finite = 0;
+            double d = xn[i] - x[i];
+            dsq += d * d;
+        }
+        double delta = sqrt(dsq);
+        if (!finite || delta > 1e6 * ref_norm) { /* :169-171 */
+            if (rows == cols) lso_gram_schmidt_orthonormal(a, n, q);
+            else memcpy(q, x, sizeof(double) * len);
+            if (iters_out) *iters_out = it + 1;
+            free(x); free(xn); free(m3);
+            return 0;
+        }
+        double* t = x; x = xn; xn = t;
+        if (delta < LSO_POLAR_TOL * ref_norm) { /* :173-176 */
+            ok = lso_orthogonality_err(x, rows, cols) <= 1e-8 * ref_norm;
+            done = 1;
+            ++it;
+            break;
+        }
+    }
+    if (!done) ok = lso_orthogonality_err(x, rows, cols) <= 1e-8 * ref_norm; /* :178-182 */
+    memcpy(q, x, sizeof(double) * len);
+    if (iters_out) *iters_out = it;
+    free(x); free(xn); free(m3);
+    return ok;
+}
+
+/* repair_left_basis_from_right, src/lie_svd_small.rs:185-208. */
+static void repair_left_basis_from_right(const double* mat, const double* v, const double* sigma, int n,
+                                         double* u_out) {
+    double smax = 0.0;
+    for (int j = 0; j < n; ++j) smax = fmax(smax, sigma[j]);
+    double tol = 1e-12 * fmax(smax, 1.0);
+    double* seed = dalloc((size_t)n * n);
+    for (int j = 0; j < n; ++j) {
+        if (sigma[j] > tol) {
+            for (int row = 0; row < n; ++row) {
+                double s = 0.0;
+                for (int k = 0; k < n; ++k) s += mat[(size_t)row * n + k] * v[(size_t)k * n + j];
+                seed[(size_t)row * n + j] = s / sigma[j];
+            }
+        }
+    }
+    lso_gram_schmidt_orthonormal(seed, n, u_out);
+    free(seed);
+}
+
+/* Stable index sort by key descending (Rust's sort_by is stable). */
+static void argsort_desc(const double* key, int n, int* order) {
+    for (int i = 0; i < n; ++i) order[i] = i;
+    for (int i = 1; i < n; ++i) { /* insertion sort: stable */
+        int cur = order[i];
+        int j = i - 1;
+        while (j >= 0 && key[order[j]] < key[cur]) {
+            order[j + 1] = order[j];
+            --j;
+        }
+        order[j + 1] = cur;
+    }
+}
+
+/* eigh_jacobi_full, src/lie_svd_small.rs:212-285: cyclic-by-rows two-sided
+ * Jacobi with the Rutishauser tangent; eigenvalues sorted descending. */
+void lso_eigh_jacobi_full(const double* p, int n, double* v_out, double* eig_out, lso_info* info) {
+    double* a = dalloc((size_t)n * n);
+    double* v = dalloc((size_t)n * n);
+    memcpy(a, p, sizeof(double) * (size_t)n * n);
+    for (int i = 0; i < n; ++i) v[(size_t)i * n + i] = 1.0;
+    int max_sweeps = 60 + 20 * n;
+    double ref_norm = frob(p, (size_t)n * n);
+    if (ref_norm < 1e-300) ref_norm = 1e-300;
+    int sweeps = 0, rotations = 0;
+    for (int sw = 0; sw < max_sweeps; ++sw) {
+        double off_sq = 0.0;
+        for (int pp = 0; pp < n; ++pp)
+            for (int qq = pp + 1; qq < n; ++qq) off_sq += a[(size_t)pp * n + qq] * a[(size_t)pp * n + qq];
+        if (sqrt(off_sq) < LSO_JACOBI_TOL * ref_norm) break;
+        ++sweeps;
+        for (int pp = 0; pp < n; ++pp) {
+            for (int qq = pp + 1; qq < n; ++qq) {
+                double apq = a[(size_t)pp * n + qq];
+                if (fabs(apq) < 1e-300) continue;
+                double theta = (a[(size_t)qq * n + qq] - a[(size_t)pp * n + pp]) / (2.0 * apq);
+                double t = theta >= 0.0 ? 1.0 / (theta + sqrt(theta * theta + 1.0))
+                                        : -1.0 / (-theta + sqrt(theta * theta + 1.0));
+                double c = 1.0 / sqrt(t * t + 1.0);
+                double s = t * c;
+                double app = a[(size_t)pp * n + pp];
+                double aqq = a[(size_t)qq * n + qq];
+                a[(size_t)pp * n + pp] = app - t * apq;
+                a[(size_t)qq * n + qq] = aqq + t * apq;
+                a[(size_t)pp * n + qq] = 0.0;
+                a[(size_t)qq * n + pp] = 0.0;
+                for (int i = 0; i < n; ++i) {
+                    if (i != pp && i != qq) {
+                        double aip = a[(size_t)i * n + pp];
+                        double aiq = a[(size_t)i * n + qq];
+                        a[(size_t)i * n + pp] = c * aip - s * aiq;
+                        a[(size_t)pp * n + i] = a[(size_t)i * n + pp];
+                        a[(size_t)i * n + qq] = s * aip + c * aiq;
+                        a[(size_t)qq * n + i] = a[(size_t)i * n + qq];
+                    }
+                }
+                for (int i = 0; i < n; ++i) {
+                    double vip = v[(size_t)i * n + pp];
+                    double viq = v[(size_t)i * n + qq];
+                    v[(size_t)i * n + pp] = c * vip - s * viq;
+                    v[(size_t)i * n + qq] = s * vip + c * viq;
+                }
+                ++rotations;
+            }
+        }
+    }
+    double* diag = dalloc(n);
+    int* order = (int*)malloc(sizeof(int) * (n ? n : 1));
+    for (int i = 0; i < n; ++i) diag[i] = a[(size_t)i * n + i];
+    argsort_desc(diag, n, order);
+    for (int nj = 0; nj < n; ++nj) {
+        int oj = order[nj];
+        eig_out[nj] = diag[oj];
+        for (int i = 0; i < n; ++i) v_out[(size_t)i * n + nj] = v[(size_t)i * n + oj];
+    }
+    if (info) {
+        info->jacobi_sweeps = sweeps;
+        info->rotations += rotations;
+    }
+    free(a); free(v); free(diag); free(order);
+}
+
+/* LieSvdSmall::solve, src/lie_svd_small.rs:296-323.  u, vt: n x n; s: n. */
+void lso_small_solve(const double* mat, int n, double* u, double* s, double* vt, lso_info* info) {
+    lso_info local;
+    memset(&local, 0, sizeof local);
+    size_t nn = (size_t)n * n;
+    double* q = dalloc(nn);
+    double* p = dalloc(nn);
+    double* v = dalloc(nn);
+    double* eig = dalloc(n);
+    local.polar_ok = lso_newton_schulz_polar(mat, n, n, q, &local.ns_iters);
+    gemm(1, 0, n, n, n, q, mat, p); /* p = q^T mat (:305) */
+    for (int i = 0; i < n; ++i) /* symmetrize (:309) */
+        for (int j = i + 1; j < n; ++j) {
+            double m = (p[(size_t)i * n + j] + p[(size_t)j * n + i]) * 0.5;
+            p[(size_t)i * n + j] = m;
+            p[(size_t)j * n + i] = m;
+        }
+    lso_eigh_jacobi_full(p, n, v, eig, &local);
+    for (int j = 0; j < n; ++j) s[j] = fmax(eig[j], 0.0); /* :313-316 (max(0)) */
+    gemm(0, 0, n, n, n, q, v, u); /* u = q v (:317) */
+    if (!local.polar_ok || lso_orthogonality_err(u, n, n) > 1e-8 * sqrt((double)n)) { /* :318-320 */
+        repair_left_basis_from_right(mat, v, s, n, u);
+        local.repaired = 1;
+    }
+    for (int i = 0; i < n; ++i) /* vt = v^T (:321) */
+        for (int j = 0; j < n; ++j) vt[(size_t)i * n + j] = v[(size_t)j * n + i];
+    if (info) *info = local;
+    free(q); free(p); free(v); free(eig);
+}
+
+/* qr_reduce, src/lie_svd_small.rs:373-418: MGS with a scale-relative rank
+ * floor; dependent columns keep a zero Q column and the tiny R pivot. */
+void lso_qr_reduce(const double* mat, int n, int d, double* q, double* r) {
+    double* v = dalloc((size_t)n * d);
+    memset(q, 0, sizeof(double) * (size_t)n * d);
+    memset(r, 0, sizeof(double) * (size_t)d * d);
+    double mat_scale = frob(mat, (size_t)n * d);
+    if (mat_scale < 1e-300) mat_scale = 1e-300;
+    double floor_ = 1e-10 * mat_scale;
+    for (int j = 0; j < d; ++j) {
+        for (int i = 0; i < n; ++i) v[(size_t)i * d + j] = mat[(size_t)i * d + j];
+        for (int k = 0; k < j; ++k) {
+            double dot = 0.0;
+            for (int i = 0; i < n; ++i) dot += q[(size_t)i * d + k] * v[(size_t)i * d + j];
+            r[(size_t)k * d + j] = dot;
+            for (int i = 0; i < n; ++i) v[(size_t)i * d + j] -= dot * q[(size_t)i * d + k];
+        }
+        double ns = 0.0;
+        for (int i = 0; i < n; ++i) ns += v[(size_t)i * d + j] * v[(size_t)i * d + j];
+        double norm = sqrt(ns);
+        r[(size_t)j * d + j] = norm;
+        if (norm >= floor_)
+            for (int i = 0; i < n; ++i) q[(size_t)i * d + j] = v[(size_t)i * d + j] / norm;
+    }
+    free(v);
+}
+
+/* LieSvdSmall::solve_rectangular, src/lie_svd_small.rs:340-360.
+ * mat: rows x cols.  k = min(rows, cols); u: rows x k, s: k, vt: k x cols. */
+void lso_small_solve_rectangular(const double* mat, int rows, int cols, double* u, double* s, double* vt,
+                                 lso_info* info) {
+    if (rows == cols) {
+        lso_small_solve(mat, rows, u, s, vt, info);
+        return;
+    }
+    if (cols > rows) { /* transpose branch (:350-354) */
+        double* t = dalloc((size_t)rows * cols);
+        for (int i = 0; i < rows; ++i)
+            for (int j = 0; j < cols; ++j) t[(size_t)j * rows + i] = mat[(size_t)i * cols + j];
+        int k = rows;
+        double* ut = dalloc((size_t)cols * k); /* cols x k */
+        double* vtt = dalloc((size_t)k * rows); /* k x rows */
+        lso_small_solve_rectangular(t, cols, rows, ut, s, vtt, info);
+        for (int i = 0; i < rows; ++i) /* u = vt_t^T : rows x k */
+            for (int j = 0; j < k; ++j) u[(size_t)i * k + j] = vtt[(size_t)j * rows + i];
+        for (int i = 0; i < k; ++i) /* vt = u_t^T : k x cols */
+            for (int j = 0; j < cols; ++j) vt[(size_t)i * cols + j] = ut[(size_t)j * k + i];
+        free(t); free(ut); free(vtt);
+        return;
+    }
+    int d = cols; /* tall (:356-359) */
+    double* q = dalloc((size_t)rows * d);
+    double* r = dalloc((size_t)d * d);
+    double* ur = dalloc((size_t)d * d);
+    lso_qr_reduce(mat, rows, d, q, r);
+    lso_small_solve(r, d, ur, s, vt, info);
+    gemm(0, 0, rows, d, d, q, ur, u);
+    free(q); free(r); free(ur);
+}
+
+/* ------------------------------------------------------------------ */
+/* LieSvdMicro                                                         */
+/* ------------------------------------------------------------------ */
+
+/* wrap_angle, src/lie_svd_micro.rs:118-126 (also wrap_svd_angle,
+ * src/lie_svd_joint.rs:293-301). */
+static double wrap_half_pi(double a) {
+    while (a > M_PI / 2) a -= M_PI;
+    while (a < -M_PI / 2) a += M_PI;
+    return a;
+}
+
+/* local_pair_svd_angles, src/lie_svd_micro.rs:128-139. */
+static void pair_svd_angles(double a, double b, double c, double d, double* tl, double* tr) {
+    double sum_angle = atan2(-(b + c), a - d);
+    double diff_angle = atan2(b - c, a + d);
+    *tl = wrap_half_pi(0.5 * (sum_angle + diff_angle));
+    *tr = wrap_half_pi(0.5 * (sum_angle - diff_angle));
+}
+
+static double offdiag_norm(const double* w, int n) { /* src/lie_svd_micro.rs:101-112 */
+    double s = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            if (i != j) s += w[i * n + j] * w[i * n + j];
+    return sqrt(s);
+}
+
+/* LieSvdMicro::solve + solve_fixed_schedule + extract_sorted_svd,
+ * src/lie_svd_micro.rs:21-95,141-214.  n <= 4.  Returns 0, or -1 if n > 4. */
+int lso_micro_solve(const double* mat, int n, double* u, double* s, double* vt, lso_info* info) {
+    lso_info local;
+    memset(&local, 0, sizeof local);
+    if (n < 0 || n > 4) return -1;
+    if (n == 0) {
+        if (info) *info = local;
+        return 0;
+    }
+    if (n == 1) { /* :32-40 */
+        double d = mat[0];
+        u[0] = d < 0.0 ? -1.0 : 1.0;
+        s[0] = fabs(d);
+        vt[0] = 1.0;
+        if (info) *info = local;
+        return 0;
+    }
+    static const int p2[1][2] = {{0, 1}};
+    static const int p3[3][2] = {{0, 1}, {0, 2}, {1, 2}};
+    static const int p4[6][2] = {{0, 1}, {2, 3}, {0, 2}, {1, 3}, {0, 3}, {1, 2}};
+    const int(*pairs)[2] = n == 2 ? p2 : (n == 3 ? p3 : p4);
+    int npairs = n == 2 ? 1 : (n == 3 ? 3 : 6);
+    int max_sweeps = n == 2 ? 1 : 24;
+    double w[16], ub[16], vb[16];
+    memcpy(w, mat, sizeof(double) * n * n);
+    memset(ub, 0, sizeof ub);
+    memset(vb, 0, sizeof vb);
+    for (int i = 0; i < n; ++i) ub[i * n + i] = vb[i * n + i] = 1.0;
+    double ref_norm = frob(mat, (size_t)n * n);
+    if (ref_norm < 1e-300) ref_norm = 1e-300;
+    double tol = LSO_MICRO_TOL * fmax(ref_norm, 1.0);
+    for (int sw = 0; sw < (max_sweeps > 1 ? max_sweeps : 1); ++sw) {
+        double energy = offdiag_norm(w, n);
+        if (energy <= tol || !isfinite(energy)) break;
+        int changed = 0;
+        for (int pi = 0; pi < npairs; ++pi) {
+            int i = pairs[pi][0], j = pairs[pi][1];
+            if (fabs(w[i * n + j]) + fabs(w[j * n + i]) <= tol) continue;
+            double tl, tr;
+            pair_svd_angles(w[i * n + i], w[i * n + j], w[j * n + i], w[j * n + j], &tl, &tr);
+            if (fabs(tl) + fabs(tr) <= 1e-18) continue;
+            double sl = sin(tl), cl = cos(tl);
+            for (int col = 0; col < n; ++col) { /* apply_left_rotor :141-162 */
+                double ai = w[i * n + col], aj = w[j * n + col];
+                w[i * n + col] = cl * ai - sl * aj;
+                w[j * n + col] = sl * ai + cl * aj;
+            }
+            for (int r = 0; r < n; ++r) {
+                double ui = ub[r * n + i], uj = ub[r * n + j];
+                ub[r * n + i] = cl * ui - sl * uj;
+                ub[r * n + j] = sl * ui + cl * uj;
+            }
+            double sr = sin(tr), cr = cos(tr);
+            for (int r = 0; r < n; ++r) { /* apply_right_rotor :164-185 */
+                double ai = w[r * n + i], aj = w[r * n + j];
+                w[r * n + i] = cr * ai - sr * aj;
+                w[r * n + j] = sr * ai + cr * aj;
+            }
+            for (int r = 0; r < n; ++r) {
+                double vi = vb[r * n + i], vj = vb[r * n + j];
+                vb[r * n + i] = cr * vi - sr * vj;
+                vb[r * n + j] = sr * vi + cr * vj;
+            }
+            changed = 1;
+            ++local.rotations;
+        }
+        if (!changed) break;
+    }
+    if (offdiag_norm(w, n) > 1e-10 * fmax(ref_norm, 1.0)) { /* :90-92 */
+        lso_small_solve(mat, n, u, s, vt, info);
+        if (info) info->micro_fallback = 1;
+        return 0;
+    }
+    /* extract_sorted_svd :187-214, stable sort by |diag| descending */
+    double key[4];
+    int order[4];
+    for (int i = 0; i < n; ++i) key[i] = fabs(w[i * n + i]);
+    argsort_desc(key, n, order);
+    for (int dst = 0; dst < n; ++dst) {
+        int src = order[dst];
+        double d = w[src * n + src];
+        s[dst] = fabs(d);
+        double sign = d < 0.0 ? -1.0 : 1.0;
+        for (int r = 0; r < n; ++r) {
+            u[r * n + dst] = sign * ub[r * n + src];
+            vt[dst * n + r] = vb[r * n + src];
+        }
+    }
+    if (info) *info = local;
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* LieSvdJoint (Phase-JADE)                                            */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    int max_sweeps;
+    double tol;
+    double min_pair_scale;
+    int line_search_steps;
+} lso_joint_params;
+
+typedef struct {
+    double initial_offdiag;
+    double final_offdiag;
+    int sweeps;
+    int rotations;
+    int rejected_rotations;
+} lso_joint_trace;
+
+/* JointDiagonalizationParams::for_n, src/lie_svd_joint.rs:26-35. */
+lso_joint_params lso_joint_params_for_n(int n) {
+    lso_joint_params p;
+    p.max_sweeps = 12 + (n > 1 ? n : 1);
+    p.tol = 1e-12;
+    p.min_pair_scale = 1e-14;
+    p.line_search_steps = 6;
+    return p;
+}
+
+/* wrap_jacobi_angle, src/lie_svd_joint.rs:506-514. */
+static double wrap_quarter_pi(double a) {
+    while (a > M_PI / 4) a -= M_PI / 2;
+    while (a < -M_PI / 4) a += M_PI / 2;
+    return a;
+}
+
+/* round_robin_layer_count / layer_pairs, src/lie_svd_joint.rs:516-539.
+ * Writes up to n/2 pairs (lo, hi) into out; returns the count. */
+int lso_layer_count(int n) { return n % 2 == 0 ? (n > 0 ? n - 1 : 0) : n; }
+
+int lso_layer_pairs(int n, int layer, int* out) {
+    int m = n % 2 == 0 ? n : n + 1;
+    int ring = m - 1;
+    int cnt = 0;
+    for (int k = 0; k < m / 2; ++k) {
+        int a, b;
+        if (k == 0) {
+            a = m - 1;
+            b = layer % ring;
+        } else {
+            a = (layer + k) % ring;
+            b = (layer + ring - k) % ring;
+        }
+        if (a < n && b < n) {
+            out[2 * cnt] = a < b ? a : b;
+            out[2 * cnt + 1] = a < b ? b : a;
+            ++cnt;
+        }
+    }
+    return cnt;
+}
+
+/* apply_left_rotor / apply_right_rotor / apply_basis_rotor,
+ * src/lie_svd_joint.rs:401-437 (rows x cols matrix m). */
+static void rot_left(double* m, int rows, int cols, int i, int j, double th) {
+    (void)rows;
+    double s = sin(th), c = cos(th);
+    for (int col = 0; col < cols; ++col) {
+        double ai = m[(size_t)i * cols + col], aj = m[(size_t)j * cols + col];
+        m[(size_t)i * cols + col] = c * ai - s * aj;
+        m[(size_t)j * cols + col] = s * ai + c * aj;
+    }
+}
+static void rot_right(double* m, int rows, int cols, int i, int j, double th) {
+    double s = sin(th), c = cos(th);
+    for (int r = 0; r < rows; ++r) {
+        double ai = m[(size_t)r * cols + i], aj = m[(size_t)r * cols + j];
+        m[(size_t)r * cols + i] = c * ai - s * aj;
+        m[(size_t)r * cols + j] = s * ai + c * aj;
+    }
+}
+
+static double offdiag_sq(const double* m, int rows, int cols) { /* :461-473 */
+    double s = 0.0;
+    for (int i = 0; i < rows; ++i)
+        for (int j = 0; j < cols; ++j)
+            if (i != j) s += m[(size_t)i * cols + j] * m[(size_t)i * cols + j];
+    return s;
+}
+
+static double local_offdiag_sq(const double* m, int rows, int cols, int i, int j) { /* :475-494 */
+    double s = 0.0;
+    for (int col = 0; col < cols; ++col) {
+        if (col != i) s += m[(size_t)i * cols + col] * m[(size_t)i * cols + col];
+        if (col != j) s += m[(size_t)j * cols + col] * m[(size_t)j * cols + col];
+    }
+    for (int r = 0; r < rows; ++r) {
+        if (r != i && r != j) {
+            s += m[(size_t)r * cols + i] * m[(size_t)r * cols + i];
+            s += m[(size_t)r * cols + j] * m[(size_t)r * cols + j];
+        }
+    }
+    return s;
+}
+
+static double joint_offdiag_norm(const double* w, int k, int rows, int cols) { /* :439-441 */
+    double s = 0.0;
+    for (int q = 0; q < k; ++q) s += offdiag_sq(w + (size_t)q * rows * cols, rows, cols);
+    return sqrt(s);
+}
+
+/* joint_pair_energy_after, src/lie_svd_joint.rs:388-399. */
+static double pair_energy_after(const double* w, int k, int n, int i, int j, double th) {
+    double s = sin(th), c = cos(th);
+    double out = 0.0;
+    for (int q = 0; q < k; ++q) {
+        const double* m = w + (size_t)q * n * n;
+        double a = m[(size_t)i * n + i];
+        double b = 0.5 * (m[(size_t)i * n + j] + m[(size_t)j * n + i]);
+        double d = m[(size_t)j * n + j];
+        double off = 0.5 * (a - d) * (2.0 * s * c) + b * (c * c - s * s);
+        out += 2.0 * off * off;
+    }
+    return out;
+}
+
+/* joint_symmetric_pair_angle, src/lie_svd_joint.rs:338-357. */
+static double joint_sym_angle(const double* w, int k, int n, int i, int j) {
+    double sxx = 0.0, syy = 0.0, sxy = 0.0;
+    for (int q = 0; q < k; ++q) {
+        const double* m = w + (size_t)q * n * n;
+        double x = 0.5 * (m[(size_t)i * n + i] - m[(size_t)j * n + j]);
+        double y = 0.5 * (m[(size_t)i * n + j] + m[(size_t)j * n + i]);
+        sxx += x * x;
+        syy += y * y;
+        sxy += x * y;
+    }
+    double theta = 0.25 * atan2(2.0 * sxy, syy - sxx);
+    double alt = wrap_quarter_pi(theta + M_PI / 4);
+    theta = wrap_quarter_pi(theta);
+    return pair_energy_after(w, k, n, i, j, alt) < pair_energy_after(w, k, n, i, j, theta) ? alt : theta;
+}
+
+/* LieSvdJoint::diagonalize_symmetric_with_params, src/lie_svd_joint.rs:71-158
+ * with accept_joint_rotor :359-386.  mats: k consecutive n x n matrices
+ * (not modified).  v_out: n x n basis; diag_out: k x n. */
+void lso_joint_diag_sym(const double* mats, int k, int n, lso_joint_params params, double* v_out,
+                        double* diag_out, lso_joint_trace* trace) {
+    lso_joint_trace tr;
+    memset(&tr, 0, sizeof tr);
+    size_t nn = (size_t)n * n;
+    memset(v_out, 0, sizeof(double) * nn);
+    for (int i = 0; i < n; ++i) v_out[(size_t)i * n + i] = 1.0;
+    if (n <= 1) {
+        for (int q = 0; q < k; ++q)
+            for (int i = 0; i < n; ++i) diag_out[(size_t)q * n + i] = mats[(size_t)q * nn + (size_t)i * n + i];
+        if (trace) *trace = tr;
+        return;
+    }
+    double* w = dalloc(nn * k);
+    memcpy(w, mats, sizeof(double) * nn * k);
+    double ref_norm = frob(w, nn * k);
+    if (ref_norm < 1e-300) ref_norm = 1e-300;
+    double pair_tol = params.min_pair_scale * fmax(ref_norm, 1.0);
+    double tol = params.tol * fmax(ref_norm, 1.0);
+    tr.initial_offdiag = joint_offdiag_norm(w, k, n, n);
+    int* pairs = (int*)malloc(sizeof(int) * (n + 1));
+    int lss = params.line_search_steps > 1 ? params.line_search_steps : 1;
+    for (int sw = 0; sw < params.max_sweeps; ++sw) {
+        tr.sweeps = sw + 1;
+        double before = joint_offdiag_norm(w, k, n, n);
+        int layers = lso_layer_count(n);
+        for (int layer = 0; layer < layers; ++layer) {
+            int np = lso_layer_pairs(n, layer, pairs);
+            for (int pi = 0; pi < np; ++pi) {
+                int i = pairs[2 * pi], j = pairs[2 * pi + 1];
+                double po = 0.0; /* joint_pair_offdiag :451-453 */
+                for (int q = 0; q < k; ++q)
+                    po += fabs(w[(size_t)q * nn + (size_t)i * n + j]) + fabs(w[(size_t)q * nn + (size_t)j * n + i]);
+                if (po <= pair_tol) continue;
+                double theta = joint_sym_angle(w, k, n, i, j);
+                if (fabs(theta) <= 1e-18) continue;
+                /* accept_joint_rotor */
+                double bef = 0.0;
+                for (int q = 0; q < k; ++q) bef += local_offdiag_sq(w + (size_t)q * nn, n, n, i, j);
+                double slack = 1e-14 * fmax(bef, 1.0);
+                double scale = 1.0;
+                int accepted = 0;
+                for (int ls = 0; ls < lss; ++ls) {
+                    double ang = theta * scale;
+                    for (int q = 0; q < k; ++q) {
+                        rot_left(w + (size_t)q * nn, n, n, i, j, ang);
+                        rot_right(w + (size_t)q * nn, n, n, i, j, ang);
+                    }
+                    double aft = 0.0;
+                    for (int q = 0; q < k; ++q) aft += local_offdiag_sq(w + (size_t)q * nn, n, n, i, j);
+                    if (aft <= bef + slack && isfinite(aft)) {
+                        rot_right(v_out, n, n, i, j, ang); /* apply_basis_rotor */
+                        accepted = 1;
+                        break;
+                    }
+                    for (int q = 0; q < k; ++q) {
+                        rot_left(w + (size_t)q * nn, n, n, i, j, -ang);
+                        rot_right(w + (size_t)q * nn, n, n, i, j, -ang);
+                    }
+                    scale *= 0.5;
+                }
+                if (accepted) ++tr.rotations;
+                else ++tr.rejected_rotations;
+            }
+        }
+        double after = joint_offdiag_norm(w, k, n, n);
+        if (after <= tol || after >= before * (1.0 - 1e-10)) break;
+    }
+    tr.final_offdiag = joint_offdiag_norm(w, k, n, n);
+    for (int q = 0; q < k; ++q)
+        for (int i = 0; i < n; ++i) diag_out[(size_t)q * n + i] = w[(size_t)q * nn + (size_t)i * n + i];
+    if (trace) *trace = tr;
+    free(w);
+    free(pairs);
+}
+
+/* ------------------------------------------------------------------ */
+/* metrics::compute, src/lib.rs:149-194                                */
+/* ------------------------------------------------------------------ */
+
+/* out[0]=rel_recon, out[1]=orth_u, out[2]=orth_v, out[3]=sigma_max_rel,
+ * out[4]=sigma_tail_rel (NaN when sigma_ref is NULL).  Square n x n. */
+void lso_metrics(const double* a, int n, const double* u, const double* s, const double* vt,
+                 const double* sigma_ref, double* out) {
+    size_t nn = (size_t)n * n;
+    double* us = dalloc(nn);
+    double* rec = dalloc(nn);
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) us[(size_t)i * n + j] = u[(size_t)i * n + j] * s[j];
+    gemm(0, 0, n, n, n, us, vt, rec);
+    double num = 0.0;
+    for (size_t i = 0; i < nn; ++i) num += (rec[i] - a[i]) * (rec[i] - a[i]);
+    double an = frob(a, nn);
+    out[0] = sqrt(num) / fmax(an, 1e-300);
+    out[1] = lso_orthogonality_err(u, n, n);
+    gemm(0, 1, n, n, n, vt, vt, rec); /* vt vt^T (:162) */
+    for (int i = 0; i < n; ++i) rec[(size_t)i * n + i] -= 1.0;
+    out[2] = frob(rec, nn);
+    out[3] = out[4] = NAN;
+    if (sigma_ref) {
+        double* got = dalloc(n);
+        double* want = dalloc(n);
+        int* og = (int*)malloc(sizeof(int) * (n ? n : 1));
+        int* ow = (int*)malloc(sizeof(int) * (n ? n : 1));
+        argsort_desc(s, n, og);
+        argsort_desc(sigma_ref, n, ow);
+        for (int i = 0; i < n; ++i) {
+            got[i] = s[og[i]];
+            want[i] = sigma_ref[ow[i]];
+        }
+        double mx = 0.0, tail = 0.0;
+        int tail_start = (n * 3) / 4;
+        for (int i = 0; i < n; ++i) {
+            double rel = fabs(got[i] - want[i]) / fmax(fabs(want[i]), 1e-300);
+            mx = fmax(mx, rel);
+            if (i >= tail_start) tail = fmax(tail, rel);
+        }
+        out[3] = mx;
+        out[4] = tail;
+        free(got); free(want); free(og); free(ow);
+    }
+    free(us);
+    free(rec);
+}
+
+/* ------------------------------------------------------------------ */
+/* Batch drivers for the CPU baseline (one matrix per OpenMP thread)   */
+/* ------------------------------------------------------------------ */
+
+/* Ragged batch of row-major matrices: matrix b has shape m[b] x n[b] at
+ * a + a_off[b]; outputs at u + u_off[b] (m x k), s + s_off[b] (k) and
+ * vt + vt_off[b] (k x n), k = min(m, n).  Square matrices with n <= 4 go
+ * to LieSvdMicro, as LieSvd's router does (src/lie_svd_adaptive.rs:93);
+ * everything else goes through solve_rectangular.  threads <= 0 means the
+ * OpenMP default. */
+void lso_svd_batch(int count, const int* m, const int* n, const int64_t* a_off, const double* a,
+                   const int64_t* u_off, double* u, const int64_t* s_off, double* s, const int64_t* vt_off,
+                   double* vt, lso_info* info, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#else
+    (void)threads;
+#endif
+    for (int b = 0; b < count; ++b) {
+        lso_info* ib = info ? info + b : NULL;
+        if (m[b] == n[b] && n[b] <= 4)
+            lso_micro_solve(a + a_off[b], n[b], u + u_off[b], s + s_off[b], vt + vt_off[b], ib);
+        else
+            lso_small_solve_rectangular(a + a_off[b], m[b], n[b], u + u_off[b], s + s_off[b], vt + vt_off[b],
+                                        ib);
+    }
+}
+
+/* Joint diagonalization of several independent families (one per thread). */
+void lso_joint_batch(int families, int k, int n, const double* mats, lso_joint_params params, double* v,
+                     double* diag, lso_joint_trace* traces, int threads) {
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 1)
+#else
+    (void)threads;
+#endif
+    for (int f = 0; f < families; ++f)
+        lso_joint_diag_sym(mats + (size_t)f * k * n * n, k, n, params, v + (size_t)f * n * n,
+                           diag + (size_t)f * k * n, traces + f);
+}
+
+/* NS polar iterations only (the CPU-baseline timing sample for the large
+ * polar workload): runs exactly `iters` iterations of :166-167 on a rows x
+ * cols matrix, in place, with an OpenMP-parallel GEMM. */
+void lso_ns_iterations(double* x, int rows, int cols, int iters, int threads) {
+    int n = cols;
+    double* g = dalloc((size_t)n * n);
+    double* xn = dalloc((size_t)rows * n);
+#ifdef _OPENMP
+    if (threads > 0) omp_set_num_threads(threads);
+#else
+    (void)threads;
+#endif
+    for (int it = 0; it < iters; ++it) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = 0; i < n; ++i) {
+            double* gi = g + (size_t)i * n;
+            for (int j = 0; j < n; ++j) gi[j] = 0.0;
+            for (int p = 0; p < rows; ++p) {
+                double xpi = x[(size_t)p * n + i];
+                const double* xp = x + (size_t)p * n;
+                for (int j = 0; j < n; ++j) gi[j] += xpi * xp[j];
+            }
+            for (int j = 0; j < n; ++j) gi[j] = (i == j ? 3.0 : 0.0) - gi[j];
+        }
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+        for (int i = 0; i < rows; ++i) {
+            double* xi = xn + (size_t)i * n;
+            for (int j = 0; j < n; ++j) xi[j] = 0.0;
+            for (int p = 0; p < n; ++p) {
+                double a = x[(size_t)i * n + p];
+                const double* gp = g + (size_t)p * n;
+                for (int j = 0; j < n; ++j) xi[j] += a * gp[j];
+            }
+            for (int j = 0; j < n; ++j) xi[j] *= 0.5;
+        }
+        memcpy(x, xn, sizeof(double) * (size_t)rows * n);
+    }
+    free(g);
+    free(xn);
+}

@@ -81,9 +81,11 @@ def main(argv=None) -> int:
     log(f"workload: {count} matrices generated in {time.time() - t0:.1f}s")
 
     dev = torch.device("cuda:0")
+    stream = torch.cuda.Stream(device=dev)  # default priority
+    torch.cuda.set_stream(stream)
     h = Handle(0)
     h.set_pointer_mode(L.LSVD_PTR_DEVICE)
-    h.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    h.set_stream(stream.cuda_stream)
     if not args.no_tune:
         big = ~((ms == ns) & (ns <= 4))
         t1 = time.time()
@@ -126,6 +128,14 @@ def main(argv=None) -> int:
     descs, order = plan.describe()
     bucket_ms = plan.time_buckets(a_pool, u_pool, s_pool, vt_pool)
     torch.cuda.synchronize()
+
+    # ---- where the cycles go (per-phase clock64 sums over the batch)
+    clocks = torch.zeros((count, 8), dtype=torch.int64, device=dev)
+    plan.phase_clocks(a_pool, u_pool, s_pool, vt_pool, clocks)
+    ck = clocks.cpu().numpy().sum(axis=0).astype(np.float64)
+    names = ["qr", "ns", "p", "jacobi", "u", "repair", "out"]
+    phase_split = {nm: round(float(ck[i] / max(ck.sum(), 1.0)), 4) for i, nm in enumerate(names)}
+    log(f"phase split: {phase_split}")
 
     # ---- results to host, sigma errors
     u_h, s_h, vt_h = u_pool.cpu().numpy(), s_pool.cpu().numpy(), vt_pool.cpu().numpy()
@@ -211,6 +221,7 @@ def main(argv=None) -> int:
         "residuals_sampled": {"rel_recon_max": worst_recon, "orth_max": worst_orth},
         "fp64_tflops_whole_batch": round(total_tflops, 4),
         "buckets": [{**dd, "ms": round(t, 4)} for dd, t in zip(descs, bucket_ms)],
+        "phase_split": phase_split,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

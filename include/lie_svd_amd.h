@@ -185,6 +185,14 @@ int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, in
 int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
                                  double* vt_pool, float* ms, int max);
 
+/* Runs the plan once (no graph) with per-matrix phase clocks: `clocks` is
+ * device memory of count x 8 int64, zeroed by the caller; entry [b][ph]
+ * receives the clock64() cycles matrix b spent in phase ph (0 QR front
+ * end, 1 Newton-Schulz, 2 P = Q^T A, 3 Jacobi, 4 U = Q V + check,
+ * 5 repair, 6 output).  Device pointer mode only. */
+int lsvd_batch_plan_phase_clocks(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
+                                 double* vt_pool, int64_t* clocks);
+
 #ifdef __cplusplus
 }
 #endif

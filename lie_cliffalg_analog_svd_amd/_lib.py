@@ -137,6 +137,7 @@ SIGNATURES = {
     "lsvd_autotune_batch": (C.c_int, [_vp, C.c_int, _i32p, _i32p, C.c_int]),
     "lsvd_batch_plan_describe": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int), _vp]),
     "lsvd_batch_plan_time_buckets": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
+    "lsvd_batch_plan_phase_clocks": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
 }
 
 

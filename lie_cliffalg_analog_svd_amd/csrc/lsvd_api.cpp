@@ -662,6 +662,17 @@ int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_
     return rc;
 }
 
+int lsvd_batch_plan_phase_clocks(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
+                                 double* vt_pool, int64_t* clocks) {
+    if (!p || !clocks) return LSVD_E_ARG;
+    lsvd_handle_t h = p->h;
+    if (p->ptr_mode != LSVD_PTR_DEVICE) return set_error(h, LSVD_E_ARG, "phase_clocks needs device pointers");
+    Pools pools{a_pool, u_pool, s_pool, vt_pool, nullptr, reinterpret_cast<long long*>(clocks)};
+    hipError_t e = plan_enqueue(p, pools, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "phase clocks run");
+}
+
 // Single-matrix entry points: a cached one-item plan per (mode, m, n).
 static int single(lsvd_handle_t h, int mode, const double* a, int m, int n, double* u, double* s, double* vt,
                   lsvd_info_t* info) {

@@ -30,7 +30,11 @@ struct Pools {
     double* s;
     double* vt;
     lsvd_info_t* info;
+    long long* timing;   // optional: kPhases clock64() deltas per matrix
 };
+
+// Phases recorded in Pools::timing (profiling aid).
+enum Phase : int { kPhQR = 0, kPhNS = 1, kPhP = 2, kPhJacobi = 3, kPhU = 4, kPhRepair = 5, kPhOut = 6, kPhases = 8 };
 
 // Solver constants of the reference (src/lie_svd_small.rs:67-69,
 // src/lie_svd_micro.rs:15).

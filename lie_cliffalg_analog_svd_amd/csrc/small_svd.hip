@@ -328,8 +328,23 @@ __device__ void sort_desc(int N, SmallShared<NT, MAXN>& sh) {
     __syncthreads();
 }
 
+// Phase clock: thread 0 accumulates clock64() deltas into Pools::timing.
+struct PhaseClock {
+    long long* out;
+    long long t;
+    __device__ PhaseClock(long long* o) : out(o), t(o && threadIdx.x == 0 ? clock64() : 0) {}
+    __device__ __forceinline__ void mark(int ph) {
+        if (out && threadIdx.x == 0) {
+            const long long now = clock64();
+            out[ph] += now - t;
+            t = now;
+        }
+    }
+};
+
 template <int NT, int MAXN>
 __device__ void solve_item(const Item& itm, const Pools& P, double* arena, SmallShared<NT, MAXN>& sh) {
+    PhaseClock clk(P.timing ? P.timing + (size_t)itm.idx * kPhases : nullptr);
     const int m0 = itm.m, n0 = itm.n;
     const bool wide = m0 < n0;
     const int M = wide ? n0 : m0, N = wide ? m0 : n0;
@@ -396,6 +411,7 @@ __device__ void solve_item(const Item& itm, const Pools& P, double* arena, Small
         __syncthreads();
     }
     const AsqView asv{UpperView{qs, N}, ain, tall};
+    clk.mark(kPhQR);
 
     // ---- Newton-Schulz polar (src/lie_svd_small.rs:158-183)
     double scale = 0.0;
@@ -446,6 +462,7 @@ __device__ void solve_item(const Item& itm, const Pools& P, double* arena, Small
     if (!done) polar_ok = orth_err<NT>(N, X, ld, sh.red) <= 1e-8 * rootn;
     info.polar_ok = polar_ok;
     info.ns_iters = iters;
+    clk.mark(kPhNS);
 
     // ---- P = sym(Q^T A)  (:305-309)
     wg_gemm<NT>(N, N, N, View{X, ld, 1}, asv, [&](int i, int j, double acc) { W[(size_t)i * ld + j] = acc; });
@@ -461,9 +478,11 @@ __device__ void solve_item(const Item& itm, const Pools& P, double* arena, Small
     }
     __syncthreads();
 
+    clk.mark(kPhP);
     // ---- Jacobi eigensolve of P, sort (:212-285, :313-316)
     int sweeps, rots;
     jacobi_eigh<NT, MAXN>(N, T, W, ld, sh, sweeps, rots);
+    clk.mark(kPhJacobi);
     for (int i = threadIdx.x; i < N; i += NT) sh.diag[i] = T[(size_t)i * ld + i];
     __syncthreads();
     sort_desc<NT, MAXN>(N, sh);
@@ -477,6 +496,7 @@ __device__ void solve_item(const Item& itm, const Pools& P, double* arena, Small
     });
     __syncthreads();
     const double oerr = orth_err<NT>(N, T, ld, sh.red);
+    clk.mark(kPhU);
     if (!polar_ok || oerr > 1e-8 * rootn) {
         // repair_left_basis_from_right (:185-208): seed = A V S^-1, then MGS.
         double smax = 0.0;
@@ -491,6 +511,7 @@ __device__ void solve_item(const Item& itm, const Pools& P, double* arena, Small
         gram_schmidt<NT, MAXN>(N, View{X, ld, 0}, T, ld, sh);
         info.repaired = 1;
     }
+    clk.mark(kPhRepair);
 
     // ---- outputs: Vt (or U for wide), then U
     if (!wide) {
@@ -531,6 +552,7 @@ __device__ void solve_item(const Item& itm, const Pools& P, double* arena, Small
         }
     }
     rots = (int)block_sum<NT>((double)rots, sh.red);
+    clk.mark(kPhOut);
     if (threadIdx.x == 0 && P.info) {
         info.jacobi_sweeps = sweeps;
         info.rotations = rots;

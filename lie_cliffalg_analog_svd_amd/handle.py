@@ -171,6 +171,11 @@ class BatchPlan:
             self._p, _ptr(a_pool), _ptr(u_pool), _ptr(s_pool), _ptr(vt_pool), ms, len(descs)))
         return [float(ms[i]) for i in range(len(descs))]
 
+    def phase_clocks(self, a_pool, u_pool, s_pool, vt_pool, clocks):
+        """One plain run with per-matrix phase clocks (device int64 [count, 8])."""
+        check(self.handle.raw, lib().lsvd_batch_plan_phase_clocks(
+            self._p, _ptr(a_pool), _ptr(u_pool), _ptr(s_pool), _ptr(vt_pool), _ptr(clocks)))
+
     def split(self, u_pool, s_pool, vt_pool):
         """Views of the packed outputs: list of (U, s, Vt) numpy arrays."""
         L = self.layout

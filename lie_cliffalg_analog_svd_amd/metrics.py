@@ -68,9 +68,10 @@ def sigma_errors(got, want, rel_floor: float = 1e-3):
 
 
 def subspace_angle_clusters(u_got, u_ref, sigma_ref, rtol: float = 1e-8):
-    """Largest principal angle (radians) between computed and reference
-    singular subspaces, per cluster of (numerically) equal singular values.
-    Returns a list of (start, stop, angle)."""
+    """sin of the largest principal angle between computed and reference
+    singular subspaces, per cluster of (numerically) equal singular values,
+    as ||(I - R R^T) G||_2 (accurate for tiny angles).  Returns a list of
+    (start, stop, sine)."""
     s = np.asarray(sigma_ref, dtype=np.float64)
     smax = max(float(s.max()) if s.size else 0.0, 1e-300)
     out = []
@@ -81,8 +82,7 @@ def subspace_angle_clusters(u_got, u_ref, sigma_ref, rtol: float = 1e-8):
             j += 1
         a = np.asarray(u_got)[:, i:j]
         b = np.asarray(u_ref)[:, i:j]
-        sv = np.linalg.svd(a.T @ b, compute_uv=False)
-        ang = float(np.arccos(np.clip(sv.min(), -1.0, 1.0))) if sv.size else 0.0
-        out.append((i, j, ang))
+        r = a - b @ (b.T @ a)
+        out.append((i, j, float(np.linalg.norm(r, 2)) if r.size else 0.0))
         i = j
     return out

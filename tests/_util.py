@@ -43,6 +43,13 @@ def assert_factors(a, u, s, vt, tol: float, what: str = ""):
     return m
 
 
+def subspace_sine(a, b) -> float:
+    """sin of the largest principal angle between span(a) and span(b)
+    (orthonormal columns): ||(I - b b^T) a||_2, accurate for tiny angles."""
+    r = a - b @ (b.T @ a)
+    return float(np.linalg.norm(r, 2)) if r.size else 0.0
+
+
 def max_subspace_angle(u_got, u_ref, sigma_ref, gap_rtol: float = 1e-6, min_rel_sigma: float = 1e-6):
     """Largest cluster subspace angle over clusters whose singular values are
     above min_rel_sigma * sigma_max (below that the subspace is set by

@@ -17,7 +17,7 @@ import pytest
 import oracle
 from lie_cliffalg_analog_svd_amd import (LieSvd, LieSvdMicro, LieSvdSmall, ShapeError, eigh_jacobi_full,
                                          metrics, profiles, rng, svd_batched)
-from tests._util import EPS, assert_factors, assert_sigma_close, imposed_spectrum, uniform_matrix
+from tests._util import EPS, assert_factors, assert_sigma_close, imposed_spectrum, subspace_sine, uniform_matrix
 
 pytestmark = pytest.mark.gpu
 
@@ -42,8 +42,7 @@ def _vector_parity(u_got, u_ref, s_ref, n, what):
         hi = s[j - 1] - s[j] if j < s.size else np.inf
         gap = min(lo, hi)
         if s[j - 1] >= 1e-8 * smax and np.isfinite(gap):
-            sv = np.linalg.svd(u_got[:, i:j].T @ u_ref[:, i:j], compute_uv=False)
-            ang = float(np.arccos(np.clip(sv.min(), -1.0, 1.0)))
+            ang = subspace_sine(u_got[:, i:j], u_ref[:, i:j])
             bound = max(1e3 * max(n, 1) * EPS * smax / gap, 1e-10)
             assert ang <= bound, f"{what}: cluster [{i},{j}) angle {ang:.3e} > {bound:.3e}"
         i = j

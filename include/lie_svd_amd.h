@@ -133,6 +133,23 @@ int lsvd_eigh_jacobi_f64(lsvd_handle_t h, const double* p, int n, double* v, dou
 int lsvd_micro_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
                          lsvd_info_t* info);
 
+/* ---- LieSvdJoint (Phase-JADE) ----------------------------------------- */
+
+/* JointDiagonalizationParams::for_n (src/lie_svd_joint.rs:26-35). */
+int lsvd_joint_params_for_n(int n, lsvd_joint_params_t* out);
+
+/* Replaces LieSvdJoint::diagonalize_symmetric_with_params
+ * (src/lie_svd_joint.rs:71-158), for `families` independent families at
+ * once.  mats: families x k x n x n (family f, matrix q at
+ * mats + (f*k + q)*n*n); v: families x n x n, the shared basis V with
+ * diag(V^T M_q V) ~ diag_q; diag: families x k x n.  params NULL means
+ * for_n(n).  trace (families entries, always host memory) may be NULL.
+ * k == 0 is the reference's "empty matrix set" panic (LSVD_E_SHAPE).
+ * n is limited to 2048 (LSVD_E_LIMIT beyond). */
+int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const double* mats,
+                            const lsvd_joint_params_t* params, double* v, double* diag,
+                            lsvd_joint_trace_t* trace);
+
 /* ---- Ragged batch (no reference counterpart: the reference does one
  *      matrix per call; this runs many LieSvd::solve / solve_rectangular
  *      calls, src/lie_svd.rs:49-51 + src/lie_svd_small.rs:340, at once) -- */

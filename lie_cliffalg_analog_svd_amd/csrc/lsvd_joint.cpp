@@ -1,0 +1,169 @@
+// lsvd_joint.cpp -- host side of the Phase-JADE entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "lsvd_api_internal.hpp"
+#include "lsvd_joint.hpp"
+
+using namespace lsvd;
+
+namespace {
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t want) {
+        if (want <= bytes && p) return hipSuccess;
+        if (p) hipFree(p);
+        p = nullptr;
+        bytes = 0;
+        hipError_t e = hipMalloc(&p, want ? want : 1);
+        if (e == hipSuccess) bytes = want;
+        return e;
+    }
+    ~DevBuf() {
+        if (p) hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+// JointDiagonalizationParams::for_n (src/lie_svd_joint.rs:26-35).
+int lsvd_joint_params_for_n(int n, lsvd_joint_params_t* out) {
+    if (!out) return LSVD_E_ARG;
+    out->max_sweeps = 12 + std::max(n, 1);
+    out->tol = 1e-12;
+    out->min_pair_scale = 1e-14;
+    out->line_search_steps = 6;
+    return LSVD_OK;
+}
+
+// Replaces LieSvdJoint::diagonalize_symmetric_with_params
+// (src/lie_svd_joint.rs:71-158) for `families` independent families.
+int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const double* mats,
+                            const lsvd_joint_params_t* params, double* v, double* diag, lsvd_joint_trace_t* trace) {
+    if (!h) return LSVD_E_ARG;
+    if (families < 0 || n < 0) return set_error(h, LSVD_E_SHAPE, "negative size");
+    if (k <= 0) return set_error(h, LSVD_E_SHAPE, "LieSvdJoint: empty matrix set");
+    if (families == 0) return LSVD_OK;
+    if (!mats || !v || !diag) return set_error(h, LSVD_E_ARG, "null pointer");
+    if (n > 2048) return set_error(h, LSVD_E_LIMIT, "LieSvdJoint: n > 2048 (LDS row staging)");
+    lsvd_joint_params_t prm;
+    if (params) prm = *params;
+    else lsvd_joint_params_for_n(n, &prm);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    hipStream_t st = h->stream;
+    const size_t nn = (size_t)n * n;
+    const size_t mats_n = (size_t)families * k * nn;
+    const int slots = (n + (n & 1)) / 2;
+
+    DevBuf bmats, bwork, bbasis, boff, bpart, brot, bfam, bdiag;
+#define JCHECK(expr)                                        \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+    const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    const double* d_mats = mats;
+    double* d_basis = v;
+    double* d_diag = diag;
+    if (host) {
+        JCHECK(bmats.ensure(sizeof(double) * mats_n));
+        JCHECK(hipMemcpyAsync(bmats.p, mats, sizeof(double) * mats_n, hipMemcpyHostToDevice, st));
+        d_mats = (const double*)bmats.p;
+        JCHECK(bbasis.ensure(sizeof(double) * families * nn));
+        d_basis = (double*)bbasis.p;
+        JCHECK(bdiag.ensure(sizeof(double) * (size_t)families * k * std::max(n, 1)));
+        d_diag = (double*)bdiag.p;
+    }
+    JCHECK(bwork.ensure(sizeof(double) * std::max<size_t>(mats_n, 1)));
+    JCHECK(boff.ensure(sizeof(double) * (size_t)families * k * std::max(n, 1)));
+    JCHECK(bpart.ensure(sizeof(double) * (size_t)families * k * 2));
+    JCHECK(brot.ensure(sizeof(JointRot) * (size_t)families * std::max(slots, 1)));
+    JCHECK(bfam.ensure(sizeof(JointFamily) * families));
+
+    JointArgs a{};
+    a.F = families;
+    a.K = k;
+    a.n = n;
+    a.slots = slots;
+    a.mats = d_mats;
+    a.work = (double*)bwork.p;
+    a.basis = d_basis;
+    a.offrow = (double*)boff.p;
+    a.part = (double*)bpart.p;
+    a.rot = (JointRot*)brot.p;
+    a.fam = (JointFamily*)bfam.p;
+    a.tol = prm.tol;
+    a.min_pair_scale = prm.min_pair_scale;
+    a.line_search_steps = prm.line_search_steps;
+    a.max_sweeps = prm.max_sweeps;
+
+    if (n == 0) {
+        for (int f = 0; f < families && trace; ++f) trace[f] = lsvd_joint_trace_t{};
+        return LSVD_OK;
+    }
+    JCHECK(joint_launch_init(a, st));
+    std::vector<JointFamily> fam(families);
+    if (n > 1 && prm.max_sweeps > 0) {
+        // One sweep = 2 launches per layer + the stop test, captured once as a
+        // graph and relaunched until every family has stopped.
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        JCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t le = joint_launch_sweep(a, st);
+        JCHECK(hipStreamEndCapture(st, &graph));
+        if (le != hipSuccess) {
+            hipGraphDestroy(graph);
+            return hip_fail(h, le, "joint sweep launch");
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(graph);
+            return hip_fail(h, e, "hipGraphInstantiate");
+        }
+        int rc = LSVD_OK;
+        for (int sweep = 0; sweep < prm.max_sweeps; ++sweep) {
+            e = hipGraphLaunch(exec, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(fam.data(), a.fam, sizeof(JointFamily) * families, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                rc = hip_fail(h, e, "joint sweep");
+                break;
+            }
+            bool all = true;
+            for (auto& s : fam) all = all && s.done;
+            if (all) break;
+        }
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        if (rc != LSVD_OK) return rc;
+    }
+    JCHECK(joint_launch_diag(a, d_diag, st));
+    JCHECK(hipMemcpyAsync(fam.data(), a.fam, sizeof(JointFamily) * families, hipMemcpyDeviceToHost, st));
+    if (host) {
+        JCHECK(hipMemcpyAsync(v, d_basis, sizeof(double) * families * nn, hipMemcpyDeviceToHost, st));
+        JCHECK(hipMemcpyAsync(diag, d_diag, sizeof(double) * (size_t)families * k * n, hipMemcpyDeviceToHost, st));
+    }
+    JCHECK(hipStreamSynchronize(st));
+#undef JCHECK
+    if (trace) {
+        for (int f = 0; f < families; ++f) {
+            trace[f].initial_offdiag = n <= 1 ? 0.0 : fam[f].initial_offdiag;
+            trace[f].final_offdiag = n <= 1 ? 0.0 : fam[f].final_offdiag;
+            trace[f].sweeps = fam[f].sweeps;
+            trace[f].rotations = fam[f].rotations;
+            trace[f].rejected_rotations = fam[f].rejected;
+        }
+    }
+    return LSVD_OK;
+}
+
+}  // extern "C"

@@ -80,3 +80,37 @@ def ragged_batch(count: int = 4096, tiny: int = 8192, seed: int = SEED_BATCH, km
 def pack(mats):
     """Packs row-major matrices into one f64 pool (the plan layout)."""
     return np.concatenate([np.ascontiguousarray(a, dtype=np.float64).ravel() for a in mats])
+
+
+def jade_family(seed: int, family: int, K: int = 64, n: int = 256, eps: float = 1e-6):
+    """One Phase-JADE family: M_k = Q diag(d_k) Q^T + eps * sym(G_k) with a
+    Haar Q, d_k ~ N(0, 1)^n and G_k iid N(0, 1).  Returns (mats (K, n, n),
+    Q, d (K, n))."""
+    s0 = rng.mix_seed(seed, family)
+    q = rng.haar(rng.mix_seed(s0, 0), n)
+    d = rng.normal(rng.mix_seed(s0, 1), K * n).reshape(K, n)
+    g = rng.normal(rng.mix_seed(s0, 2), K * n * n).reshape(K, n, n)
+    e = 0.5 * (g + np.transpose(g, (0, 2, 1)))
+    mats = np.einsum("ij,kj,lj->kil", q, d, q, optimize=True) + eps * e
+    return np.ascontiguousarray(mats), q, d
+
+
+def jade_batch(families: int = 32, K: int = 64, n: int = 256, eps: float = 1e-6, seed: int = SEED_JADE):
+    fams, qs, ds = [], [], []
+    for f in range(families):
+        m, q, d = jade_family(seed, f, K, n, eps)
+        fams.append(m)
+        qs.append(q)
+        ds.append(d)
+    return np.stack(fams), qs, ds
+
+
+def recovery_error(q, v):
+    """Distance between planted Q and recovered V up to column permutation
+    and sign: match columns greedily by |Q^T V| (a permutation when the
+    recovery is good) and return max |1 - |<q_i, v_pi(i)>||."""
+    c = np.abs(q.T @ v)
+    perm = c.argmax(axis=1)
+    if len(set(perm.tolist())) != len(perm):
+        return 1.0
+    return float(np.max(np.abs(1.0 - c[np.arange(len(perm)), perm])))

@@ -133,6 +133,33 @@ int lsvd_eigh_jacobi_f64(lsvd_handle_t h, const double* p, int n, double* v, dou
 int lsvd_micro_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
                          lsvd_info_t* info);
 
+/* ---- Newton-Schulz polar factor --------------------------------------- */
+
+/* Replaces newton_schulz_polar (src/lie_svd_small.rs:158-183), which the
+ * reference runs on square matrices, for tall or square m x n (m >= n):
+ * q (m x n) gets the orthonormal polar factor.
+ *   LSVD_PREC_F64  : the reference iteration in FP64 (FP64 MFMA products).
+ *   LSVD_PREC_F32  : FP32 MFMA products until the iteration stalls, then
+ *   LSVD_PREC_BF16   (BF16 MFMA, FP32 accumulate) FP64 refinement iterations
+ *                    to the reference's FP64 stopping rule.
+ * Low-precision products perturb the singular directions the FP64
+ * refinement cannot restore; info->orth_err is ||Q^T Q - I||_F and the
+ * caller states the error against the FP64 factor.  The Gram-Schmidt
+ * fallback of the reference is not run at this size: on divergence
+ * polar_ok is 0 and q holds the last iterate. */
+int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision, double* q,
+                   lsvd_polar_info_t* info);
+
+/* FP64 MFMA product C (m x n) = op(A) B, op(A) = A^T (A stored k x m) when
+ * trans_a, else A (m x k); B is k x n.  Device pointer mode only.
+ * (Diagnostic; no reference counterpart.) */
+int lsvd_gemm_f64(lsvd_handle_t h, int trans_a, int m, int n, int k, const double* a, const double* b, double* c);
+
+/* The same with FP32 (LSVD_PREC_F32) or BF16-rounded (LSVD_PREC_BF16)
+ * operands and FP32 accumulation, FP32 data.  Device pointers only. */
+int lsvd_gemm_lowp(lsvd_handle_t h, int precision, int trans_a, int m, int n, int k, const float* a,
+                   const float* b, float* c);
+
 /* ---- LieSvdJoint (Phase-JADE) ----------------------------------------- */
 
 /* JointDiagonalizationParams::for_n (src/lie_svd_joint.rs:26-35). */

@@ -12,6 +12,7 @@ from .handle import BatchPlan, Handle, default_handle, pack_offsets
 from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall, eigh_jacobi_full, run_small,
                       svd_batched)
 from . import metrics, profiles
+from .polar import newton_schulz_polar
 from .joint import (JointDiagonalizationParams, JointDiagonalizationTrace, LieSvdJoint,
                     diagonalize_families)
 

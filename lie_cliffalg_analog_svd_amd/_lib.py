@@ -135,6 +135,9 @@ SIGNATURES = {
     "lsvd_svd_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _vp, _i64p, _vp, _i64p, _vp, _i64p,
                                        _vp, C.c_int, _vp]),
     "lsvd_autotune_batch": (C.c_int, [_vp, C.c_int, _i32p, _i32p, C.c_int]),
+    "lsvd_polar_f64": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdPolarInfo)]),
+    "lsvd_gemm_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "lsvd_gemm_lowp": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "lsvd_joint_params_for_n": (C.c_int, [C.c_int, C.POINTER(LsvdJointParams)]),
     "lsvd_joint_diag_sym_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdJointParams), _vp,
                                           _vp, _vp]),

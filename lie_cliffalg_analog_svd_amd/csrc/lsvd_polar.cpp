@@ -1,0 +1,183 @@
+// lsvd_polar.cpp -- host driver of the Newton-Schulz polar decomposition
+// for large matrices (newton_schulz_polar, src/lie_svd_small.rs:158-183).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "lsvd_api_internal.hpp"
+#include "lsvd_polar.hpp"
+
+using namespace lsvd;
+
+namespace {
+
+struct Buf {
+    void* p = nullptr;
+    ~Buf() {
+        if (p) hipFree(p);
+    }
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
+};
+
+}  // namespace
+
+extern "C" {
+
+// NS polar factor of a tall or square m x n matrix (m >= n).
+//   LSVD_PREC_F64 : the reference iteration in FP64 (X0 = A/||A||_F, X <- X(3I - X^T X)/2,
+//                   stop at ||dX||_F < 1e-13 sqrt(n), polar_ok = ||Q^T Q - I||_F <= 1e-8 sqrt(n)).
+//   LSVD_PREC_F32 / LSVD_PREC_BF16 : the same iteration with FP32 / BF16 MFMA products
+//                   (FP32 accumulation) until it stalls at that precision, then FP64 refinement
+//                   iterations with the FP64 stopping rule.
+int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision, double* q,
+                   lsvd_polar_info_t* info) {
+    if (!h) return LSVD_E_ARG;
+    if (m < n || n < 0) return set_error(h, LSVD_E_SHAPE, "lsvd_polar_f64: needs m >= n");
+    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16)
+        return set_error(h, LSVD_E_ARG, "unknown precision");
+    lsvd_polar_info_t inf{};
+    if (n == 0) {
+        inf.polar_ok = 1;
+        if (info) *info = inf;
+        return LSVD_OK;
+    }
+    if (!a || !q) return set_error(h, LSVD_E_ARG, "null pointer");
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    hipStream_t st = h->stream;
+    const size_t mn = (size_t)m * n, nn = (size_t)n * n;
+    const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    const bool lowp = precision != LSVD_PREC_F64;
+    const int lp = precision == LSVD_PREC_F32 ? kPolF32 : kPolBF16;
+    Buf bA, bX, bX2, bM, bXf, bXf2, bMf, bS;
+#define PCHK(expr)                                          \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+    const double* dA = a;
+    if (host) {
+        PCHK(bA.alloc(sizeof(double) * mn));
+        PCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * mn, hipMemcpyHostToDevice, st));
+        dA = (const double*)bA.p;
+    }
+    PCHK(bX.alloc(sizeof(double) * mn));
+    PCHK(bX2.alloc(sizeof(double) * mn));
+    PCHK(bM.alloc(sizeof(double) * nn));
+    PCHK(bS.alloc(sizeof(double) * 4));
+    if (lowp) {
+        PCHK(bXf.alloc(sizeof(float) * mn));
+        PCHK(bXf2.alloc(sizeof(float) * mn));
+        PCHK(bMf.alloc(sizeof(float) * nn));
+    }
+    double* S = (double*)bS.p;  // [0] ||A||^2, [1] ||dX||^2, [2] orth^2, [3] bad flag (as int)
+    int* bad = (int*)(S + 3);
+    double hs[4];
+    PCHK(polar_zero(S, 4, st));
+    PCHK(polar_sumsq(dA, mn, S, st));
+    const double rootn = std::sqrt((double)n);
+    double delta = 0.0;
+    int it_low = 0, it64 = 0;
+    bool diverged = false;
+    if (lowp) {
+        PCHK(polar_scale(dA, mn, S, bXf.p, true, st));
+        void* X = bXf.p;
+        void* Xn = bXf2.p;
+        // the low-precision iteration stalls near its rounding floor
+        const double floor_tol = (lp == kPolBF16 ? 2e-2 : 1e-5) * rootn;
+        double prev = 1e300;
+        for (int it = 0; it < 120; ++it) {
+            PCHK(polar_zero(S + 1, 3, st));
+            PCHK(ns_gram(lp, m, n, X, bMf.p, st));
+            PCHK(ns_update(lp, m, n, X, bMf.p, Xn, S + 1, bad, st));
+            PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
+            PCHK(hipStreamSynchronize(st));
+            ++it_low;
+            delta = std::sqrt(hs[1]);
+            int b = 0;
+            std::memcpy(&b, &hs[3], sizeof(int));
+            if (b || !(delta <= 1e6 * rootn)) {  // non-finite or diverging: redo in FP64
+                diverged = true;
+                break;
+            }
+            std::swap(X, Xn);
+            if (delta < floor_tol || (delta < 0.5 * rootn && delta > 0.7 * prev)) break;
+            prev = delta;
+        }
+        if (diverged) {
+            PCHK(polar_scale(dA, mn, S, bX.p, false, st));
+        } else {
+            PCHK(polar_widen((const float*)X, mn, (double*)bX.p, st));
+        }
+    } else {
+        PCHK(polar_scale(dA, mn, S, bX.p, false, st));
+    }
+    double* X = (double*)bX.p;
+    double* Xn = (double*)bX2.p;
+    bool converged = false, bad64 = false;
+    const int budget = 120;
+    for (int it = 0; it < budget; ++it) {
+        PCHK(polar_zero(S + 1, 3, st));
+        PCHK(ns_gram(kPolF64, m, n, X, bM.p, st));
+        PCHK(ns_update(kPolF64, m, n, X, bM.p, Xn, S + 1, bad, st));
+        PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
+        PCHK(hipStreamSynchronize(st));
+        ++it64;
+        delta = std::sqrt(hs[1]);
+        int b = 0;
+        std::memcpy(&b, &hs[3], sizeof(int));
+        if (b || delta > 1e6 * rootn) {  // (:169-171); the Gram-Schmidt fallback is not run at this size
+            bad64 = true;
+            break;
+        }
+        std::swap(X, Xn);
+        if (delta < 1e-13 * rootn) {
+            converged = true;
+            break;
+        }
+    }
+    PCHK(polar_zero(S + 2, 1, st));
+    PCHK(ns_orth(m, n, X, S + 2, st));
+    PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
+    PCHK(hipStreamSynchronize(st));
+    inf.orth_err = std::sqrt(std::max(hs[2], 0.0));
+    inf.polar_ok = (!bad64 && inf.orth_err <= 1e-8 * rootn) ? 1 : 0;
+    (void)converged;
+    inf.lowp_iters = it_low;
+    inf.f64_iters = it64;
+    inf.last_delta = delta;
+    if (host)
+        PCHK(hipMemcpyAsync(q, X, sizeof(double) * mn, hipMemcpyDeviceToHost, st));
+    else
+        PCHK(hipMemcpyAsync(q, X, sizeof(double) * mn, hipMemcpyDeviceToDevice, st));
+    PCHK(hipStreamSynchronize(st));
+#undef PCHK
+    if (info) *info = inf;
+    return LSVD_OK;
+}
+
+// Plain FP64 product C = op(A) B (diagnostic; used by the GEMM parity tests).
+int lsvd_gemm_f64(lsvd_handle_t h, int trans_a, int m, int n, int k, const double* a, const double* b, double* c) {
+    if (!h || !a || !b || !c) return LSVD_E_ARG;
+    if (m <= 0 || n <= 0 || k <= 0) return set_error(h, LSVD_E_SHAPE, "empty product");
+    if (h->ptr_mode != LSVD_PTR_DEVICE) return set_error(h, LSVD_E_ARG, "lsvd_gemm_f64 needs device pointers");
+    hipError_t e = gemm_f64(trans_a != 0, m, n, k, a, b, c, 1.0, h->stream);
+    return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "gemm_f64");
+}
+
+// Low-precision product (prec LSVD_PREC_F32 or LSVD_PREC_BF16), FP32 in/out.
+int lsvd_gemm_lowp(lsvd_handle_t h, int precision, int trans_a, int m, int n, int k, const float* a, const float* b,
+                   float* c) {
+    if (!h || !a || !b || !c) return LSVD_E_ARG;
+    if (m <= 0 || n <= 0 || k <= 0) return set_error(h, LSVD_E_SHAPE, "empty product");
+    if (precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16) return set_error(h, LSVD_E_ARG, "precision");
+    if (h->ptr_mode != LSVD_PTR_DEVICE) return set_error(h, LSVD_E_ARG, "lsvd_gemm_lowp needs device pointers");
+    hipError_t e = gemm_lowp(precision == LSVD_PREC_F32 ? kPolF32 : kPolBF16, trans_a != 0, m, n, k, a, b, c,
+                             h->stream);
+    return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "gemm_lowp");
+}
+
+}  // extern "C"

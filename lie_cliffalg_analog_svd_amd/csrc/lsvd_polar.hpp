@@ -1,0 +1,30 @@
+// lsvd_polar.hpp -- launch interface of the large-matrix polar kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace lsvd {
+
+enum PolarPrec { kPolF64 = 0, kPolF32 = 1, kPolBF16 = 2 };
+enum Epilogue { kEpiStore = 0, kEpi3IMinus = 1, kEpiHalfDelta = 2, kEpiOrth = 3 };
+
+// M = 3I - X^T X (n x n) from X (m x n); X/M are f64 (kPolF64) or f32.
+hipError_t ns_gram(int prec, int m, int n, const void* X, void* M, hipStream_t st);
+// Xn = 0.5 X M; *dsq += ||Xn - X||_F^2; *bad |= any non-finite entry of Xn.
+hipError_t ns_update(int prec, int m, int n, const void* X, const void* M, void* Xn, double* dsq, int* bad,
+                     hipStream_t st);
+// *sum += ||X^T X - I||_F^2 in FP64.
+hipError_t ns_orth(int m, int n, const double* X, double* sum, hipStream_t st);
+// C = alpha op(A) B, FP64; op(A) = A^T when trans_a (A stored K x M).
+hipError_t gemm_f64(bool trans_a, int M, int N, int K, const double* A, const double* B, double* C, double alpha,
+                    hipStream_t st);
+// C = op(A) B with FP32 (kPolF32) or BF16-rounded (kPolBF16) operands, FP32 out.
+hipError_t gemm_lowp(int prec, bool trans_a, int M, int N, int K, const float* A, const float* B, float* C,
+                     hipStream_t st);
+hipError_t polar_sumsq(const double* x, size_t n, double* out, hipStream_t st);
+hipError_t polar_scale(const double* x, size_t n, const double* sumsq, void* y, bool f32, hipStream_t st);
+hipError_t polar_widen(const float* x, size_t n, double* y, hipStream_t st);
+hipError_t polar_zero(double* p, int n, hipStream_t st);
+
+}  // namespace lsvd

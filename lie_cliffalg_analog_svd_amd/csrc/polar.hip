@@ -1,0 +1,360 @@
+// polar.hip -- MFMA GEMMs of the Newton-Schulz polar iteration for large
+// matrices (newton_schulz_polar, src/lie_svd_small.rs:158-183, generalised
+// to tall m x n), plus the small helper kernels of the iteration.
+//
+// One NS step is two products with fused epilogues:
+//   M     = 3I - X^T X      (n x n, symmetric: only upper tiles are computed,
+//                            the epilogue writes both halves)
+//   X_new = 0.5 X M         (m x n; the epilogue accumulates ||X_new - X||_F^2
+//                            and a non-finite flag for the stopping test)
+// Three arithmetic variants share the tiling:
+//   f64  : v_mfma_f64_16x16x4f64, FP64 in LDS           (plain path, refinement)
+//   f32  : v_mfma_f32_16x16x4f32, FP32 in LDS
+//   bf16 : v_mfma_f32_16x16x32_bf16, FP32 data rounded to BF16 when staged
+//          into LDS, FP32 accumulation
+// Workgroup: 256 threads = 2 x 2 wavefronts; each wavefront owns a
+// (BM/2) x (BN/2) block of 16 x 16 MFMA tiles.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "lsvd_device.hpp"
+#include "lsvd_polar.hpp"
+
+namespace lsvd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+
+template <class T>
+struct Acc4;
+template <>
+struct Acc4<double> {
+    typedef d4 type;
+};
+template <>
+struct Acc4<float> {
+    typedef f4 type;
+};
+
+__device__ __forceinline__ d4 mfma4(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+__device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+
+// Epilogue: element (i, j) of the product, value `acc` (already in T).
+struct EpiArgs {
+    void* c;          // output (T*), ldc = N
+    const void* xold; // kHalfDelta: previous iterate (T*), same shape as c
+    double* sum;      // kHalfDelta / kOrth: accumulated squared norm
+    int* bad;         // kHalfDelta: non-finite flag
+    double alpha;
+};
+
+template <class T, int EPI>
+__device__ __forceinline__ void epi_apply(const EpiArgs& e, int N, int i, int j, T acc, double& part, int& bad,
+                                          bool mirror) {
+    T* c = reinterpret_cast<T*>(e.c);
+    if (EPI == kEpiStore) {
+        const T v = (T)(e.alpha * (double)acc);
+        c[(size_t)i * N + j] = v;
+        if (mirror) c[(size_t)j * N + i] = v;
+    } else if (EPI == kEpi3IMinus) {
+        const T v = (T)((i == j ? 3.0 : 0.0) - (double)acc);
+        c[(size_t)i * N + j] = v;
+        if (mirror) c[(size_t)j * N + i] = v;
+    } else if (EPI == kEpiHalfDelta) {
+        const T v = (T)(0.5 * (double)acc);
+        const T* xo = reinterpret_cast<const T*>(e.xold);
+        const double d = (double)v - (double)xo[(size_t)i * N + j];
+        c[(size_t)i * N + j] = v;
+        part += d * d;
+        bad |= !isfinite((double)v);
+    } else {  // kEpiOrth: ||X^T X - I||_F^2, symmetric tiles count twice off the diagonal
+        const double d = (double)acc - (i == j ? 1.0 : 0.0);
+        part += (mirror ? 2.0 : 1.0) * d * d;
+    }
+}
+
+template <int EPI>
+__device__ __forceinline__ void epi_finish(const EpiArgs& e, double part, int bad) {
+    if (EPI != kEpiHalfDelta && EPI != kEpiOrth) return;
+    __shared__ double red[4];
+    part = block_sum<256>(part, red);
+    if (EPI == kEpiHalfDelta) bad = block_or<256>(bad, red);
+    if (threadIdx.x == 0) {
+        atomicAdd(e.sum, part);
+        if (EPI == kEpiHalfDelta && bad) atomicOr(e.bad, 1);
+    }
+}
+
+// Maps a linear block id to an upper-triangular tile pair (ti <= tj).
+__device__ __forceinline__ void tri_tile(int b, int& ti, int& tj) {
+    int j = (int)((sqrt(8.0 * b + 1.0) - 1.0) * 0.5);
+    while (j * (j + 1) / 2 > b) --j;
+    while ((j + 1) * (j + 2) / 2 <= b) ++j;
+    tj = j;
+    ti = b - j * (j + 1) / 2;
+}
+
+// ---- f64 / f32: C (M x N) = op(A) B, A stored K x M (TRANS_A) or M x K --------
+// BM = BN = 64, BK = 16.  SYM: square symmetric output, upper tiles only.
+template <class T, bool TRANS_A, bool SYM, int EPI>
+__global__ __launch_bounds__(256) void gemm_mfma4_kernel(int M, int N, int K, const T* __restrict__ A,
+                                                          const T* __restrict__ B, EpiArgs e) {
+    constexpr int BM = 64, BN = 64, BK = 16, LDA = BM + 1, LDB = BN + 1;
+    __shared__ T As[BK * LDA];
+    __shared__ T Bs[BK * LDB];
+    typedef typename Acc4<T>::type acc_t;
+    int ti, tj;
+    if (SYM) {
+        tri_tile(blockIdx.x, ti, tj);
+    } else {
+        ti = blockIdx.y;
+        tj = blockIdx.x;
+    }
+    const int i0 = ti * BM, j0 = tj * BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    acc_t acc[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e_ = t + 256 * r;
+            if (TRANS_A) {  // op(A)[i][k] = A[k][i], rows of A contiguous in i
+                const int kk = e_ / BM, ii = e_ % BM;
+                const int gk = k0 + kk, gi = i0 + ii;
+                As[kk * LDA + ii] = (gk < K && gi < M) ? A[(size_t)gk * M + gi] : (T)0;
+            } else {
+                const int ii = e_ / BK, kk = e_ % BK;
+                const int gk = k0 + kk, gi = i0 + ii;
+                As[kk * LDA + ii] = (gk < K && gi < M) ? A[(size_t)gi * K + gk] : (T)0;
+            }
+            const int kk = e_ / BN, jj = e_ % BN;
+            const int gk = k0 + kk, gj = j0 + jj;
+            Bs[kk * LDB + jj] = (gk < K && gj < N) ? B[(size_t)gk * N + gj] : (T)0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            const int kr = kk + (lane >> 4), c16 = lane & 15;
+            T av[2], bv[2];
+#pragma unroll
+            for (int a = 0; a < 2; ++a) av[a] = As[kr * LDA + wm * 32 + a * 16 + c16];
+#pragma unroll
+            for (int b = 0; b < 2; ++b) bv[b] = Bs[kr * LDB + wn * 32 + b * 16 + c16];
+#pragma unroll
+            for (int a = 0; a < 2; ++a)
+#pragma unroll
+                for (int b = 0; b < 2; ++b) acc[a][b] = mfma4(av[a], bv[b], acc[a][b]);
+        }
+        __syncthreads();
+    }
+    double part = 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wm * 32 + a * 16 + 4 * (lane >> 4) + r;
+                const int j = j0 + wn * 32 + b * 16 + (lane & 15);
+                if (i < M && j < N && (!SYM || ti < tj || i <= j))
+                    epi_apply<T, EPI>(e, N, i, j, acc[a][b][r], part, bad, SYM && i != j);
+            }
+    epi_finish<EPI>(e, part, bad);
+}
+
+// ---- bf16 operands, f32 data and accumulation --------------------------------
+// BM = BN = 128, BK = 32; LDS tiles are [row][k] with k contiguous so one lane
+// reads its 8 consecutive k values as one 16-byte load.
+template <bool TRANS_A, bool SYM, int EPI>
+__global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, const float* __restrict__ A,
+                                                         const float* __restrict__ B, EpiArgs e) {
+    constexpr int BM = 128, BN = 128, BK = 32, LDK = BK + 8;
+    __shared__ __align__(16) __bf16 As[BM * LDK];
+    __shared__ __align__(16) __bf16 Bs[BN * LDK];
+    int ti, tj;
+    if (SYM) {
+        tri_tile(blockIdx.x, ti, tj);
+    } else {
+        ti = blockIdx.y;
+        tj = blockIdx.x;
+    }
+    const int i0 = ti * BM, j0 = tj * BN;
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    f4 acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int e_ = t + 256 * r;
+            if (TRANS_A) {
+                const int kk = e_ / BM, ii = e_ % BM;
+                const int gk = k0 + kk, gi = i0 + ii;
+                As[ii * LDK + kk] = (__bf16)((gk < K && gi < M) ? A[(size_t)gk * M + gi] : 0.0f);
+            } else {
+                const int ii = e_ / BK, kk = e_ % BK;
+                const int gk = k0 + kk, gi = i0 + ii;
+                As[ii * LDK + kk] = (__bf16)((gk < K && gi < M) ? A[(size_t)gi * K + gk] : 0.0f);
+            }
+            const int kk = e_ / BN, jj = e_ % BN;
+            const int gk = k0 + kk, gj = j0 + jj;
+            Bs[jj * LDK + kk] = (__bf16)((gk < K && gj < N) ? B[(size_t)gk * N + gj] : 0.0f);
+        }
+        __syncthreads();
+        const int kq = 8 * (lane >> 4), c16 = lane & 15;
+        bf8 av[4], bv[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) av[a] = *reinterpret_cast<const bf8*>(&As[(wm * 64 + a * 16 + c16) * LDK + kq]);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) bv[b] = *reinterpret_cast<const bf8*>(&Bs[(wn * 64 + b * 16 + c16) * LDK + kq]);
+#pragma unroll
+        for (int a = 0; a < 4; ++a)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av[a], bv[b], acc[a][b], 0, 0, 0);
+        __syncthreads();
+    }
+    double part = 0.0;
+    int bad = 0;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = i0 + wm * 64 + a * 16 + 4 * (lane >> 4) + r;
+                const int j = j0 + wn * 64 + b * 16 + (lane & 15);
+                if (i < M && j < N && (!SYM || ti < tj || i <= j))
+                    epi_apply<float, EPI>(e, N, i, j, acc[a][b][r], part, bad, SYM && i != j);
+            }
+    epi_finish<EPI>(e, part, bad);
+}
+
+// ---- helpers --------------------------------------------------------------------
+__global__ void sumsq_kernel(const double* x, size_t n, double* out) {
+    __shared__ double red[4];
+    double s = 0.0;
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        s += x[e] * x[e];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) atomicAdd(out, s);
+}
+
+// y = x / max(sqrt(*sumsq), 1e-300), into f64 or f32 storage
+template <class T>
+__global__ void scale_kernel(const double* x, size_t n, const double* sumsq, T* y) {
+    const double s = fmax(sqrt(*sumsq), 1e-300);
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        y[e] = (T)(x[e] / s);
+}
+
+__global__ void f32_to_f64_kernel(const float* x, size_t n, double* y) {
+    for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+        y[e] = (double)x[e];
+}
+
+__global__ void zero_kernel(double* p, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = 0.0;
+}
+
+// ---- launch wrappers ---------------------------------------------------------------
+template <class T, bool TA, bool SYM, int EPI>
+static hipError_t launch4(int M, int N, int K, const T* A, const T* B, const EpiArgs& e, hipStream_t st) {
+    dim3 grid;
+    if (SYM) {
+        const int tn = (N + 63) / 64;
+        grid = dim3(tn * (tn + 1) / 2);
+    } else {
+        grid = dim3((N + 63) / 64, (M + 63) / 64);
+    }
+    hipLaunchKernelGGL((gemm_mfma4_kernel<T, TA, SYM, EPI>), grid, dim3(256), 0, st, M, N, K, A, B, e);
+    return hipGetLastError();
+}
+
+template <bool TA, bool SYM, int EPI>
+static hipError_t launch_bf16(int M, int N, int K, const float* A, const float* B, const EpiArgs& e, hipStream_t st) {
+    dim3 grid;
+    if (SYM) {
+        const int tn = (N + 127) / 128;
+        grid = dim3(tn * (tn + 1) / 2);
+    } else {
+        grid = dim3((N + 127) / 128, (M + 127) / 128);
+    }
+    hipLaunchKernelGGL((gemm_bf16_kernel<TA, SYM, EPI>), grid, dim3(256), 0, st, M, N, K, A, B, e);
+    return hipGetLastError();
+}
+
+// M = 3I - X^T X (n x n) from X (m x n)
+hipError_t ns_gram(int prec, int m, int n, const void* X, void* Mout, hipStream_t st) {
+    EpiArgs e{Mout, nullptr, nullptr, nullptr, 1.0};
+    if (prec == kPolF64) return launch4<double, true, true, kEpi3IMinus>(n, n, m, (const double*)X, (const double*)X, e, st);
+    if (prec == kPolF32) return launch4<float, true, true, kEpi3IMinus>(n, n, m, (const float*)X, (const float*)X, e, st);
+    return launch_bf16<true, true, kEpi3IMinus>(n, n, m, (const float*)X, (const float*)X, e, st);
+}
+
+// Xn = 0.5 X M, accumulating ||Xn - X||^2 into *dsq and non-finite into *bad
+hipError_t ns_update(int prec, int m, int n, const void* X, const void* Mm, void* Xn, double* dsq, int* bad,
+                     hipStream_t st) {
+    EpiArgs e{Xn, X, dsq, bad, 1.0};
+    if (prec == kPolF64) return launch4<double, false, false, kEpiHalfDelta>(m, n, n, (const double*)X, (const double*)Mm, e, st);
+    if (prec == kPolF32) return launch4<float, false, false, kEpiHalfDelta>(m, n, n, (const float*)X, (const float*)Mm, e, st);
+    return launch_bf16<false, false, kEpiHalfDelta>(m, n, n, (const float*)X, (const float*)Mm, e, st);
+}
+
+// *sum += ||X^T X - I||_F^2 (FP64)
+hipError_t ns_orth(int m, int n, const double* X, double* sum, hipStream_t st) {
+    EpiArgs e{nullptr, nullptr, sum, nullptr, 1.0};
+    return launch4<double, true, true, kEpiOrth>(n, n, m, X, X, e, st);
+}
+
+// Plain C = op(A) B in FP64 (tests and the adjoint)
+hipError_t gemm_f64(bool trans_a, int M, int N, int K, const double* A, const double* B, double* C, double alpha,
+                    hipStream_t st) {
+    EpiArgs e{C, nullptr, nullptr, nullptr, alpha};
+    if (trans_a) return launch4<double, true, false, kEpiStore>(M, N, K, A, B, e, st);
+    return launch4<double, false, false, kEpiStore>(M, N, K, A, B, e, st);
+}
+
+hipError_t gemm_lowp(int prec, bool trans_a, int M, int N, int K, const float* A, const float* B, float* C,
+                     hipStream_t st) {
+    EpiArgs e{C, nullptr, nullptr, nullptr, 1.0};
+    if (prec == kPolF32) {
+        if (trans_a) return launch4<float, true, false, kEpiStore>(M, N, K, A, B, e, st);
+        return launch4<float, false, false, kEpiStore>(M, N, K, A, B, e, st);
+    }
+    if (trans_a) return launch_bf16<true, false, kEpiStore>(M, N, K, A, B, e, st);
+    return launch_bf16<false, false, kEpiStore>(M, N, K, A, B, e, st);
+}
+
+hipError_t polar_sumsq(const double* x, size_t n, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(sumsq_kernel, dim3(1024), dim3(256), 0, st, x, n, out);
+    return hipGetLastError();
+}
+
+hipError_t polar_scale(const double* x, size_t n, const double* sumsq, void* y, bool f32, hipStream_t st) {
+    if (f32)
+        hipLaunchKernelGGL(scale_kernel<float>, dim3(1024), dim3(256), 0, st, x, n, sumsq, (float*)y);
+    else
+        hipLaunchKernelGGL(scale_kernel<double>, dim3(1024), dim3(256), 0, st, x, n, sumsq, (double*)y);
+    return hipGetLastError();
+}
+
+hipError_t polar_widen(const float* x, size_t n, double* y, hipStream_t st) {
+    hipLaunchKernelGGL(f32_to_f64_kernel, dim3(1024), dim3(256), 0, st, x, n, y);
+    return hipGetLastError();
+}
+
+hipError_t polar_zero(double* p, int n, hipStream_t st) {
+    hipLaunchKernelGGL(zero_kernel, dim3(1), dim3(64), 0, st, p, n);
+    return hipGetLastError();
+}
+
+}  // namespace lsvd

@@ -114,3 +114,12 @@ def recovery_error(q, v):
     if len(set(perm.tolist())) != len(perm):
         return 1.0
     return float(np.max(np.abs(1.0 - c[np.arange(len(perm)), perm])))
+
+
+def polar_matrix(m: int, n: int, seed: int = SEED_POLAR, lo: float = 1e-6, hi: float = 1.0):
+    """A = U diag(s) V^T with Haar U (m x n), V (n x n), s log-uniform in
+    [lo, hi].  Returns (A, s, U, V).  The exact polar factor is U V^T."""
+    s = np.exp(np.log(lo) + (np.log(hi) - np.log(lo)) * rng.uniform(rng.mix_seed(seed, 7), n))
+    u = rng.haar(rng.mix_seed(seed, m * 131 + n), m, n)
+    v = rng.haar(rng.mix_seed(seed, n * 977 + 3), n)
+    return (u * s[None, :]) @ v.T, s, u, v

@@ -86,12 +86,14 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
         PCHK(polar_scale(dA, mn, S, bXf.p, true, st));
         void* X = bXf.p;
         void* Xn = bXf2.p;
-        // the low-precision iteration stalls near its rounding floor
-        const double floor_tol = (lp == kPolBF16 ? 2e-2 : 1e-5) * rootn;
-        double prev = 1e300;
+        // The low-precision phase runs until ||X^T X - I||_F (monotone in
+        // exact arithmetic) reaches its rounding floor or stops improving.
+        const double floor_tol = (lp == kPolBF16 ? 1e-3 : 1e-6) * rootn;
+        double best = 1e300;
+        int stall = 0;
         for (int it = 0; it < 120; ++it) {
             PCHK(polar_zero(S + 1, 3, st));
-            PCHK(ns_gram(lp, m, n, X, bMf.p, st));
+            PCHK(ns_gram(lp, m, n, X, bMf.p, S + 2, st));
             PCHK(ns_update(lp, m, n, X, bMf.p, Xn, S + 1, bad, st));
             PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
             PCHK(hipStreamSynchronize(st));
@@ -104,8 +106,14 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
                 break;
             }
             std::swap(X, Xn);
-            if (delta < floor_tol || (delta < 0.5 * rootn && delta > 0.7 * prev)) break;
-            prev = delta;
+            const double orth = std::sqrt(std::max(hs[2], 0.0));  // of the iterate before this step
+            if (orth < floor_tol) break;
+            if (orth < 0.99 * best) {
+                best = orth;
+                stall = 0;
+            } else if (++stall >= 3) {
+                break;
+            }
         }
         if (diverged) {
             PCHK(polar_scale(dA, mn, S, bX.p, false, st));
@@ -121,7 +129,7 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
     const int budget = 120;
     for (int it = 0; it < budget; ++it) {
         PCHK(polar_zero(S + 1, 3, st));
-        PCHK(ns_gram(kPolF64, m, n, X, bM.p, st));
+        PCHK(ns_gram(kPolF64, m, n, X, bM.p, nullptr, st));
         PCHK(ns_update(kPolF64, m, n, X, bM.p, Xn, S + 1, bad, st));
         PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
         PCHK(hipStreamSynchronize(st));

@@ -37,6 +37,18 @@ struct Acc4<float> {
     typedef f4 type;
 };
 
+// Row of accumulator register r in lane `lane` within a 16 x 16 MFMA tile
+// (column = lane % 16).  Measured on gfx950: the FP64 16x16x4 instruction
+// interleaves rows across lane groups (row = 4 r + lane / 16), the FP32 and
+// BF16 16x16 instructions keep 4 consecutive rows per lane group
+// (row = 4 (lane / 16) + r).
+template <class T>
+__device__ __forceinline__ int acc_row(int lane, int r);
+template <>
+__device__ __forceinline__ int acc_row<double>(int lane, int r) { return 4 * r + (lane >> 4); }
+template <>
+__device__ __forceinline__ int acc_row<float>(int lane, int r) { return 4 * (lane >> 4) + r; }
+
 __device__ __forceinline__ d4 mfma4(double a, double b, d4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
 __device__ __forceinline__ f4 mfma4(float a, float b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
 
@@ -61,6 +73,10 @@ __device__ __forceinline__ void epi_apply(const EpiArgs& e, int N, int i, int j,
         const T v = (T)((i == j ? 3.0 : 0.0) - (double)acc);
         c[(size_t)i * N + j] = v;
         if (mirror) c[(size_t)j * N + i] = v;
+        if (e.sum) {  // ||X^T X - I||_F^2 as a by-product
+            const double d = (double)acc - (i == j ? 1.0 : 0.0);
+            part += (mirror ? 2.0 : 1.0) * d * d;
+        }
     } else if (EPI == kEpiHalfDelta) {
         const T v = (T)(0.5 * (double)acc);
         const T* xo = reinterpret_cast<const T*>(e.xold);
@@ -76,7 +92,8 @@ __device__ __forceinline__ void epi_apply(const EpiArgs& e, int N, int i, int j,
 
 template <int EPI>
 __device__ __forceinline__ void epi_finish(const EpiArgs& e, double part, int bad) {
-    if (EPI != kEpiHalfDelta && EPI != kEpiOrth) return;
+    if (EPI != kEpiHalfDelta && EPI != kEpiOrth && EPI != kEpi3IMinus) return;
+    if (EPI == kEpi3IMinus && !e.sum) return;  // uniform: kernel argument
     __shared__ double red[4];
     part = block_sum<256>(part, red);
     if (EPI == kEpiHalfDelta) bad = block_or<256>(bad, red);
@@ -160,7 +177,7 @@ __global__ __launch_bounds__(256) void gemm_mfma4_kernel(int M, int N, int K, co
         for (int b = 0; b < 2; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wm * 32 + a * 16 + 4 * (lane >> 4) + r;
+                const int i = i0 + wm * 32 + a * 16 + acc_row<T>(lane, r);
                 const int j = j0 + wn * 32 + b * 16 + (lane & 15);
                 if (i < M && j < N && (!SYM || ti < tj || i <= j))
                     epi_apply<T, EPI>(e, N, i, j, acc[a][b][r], part, bad, SYM && i != j);
@@ -230,7 +247,7 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wm * 64 + a * 16 + 4 * (lane >> 4) + r;
+                const int i = i0 + wm * 64 + a * 16 + acc_row<float>(lane, r);
                 const int j = j0 + wn * 64 + b * 16 + (lane & 15);
                 if (i < M && j < N && (!SYM || ti < tj || i <= j))
                     epi_apply<float, EPI>(e, N, i, j, acc[a][b][r], part, bad, SYM && i != j);
@@ -292,9 +309,9 @@ static hipError_t launch_bf16(int M, int N, int K, const float* A, const float* 
     return hipGetLastError();
 }
 
-// M = 3I - X^T X (n x n) from X (m x n)
-hipError_t ns_gram(int prec, int m, int n, const void* X, void* Mout, hipStream_t st) {
-    EpiArgs e{Mout, nullptr, nullptr, nullptr, 1.0};
+// M = 3I - X^T X (n x n) from X (m x n); *orth_sq += ||X^T X - I||_F^2 when non-null
+hipError_t ns_gram(int prec, int m, int n, const void* X, void* Mout, double* orth_sq, hipStream_t st) {
+    EpiArgs e{Mout, nullptr, orth_sq, nullptr, 1.0};
     if (prec == kPolF64) return launch4<double, true, true, kEpi3IMinus>(n, n, m, (const double*)X, (const double*)X, e, st);
     if (prec == kPolF32) return launch4<float, true, true, kEpi3IMinus>(n, n, m, (const float*)X, (const float*)X, e, st);
     return launch_bf16<true, true, kEpi3IMinus>(n, n, m, (const float*)X, (const float*)X, e, st);

@@ -48,6 +48,55 @@ def algorithmic_flops(m: int, n: int, k_ns: int, sweeps: int) -> float:
     return 4.0 * M * N * N * (M > N) + (4.0 * k_ns + 4.0 + 6.0 * sweeps) * N ** 3
 
 
+def polar_leg(h, dev, args):
+    """Large NS polar: 4096 x 4096 and 8192 x 2048, A = U diag(s) V^T with Haar
+    factors and s log-uniform in [1e-6, 1].  Plain FP64 vs BF16 MFMA
+    iterations + FP64 refinement; polar TFLOP/s = iterations x 4 m n^2 / time."""
+    import torch
+
+    from lie_cliffalg_analog_svd_amd import LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, polar, workloads
+    res = {}
+    for (m, n) in [(4096, 4096), (8192, 2048)]:
+        t0 = time.time()
+        a, s, u, v = workloads.polar_matrix(m, n)
+        exact = u @ v.T
+        log(f"polar {m}x{n}: input generated in {time.time() - t0:.1f}s")
+        ad = torch.from_numpy(a).to(dev)
+        entry = {}
+        q64 = None
+        for name, prec in [("f64", LSVD_PREC_F64), ("bf16", LSVD_PREC_BF16), ("f32", LSVD_PREC_F32)]:
+            qd = torch.empty((m, n), dtype=torch.float64, device=dev)
+            info = polar.polar_device(h, ad, m, n, qd, prec)  # warm-up + counts
+            torch.cuda.synchronize()
+            times = []
+            for _ in range(args.polar_reps):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                info = polar.polar_device(h, ad, m, n, qd, prec)
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - t1)
+            sec = min(times)
+            iters = info["lowp_iters"] + info["f64_iters"]
+            flops = iters * 4.0 * m * n * n
+            q = qd.cpu().numpy()
+            if name == "f64":
+                q64 = q
+            entry[name] = {
+                "seconds": round(sec, 4), "tflops": round(flops / sec / 1e12, 3),
+                "lowp_iters": info["lowp_iters"], "f64_iters": info["f64_iters"], "polar_ok": info["polar_ok"],
+                "orth_err_fro": info["orth_err"],
+                "err_vs_f64_fro_over_sqrt_n": float(np.linalg.norm(q - q64) / np.sqrt(n)),
+                "err_vs_exact_fro_over_sqrt_n": float(np.linalg.norm(q - exact) / np.sqrt(n)),
+            }
+            log(f"polar {m}x{n} {name}: {entry[name]}")
+            del qd
+        entry["speedup_bf16_vs_f64"] = round(entry["f64"]["seconds"] / entry["bf16"]["seconds"], 3)
+        entry["speedup_f32_vs_f64"] = round(entry["f64"]["seconds"] / entry["f32"]["seconds"], 3)
+        res[f"{m}x{n}"] = entry
+        del ad
+    return res
+
+
 def main(argv=None) -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +107,8 @@ def main(argv=None) -> int:
     ap.add_argument("--tiny", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
+    ap.add_argument("--no-polar", action="store_true", help="skip the large polar leg")
+    ap.add_argument("--polar-reps", type=int, default=2)
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -200,6 +251,8 @@ def main(argv=None) -> int:
                           f"{d['threads']} threads x {d['grid']} workgroups)",
                 "launch_ms": round(bucket_ms[dom], 4)}
 
+    polar_res = None if args.no_polar else polar_leg(h, dev, args)
+
     out = {
         "metric": "batched SVD matrices/s + polar TFLOP/s, chosen vs plain FP64, at stated sigma err",
         "value": round(value, 1),
@@ -224,6 +277,7 @@ def main(argv=None) -> int:
         "phase_split": phase_split,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "polar": polar_res,
     }
     print(json.dumps(out), flush=True)
     plan.close()

@@ -160,6 +160,26 @@ int lsvd_gemm_f64(lsvd_handle_t h, int trans_a, int m, int n, int k, const doubl
 int lsvd_gemm_lowp(lsvd_handle_t h, int precision, int trans_a, int m, int n, int k, const float* a,
                    const float* b, float* c);
 
+/* ---- Adjoints (no reference counterpart: the reference has no
+ *      derivatives) -------------------------------------------------------- */
+
+/* Reverse-mode adjoint of the thin SVD (U, s, Vt) = solve_rectangular(A)
+ * for `count` matrices of one shape m x n, m >= n (packed back to back).
+ * u/s/vt: the forward outputs, or all three NULL to recompute the forward
+ * pass (no storage of factors needed).  gu (m x n), gs (n), gvt (n x n):
+ * cotangents of U, s and Vt; each may be NULL (zero).  grad (m x n) gets
+ * dL/dA.  The 1/(s_i - s_j) factor of the singular-vector terms is guarded
+ * as d/(d^2 + eps^2), eps = guard * s_max (guard 0 disables it; 1/(s_i +
+ * s_j) and 1/s are treated the same way). */
+int lsvd_svd_vjp_f64(lsvd_handle_t h, int count, int m, int n, const double* a, const double* u,
+                     const double* s, const double* vt, const double* gu, const double* gs, const double* gvt,
+                     double guard, double* grad);
+
+/* Adjoint of the polar factor Q = U Vt of each m x n (m >= n) matrix:
+ * qbar (m x n) is dL/dQ; grad gets dL/dA.  The forward SVD is recomputed. */
+int lsvd_polar_vjp_f64(lsvd_handle_t h, int count, int m, int n, const double* a, const double* qbar,
+                       double guard, double* grad);
+
 /* ---- LieSvdJoint (Phase-JADE) ----------------------------------------- */
 
 /* JointDiagonalizationParams::for_n (src/lie_svd_joint.rs:26-35). */

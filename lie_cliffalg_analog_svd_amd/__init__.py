@@ -13,6 +13,7 @@ from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall
                       svd_batched)
 from . import metrics, profiles
 from .polar import newton_schulz_polar
+from .adjoint import polar_vjp, svd_loss_grad, svd_vjp
 from .joint import (JointDiagonalizationParams, JointDiagonalizationTrace, LieSvdJoint,
                     diagonalize_families)
 

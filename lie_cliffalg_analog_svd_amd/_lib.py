@@ -138,6 +138,9 @@ SIGNATURES = {
     "lsvd_polar_f64": (C.c_int, [_vp, _vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdPolarInfo)]),
     "lsvd_gemm_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
     "lsvd_gemm_lowp": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp]),
+    "lsvd_svd_vjp_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp, _vp, C.c_double,
+                                   _vp]),
+    "lsvd_polar_vjp_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, _vp, C.c_double, _vp]),
     "lsvd_joint_params_for_n": (C.c_int, [C.c_int, C.POINTER(LsvdJointParams)]),
     "lsvd_joint_diag_sym_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdJointParams), _vp,
                                           _vp, _vp]),

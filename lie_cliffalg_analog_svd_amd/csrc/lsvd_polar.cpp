@@ -88,6 +88,9 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
         void* Xn = bXf2.p;
         // The low-precision phase runs until ||X^T X - I||_F (monotone in
         // exact arithmetic) reaches its rounding floor or stops improving.
+        // Every direction still escaping from a small singular value adds
+        // about 1 to it, so the stall test only applies below 0.5, where
+        // all directions are in the quadratic regime.
         const double floor_tol = (lp == kPolBF16 ? 1e-3 : 1e-6) * rootn;
         double best = 1e300;
         int stall = 0;
@@ -111,7 +114,7 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
             if (orth < 0.99 * best) {
                 best = orth;
                 stall = 0;
-            } else if (++stall >= 3) {
+            } else if (orth < 0.5 && ++stall >= 3) {
                 break;
             }
         }

@@ -100,6 +100,8 @@ def lib():
         L.lso_joint_batch.argtypes = [C.c_int, C.c_int, C.c_int, dp, JointParams, dp, dp,
                                       C.POINTER(JointTrace), C.c_int]
         L.lso_ns_iterations.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
+        L.lso_svd_loss_grad.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp]
+        L.lso_svd_loss_grad.restype = C.c_double
         _lib = L
     return _lib
 
@@ -279,3 +281,13 @@ def ns_iterations(x, iters: int, threads: int = 0):
     m, n = x.shape
     lib().lso_ns_iterations(_dp(x), m, n, iters, threads)
     return x
+
+
+def svd_loss_grad(a, w, g):
+    """FP64 analytic gradient of L = sum_i w_i s_i + <G, U Vt> (no reference
+    counterpart; closed form in lsvd_oracle.c).  Returns (loss, dL/dA)."""
+    a, w, g = _c(a), _c(w), _c(g)
+    m, n = a.shape
+    grad = np.zeros((m, n))
+    loss = lib().lso_svd_loss_grad(_dp(a), m, n, _dp(w), _dp(g), _dp(grad))
+    return loss, grad

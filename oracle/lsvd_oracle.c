@@ -861,3 +861,42 @@ void lso_ns_iterations(double* x, int rows, int cols, int iters, int threads) {
     free(g);
     free(xn);
 }
+
+/* ------------------------------------------------------------------ */
+/* FP64 analytic gradient of L = sum_i w_i s_i + <G, U Vt> (test oracle)  */
+/* ------------------------------------------------------------------ */
+
+/* No reference counterpart (the reference has no derivatives).  For the
+ * thin SVD A = U diag(s) Vt (m >= n) from LieSvdSmall::solve_rectangular:
+ *   dL/dA = U (diag(w) + K) Vt + (I - U U^T) G V diag(1/s) Vt,
+ *   X = U^T G V,  K_ij = (X_ij - X_ji) / (s_i + s_j).
+ * Returns the loss; grad is m x n. */
+double lso_svd_loss_grad(const double* a, int m, int n, const double* w, const double* g, double* grad) {
+    double* u = dalloc((size_t)m * n);
+    double* s = dalloc(n);
+    double* vt = dalloc((size_t)n * n);
+    lso_small_solve_rectangular(a, m, n, u, s, vt, NULL);
+    double* q = dalloc((size_t)m * n); /* U Vt */
+    gemm(0, 0, m, n, n, u, vt, q);
+    double loss = 0.0;
+    for (int i = 0; i < n; ++i) loss += w[i] * s[i];
+    for (size_t e = 0; e < (size_t)m * n; ++e) loss += g[e] * q[e];
+    double* gv = dalloc((size_t)m * n); /* G V = G Vt^T */
+    gemm(0, 1, m, n, n, g, vt, gv);
+    double* x = dalloc((size_t)n * n); /* U^T G V */
+    gemm(1, 0, n, n, m, u, gv, x);
+    double* mid = dalloc((size_t)n * n); /* diag(w) + K */
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j)
+            mid[(size_t)i * n + j] = i == j ? w[i] : (x[(size_t)i * n + j] - x[(size_t)j * n + i]) / (s[i] + s[j]);
+    /* T = U mid + (G V - U X) diag(1/s) */
+    double* t = dalloc((size_t)m * n);
+    double* ux = dalloc((size_t)m * n);
+    gemm(0, 0, m, n, n, u, mid, t);
+    gemm(0, 0, m, n, n, u, x, ux);
+    for (int i = 0; i < m; ++i)
+        for (int j = 0; j < n; ++j) t[(size_t)i * n + j] += (gv[(size_t)i * n + j] - ux[(size_t)i * n + j]) / s[j];
+    gemm(0, 0, m, n, n, t, vt, grad);
+    free(u); free(s); free(vt); free(q); free(gv); free(x); free(mid); free(t); free(ux);
+    return loss;
+}

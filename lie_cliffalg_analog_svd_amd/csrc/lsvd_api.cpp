@@ -362,6 +362,34 @@ static int plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, dou
     return LSVD_OK;
 }
 
+namespace lsvd {
+int svd_device_uniform(lsvd_handle_t h, int count, int m, int n, const double* a, double* u, double* s, double* vt) {
+    std::vector<int32_t> mm(count, m), nn(count, n), modes(count, kModeSvd);
+    std::vector<int64_t> ao(count), uo(count), so(count), vo(count);
+    const int k = std::min(m, n);
+    for (int b = 0; b < count; ++b) {
+        ao[b] = (int64_t)b * m * n;
+        uo[b] = (int64_t)b * m * k;
+        so[b] = (int64_t)b * k;
+        vo[b] = (int64_t)b * k * n;
+    }
+    const int saved = h->ptr_mode;
+    h->ptr_mode = LSVD_PTR_DEVICE;
+    lsvd_plan_t p = nullptr;
+    int rc = plan_build(h, count, mm.data(), nn.data(), ao.data(), uo.data(), so.data(), vo.data(), modes.data(),
+                        LSVD_PREC_F64, &p);
+    h->ptr_mode = saved;
+    if (rc != LSVD_OK) return rc;
+    rc = plan_execute(p, a, u, s, vt, nullptr, false);
+    if (rc == LSVD_OK) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = hip_fail(h, e, "hipStreamSynchronize");
+    }
+    lsvd_batch_plan_destroy(p);
+    return rc;
+}
+}  // namespace lsvd
+
 // ---- C ABI ------------------------------------------------------------------
 
 extern "C" {
@@ -673,33 +701,6 @@ int lsvd_batch_plan_phase_clocks(lsvd_plan_t p, const double* a_pool, double* u_
     return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "phase clocks run");
 }
 
-namespace lsvd {
-int svd_device_uniform(lsvd_handle_t h, int count, int m, int n, const double* a, double* u, double* s, double* vt) {
-    std::vector<int32_t> mm(count, m), nn(count, n), modes(count, kModeSvd);
-    std::vector<int64_t> ao(count), uo(count), so(count), vo(count);
-    const int k = std::min(m, n);
-    for (int b = 0; b < count; ++b) {
-        ao[b] = (int64_t)b * m * n;
-        uo[b] = (int64_t)b * m * k;
-        so[b] = (int64_t)b * k;
-        vo[b] = (int64_t)b * k * n;
-    }
-    const int saved = h->ptr_mode;
-    h->ptr_mode = LSVD_PTR_DEVICE;
-    lsvd_plan_t p = nullptr;
-    int rc = plan_build(h, count, mm.data(), nn.data(), ao.data(), uo.data(), so.data(), vo.data(), modes.data(),
-                        LSVD_PREC_F64, &p);
-    h->ptr_mode = saved;
-    if (rc != LSVD_OK) return rc;
-    rc = plan_execute(p, a, u, s, vt, nullptr, false);
-    if (rc == LSVD_OK) {
-        hipError_t e = hipStreamSynchronize(h->stream);
-        if (e != hipSuccess) rc = hip_fail(h, e, "hipStreamSynchronize");
-    }
-    lsvd_batch_plan_destroy(p);
-    return rc;
-}
-}  // namespace lsvd
 
 // Single-matrix entry points: a cached one-item plan per (mode, m, n).
 static int single(lsvd_handle_t h, int mode, const double* a, int m, int n, double* u, double* s, double* vt,

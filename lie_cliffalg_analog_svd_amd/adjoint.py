@@ -28,19 +28,24 @@ def svd_vjp(a, gu=None, gs=None, gvt=None, forward=None, guard: float = 1e-12, h
     """a: (B, m, n); gu (B, m, n), gs (B, n), gvt (B, n, n) may be None;
     forward: optional (U, s, Vt) batches.  Returns dL/dA (B, m, n)."""
     a = np.ascontiguousarray(np.asarray(a, dtype=np.float64))
-    if a.ndim == 2:
+    single = a.ndim == 2
+    if single:
         a = a[None]
     B, m, n = a.shape
     if m < n:
         raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "svd_vjp: needs m >= n")
-    gu = _batch(gu, (m, n), "gu")
-    gs = _batch(gs, (n,), "gs")
-    gvt = _batch(gvt, (n, n), "gvt")
+
+    def lift(x, nd):  # accept unbatched arrays for a single matrix
+        return None if x is None else (np.asarray(x)[None] if np.ndim(x) == nd else x)
+
+    gu = _batch(lift(gu, 2), (m, n), "gu")
+    gs = _batch(lift(gs, 1), (n,), "gs")
+    gvt = _batch(lift(gvt, 2), (n, n), "gvt")
     u = s = vt = None
     if forward is not None:
-        u = _batch(forward[0], (m, n), "U")
-        s = _batch(forward[1], (n,), "s")
-        vt = _batch(forward[2], (n, n), "Vt")
+        u = _batch(lift(forward[0], 2), (m, n), "U")
+        s = _batch(lift(forward[1], 1), (n,), "s")
+        vt = _batch(lift(forward[2], 2), (n, n), "Vt")
     h = handle or default_handle()
     grad = np.zeros((B, m, n))
     st = _lib.lib().lsvd_svd_vjp_f64(h.raw, B, m, n, _ptr(a), _ptr(u), _ptr(s), _ptr(vt), _ptr(gu), _ptr(gs),

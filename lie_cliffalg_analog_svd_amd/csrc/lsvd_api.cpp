@@ -85,6 +85,16 @@ static double item_cost(int M, int N) {
 // Sets the launch shape of a bucket for `variant`; returns the global arena
 // doubles it needs (0 for LDS variants).
 static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
+    if (variant >= kV2Base) {  // on-chip kernel: LDS sized by the bucket's largest M
+        const int np = variant - kV2Base;
+        bk.variant = variant;
+        const size_t lds = v2_lds_bytes(np, bk.maxM);
+        const int per_cu = small2_blocks_per_cu(np, lds);
+        bk.grid = std::max(1, std::min(bk.count, per_cu * h->num_cus));
+        bk.lds_bytes = lds;
+        bk.gws_stride = (int64_t)((3 * (size_t)np * np + 31) / 32 * 32);
+        return (size_t)bk.gws_stride * bk.grid;
+    }
     const SmallVariant* v = find_variant(variant);
     bk.variant = variant;
     const size_t lds = v->lds ? bk.arena_bytes : 0;
@@ -159,14 +169,19 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
             continue;
         }
         const int M = std::max(mb, nb), N = std::min(mb, nb);
-        const int cls = n_class(N);
-        if (cls < 0) {
-            delete p;
-            return set_error(h, LSVD_E_LIMIT, "matrix " + std::to_string(b) + ": min(m, n) > 1024");
+        int key;
+        if (N <= 128 && M <= 128) {
+            key = 1000 + (N + 15) / 16 * 16;  // on-chip kernel, padded size class
+        } else {
+            const int cls = n_class(N);
+            if (cls < 0) {
+                delete p;
+                return set_error(h, LSVD_E_LIMIT, "matrix " + std::to_string(b) + ": min(m, n) > 1024");
+            }
+            // split classes further by whether the arena is large (tall M)
+            const size_t arena = small_arena_doubles(M, N) * 8;
+            key = cls * 2 + (arena > lds_budget_bytes() ? 1 : 0);
         }
-        // split classes further by whether the arena is large (tall M)
-        const size_t arena = small_arena_doubles(M, N) * 8;
-        const int key = cls * 2 + (arena > lds_budget_bytes() ? 1 : 0);
         by_class[key].push_back({it, M, N, item_cost(M, N)});
     }
     p->a_len = a_end;
@@ -184,18 +199,25 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         Bucket bk{};
         bk.first = (int)all.size();
         bk.count = (int)v.size();
-        int maxN = 0;
+        int maxN = 0, maxM = 0;
         size_t arena = 0;
         for (auto& t : v) {
             all.push_back(t.it);
             maxN = std::max(maxN, t.N);
+            maxM = std::max(maxM, t.M);
             arena = std::max(arena, small_arena_doubles(t.M, t.N) * 8);
         }
         bk.maxN = maxN;
+        bk.maxM = maxM;
         bk.arena_bytes = arena;
-        int vid = default_variant(maxN, arena);
-        auto tk = h->tune.find(tune_key(maxN, arena, precision));
-        if (tk != h->tune.end() && variant_accepts(tk->second, maxN, arena)) vid = tk->second;
+        int vid;
+        if (kv.first >= 1000) {
+            vid = kV2Base + (kv.first - 1000);
+        } else {
+            vid = default_variant(maxN, arena);
+            auto tk = h->tune.find(tune_key(maxN, arena, precision));
+            if (tk != h->tune.end() && variant_accepts(tk->second, maxN, arena)) vid = tk->second;
+        }
         if (vid < 0) {
             delete p;
             return set_error(h, LSVD_E_LIMIT, "no kernel variant for N=" + std::to_string(maxN));
@@ -575,6 +597,7 @@ int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int3
     const SmallVariant* vs = small_variants(&nv);
     for (size_t bi = 0; bi < p->buckets.size(); ++bi) {
         Bucket base = p->buckets[bi];
+        if (base.variant >= kV2Base) continue;  // one on-chip variant per size class
         int best = base.variant;
         float best_ms = 1e30f;
         for (int vi = 0; vi < nv; ++vi) {
@@ -623,8 +646,9 @@ int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, in
     std::vector<lsvd_bucket_desc_t> d;
     if (p->n_micro > 0) d.push_back({0, p->n_micro, 4, -1, (p->n_micro + 255) / 256, 256, 0});
     for (auto& bk : p->buckets) {
-        const SmallVariant* v = find_variant(bk.variant);
-        d.push_back({bk.first, bk.count, bk.maxN, bk.variant, bk.grid, v ? v->nt : 0, (int64_t)bk.lds_bytes});
+        const SmallVariant* v = bk.variant >= kV2Base ? nullptr : find_variant(bk.variant);
+        const int nt = bk.variant >= kV2Base ? v2_threads(bk.variant - kV2Base) : (v ? v->nt : 0);
+        d.push_back({bk.first, bk.count, bk.maxN, bk.variant, bk.grid, nt, (int64_t)bk.lds_bytes});
     }
     *nbuckets = (int)d.size();
     for (int i = 0; i < (int)d.size() && i < max && out; ++i) out[i] = d[i];

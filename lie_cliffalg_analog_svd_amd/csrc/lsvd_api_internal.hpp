@@ -15,6 +15,7 @@ struct Bucket {
     int first;           // index of the first item in the device item array
     int count;
     int maxN;
+    int maxM;
     size_t arena_bytes;  // largest per-matrix arena in the bucket
     int variant;         // small_svd variant id
     int grid;

@@ -38,6 +38,14 @@ hipError_t launch_small_variant(int id, const SmallLaunch& L, hipStream_t st);
 int small_variant_blocks_per_cu(int id, size_t lds_bytes);
 hipError_t launch_reset_counters(int* counters, int n, hipStream_t st);
 
+// ---- on-chip batched LieSvdSmall for N, M <= 128 (small_svd2.hip) -------
+// np: padded size 16, 32, ..., 128 (N <= np).  Variant id = 100 + np.
+int v2_threads(int np);
+size_t v2_lds_bytes(int np, int max_m);
+hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st);
+int small2_blocks_per_cu(int np, size_t lds_bytes);
+constexpr int kV2Base = 100;
+
 // ---- LieSvdMicro (micro_svd.hip) ----------------------------------------
 
 struct MicroLaunch {

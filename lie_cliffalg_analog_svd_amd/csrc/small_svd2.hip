@@ -1,0 +1,607 @@
+// small_svd2.hip -- on-chip batched LieSvdSmall for N <= 128, M <= 128.
+//
+// Same algorithm as small_svd.hip (src/lie_svd_small.rs:296-360), with
+// the working set kept on the CU:
+//   - matrices are zero-padded to NP = 16 * ceil(N / 16) (the padding stays
+//     zero under every step below, and padded Jacobi pairs have a_pq = 0, so
+//     the reference's |a_pq| < 1e-300 skip leaves them alone);
+//   - Newton-Schulz, P = Q^T R and U_r = Q V run on v_mfma_f64_16x16x4f64 in
+//     16-column panels: X lives in LDS, each finished tile of the new iterate
+//     / product goes to the staging slot and is copied back after a barrier;
+//   - P lives in LDS for the Jacobi sweeps; V lives in registers in the
+//     "position space" of the round-robin ring: every layer's pairs sit at
+//     fixed register slots (2k, 2k+1) and moving to the next layer is a
+//     fixed cyclic shift of the slots (register moves plus one exchange with
+//     each neighbouring lane), so V needs neither LDS traffic nor dynamic
+//     register indexing.  A sweep is NP - 1 layers, after which the ring is
+//     back in its starting arrangement;
+//   - Q is parked in a per-workgroup global slot during the sweeps, and the
+//     tiles of each product are staged in a second slot (L2 resident), so
+//     only one or two MFMA accumulators are live at a time.
+// Wide inputs are solved transposed (as in small_svd.hip).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "lsvd_device.hpp"
+#include "lsvd_kernels.hpp"
+#include "lsvd_small_common.hpp"
+
+namespace lsvd {
+
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <int NP>
+struct Cfg2 {
+    static constexpr int NT = NP <= 16 ? 64 : (NP <= 32 ? 128 : (NP <= 64 ? 256 : 512));
+    static constexpr int NW = NT / 64;
+    static constexpr int TPR = 4;                 // lanes per row of V
+    static constexpr int CH = NP / TPR;           // ring positions per lane
+    static constexpr int NTILE = NP / 16;
+    static constexpr int RT = (NTILE + NW - 1) / NW;  // tiles per wave per panel
+    static constexpr int LDM = NP + 1;
+    static constexpr int SLOTS = NP / 2;
+    static constexpr int LDP = 17;  // panel leading dimension (16 columns)
+    static constexpr int PANEL = (NP * LDP > 16 * LDM) ? NP * LDP : 16 * LDM;
+};
+
+template <int NP>
+struct Sh2 {
+    double red[8];
+    double diag[NP];
+    double tau[NP];
+    double scr[NP];
+    double rc[NP / 2], rs[NP / 2], rsv[NP / 2], rt[NP / 2];
+    int rp[NP / 2], rq[NP / 2], rf[NP / 2];
+    int rank[NP];
+    int item;
+};
+
+__device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// One 16 x 16 tile: acc += A (16 x K) B (K x 16); fa(i, k), fb(k, j).
+// Operand lanes: A[i][k] at lane (i + 16 k'), B[k][j] at lane (j + 16 k').
+template <int K, class FA, class FB>
+__device__ __forceinline__ d4v tile_mm(const FA& fa, const FB& fb, d4v acc) {
+    const int lane = threadIdx.x & 63;
+    const int i = lane & 15, kk = lane >> 4;
+#pragma unroll 4
+    for (int k0 = 0; k0 < K; k0 += 4) acc = mfma64(fa(i, k0 + kk), fb(k0 + kk, i), acc);
+    return acc;
+}
+
+// Accumulator element r of a lane sits at row 4 r + lane / 16, column lane % 16
+// (measured layout of the FP64 16x16x4 instruction).
+__device__ __forceinline__ int trow(int r) { return 4 * r + ((threadIdx.x & 63) >> 4); }
+__device__ __forceinline__ int tcol() { return threadIdx.x & 15; }
+
+// Ring slot -> indices at layer L (circle method over m = NP positions).
+__device__ __forceinline__ void ring_pair(int m, int L, int k, int& a, int& b) {
+    const int ring = m - 1;
+    if (k == 0) {
+        a = m - 1;
+        b = L % ring;
+    } else {
+        a = (L + k) % ring;
+        b = (L + ring - k) % ring;
+    }
+}
+
+// Index held by ring position g in the layer-0 arrangement.
+__device__ __forceinline__ int ring_idx0(int m, int g) {
+    const int k = g >> 1;
+    if ((g & 1) == 0) return k == 0 ? m - 1 : k;
+    return k == 0 ? 0 : m - 1 - k;
+}
+
+template <int NP>
+__device__ void solve_item2(const Item& itm, const Pools& P, double* main, double* panel, double* xs, double* vs,
+                            double* ys, Sh2<NP>& sh) {
+    using C = Cfg2<NP>;
+    constexpr int NT = C::NT, NW = C::NW, LDM = C::LDM, NTILE = C::NTILE, RT = C::RT;
+    constexpr int TPR = C::TPR, CH = C::CH, SLOTS = C::SLOTS, LDP = C::LDP;
+    const int tid = threadIdx.x, w = tid >> 6;
+    const int m0 = itm.m, n0 = itm.n;
+    const bool wide = m0 < n0;
+    const int M = wide ? n0 : m0, N = wide ? m0 : n0;
+    const double* A = P.a + itm.a_off;
+    const View ain{A, n0, wide ? 1 : 0};
+    double* qs = wide ? P.vt + itm.vt_off : P.u + itm.u_off;
+    const bool tall = M > N;
+    const bool eigh = itm.mode == kModeEigh;
+    const double rootn = sqrt((double)N);
+    lsvd_info_t info = {};
+    const d4v zero4 = {0.0, 0.0, 0.0, 0.0};
+
+    // ---- front end (tall/wide: Householder QR in LDS, compact form to qs)
+    if (tall && !eigh) {
+        for (int e = tid; e < M * N; e += NT) main[(size_t)(e / N) * LDM + e % N] = ain(e / N, e % N);
+        __syncthreads();
+        householder_qr_g<NT>(M, N, main, LDM, sh.red, sh.tau, sh.scr);
+        for (int e = tid; e < M * N; e += NT) qs[(size_t)(e / N) * N + e % N] = main[(size_t)(e / N) * LDM + e % N];
+        __syncthreads();
+    }
+    struct Asq {
+        UpperView r;
+        View a;
+        bool tall;
+        __device__ double operator()(int i, int j) const { return tall ? r(i, j) : a(i, j); }
+    } asv{UpperView{qs, N}, ain, tall};
+
+    int polar_ok = 0, iters = 0;
+    if (!eigh) {
+        // ---- Newton-Schulz polar (src/lie_svd_small.rs:158-183)
+        double sc = 0.0;
+        for (int e = tid; e < N * N; e += NT) {
+            const double x = asv(e / N, e % N);
+            sc += x * x;
+        }
+        sc = fmax(sqrt(block_sum<NT>(sc, sh.red)), 1e-300);
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int i = e / NP, j = e % NP;
+            main[i * LDM + j] = (i < N && j < N) ? asv(i, j) / sc : 0.0;
+        }
+        __syncthreads();
+        int diverged = 0;
+        for (int it = 0; it < kPolarMaxIters; ++it) {
+#pragma unroll 1
+            for (int J = 0; J < NTILE; ++J) {
+                // panel = (3I - X^T X)[:, J]
+#pragma unroll
+                for (int a = 0; a < RT; ++a) {
+                    const int I = w + a * NW;
+                    if (I < NTILE) {
+                        const d4v g = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                                  [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = I * 16 + trow(r), col = tcol();
+                            panel[row * LDP + col] = (row == J * 16 + col ? 3.0 : 0.0) - g[r];
+                        }
+                    }
+                }
+                __syncthreads();
+                // X_new[:, J] = 0.5 X panel -> staging slot
+#pragma unroll
+                for (int a = 0; a < RT; ++a) {
+                    const int I = w + a * NW;
+                    if (I < NTILE) {
+                        const d4v x = tile_mm<NP>([&](int i, int k) { return main[(I * 16 + i) * LDM + k]; },
+                                                  [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) ys[(I * 16 + trow(r)) * NP + J * 16 + tcol()] = 0.5 * x[r];
+                    }
+                }
+                __syncthreads();
+            }
+            double dsq = 0.0;
+            int bad = 0;
+            for (int e = tid; e < NP * NP; e += NT) {
+                const double xn = ys[e];
+                const double d = xn - main[(e / NP) * LDM + e % NP];
+                dsq += d * d;
+                bad |= !isfinite(xn);
+            }
+            dsq = block_sum<NT>(dsq, sh.red);
+            bad = block_or<NT>(bad, sh.red);
+            iters = it + 1;
+            const double delta = sqrt(dsq);
+            if (bad || delta > 1e6 * rootn) {  // (:169-171): Gram-Schmidt of A
+                gram_schmidt_g<NT>(N, asv, main, LDM, sh.red);
+                diverged = 1;
+                break;
+            }
+            for (int e = tid; e < NP * NP; e += NT) main[(e / NP) * LDM + e % NP] = ys[e];
+            __syncthreads();
+            if (delta < kPolarTol * rootn) break;
+        }
+        // orthogonality_err (:146-150) of the final X, N x N part
+        {
+            double s = 0.0;
+#pragma unroll
+            for (int a = 0; a < RT; ++a) {
+                const int I = w + a * NW;
+#pragma unroll
+                for (int J = 0; J < NTILE; ++J)
+                    if (I < NTILE) {
+                        const d4v g = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                                  [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+#pragma unroll
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = I * 16 + trow(r), col = J * 16 + tcol();
+                            if (row < N && col < N) {
+                                const double d = g[r] - (row == col ? 1.0 : 0.0);
+                                s += d * d;
+                            }
+                        }
+                    }
+            }
+            const double oe = sqrt(block_sum<NT>(s, sh.red));
+            // trust test (:174, :178); a diverged iteration is never trusted (:170)
+            polar_ok = diverged ? 0 : (oe <= 1e-8 * rootn);
+        }
+        info.ns_iters = iters;
+
+        // ---- P = sym(Q^T A) (:305-309); Q parked in the global slot xs
+#pragma unroll 1
+        for (int J = 0; J < NTILE; ++J) {
+            for (int e = tid; e < NP * 16; e += NT) {
+                const int k = e >> 4, j = e & 15;
+                panel[k * LDP + j] = (k < N && J * 16 + j < N) ? asv(k, J * 16 + j) : 0.0;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int a = 0; a < RT; ++a) {
+                const int I = w + a * NW;
+                if (I < NTILE) {
+                    const d4v pv = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                               [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) ys[(I * 16 + trow(r)) * NP + J * 16 + tcol()] = pv[r];
+                }
+            }
+            __syncthreads();
+        }
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int o = (e / NP) * LDM + e % NP;
+            xs[e] = main[o];
+            main[o] = ys[e];
+        }
+        __syncthreads();
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int i = e / NP, j = e % NP;
+            if (i < j) {
+                const double mm = (main[i * LDM + j] + main[j * LDM + i]) * 0.5;
+                main[i * LDM + j] = mm;
+                main[j * LDM + i] = mm;
+            }
+        }
+        __syncthreads();
+    } else {
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int i = e / NP, j = e % NP;
+            main[i * LDM + j] = (i < N && j < N) ? A[(size_t)i * N + j] : 0.0;
+        }
+        __syncthreads();
+    }
+    info.polar_ok = polar_ok;
+
+    // ---- Jacobi on P (:212-285): P in LDS, V in registers (ring positions)
+    const int vrow = tid / TPR, vch = tid % TPR;
+    double v[CH];
+#pragma unroll
+    for (int j = 0; j < CH; ++j) {
+        const int idx = ring_idx0(NP, vch * CH + j);
+        v[j] = (vrow < N && idx == vrow) ? 1.0 : 0.0;
+    }
+    double ref = 0.0;
+    for (int e = tid; e < NP * NP; e += NT) {
+        const double x = main[(e / NP) * LDM + e % NP];
+        ref += x * x;
+    }
+    ref = fmax(sqrt(block_sum<NT>(ref, sh.red)), 1e-300);
+    const int max_sweeps = 60 + 20 * N;
+    int sweeps = 0, rots = 0;
+    const int lane = tid & 63;
+    const int nbr_next = (lane + 1) & 63, nbr_prev = (lane + 63) & 63;
+    constexpr int nblk = SLOTS * (SLOTS + 1) / 2;
+    for (int sw = 0; sw < max_sweeps && N > 1; ++sw) {
+        double off = 0.0;
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int p = e / NP, q = e % NP;
+            if (q > p) off += main[p * LDM + q] * main[p * LDM + q];
+        }
+        off = block_sum<NT>(off, sh.red);
+        if (!(sqrt(off) >= kJacobiTol * ref)) break;
+        ++sweeps;
+        for (int L = 0; L < NP - 1; ++L) {
+            // phase A: rotation of every slot of layer L
+            for (int k = tid; k < SLOTS; k += NT) {
+                int a, b;
+                ring_pair(NP, L, k, a, b);
+                const int p = a < b ? a : b, q = a < b ? b : a;
+                double c = 1.0, s = 0.0, tt = 0.0;
+                int rot = 0;
+                const double apq = main[p * LDM + q];
+                if (fabs(apq) >= 1e-300) {
+                    const double app = main[p * LDM + p], aqq = main[q * LDM + q];
+                    const double th = (aqq - app) / (2.0 * apq);
+                    tt = th >= 0.0 ? 1.0 / (th + sqrt(th * th + 1.0)) : -1.0 / (-th + sqrt(th * th + 1.0));
+                    c = 1.0 / sqrt(tt * tt + 1.0);
+                    s = tt * c;
+                    rot = 1;
+                    ++rots;
+                }
+                sh.rp[k] = p;
+                sh.rq[k] = q;
+                sh.rf[k] = rot;
+                sh.rc[k] = c;
+                sh.rs[k] = s;
+                sh.rsv[k] = a < b ? s : -s;  // ring slot order may be (q, p)
+                sh.rt[k] = tt;
+            }
+            __syncthreads();
+            // phase B1: V rows -- rotate own slots, then shift the ring
+#pragma unroll
+            for (int j = 0; j < CH / 2; ++j) {
+                const int k = vch * (CH / 2) + j;
+                const double c = sh.rc[k], s = sh.rsv[k];
+                const double x = v[2 * j], y = v[2 * j + 1];
+                v[2 * j] = c * x - s * y;
+                v[2 * j + 1] = s * x + c * y;
+            }
+            {
+                // in-place ring shift (verified against the circle-method
+                // schedule): even slots move down by one pair, odd slots up
+                const double nfe = __shfl(v[0], nbr_next, 64);       // next lane's first even
+                const double plo = __shfl(v[CH - 1], nbr_prev, 64);  // previous lane's last odd
+                const double e2 = v[2], olast = v[CH - 1];
+#pragma unroll
+                for (int j = 0; j + 2 < CH; j += 2) v[j] = (vch == 0 && j == 0) ? v[0] : v[j + 2];
+                v[CH - 2] = (vch == TPR - 1) ? olast : nfe;
+#pragma unroll
+                for (int j = CH - 1; j >= 3; j -= 2) v[j] = v[j - 2];
+                v[1] = (vch == 0) ? e2 : plo;
+            }
+            // phase B2: P 2x2 blocks (pair a rows, pair b columns), a <= b
+            for (int bI = tid; bI < nblk; bI += NT) {
+                int kb = (int)((sqrt(8.0 * bI + 1.0) - 1.0) * 0.5);
+                while (kb * (kb + 1) / 2 > bI) --kb;
+                while ((kb + 1) * (kb + 2) / 2 <= bI) ++kb;
+                const int ka = bI - kb * (kb + 1) / 2;
+                const int pa = sh.rp[ka], qa = sh.rq[ka], pb = sh.rp[kb], qb = sh.rq[kb];
+                if (ka == kb) {
+                    if (sh.rf[ka]) {  // (:244-249)
+                        const double apq = main[pa * LDM + qa];
+                        const double tt = sh.rt[ka];
+                        main[pa * LDM + pa] -= tt * apq;
+                        main[qa * LDM + qa] += tt * apq;
+                        main[pa * LDM + qa] = 0.0;
+                        main[qa * LDM + pa] = 0.0;
+                    }
+                    continue;
+                }
+                if (!sh.rf[ka] && !sh.rf[kb]) continue;
+                const double ca = sh.rc[ka], sa = sh.rs[ka], cb = sh.rc[kb], sb = sh.rs[kb];
+                const double b00 = main[pa * LDM + pb], b01 = main[pa * LDM + qb];
+                const double b10 = main[qa * LDM + pb], b11 = main[qa * LDM + qb];
+                const double l00 = ca * b00 - sa * b10, l01 = ca * b01 - sa * b11;
+                const double l10 = sa * b00 + ca * b10, l11 = sa * b01 + ca * b11;
+                const double r00 = cb * l00 - sb * l01, r01 = sb * l00 + cb * l01;
+                const double r10 = cb * l10 - sb * l11, r11 = sb * l10 + cb * l11;
+                main[pa * LDM + pb] = r00;
+                main[pb * LDM + pa] = r00;
+                main[pa * LDM + qb] = r01;
+                main[qb * LDM + pa] = r01;
+                main[qa * LDM + pb] = r10;
+                main[pb * LDM + qa] = r10;
+                main[qa * LDM + qb] = r11;
+                main[qb * LDM + qa] = r11;
+            }
+            __syncthreads();
+        }
+    }
+    for (int i = tid; i < N; i += NT) sh.diag[i] = main[i * LDM + i];
+    __syncthreads();
+    for (int i = tid; i < N; i += NT) {  // rank sort, eigenvalues descending
+        const double di = isnan(sh.diag[i]) ? -INFINITY : sh.diag[i];
+        int r = 0;
+        for (int j = 0; j < N; ++j) {
+            const double dj = isnan(sh.diag[j]) ? -INFINITY : sh.diag[j];
+            r += (dj > di) || (dj == di && j < i);
+        }
+        sh.rank[i] = r;
+    }
+    // V from registers to LDS (index space), zero padding
+    for (int e = tid; e < NP * NP; e += NT) main[(e / NP) * LDM + e % NP] = 0.0;
+    __syncthreads();
+    if (vrow < N) {
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int idx = ring_idx0(NP, vch * CH + j);
+            if (idx < N) main[vrow * LDM + idx] = v[j];
+        }
+    }
+    __syncthreads();
+    rots = (int)block_sum<NT>((double)rots, sh.red);
+    info.jacobi_sweeps = sweeps;
+    info.rotations = rots;
+
+    if (eigh) {
+        double* vout = P.u + itm.u_off;
+        double* eout = P.s + itm.s_off;
+        for (int i = tid; i < N; i += NT) eout[sh.rank[i]] = sh.diag[i];
+        for (int e = tid; e < N * N; e += NT) vout[(size_t)(e / N) * N + sh.rank[e % N]] = main[(e / N) * LDM + e % N];
+        if (tid == 0 && P.info) P.info[itm.idx] = info;
+        __syncthreads();
+        return;
+    }
+
+    // ---- sigma, Vt (or U for wide), and a global copy of V for the repair
+    double* sdst = P.s + itm.s_off;
+    for (int i = tid; i < N; i += NT) sdst[sh.rank[i]] = fmax(sh.diag[i], 0.0);
+    double* vdst = wide ? P.u + itm.u_off : P.vt + itm.vt_off;
+    for (int e = tid; e < N * N; e += NT) {
+        const int r = e / N, i = e % N;
+        const double x = main[r * LDM + i];
+        if (!wide) vdst[(size_t)sh.rank[i] * N + r] = x;
+        else vdst[(size_t)r * N + sh.rank[i]] = x;
+        vs[(size_t)r * N + sh.rank[i]] = x;  // sorted V
+    }
+
+    // ---- U_r = Q V (:317), columns in sorted order, tiles staged in ys
+#pragma unroll 1
+    for (int I = 0; I < NTILE; ++I) {
+        for (int e = tid; e < 16 * NP; e += NT) {
+            const int i = e / NP, k = e % NP;
+            panel[i * LDM + k] = xs[(size_t)(I * 16 + i) * NP + k];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int a = 0; a < RT; ++a) {
+            const int J = w + a * NW;
+            if (J < NTILE) {
+                const d4v uv = tile_mm<NP>([&](int i, int k) { return panel[i * LDM + k]; },
+                                           [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ys[(I * 16 + trow(r)) * NP + J * 16 + tcol()] = uv[r];
+            }
+        }
+        __syncthreads();
+    }
+    const int rows_cap = M > NP ? M : NP;
+    for (int e = tid; e < rows_cap * NP; e += NT) main[(e / NP) * LDM + e % NP] = 0.0;
+    __syncthreads();
+    for (int e = tid; e < N * N; e += NT) {
+        const int row = e / N, col = e % N;
+        main[row * LDM + sh.rank[col]] = ys[row * NP + col];
+    }
+    __syncthreads();
+    // orthogonality of U_r, repair if needed (:318-320)
+    double oe = 0.0;
+#pragma unroll
+    for (int a = 0; a < RT; ++a) {
+        const int I = w + a * NW;
+#pragma unroll
+        for (int J = 0; J < NTILE; ++J)
+            if (I < NTILE) {
+                const d4v g = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                          [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int row = I * 16 + trow(r), col = J * 16 + tcol();
+                    if (row < N && col < N) {
+                        const double d = g[r] - (row == col ? 1.0 : 0.0);
+                        oe += d * d;
+                    }
+                }
+            }
+    }
+    oe = sqrt(block_sum<NT>(oe, sh.red));
+    if (!polar_ok || oe > 1e-8 * rootn) {
+        // repair_left_basis_from_right (:185-208): seed = A V S^-1, MGS
+        double smax = 0.0;
+        for (int i = tid; i < N; i += NT) smax = fmax(smax, fmax(sh.diag[i], 0.0));
+        smax = block_max<NT>(smax, sh.red);
+        const double stol = 1e-12 * fmax(smax, 1.0);
+        for (int i = tid; i < N; i += NT) sh.scr[sh.rank[i]] = fmax(sh.diag[i], 0.0);
+        __syncthreads();
+        wg_gemm<NT>(N, N, N, asv, View{vs, N, 0}, [&](int i, int j, double acc) {
+            const double sj = sh.scr[j];
+            xs[(size_t)i * N + j] = sj > stol ? acc / sj : 0.0;
+        });
+        __syncthreads();
+        gram_schmidt_g<NT>(N, View{xs, N, 0}, main, LDM, sh.red);
+        info.repaired = 1;
+    }
+    __syncthreads();
+    if (tall) apply_q_g<NT>(M, N, qs, main, LDM, sh.tau, sh.scr);
+    if (!wide) {
+        double* u = P.u + itm.u_off;
+        for (int e = tid; e < M * N; e += NT) u[(size_t)(e / N) * N + e % N] = main[(e / N) * LDM + e % N];
+    } else {
+        double* vt = P.vt + itm.vt_off;  // (N x M) = U_t^T
+        for (int e = tid; e < M * N; e += NT) vt[(size_t)(e % N) * M + e / N] = main[(e / N) * LDM + e % N];
+    }
+    if (tid == 0 && P.info) {
+        info.path = itm.mode == kModeMicroFallback ? 2 : 0;
+        info.micro_fallback = itm.mode == kModeMicroFallback ? 1 : 0;
+        P.info[itm.idx] = info;
+    }
+    __syncthreads();
+}
+
+template <int NP>
+__global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __restrict__ items, const int* count_ptr,
+                                                                  int* counter, Pools pools, double* gws,
+                                                                  int64_t gws_stride) {
+    extern __shared__ double dyn_lds[];
+    __shared__ Sh2<NP> sh;
+    const int count = *count_ptr;
+    double* xs = gws + (size_t)blockIdx.x * gws_stride;
+    double* vs = xs + (size_t)NP * NP;
+    double* ys = vs + (size_t)NP * NP;
+    for (;;) {
+        if (threadIdx.x == 0) sh.item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = sh.item;
+        __syncthreads();
+        if (it >= count) break;
+        const Item itm = items[it];
+        const int M = itm.m > itm.n ? itm.m : itm.n;
+        const int rows = M > NP ? M : NP;
+        double* main = dyn_lds;
+        double* panel = dyn_lds + (size_t)rows * Cfg2<NP>::LDM;
+        solve_item2<NP>(itm, pools, main, panel, xs, vs, ys, sh);
+    }
+}
+
+template <int NP>
+static hipError_t launch2(const SmallLaunch& L, hipStream_t st) {
+    if (L.lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&small_svd2_kernel<NP>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((small_svd2_kernel<NP>), dim3(L.grid), dim3(Cfg2<NP>::NT), L.lds_bytes, st, L.items,
+                       L.count_ptr, L.counter, L.pools, L.gws, L.gws_stride);
+    return hipGetLastError();
+}
+
+template <int NP>
+static int blocks2(size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, small_svd2_kernel<NP>, Cfg2<NP>::NT, lds) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
+}
+
+#define LSVD_V2_SIZES(X) X(16) X(32) X(48) X(64) X(80) X(96) X(112) X(128)
+
+int v2_threads(int np) {
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return Cfg2<np_>::NT;
+        LSVD_V2_SIZES(X)
+#undef X
+    }
+    return 0;
+}
+
+size_t v2_lds_bytes(int np, int M) {
+    const int rows = M > np ? M : np;
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return sizeof(double) * ((size_t)rows * Cfg2<np_>::LDM + Cfg2<np_>::PANEL);
+        LSVD_V2_SIZES(X)
+#undef X
+    }
+    return 0;
+}
+
+hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st) {
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return launch2<np_>(L, st);
+        LSVD_V2_SIZES(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+int small2_blocks_per_cu(int np, size_t lds) {
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return blocks2<np_>(lds);
+        LSVD_V2_SIZES(X)
+#undef X
+    }
+    return 1;
+}
+
+}  // namespace lsvd

@@ -281,19 +281,53 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
 
 namespace lsvd {
 
-// Enqueues the whole plan on `st` with the given device pools.
+// Side streams and fork/join events, one per concurrent launch.  Created
+// outside any graph capture (plan_run_device calls this first).
+hipError_t plan_ensure_streams(lsvd_plan_t p) {
+    const int lanes = (int)p->buckets.size() + (p->n_micro > 0 ? 1 : 0);
+    for (int i = (int)p->side.size(); i < lanes; ++i) {
+        hipStream_t s2;
+        hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
+        if (e != hipSuccess) return e;
+        p->side.push_back(s2);
+    }
+    while ((int)p->ev.size() < lanes + 1) {
+        hipEvent_t ev;
+        hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (e != hipSuccess) return e;
+        p->ev.push_back(ev);
+    }
+    return hipSuccess;
+}
+
+// Enqueues the whole plan on `st` with the given device pools.  The micro
+// launch and every bucket run concurrently on side streams (fork/join with
+// events, which a hipGraph capture turns into parallel branches), largest
+// per-matrix work first, so the CUs a large-N bucket leaves idle take the
+// small-N buckets; the micro escalations drain after the join.
 hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
     hipError_t e = launch_reset_counters(p->d_counters, p->n_counters, st);
     if (e != hipSuccess) return e;
     const int nb = (int)p->buckets.size();
     int* fb_count = p->d_counters + nb + 1;
+    e = plan_ensure_streams(p);
+    if (e != hipSuccess) return e;
+    e = hipEventRecord(p->ev[0], st);  // fork point
+    if (e != hipSuccess) return e;
+    int lane = 0;
     if (p->n_micro > 0) {
+        hipStream_t s2 = p->side[lane++];
+        e = hipStreamWaitEvent(s2, p->ev[0], 0);
+        if (e != hipSuccess) return e;
         MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, fb_count};
-        e = launch_micro(ml, st);
+        e = launch_micro(ml, s2);
         if (e != hipSuccess) return e;
     }
-    for (int i = 0; i < nb; ++i) {
+    for (int i = nb - 1; i >= 0; --i) {  // buckets are in increasing size order
         const Bucket& bk = p->buckets[i];
+        hipStream_t s2 = p->side[lane++];
+        e = hipStreamWaitEvent(s2, p->ev[0], 0);
+        if (e != hipSuccess) return e;
         SmallLaunch L{};
         L.grid = bk.grid;
         L.lds_bytes = bk.lds_bytes;
@@ -303,7 +337,13 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
         L.pools = pools;
         L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
         L.gws_stride = bk.gws_stride;
-        e = launch_small_variant(bk.variant, L, st);
+        e = launch_small_variant(bk.variant, L, s2);
+        if (e != hipSuccess) return e;
+    }
+    for (int l = 0; l < lane; ++l) {  // join
+        e = hipEventRecord(p->ev[l + 1], p->side[l]);
+        if (e != hipSuccess) return e;
+        e = hipStreamWaitEvent(st, p->ev[l + 1], 0);
         if (e != hipSuccess) return e;
     }
     if (p->n_micro > 0) {
@@ -338,6 +378,8 @@ static int plan_run_device(lsvd_plan_t p, const Pools& pools, bool use_graph) {
         if (p->graph) hipGraphDestroy(p->graph);
         p->graph_exec = nullptr;
         p->graph = nullptr;
+        e = plan_ensure_streams(p);
+        if (e != hipSuccess) return hip_fail(h, e, "hipStreamCreate");
         e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (e != hipSuccess) return hip_fail(h, e, "hipStreamBeginCapture");
         hipError_t le = plan_enqueue(p, pools, st);
@@ -514,6 +556,8 @@ int lsvd_batch_plan_destroy(lsvd_plan_t p) {
     hipSetDevice(p->h->device);
     if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
     if (p->graph) hipGraphDestroy(p->graph);
+    for (hipStream_t s2 : p->side) hipStreamDestroy(s2);
+    for (hipEvent_t ev : p->ev) hipEventDestroy(ev);
     void* ptrs[] = {p->d_items, p->d_counters, p->d_consts, p->d_gws, p->d_a, p->d_u, p->d_s, p->d_vt, p->d_info};
     for (void* q : ptrs)
         if (q) hipFree(q);

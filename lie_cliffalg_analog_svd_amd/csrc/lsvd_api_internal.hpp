@@ -70,6 +70,8 @@ struct lsvd_plan_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     lsvd::Pools graph_pools = {};
+    std::vector<hipStream_t> side;     // side streams for concurrent buckets
+    std::vector<hipEvent_t> ev;        // fork/join events
 };
 
 namespace lsvd {

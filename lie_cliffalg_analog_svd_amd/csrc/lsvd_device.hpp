@@ -15,6 +15,17 @@ namespace lsvd {
 
 constexpr int kWave = 64;
 
+// Thread index through an opaque move.  The batched kernels loop over work
+// items inside one launch; reading the index this way makes every address
+// derived from it a per-item value, so the compiler recomputes it where it
+// is used instead of hoisting all phases' addresses out of the item loop
+// (which keeps them live across the whole kernel and forces spills).
+__device__ __forceinline__ int tix() {
+    int t;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(t) : "v"((int)threadIdx.x));
+    return t;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
@@ -38,8 +49,8 @@ __device__ __forceinline__ int wave_or(int v) {
 template <int NT>
 __device__ __forceinline__ double block_sum(double v, double* red) {
     v = wave_sum(v);
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) red[w] = v;
+    const int w = tix() / kWave;
+    if ((tix() & (kWave - 1)) == 0) red[w] = v;
     __syncthreads();
     double s = 0.0;
 #pragma unroll
@@ -51,8 +62,8 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* red) {
     v = wave_max(v);
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) red[w] = v;
+    const int w = tix() / kWave;
+    if ((tix() & (kWave - 1)) == 0) red[w] = v;
     __syncthreads();
     double s = red[0];
 #pragma unroll
@@ -65,8 +76,8 @@ template <int NT>
 __device__ __forceinline__ int block_or(int v, double* red) {
     v = wave_or(v);
     int* ri = reinterpret_cast<int*>(red);
-    const int w = threadIdx.x / kWave;
-    if ((threadIdx.x & (kWave - 1)) == 0) ri[w] = v;
+    const int w = tix() / kWave;
+    if ((tix() & (kWave - 1)) == 0) ri[w] = v;
     __syncthreads();
     int s = 0;
 #pragma unroll
@@ -103,7 +114,7 @@ template <int NT, int TM, int TN, class LA, class LB, class Epi>
 __device__ __forceinline__ void wg_gemm_t(int M, int N, int K, const LA& la, const LB& lb, Epi&& epi) {
     const int tm = (M + TM - 1) / TM;
     const int tn = (N + TN - 1) / TN;
-    for (int t = threadIdx.x; t < tm * tn; t += NT) {
+    for (int t = tix(); t < tm * tn; t += NT) {
         const int i0 = (t / tn) * TM;
         const int j0 = (t % tn) * TN;
         double acc[TM][TN];

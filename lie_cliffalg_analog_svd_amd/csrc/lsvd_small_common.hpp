@@ -20,27 +20,27 @@ namespace lsvd {
 template <int NT, class V>
 __device__ void gram_schmidt_g(int N, const V& a, double* out, int ld, double* red) {
     for (int j = 0; j < N; ++j) {
-        for (int i = threadIdx.x; i < N; i += NT) out[(size_t)i * ld + j] = a(i, j);
+        for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] = a(i, j);
         __syncthreads();
         for (int k = 0; k < j; ++k) {
             double d = 0.0;
-            for (int i = threadIdx.x; i < N; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
+            for (int i = tix(); i < N; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
             d = block_sum<NT>(d, red);
-            for (int i = threadIdx.x; i < N; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
+            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
             __syncthreads();
         }
         double ns = 0.0;
-        for (int i = threadIdx.x; i < N; i += NT) ns += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
+        for (int i = tix(); i < N; i += NT) ns += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
         const double norm = sqrt(block_sum<NT>(ns, red));
         if (norm >= 1e-10) {
-            for (int i = threadIdx.x; i < N; i += NT) out[(size_t)i * ld + j] /= norm;
+            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] /= norm;
             __syncthreads();
             continue;
         }
         // Best standard-basis candidate: thread k evaluates e_k.
         double best = -1.0;
         int bestk = 0;
-        for (int k = threadIdx.x; k < N; k += NT) {
+        for (int k = tix(); k < N; k += NT) {
             double r = 1.0;
             for (int c = 0; c < j; ++c) r -= out[(size_t)k * ld + c] * out[(size_t)k * ld + c];
             const double en = sqrt(fmax(r, 0.0));
@@ -52,13 +52,13 @@ __device__ void gram_schmidt_g(int N, const V& a, double* out, int ld, double* r
         // argmax with lowest index on ties (the reference keeps the first max)
         double* rd = red;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (tix() == 0) {
             rd[0] = -1.0;
             reinterpret_cast<int*>(rd)[2] = 0;
         }
         __syncthreads();
         for (int w = 0; w < NT; ++w) {  // serialised, fallback path only
-            if ((int)threadIdx.x == w && best > rd[0]) {
+            if (tix() == w && best > rd[0]) {
                 rd[0] = best;
                 reinterpret_cast<int*>(rd)[2] = bestk;
             }
@@ -68,21 +68,21 @@ __device__ void gram_schmidt_g(int N, const V& a, double* out, int ld, double* r
         const int bk = reinterpret_cast<int*>(rd)[2];
         __syncthreads();
         if (bn > 0.0 && isfinite(bn)) {
-            for (int i = threadIdx.x; i < N; i += NT) out[(size_t)i * ld + j] = (i == bk) ? 1.0 : 0.0;
+            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] = (i == bk) ? 1.0 : 0.0;
             __syncthreads();
             for (int k = 0; k < j; ++k) {
                 double d = 0.0;
-                for (int i = threadIdx.x; i < N; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
+                for (int i = tix(); i < N; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
                 d = block_sum<NT>(d, red);
-                for (int i = threadIdx.x; i < N; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
+                for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
                 __syncthreads();
             }
             double ns2 = 0.0;
-            for (int i = threadIdx.x; i < N; i += NT) ns2 += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
+            for (int i = tix(); i < N; i += NT) ns2 += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
             const double nn = sqrt(block_sum<NT>(ns2, red));
-            for (int i = threadIdx.x; i < N; i += NT) out[(size_t)i * ld + j] /= nn;
+            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] /= nn;
         } else {
-            for (int i = threadIdx.x; i < N; i += NT) out[(size_t)i * ld + j] = (i == j) ? 1.0 : 0.0;
+            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] = (i == j) ? 1.0 : 0.0;
         }
         __syncthreads();
     }
@@ -94,7 +94,7 @@ template <int NT>
 __device__ void householder_qr_g(int M, int N, double* a, int ld, double* red, double* tau_s, double* scratch) {
     for (int j = 0; j < N; ++j) {
         double xs = 0.0;
-        for (int i = j + 1 + threadIdx.x; i < M; i += NT) xs += a[(size_t)i * ld + j] * a[(size_t)i * ld + j];
+        for (int i = j + 1 + tix(); i < M; i += NT) xs += a[(size_t)i * ld + j] * a[(size_t)i * ld + j];
         const double xnorm2 = block_sum<NT>(xs, red);
         const double alpha = a[(size_t)j * ld + j];
         double tau = 0.0, beta = alpha, scal = 0.0;
@@ -103,12 +103,12 @@ __device__ void householder_qr_g(int M, int N, double* a, int ld, double* red, d
             tau = (beta - alpha) / beta;
             scal = 1.0 / (alpha - beta);
         }
-        if (threadIdx.x == 0) tau_s[j] = tau;
-        for (int i = j + 1 + threadIdx.x; i < M; i += NT) a[(size_t)i * ld + j] *= scal;
+        if (tix() == 0) tau_s[j] = tau;
+        for (int i = j + 1 + tix(); i < M; i += NT) a[(size_t)i * ld + j] *= scal;
         __syncthreads();
         if (tau != 0.0) {
             // w_c = v^T a[:, c] for c > j, then a[:, c] -= tau v w_c
-            for (int c = j + 1 + threadIdx.x; c < N; c += NT) {
+            for (int c = j + 1 + tix(); c < N; c += NT) {
                 double w = a[(size_t)j * ld + c];
                 for (int i = j + 1; i < M; ++i) w += a[(size_t)i * ld + j] * a[(size_t)i * ld + c];
                 a[(size_t)j * ld + c] -= tau * w;
@@ -116,12 +116,12 @@ __device__ void householder_qr_g(int M, int N, double* a, int ld, double* red, d
             }
             __syncthreads();
             const int cols = N - j - 1;
-            for (int e = threadIdx.x; e < (M - j - 1) * cols; e += NT) {
+            for (int e = tix(); e < (M - j - 1) * cols; e += NT) {
                 const int i = j + 1 + e / cols, c = j + 1 + e % cols;
                 a[(size_t)i * ld + c] -= a[(size_t)i * ld + j] * scratch[c];
             }
         }
-        if (threadIdx.x == 0) a[(size_t)j * ld + j] = beta;
+        if (tix() == 0) a[(size_t)j * ld + j] = beta;
         __syncthreads();
     }
 }
@@ -134,13 +134,13 @@ __device__ void apply_q_g(int M, int N, const double* qs, double* y, int ld, con
         const double tau = tau_s[j];
         if (tau == 0.0) continue;  // uniform: tau is in LDS
         // w_c = y[j][c] + sum_{i>j} v_i y[i][c];  y[i][c] -= tau v_i w_c
-        for (int c = threadIdx.x; c < N; c += NT) {
+        for (int c = tix(); c < N; c += NT) {
             double w = y[(size_t)j * ld + c];
             for (int i = j + 1; i < M; ++i) w += qs[(size_t)i * N + j] * y[(size_t)i * ld + c];
             scratch[c] = tau * w;
         }
         __syncthreads();
-        for (int e = threadIdx.x; e < (M - j) * N; e += NT) {
+        for (int e = tix(); e < (M - j) * N; e += NT) {
             const int i = j + e / N, c = e % N;
             const double v = (i == j) ? 1.0 : qs[(size_t)i * N + j];
             y[(size_t)i * ld + c] -= v * scratch[c];

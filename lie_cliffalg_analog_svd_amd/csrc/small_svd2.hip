@@ -32,9 +32,9 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 
 template <int NP>
 struct Cfg2 {
-    static constexpr int NT = NP <= 16 ? 64 : (NP <= 32 ? 128 : (NP <= 64 ? 256 : 512));
+    static constexpr int NT = NP <= 16 ? 64 : (NP <= 32 ? 128 : (NP <= 64 ? 256 : 1024));
     static constexpr int NW = NT / 64;
-    static constexpr int TPR = 4;                 // lanes per row of V
+    static constexpr int TPR = NP <= 64 ? 4 : 8;  // lanes per row of V
     static constexpr int CH = NP / TPR;           // ring positions per lane
     static constexpr int NTILE = NP / 16;
     static constexpr int RT = (NTILE + NW - 1) / NW;  // tiles per wave per panel
@@ -46,12 +46,13 @@ struct Cfg2 {
 
 template <int NP>
 struct Sh2 {
-    double red[8];
+    double red[Cfg2<NP>::NW];
     double diag[NP];
     double tau[NP];
     double scr[NP];
-    double rc[NP / 2], rs[NP / 2], rsv[NP / 2], rt[NP / 2];
-    int rp[NP / 2], rq[NP / 2], rf[NP / 2];
+    double2 cs[2][NP / 2];  // per layer parity: (c, s oriented to the slot order)
+    double tv[2][NP / 2];   // oriented t of the slot's diagonal update
+    int rf[2][NP / 2];      // rotation applied (|a_pq| >= 1e-300)
     int rank[NP];
     int item;
 };
@@ -64,7 +65,7 @@ __device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
 // Operand lanes: A[i][k] at lane (i + 16 k'), B[k][j] at lane (j + 16 k').
 template <int K, class FA, class FB>
 __device__ __forceinline__ d4v tile_mm(const FA& fa, const FB& fb, d4v acc) {
-    const int lane = threadIdx.x & 63;
+    const int lane = tix() & 63;
     const int i = lane & 15, kk = lane >> 4;
 #pragma unroll 4
     for (int k0 = 0; k0 < K; k0 += 4) acc = mfma64(fa(i, k0 + kk), fb(k0 + kk, i), acc);
@@ -73,8 +74,8 @@ __device__ __forceinline__ d4v tile_mm(const FA& fa, const FB& fb, d4v acc) {
 
 // Accumulator element r of a lane sits at row 4 r + lane / 16, column lane % 16
 // (measured layout of the FP64 16x16x4 instruction).
-__device__ __forceinline__ int trow(int r) { return 4 * r + ((threadIdx.x & 63) >> 4); }
-__device__ __forceinline__ int tcol() { return threadIdx.x & 15; }
+__device__ __forceinline__ int trow(int r) { return 4 * r + ((tix() & 63) >> 4); }
+__device__ __forceinline__ int tcol() { return tix() & 15; }
 
 // Ring slot -> indices at layer L (circle method over m = NP positions).
 __device__ __forceinline__ void ring_pair(int m, int L, int k, int& a, int& b) {
@@ -95,13 +96,43 @@ __device__ __forceinline__ int ring_idx0(int m, int g) {
     return k == 0 ? 0 : m - 1 - k;
 }
 
+// Position a ring position moves to when the ring advances one layer
+// (the inverse of the shift applied to V below).
+__device__ __forceinline__ int ring_dst(int m, int h) {
+    if (h == 0) return 0;
+    if (h == 2) return 1;
+    if (h == m - 1) return m - 2;
+    return (h & 1) ? h + 2 : h - 2;
+}
+
+// Ring position of index i in the layer-0 arrangement (inverse of ring_idx0).
+__device__ __forceinline__ int ring_pos0(int m, int i) {
+    if (i == m - 1) return 0;
+    if (i == 0) return 1;
+    if (2 * i <= m - 2) return 2 * i;
+    return 2 * (m - 1 - i) + 1;
+}
+
+struct PhaseClock2 {
+    long long* out;
+    long long t;
+    __device__ PhaseClock2(long long* o) : out(o), t(o && threadIdx.x == 0 ? clock64() : 0) {}
+    __device__ __forceinline__ void mark(int ph) {
+        if (out && threadIdx.x == 0) {
+            const long long now = clock64();
+            out[ph] += now - t;
+            t = now;
+        }
+    }
+};
+
 template <int NP>
 __device__ void solve_item2(const Item& itm, const Pools& P, double* main, double* panel, double* xs, double* vs,
                             double* ys, Sh2<NP>& sh) {
     using C = Cfg2<NP>;
     constexpr int NT = C::NT, NW = C::NW, LDM = C::LDM, NTILE = C::NTILE, RT = C::RT;
     constexpr int TPR = C::TPR, CH = C::CH, SLOTS = C::SLOTS, LDP = C::LDP;
-    const int tid = threadIdx.x, w = tid >> 6;
+    const int tid = tix(), w = tid >> 6;
     const int m0 = itm.m, n0 = itm.n;
     const bool wide = m0 < n0;
     const int M = wide ? n0 : m0, N = wide ? m0 : n0;
@@ -113,6 +144,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     const double rootn = sqrt((double)N);
     lsvd_info_t info = {};
     const d4v zero4 = {0.0, 0.0, 0.0, 0.0};
+    PhaseClock2 clk(P.timing ? P.timing + (size_t)itm.idx * kPhases : nullptr);
 
     // ---- front end (tall/wide: Householder QR in LDS, compact form to qs)
     if (tall && !eigh) {
@@ -128,6 +160,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         bool tall;
         __device__ double operator()(int i, int j) const { return tall ? r(i, j) : a(i, j); }
     } asv{UpperView{qs, N}, ain, tall};
+    clk.mark(kPhQR);
 
     int polar_ok = 0, iters = 0;
     if (!eigh) {
@@ -222,6 +255,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
             polar_ok = diverged ? 0 : (oe <= 1e-8 * rootn);
         }
         info.ns_iters = iters;
+        clk.mark(kPhNS);
 
         // ---- P = sym(Q^T A) (:305-309); Q parked in the global slot xs
 #pragma unroll 1
@@ -266,8 +300,23 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         __syncthreads();
     }
     info.polar_ok = polar_ok;
+    clk.mark(kPhP);
 
-    // ---- Jacobi on P (:212-285): P in LDS, V in registers (ring positions)
+    // ---- Jacobi on P (:212-285) in ring-position space.  P is stored
+    // permuted so that the pairs of every layer sit at positions (2k, 2k+1),
+    // and only its upper triangle is kept.  Each thread owns fixed 2x2
+    // blocks (k_a <= k_b) of that layout, rotates them and writes them to
+    // where the next layer needs them (transposed into the upper triangle
+    // when the move crosses the diagonal): one read, four writes per block
+    // and two barriers per layer.  V's update for a layer is deferred to the
+    // next layer's first phase, where it overlaps the angle computation.
+    for (int e = tid; e < NP * NP; e += NT) ys[e] = main[(e / NP) * LDM + e % NP];
+    __syncthreads();
+    for (int e = tid; e < NP * NP; e += NT) {
+        const int g = e / NP, hh = e % NP;
+        main[g * LDM + hh] = ys[ring_idx0(NP, g) * NP + ring_idx0(NP, hh)];
+    }
+    __syncthreads();
     const int vrow = tid / TPR, vch = tid % TPR;
     double v[CH];
 #pragma unroll
@@ -281,108 +330,136 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         ref += x * x;
     }
     ref = fmax(sqrt(block_sum<NT>(ref, sh.red)), 1e-300);
+    constexpr int NBLK = SLOTS * (SLOTS + 1) / 2;
+    constexpr int MB = (NBLK + NT - 1) / NT;
+    // per block: packed (k_a, k_b) and the four upper-triangle write offsets
+    int bk[MB], bw01[MB], bw23[MB];
+#pragma unroll
+    for (int i = 0; i < MB; ++i) {
+        const int bI = tid + i * NT;
+        int ka = 0, rem = bI;
+        while (ka < SLOTS && rem >= SLOTS - ka) {
+            rem -= SLOTS - ka;
+            ++ka;
+        }
+        const int kb = ka + rem;
+        const int da0 = ring_dst(NP, 2 * ka), da1 = ring_dst(NP, 2 * ka + 1);
+        const int db0 = ring_dst(NP, 2 * kb), db1 = ring_dst(NP, 2 * kb + 1);
+        auto up = [](int r, int c) { return r <= c ? r * LDM + c : c * LDM + r; };
+        bk[i] = bI < NBLK ? (ka | (kb << 8)) : -1;
+        bw01[i] = up(da0, db0) | (up(da0, db1) << 16);
+        bw23[i] = up(da1, db0) | (up(da1, db1) << 16);
+    }
     const int max_sweeps = 60 + 20 * N;
     int sweeps = 0, rots = 0;
     const int lane = tid & 63;
     const int nbr_next = (lane + 1) & 63, nbr_prev = (lane + 63) & 63;
-    constexpr int nblk = SLOTS * (SLOTS + 1) / 2;
+    const int ka_thread = tid - (NT - 64);  // the last wave computes the angles
+    // rotate V's own slots with layer `buf`'s angles, then shift the ring
+    auto v_update = [&](int buf) {
+#pragma unroll
+        for (int j = 0; j < CH / 2; ++j) {
+            const double2 r = sh.cs[buf][vch * (CH / 2) + j];
+            const double x = v[2 * j], y = v[2 * j + 1];
+            v[2 * j] = r.x * x - r.y * y;
+            v[2 * j + 1] = r.y * x + r.x * y;
+        }
+        // in-place ring shift (verified against the circle-method schedule):
+        // even slots move down by one pair, odd slots up
+        const double nfe = __shfl(v[0], nbr_next, 64);       // next lane's first even
+        const double plo = __shfl(v[CH - 1], nbr_prev, 64);  // previous lane's last odd
+        const double e2 = v[2], olast = v[CH - 1];
+#pragma unroll
+        for (int j = 0; j + 2 < CH; j += 2) v[j] = (vch == 0 && j == 0) ? v[0] : v[j + 2];
+        v[CH - 2] = (vch == TPR - 1) ? olast : nfe;
+#pragma unroll
+        for (int j = CH - 1; j >= 3; j -= 2) v[j] = v[j - 2];
+        v[1] = (vch == 0) ? e2 : plo;
+    };
     for (int sw = 0; sw < max_sweeps && N > 1; ++sw) {
         double off = 0.0;
         for (int e = tid; e < NP * NP; e += NT) {
-            const int p = e / NP, q = e % NP;
-            if (q > p) off += main[p * LDM + q] * main[p * LDM + q];
+            const int g = e / NP, hh = e % NP;
+            const double x = main[g * LDM + hh];
+            if (g < hh) off += x * x;
         }
         off = block_sum<NT>(off, sh.red);
         if (!(sqrt(off) >= kJacobiTol * ref)) break;
         ++sweeps;
+#pragma unroll 1
         for (int L = 0; L < NP - 1; ++L) {
-            // phase A: rotation of every slot of layer L
-            for (int k = tid; k < SLOTS; k += NT) {
+            const int buf = L & 1;
+            // phase A: rotation of every slot of layer L (:217-243)
+            if (ka_thread >= 0 && ka_thread < SLOTS) {
+                const int k = ka_thread;
                 int a, b;
                 ring_pair(NP, L, k, a, b);
-                const int p = a < b ? a : b, q = a < b ? b : a;
-                double c = 1.0, s = 0.0, tt = 0.0;
+                const bool fwd = a < b;  // position 2k holds index a
+                const double x00 = main[2 * k * LDM + 2 * k], x11 = main[(2 * k + 1) * LDM + 2 * k + 1];
+                const double apq = main[2 * k * LDM + 2 * k + 1];
+                double c = 1.0, sn = 0.0, tt = 0.0;
                 int rot = 0;
-                const double apq = main[p * LDM + q];
                 if (fabs(apq) >= 1e-300) {
-                    const double app = main[p * LDM + p], aqq = main[q * LDM + q];
+                    const double app = fwd ? x00 : x11, aqq = fwd ? x11 : x00;
                     const double th = (aqq - app) / (2.0 * apq);
                     tt = th >= 0.0 ? 1.0 / (th + sqrt(th * th + 1.0)) : -1.0 / (-th + sqrt(th * th + 1.0));
                     c = 1.0 / sqrt(tt * tt + 1.0);
-                    s = tt * c;
+                    sn = tt * c;
                     rot = 1;
                     ++rots;
                 }
-                sh.rp[k] = p;
-                sh.rq[k] = q;
-                sh.rf[k] = rot;
-                sh.rc[k] = c;
-                sh.rs[k] = s;
-                sh.rsv[k] = a < b ? s : -s;  // ring slot order may be (q, p)
-                sh.rt[k] = tt;
+                sh.cs[buf][k] = make_double2(c, fwd ? sn : -sn);
+                sh.tv[buf][k] = fwd ? tt : -tt;
+                sh.rf[buf][k] = rot;
             }
+            double blk[MB][4];
+#pragma unroll
+            for (int i = 0; i < MB; ++i) {
+                if (bk[i] >= 0) {
+                    const int ka = bk[i] & 255, kb = bk[i] >> 8;
+                    const double* r0 = main + 2 * ka * LDM + 2 * kb;
+                    blk[i][0] = r0[0];
+                    blk[i][1] = r0[1];
+                    blk[i][2] = ka == kb ? r0[1] : r0[LDM];
+                    blk[i][3] = r0[LDM + 1];
+                }
+            }
+            if (L > 0) v_update(buf ^ 1);
             __syncthreads();
-            // phase B1: V rows -- rotate own slots, then shift the ring
+            // P blocks: rotate (:244-285) and move to the next layer's positions
 #pragma unroll
-            for (int j = 0; j < CH / 2; ++j) {
-                const int k = vch * (CH / 2) + j;
-                const double c = sh.rc[k], s = sh.rsv[k];
-                const double x = v[2 * j], y = v[2 * j + 1];
-                v[2 * j] = c * x - s * y;
-                v[2 * j + 1] = s * x + c * y;
-            }
-            {
-                // in-place ring shift (verified against the circle-method
-                // schedule): even slots move down by one pair, odd slots up
-                const double nfe = __shfl(v[0], nbr_next, 64);       // next lane's first even
-                const double plo = __shfl(v[CH - 1], nbr_prev, 64);  // previous lane's last odd
-                const double e2 = v[2], olast = v[CH - 1];
-#pragma unroll
-                for (int j = 0; j + 2 < CH; j += 2) v[j] = (vch == 0 && j == 0) ? v[0] : v[j + 2];
-                v[CH - 2] = (vch == TPR - 1) ? olast : nfe;
-#pragma unroll
-                for (int j = CH - 1; j >= 3; j -= 2) v[j] = v[j - 2];
-                v[1] = (vch == 0) ? e2 : plo;
-            }
-            // phase B2: P 2x2 blocks (pair a rows, pair b columns), a <= b
-            for (int bI = tid; bI < nblk; bI += NT) {
-                int kb = (int)((sqrt(8.0 * bI + 1.0) - 1.0) * 0.5);
-                while (kb * (kb + 1) / 2 > bI) --kb;
-                while ((kb + 1) * (kb + 2) / 2 <= bI) ++kb;
-                const int ka = bI - kb * (kb + 1) / 2;
-                const int pa = sh.rp[ka], qa = sh.rq[ka], pb = sh.rp[kb], qb = sh.rq[kb];
+            for (int i = 0; i < MB; ++i) {
+                if (bk[i] < 0) continue;
+                const int ka = bk[i] & 255, kb = bk[i] >> 8;
+                const int w0 = bw01[i] & 0xffff, w1 = bw01[i] >> 16, w2 = bw23[i] & 0xffff, w3 = bw23[i] >> 16;
+                const double b00 = blk[i][0], b01 = blk[i][1], b10 = blk[i][2], b11 = blk[i][3];
                 if (ka == kb) {
-                    if (sh.rf[ka]) {  // (:244-249)
-                        const double apq = main[pa * LDM + qa];
-                        const double tt = sh.rt[ka];
-                        main[pa * LDM + pa] -= tt * apq;
-                        main[qa * LDM + qa] += tt * apq;
-                        main[pa * LDM + qa] = 0.0;
-                        main[qa * LDM + pa] = 0.0;
+                    double n00 = b00, n11 = b11, n01 = b01;
+                    if (sh.rf[buf][ka]) {  // (:244-249)
+                        const double tt = sh.tv[buf][ka];
+                        n00 = b00 - tt * b01;
+                        n11 = b11 + tt * b01;
+                        n01 = 0.0;
                     }
+                    main[w0] = n00;
+                    main[w1] = n01;
+                    main[w3] = n11;
                     continue;
                 }
-                if (!sh.rf[ka] && !sh.rf[kb]) continue;
-                const double ca = sh.rc[ka], sa = sh.rs[ka], cb = sh.rc[kb], sb = sh.rs[kb];
-                const double b00 = main[pa * LDM + pb], b01 = main[pa * LDM + qb];
-                const double b10 = main[qa * LDM + pb], b11 = main[qa * LDM + qb];
-                const double l00 = ca * b00 - sa * b10, l01 = ca * b01 - sa * b11;
-                const double l10 = sa * b00 + ca * b10, l11 = sa * b01 + ca * b11;
-                const double r00 = cb * l00 - sb * l01, r01 = sb * l00 + cb * l01;
-                const double r10 = cb * l10 - sb * l11, r11 = sb * l10 + cb * l11;
-                main[pa * LDM + pb] = r00;
-                main[pb * LDM + pa] = r00;
-                main[pa * LDM + qb] = r01;
-                main[qb * LDM + pa] = r01;
-                main[qa * LDM + pb] = r10;
-                main[pb * LDM + qa] = r10;
-                main[qa * LDM + qb] = r11;
-                main[qb * LDM + qa] = r11;
+                const double2 ra = sh.cs[buf][ka], rb = sh.cs[buf][kb];
+                const double l00 = ra.x * b00 - ra.y * b10, l01 = ra.x * b01 - ra.y * b11;
+                const double l10 = ra.y * b00 + ra.x * b10, l11 = ra.y * b01 + ra.x * b11;
+                main[w0] = rb.x * l00 - rb.y * l01;
+                main[w1] = rb.y * l00 + rb.x * l01;
+                main[w2] = rb.x * l10 - rb.y * l11;
+                main[w3] = rb.y * l10 + rb.x * l11;
             }
             __syncthreads();
         }
+        v_update((NP - 2) & 1);  // the sweep's last layer
     }
-    for (int i = tid; i < N; i += NT) sh.diag[i] = main[i * LDM + i];
+    clk.mark(kPhJacobi);
+    for (int i = tid; i < N; i += NT) sh.diag[i] = main[ring_pos0(NP, i) * (LDM + 1)];
     __syncthreads();
     for (int i = tid; i < N; i += NT) {  // rank sort, eigenvalues descending
         const double di = isnan(sh.diag[i]) ? -INFINITY : sh.diag[i];
@@ -479,6 +556,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
             }
     }
     oe = sqrt(block_sum<NT>(oe, sh.red));
+    clk.mark(kPhU);
     if (!polar_ok || oe > 1e-8 * rootn) {
         // repair_left_basis_from_right (:185-208): seed = A V S^-1, MGS
         double smax = 0.0;
@@ -496,6 +574,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         info.repaired = 1;
     }
     __syncthreads();
+    clk.mark(kPhRepair);
     if (tall) apply_q_g<NT>(M, N, qs, main, LDM, sh.tau, sh.scr);
     if (!wide) {
         double* u = P.u + itm.u_off;
@@ -504,6 +583,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         double* vt = P.vt + itm.vt_off;  // (N x M) = U_t^T
         for (int e = tid; e < M * N; e += NT) vt[(size_t)(e % N) * M + e / N] = main[(e / N) * LDM + e % N];
     }
+    clk.mark(kPhOut);
     if (tid == 0 && P.info) {
         info.path = itm.mode == kModeMicroFallback ? 2 : 0;
         info.micro_fallback = itm.mode == kModeMicroFallback ? 1 : 0;

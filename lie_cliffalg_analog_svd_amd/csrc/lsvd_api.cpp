@@ -77,9 +77,11 @@ static const SmallVariant* find_variant(int vid) {
 
 // Work estimate used for longest-first ordering (flops up to a constant):
 // QR + compose ~ 4 M N^2, NS ~ 4 k N^3 with k ~ 30, Jacobi ~ 6 S N^3, S ~ 8.
+// Relative cost used to order the queues (largest first): about 42
+// Newton-Schulz steps of two M x N x N products, Jacobi sweeps of ~12 N^3.
 static double item_cost(int M, int N) {
     const double n3 = (double)N * N * N;
-    return 4.0 * M * (double)N * N * (M > N) + (120.0 + 48.0 + 4.0) * n3;
+    return 86.0 * M * (double)N * N + 48.0 * n3;
 }
 
 // Sets the launch shape of a bucket for `variant`; returns the global arena
@@ -92,7 +94,7 @@ static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
         const int per_cu = small2_blocks_per_cu(np, lds);
         bk.grid = std::max(1, std::min(bk.count, per_cu * h->num_cus));
         bk.lds_bytes = lds;
-        bk.gws_stride = (int64_t)((3 * (size_t)np * np + 31) / 32 * 32);
+        bk.gws_stride = v2_slot_doubles(np, bk.maxM);
         return (size_t)bk.gws_stride * bk.grid;
     }
     const SmallVariant* v = find_variant(variant);
@@ -203,6 +205,8 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         size_t arena = 0;
         for (auto& t : v) {
             all.push_back(t.it);
+            bk.cost_sum += t.cost;
+            bk.cost_max = std::max(bk.cost_max, t.cost);
             maxN = std::max(maxN, t.N);
             maxM = std::max(maxM, t.M);
             arena = std::max(arena, small_arena_doubles(t.M, t.N) * 8);
@@ -281,10 +285,16 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
 
 namespace lsvd {
 
-// Side streams and fork/join events, one per concurrent launch.  Created
+// Side streams and fork/join events for the concurrent lanes.  Created
 // outside any graph capture (plan_run_device calls this first).
+constexpr int kPlanLanes = 4;  // one per hardware queue HIP gives a process
+
+static int plan_lanes(lsvd_plan_t p) {
+    return std::min(kPlanLanes, (int)p->buckets.size() + (p->n_micro > 0 ? 1 : 0));
+}
+
 hipError_t plan_ensure_streams(lsvd_plan_t p) {
-    const int lanes = (int)p->buckets.size() + (p->n_micro > 0 ? 1 : 0);
+    const int lanes = plan_lanes(p);
     for (int i = (int)p->side.size(); i < lanes; ++i) {
         hipStream_t s2;
         hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
@@ -300,11 +310,13 @@ hipError_t plan_ensure_streams(lsvd_plan_t p) {
     return hipSuccess;
 }
 
-// Enqueues the whole plan on `st` with the given device pools.  The micro
-// launch and every bucket run concurrently on side streams (fork/join with
-// events, which a hipGraph capture turns into parallel branches), largest
-// per-matrix work first, so the CUs a large-N bucket leaves idle take the
-// small-N buckets; the micro escalations drain after the join.
+// Enqueues the whole plan on `st` with the given device pools.  The
+// buckets (and the micro launch) are spread over up to four concurrent
+// lanes (fork/join with events, which a hipGraph capture turns into
+// parallel branches): longest estimated bucket first, each onto the lane
+// with the least estimated work, so a lane never queues a long bucket
+// behind another one while the CUs a large-N bucket leaves idle take the
+// small-N buckets.  The micro escalations drain after the join.
 hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
     hipError_t e = launch_reset_counters(p->d_counters, p->n_counters, st);
     if (e != hipSuccess) return e;
@@ -312,22 +324,26 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
     int* fb_count = p->d_counters + nb + 1;
     e = plan_ensure_streams(p);
     if (e != hipSuccess) return e;
+    const int lanes = plan_lanes(p);
+    std::vector<int> order(nb);
+    for (int i = 0; i < nb; ++i) order[i] = i;
+    auto est = [&](int i) {
+        const Bucket& bk = p->buckets[i];
+        return bk.cost_sum / std::max(bk.grid, 1) + bk.cost_max;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return est(x) > est(y); });
+    std::vector<double> load(std::max(lanes, 1), 0.0);
+    auto least = [&]() { return (int)(std::min_element(load.begin(), load.end()) - load.begin()); };
     e = hipEventRecord(p->ev[0], st);  // fork point
     if (e != hipSuccess) return e;
-    int lane = 0;
-    if (p->n_micro > 0) {
-        hipStream_t s2 = p->side[lane++];
-        e = hipStreamWaitEvent(s2, p->ev[0], 0);
-        if (e != hipSuccess) return e;
-        MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, fb_count};
-        e = launch_micro(ml, s2);
+    for (int l = 0; l < lanes; ++l) {
+        e = hipStreamWaitEvent(p->side[l], p->ev[0], 0);
         if (e != hipSuccess) return e;
     }
-    for (int i = nb - 1; i >= 0; --i) {  // buckets are in increasing size order
+    for (int i : order) {
         const Bucket& bk = p->buckets[i];
-        hipStream_t s2 = p->side[lane++];
-        e = hipStreamWaitEvent(s2, p->ev[0], 0);
-        if (e != hipSuccess) return e;
+        const int l = least();
+        load[l] += est(i);
         SmallLaunch L{};
         L.grid = bk.grid;
         L.lds_bytes = bk.lds_bytes;
@@ -337,10 +353,15 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
         L.pools = pools;
         L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
         L.gws_stride = bk.gws_stride;
-        e = launch_small_variant(bk.variant, L, s2);
+        e = launch_small_variant(bk.variant, L, p->side[l]);
         if (e != hipSuccess) return e;
     }
-    for (int l = 0; l < lane; ++l) {  // join
+    if (p->n_micro > 0) {
+        MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, fb_count};
+        e = launch_micro(ml, p->side[least()]);
+        if (e != hipSuccess) return e;
+    }
+    for (int l = 0; l < lanes; ++l) {  // join
         e = hipEventRecord(p->ev[l + 1], p->side[l]);
         if (e != hipSuccess) return e;
         e = hipStreamWaitEvent(st, p->ev[l + 1], 0);

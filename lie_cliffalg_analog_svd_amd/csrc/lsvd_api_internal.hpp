@@ -22,6 +22,8 @@ struct Bucket {
     size_t lds_bytes;
     int64_t gws_offset;  // doubles into the plan's global arena
     int64_t gws_stride;
+    double cost_sum;     // item_cost summed over the bucket
+    double cost_max;
 };
 
 inline uint64_t tune_key(int maxN, size_t arena_bytes, int precision) {

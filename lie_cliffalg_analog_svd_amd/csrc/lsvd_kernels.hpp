@@ -42,6 +42,7 @@ hipError_t launch_reset_counters(int* counters, int n, hipStream_t st);
 // np: padded size 16, 32, ..., 128 (N <= np).  Variant id = 100 + np.
 int v2_threads(int np);
 size_t v2_lds_bytes(int np, int max_m);
+int64_t v2_slot_doubles(int np, int max_m);
 hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st);
 int small2_blocks_per_cu(int np, size_t lds_bytes);
 constexpr int kV2Base = 100;

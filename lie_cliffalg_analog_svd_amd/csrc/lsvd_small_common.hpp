@@ -12,35 +12,35 @@
 
 namespace lsvd {
 
-// gram_schmidt_orthonormal (src/lie_svd_small.rs:82-144): MGS over the
-// columns of `a` into `out` (N x ld).  Rank-deficient columns take the
-// standard-basis direction with the largest residual; its norm is
-// evaluated as 1 - sum_c out[k][c]^2, which equals the reference's MGS
-// projection of e_k up to rounding.
+// gram_schmidt_orthonormal (src/lie_svd_small.rs:82-144): MGS over the N
+// columns (length M >= N) of `a` into `out` (M x N, leading dimension ld).
+// Rank-deficient columns take the standard-basis direction e_k (k < M) with
+// the largest residual; its norm is evaluated as 1 - sum_c out[k][c]^2,
+// which equals the reference's MGS projection of e_k up to rounding.
 template <int NT, class V>
-__device__ void gram_schmidt_g(int N, const V& a, double* out, int ld, double* red) {
+__device__ void gram_schmidt_g(int M, int N, const V& a, double* out, int ld, double* red) {
     for (int j = 0; j < N; ++j) {
-        for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] = a(i, j);
+        for (int i = tix(); i < M; i += NT) out[(size_t)i * ld + j] = a(i, j);
         __syncthreads();
         for (int k = 0; k < j; ++k) {
             double d = 0.0;
-            for (int i = tix(); i < N; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
+            for (int i = tix(); i < M; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
             d = block_sum<NT>(d, red);
-            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
+            for (int i = tix(); i < M; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
             __syncthreads();
         }
         double ns = 0.0;
-        for (int i = tix(); i < N; i += NT) ns += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
+        for (int i = tix(); i < M; i += NT) ns += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
         const double norm = sqrt(block_sum<NT>(ns, red));
         if (norm >= 1e-10) {
-            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] /= norm;
+            for (int i = tix(); i < M; i += NT) out[(size_t)i * ld + j] /= norm;
             __syncthreads();
             continue;
         }
         // Best standard-basis candidate: thread k evaluates e_k.
         double best = -1.0;
         int bestk = 0;
-        for (int k = tix(); k < N; k += NT) {
+        for (int k = tix(); k < M; k += NT) {
             double r = 1.0;
             for (int c = 0; c < j; ++c) r -= out[(size_t)k * ld + c] * out[(size_t)k * ld + c];
             const double en = sqrt(fmax(r, 0.0));
@@ -68,21 +68,21 @@ __device__ void gram_schmidt_g(int N, const V& a, double* out, int ld, double* r
         const int bk = reinterpret_cast<int*>(rd)[2];
         __syncthreads();
         if (bn > 0.0 && isfinite(bn)) {
-            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] = (i == bk) ? 1.0 : 0.0;
+            for (int i = tix(); i < M; i += NT) out[(size_t)i * ld + j] = (i == bk) ? 1.0 : 0.0;
             __syncthreads();
             for (int k = 0; k < j; ++k) {
                 double d = 0.0;
-                for (int i = tix(); i < N; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
+                for (int i = tix(); i < M; i += NT) d += out[(size_t)i * ld + k] * out[(size_t)i * ld + j];
                 d = block_sum<NT>(d, red);
-                for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
+                for (int i = tix(); i < M; i += NT) out[(size_t)i * ld + j] -= d * out[(size_t)i * ld + k];
                 __syncthreads();
             }
             double ns2 = 0.0;
-            for (int i = tix(); i < N; i += NT) ns2 += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
+            for (int i = tix(); i < M; i += NT) ns2 += out[(size_t)i * ld + j] * out[(size_t)i * ld + j];
             const double nn = sqrt(block_sum<NT>(ns2, red));
-            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] /= nn;
+            for (int i = tix(); i < M; i += NT) out[(size_t)i * ld + j] /= nn;
         } else {
-            for (int i = tix(); i < N; i += NT) out[(size_t)i * ld + j] = (i == j) ? 1.0 : 0.0;
+            for (int i = tix(); i < M; i += NT) out[(size_t)i * ld + j] = (i == j) ? 1.0 : 0.0;
         }
         __syncthreads();
     }

@@ -72,7 +72,7 @@ __device__ double orth_err(int N, const double* q, int ld, double* red) {
 // Thin wrappers over the shared building blocks (lsvd_small_common.hpp).
 template <int NT, int MAXN, class V>
 __device__ void gram_schmidt(int N, const V& a, double* out, int ld, SmallShared<NT, MAXN>& sh) {
-    gram_schmidt_g<NT>(N, a, out, ld, sh.red);
+    gram_schmidt_g<NT>(N, N, a, out, ld, sh.red);
 }
 
 template <int NT, int MAXN>

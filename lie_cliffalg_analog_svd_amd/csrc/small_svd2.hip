@@ -30,14 +30,21 @@ namespace lsvd {
 
 typedef double d4v __attribute__((ext_vector_type(4)));
 
+// Largest padded size whose Newton-Schulz step may keep Y = 3I - X^T X in
+// LDS next to X (the host sizes LDS for it when it fits).
+constexpr int kV2YModeMaxNP = 96;
+constexpr size_t kV2LdsBudget = 150 * 1024;
+
 template <int NP>
 struct Cfg2 {
-    static constexpr int NT = NP <= 16 ? 64 : (NP <= 32 ? 128 : (NP <= 64 ? 256 : 1024));
+    // measured per-matrix latency is lowest with 4 (N <= 32), 8 (N <= 64)
+    // and 16 (N > 64) waves per matrix
+    static constexpr int NT = NP <= 16 ? 64 : (NP <= 32 ? 256 : (NP <= 64 ? 512 : 1024));
     static constexpr int NW = NT / 64;
-    static constexpr int TPR = NP <= 64 ? 4 : 8;  // lanes per row of V
+    static constexpr int TPR = NP <= 16 ? 4 : 8;  // lanes per row of V (divides 64)
     static constexpr int CH = NP / TPR;           // ring positions per lane
+    static_assert(CH % 2 == 0 && CH >= 4 && NP * TPR <= NT, "V ring layout");
     static constexpr int NTILE = NP / 16;
-    static constexpr int RT = (NTILE + NW - 1) / NW;  // tiles per wave per panel
     static constexpr int LDM = NP + 1;
     static constexpr int SLOTS = NP / 2;
     static constexpr int LDP = 17;  // panel leading dimension (16 columns)
@@ -61,10 +68,11 @@ __device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// One 16 x 16 tile: acc += A (16 x K) B (K x 16); fa(i, k), fb(k, j).
+// One 16 x 16 tile: acc += A (16 x K) B (K x 16); fa(i, k), fb(k, j);
+// K is a multiple of 4.
 // Operand lanes: A[i][k] at lane (i + 16 k'), B[k][j] at lane (j + 16 k').
-template <int K, class FA, class FB>
-__device__ __forceinline__ d4v tile_mm(const FA& fa, const FB& fb, d4v acc) {
+template <class FA, class FB>
+__device__ __forceinline__ d4v tile_mm(int K, const FA& fa, const FB& fb, d4v acc) {
     const int lane = tix() & 63;
     const int i = lane & 15, kk = lane >> 4;
 #pragma unroll 4
@@ -127,10 +135,10 @@ struct PhaseClock2 {
 };
 
 template <int NP>
-__device__ void solve_item2(const Item& itm, const Pools& P, double* main, double* panel, double* xs, double* vs,
-                            double* ys, Sh2<NP>& sh) {
+__device__ void solve_item2(const Item& itm, const Pools& P, double* main, double* panel, int panel_rows,
+                            double* xs, double* vs, double* ys, Sh2<NP>& sh) {
     using C = Cfg2<NP>;
-    constexpr int NT = C::NT, NW = C::NW, LDM = C::LDM, NTILE = C::NTILE, RT = C::RT;
+    constexpr int NT = C::NT, NW = C::NW, LDM = C::LDM, NTILE = C::NTILE;
     constexpr int TPR = C::TPR, CH = C::CH, SLOTS = C::SLOTS, LDP = C::LDP;
     const int tid = tix(), w = tid >> 6;
     const int m0 = itm.m, n0 = itm.n;
@@ -138,117 +146,143 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     const int M = wide ? n0 : m0, N = wide ? m0 : n0;
     const double* A = P.a + itm.a_off;
     const View ain{A, n0, wide ? 1 : 0};
-    double* qs = wide ? P.vt + itm.vt_off : P.u + itm.u_off;
-    const bool tall = M > N;
     const bool eigh = itm.mode == kModeEigh;
     const double rootn = sqrt((double)N);
     lsvd_info_t info = {};
     const d4v zero4 = {0.0, 0.0, 0.0, 0.0};
     PhaseClock2 clk(P.timing ? P.timing + (size_t)itm.idx * kPhases : nullptr);
 
-    // ---- front end (tall/wide: Householder QR in LDS, compact form to qs)
-    if (tall && !eigh) {
-        for (int e = tid; e < M * N; e += NT) main[(size_t)(e / N) * LDM + e % N] = ain(e / N, e % N);
-        __syncthreads();
-        householder_qr_g<NT>(M, N, main, LDM, sh.red, sh.tau, sh.scr);
-        for (int e = tid; e < M * N; e += NT) qs[(size_t)(e / N) * N + e % N] = main[(size_t)(e / N) * LDM + e % N];
-        __syncthreads();
-    }
-    struct Asq {
-        UpperView r;
-        View a;
-        bool tall;
-        __device__ double operator()(int i, int j) const { return tall ? r(i, j) : a(i, j); }
-    } asv{UpperView{qs, N}, ain, tall};
+    // Tall and wide inputs are not reduced by a QR first: the polar
+    // iteration, P = sym(Q^T A) and U = Q V run on the M x N matrix itself.
+    // With A = Q_R R this is the reference's iteration on R (:356-359)
+    // carried by the orthonormal Q_R, so sigma and V are those of R and U is
+    // Q_R U_R, without the two sequential Householder passes.
+    const int MP = (M + 15) & ~15, MT = MP >> 4;
     clk.mark(kPhQR);
 
     int polar_ok = 0, iters = 0;
     if (!eigh) {
         // ---- Newton-Schulz polar (src/lie_svd_small.rs:158-183)
         double sc = 0.0;
-        for (int e = tid; e < N * N; e += NT) {
-            const double x = asv(e / N, e % N);
+        for (int e = tid; e < M * N; e += NT) {
+            const double x = ain(e / N, e % N);
             sc += x * x;
         }
         sc = fmax(sqrt(block_sum<NT>(sc, sh.red)), 1e-300);
-        for (int e = tid; e < NP * NP; e += NT) {
+        for (int e = tid; e < MP * NP; e += NT) {
             const int i = e / NP, j = e % NP;
-            main[i * LDM + j] = (i < N && j < N) ? asv(i, j) / sc : 0.0;
+            main[i * LDM + j] = (i < M && j < N) ? ain(i, j) / sc : 0.0;
         }
         __syncthreads();
         int diverged = 0;
+        // Y = 3I - X^T X fits next to X: compute its upper tiles once per step
+        // and update X in place, row tile by row tile (no staging)
+        const bool ymode = NP <= kV2YModeMaxNP && panel_rows >= NP;
         for (int it = 0; it < kPolarMaxIters; ++it) {
-#pragma unroll 1
-            for (int J = 0; J < NTILE; ++J) {
-                // panel = (3I - X^T X)[:, J]
+            double dsq = 0.0;
+            int bad = 0;
+            if constexpr (NP <= kV2YModeMaxNP) {
+                if (ymode) {
+                    constexpr int NTRI = NTILE * (NTILE + 1) / 2;
+                    for (int t = w; t < NTRI; t += NW) {
+                        int I = 0, rem = t;
+                        while (rem >= NTILE - I) {
+                            rem -= NTILE - I;
+                            ++I;
+                        }
+                        const int J = I + rem;
+                        const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                              [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
 #pragma unroll
-                for (int a = 0; a < RT; ++a) {
-                    const int I = w + a * NW;
-                    if (I < NTILE) {
-                        const d4v g = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
-                                                  [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = I * 16 + trow(r), col = J * 16 + tcol();
+                            const double y = (row == col ? 3.0 : 0.0) - g[r];
+                            panel[row * LDM + col] = y;
+                            panel[col * LDM + row] = y;
+                        }
+                    }
+                    __syncthreads();
+                    for (int R = w; R < MT; R += NW) {
+                        d4v acc[NTILE];
+#pragma unroll
+                        for (int J = 0; J < NTILE; ++J)
+                            acc[J] = tile_mm(NP, [&](int i, int k) { return main[(R * 16 + i) * LDM + k]; },
+                                             [&](int k, int j) { return panel[k * LDM + J * 16 + j]; }, zero4);
+#pragma unroll
+                        for (int J = 0; J < NTILE; ++J)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int o = (R * 16 + trow(r)) * LDM + J * 16 + tcol();
+                                const double xn = 0.5 * acc[J][r];
+                                const double d = xn - main[o];
+                                main[o] = xn;
+                                bad |= !isfinite(xn);
+                                dsq += d * d;
+                            }
+                    }
+                }
+            }
+            if (!ymode) {
+#pragma unroll 1
+                for (int J = 0; J < NTILE; ++J) {
+                    // panel = (3I - X^T X)[:, J]
+                    for (int I = w; I < NTILE; I += NW) {
+                        const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                              [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int row = I * 16 + trow(r), col = tcol();
                             panel[row * LDP + col] = (row == J * 16 + col ? 3.0 : 0.0) - g[r];
                         }
                     }
-                }
-                __syncthreads();
-                // X_new[:, J] = 0.5 X panel -> staging slot
+                    __syncthreads();
+                    // X_new[:, J] = 0.5 X panel -> staging slot; |X_new - X|^2 on the fly
+                    for (int R = w; R < MT; R += NW) {
+                        const d4v x = tile_mm(NP, [&](int i, int k) { return main[(R * 16 + i) * LDM + k]; },
+                                              [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
 #pragma unroll
-                for (int a = 0; a < RT; ++a) {
-                    const int I = w + a * NW;
-                    if (I < NTILE) {
-                        const d4v x = tile_mm<NP>([&](int i, int k) { return main[(I * 16 + i) * LDM + k]; },
-                                                  [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) ys[(I * 16 + trow(r)) * NP + J * 16 + tcol()] = 0.5 * x[r];
+                        for (int r = 0; r < 4; ++r) {
+                            const int row = R * 16 + trow(r), col = J * 16 + tcol();
+                            const double xn = 0.5 * x[r];
+                            const double d = xn - main[row * LDM + col];
+                            ys[row * NP + col] = xn;
+                            bad |= !isfinite(xn);
+                            dsq += d * d;
+                        }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
             }
-            double dsq = 0.0;
-            int bad = 0;
-            for (int e = tid; e < NP * NP; e += NT) {
-                const double xn = ys[e];
-                const double d = xn - main[(e / NP) * LDM + e % NP];
-                dsq += d * d;
-                bad |= !isfinite(xn);
-            }
-            dsq = block_sum<NT>(dsq, sh.red);
-            bad = block_or<NT>(bad, sh.red);
+            // a non-finite entry makes the step infinite
+            dsq = block_sum<NT>(bad ? INFINITY : dsq, sh.red);
             iters = it + 1;
             const double delta = sqrt(dsq);
-            if (bad || delta > 1e6 * rootn) {  // (:169-171): Gram-Schmidt of A
-                gram_schmidt_g<NT>(N, asv, main, LDM, sh.red);
+            if (!(delta <= 1e6 * rootn)) {  // (:169-171): Gram-Schmidt of A
+                gram_schmidt_g<NT>(M, N, ain, main, LDM, sh.red);
                 diverged = 1;
                 break;
             }
-            for (int e = tid; e < NP * NP; e += NT) main[(e / NP) * LDM + e % NP] = ys[e];
-            __syncthreads();
+            if (!ymode) {
+                for (int e = tid; e < MP * NP; e += NT) main[(e / NP) * LDM + e % NP] = ys[e];
+                __syncthreads();
+            }
             if (delta < kPolarTol * rootn) break;
         }
-        // orthogonality_err (:146-150) of the final X, N x N part
+        // orthogonality_err (:146-150) of the final X: X^T X - I, N x N part
         {
             double s = 0.0;
+            for (int t = w; t < NTILE * NTILE; t += NW) {
+                const int I = t / NTILE, J = t % NTILE;
+                const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                      [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
 #pragma unroll
-            for (int a = 0; a < RT; ++a) {
-                const int I = w + a * NW;
-#pragma unroll
-                for (int J = 0; J < NTILE; ++J)
-                    if (I < NTILE) {
-                        const d4v g = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
-                                                  [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
-#pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = I * 16 + trow(r), col = J * 16 + tcol();
-                            if (row < N && col < N) {
-                                const double d = g[r] - (row == col ? 1.0 : 0.0);
-                                s += d * d;
-                            }
-                        }
+                for (int r = 0; r < 4; ++r) {
+                    const int row = I * 16 + trow(r), col = J * 16 + tcol();
+                    if (row < N && col < N) {
+                        const double d = g[r] - (row == col ? 1.0 : 0.0);
+                        s += d * d;
                     }
+                }
             }
             const double oe = sqrt(block_sum<NT>(s, sh.red));
             // trust test (:174, :178); a diverged iteration is never trusted (:170)
@@ -260,27 +294,23 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         // ---- P = sym(Q^T A) (:305-309); Q parked in the global slot xs
 #pragma unroll 1
         for (int J = 0; J < NTILE; ++J) {
-            for (int e = tid; e < NP * 16; e += NT) {
+            for (int e = tid; e < MP * 16; e += NT) {
                 const int k = e >> 4, j = e & 15;
-                panel[k * LDP + j] = (k < N && J * 16 + j < N) ? asv(k, J * 16 + j) : 0.0;
+                panel[k * LDP + j] = (k < M && J * 16 + j < N) ? ain(k, J * 16 + j) : 0.0;
             }
             __syncthreads();
+            for (int I = w; I < NTILE; I += NW) {
+                const d4v pv = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                       [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
 #pragma unroll
-            for (int a = 0; a < RT; ++a) {
-                const int I = w + a * NW;
-                if (I < NTILE) {
-                    const d4v pv = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
-                                               [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) ys[(I * 16 + trow(r)) * NP + J * 16 + tcol()] = pv[r];
-                }
+                for (int r = 0; r < 4; ++r) ys[(I * 16 + trow(r)) * NP + J * 16 + tcol()] = pv[r];
             }
             __syncthreads();
         }
-        for (int e = tid; e < NP * NP; e += NT) {
+        for (int e = tid; e < MP * NP; e += NT) {  // each element read, then overwritten, by one thread
             const int o = (e / NP) * LDM + e % NP;
             xs[e] = main[o];
-            main[o] = ys[e];
+            if (e < NP * NP) main[o] = ys[e];
         }
         __syncthreads();
         for (int e = tid; e < NP * NP; e += NT) {
@@ -507,53 +537,50 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         vs[(size_t)r * N + sh.rank[i]] = x;  // sorted V
     }
 
-    // ---- U_r = Q V (:317), columns in sorted order, tiles staged in ys
+    // ---- U = Q V (:317), columns in sorted order.  Q's rows are staged
+    // through the LDS panel, `cr` row tiles at a time; tiles go to ys.
+    {
+        const int cr_cap = panel_rows / 16 > 0 ? panel_rows / 16 : 1;
 #pragma unroll 1
-    for (int I = 0; I < NTILE; ++I) {
-        for (int e = tid; e < 16 * NP; e += NT) {
-            const int i = e / NP, k = e % NP;
-            panel[i * LDM + k] = xs[(size_t)(I * 16 + i) * NP + k];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int a = 0; a < RT; ++a) {
-            const int J = w + a * NW;
-            if (J < NTILE) {
-                const d4v uv = tile_mm<NP>([&](int i, int k) { return panel[i * LDM + k]; },
-                                           [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) ys[(I * 16 + trow(r)) * NP + J * 16 + tcol()] = uv[r];
+        for (int I0 = 0; I0 < MT; I0 += cr_cap) {
+            const int cr = MT - I0 < cr_cap ? MT - I0 : cr_cap;
+            for (int e = tid; e < cr * 16 * NP; e += NT) {
+                const int i = e / NP, k = e % NP;
+                panel[i * LDM + k] = xs[(size_t)(I0 * 16 + i) * NP + k];
             }
+            __syncthreads();
+            for (int t = w; t < cr * NTILE; t += NW) {
+                const int ri = t / NTILE, J = t % NTILE;
+                const d4v uv = tile_mm(NP, [&](int i, int k) { return panel[(ri * 16 + i) * LDM + k]; },
+                                       [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+#pragma unroll
+                for (int r = 0; r < 4; ++r) ys[((I0 + ri) * 16 + trow(r)) * NP + J * 16 + tcol()] = uv[r];
+            }
+            __syncthreads();
         }
-        __syncthreads();
     }
-    const int rows_cap = M > NP ? M : NP;
+    const int rows_cap = MP > NP ? MP : NP;
     for (int e = tid; e < rows_cap * NP; e += NT) main[(e / NP) * LDM + e % NP] = 0.0;
     __syncthreads();
-    for (int e = tid; e < N * N; e += NT) {
+    for (int e = tid; e < M * N; e += NT) {
         const int row = e / N, col = e % N;
         main[row * LDM + sh.rank[col]] = ys[row * NP + col];
     }
     __syncthreads();
-    // orthogonality of U_r, repair if needed (:318-320)
+    // orthogonality of U, repair if needed (:318-320)
     double oe = 0.0;
+    for (int t = w; t < NTILE * NTILE; t += NW) {
+        const int I = t / NTILE, J = t % NTILE;
+        const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                              [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
 #pragma unroll
-    for (int a = 0; a < RT; ++a) {
-        const int I = w + a * NW;
-#pragma unroll
-        for (int J = 0; J < NTILE; ++J)
-            if (I < NTILE) {
-                const d4v g = tile_mm<NP>([&](int i, int k) { return main[k * LDM + I * 16 + i]; },
-                                          [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
-#pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int row = I * 16 + trow(r), col = J * 16 + tcol();
-                    if (row < N && col < N) {
-                        const double d = g[r] - (row == col ? 1.0 : 0.0);
-                        oe += d * d;
-                    }
-                }
+        for (int r = 0; r < 4; ++r) {
+            const int row = I * 16 + trow(r), col = J * 16 + tcol();
+            if (row < N && col < N) {
+                const double d = g[r] - (row == col ? 1.0 : 0.0);
+                oe += d * d;
             }
+        }
     }
     oe = sqrt(block_sum<NT>(oe, sh.red));
     clk.mark(kPhU);
@@ -565,17 +592,16 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         const double stol = 1e-12 * fmax(smax, 1.0);
         for (int i = tid; i < N; i += NT) sh.scr[sh.rank[i]] = fmax(sh.diag[i], 0.0);
         __syncthreads();
-        wg_gemm<NT>(N, N, N, asv, View{vs, N, 0}, [&](int i, int j, double acc) {
+        wg_gemm<NT>(M, N, N, ain, View{vs, N, 0}, [&](int i, int j, double acc) {
             const double sj = sh.scr[j];
             xs[(size_t)i * N + j] = sj > stol ? acc / sj : 0.0;
         });
         __syncthreads();
-        gram_schmidt_g<NT>(N, View{xs, N, 0}, main, LDM, sh.red);
+        gram_schmidt_g<NT>(M, N, View{xs, N, 0}, main, LDM, sh.red);
         info.repaired = 1;
     }
     __syncthreads();
     clk.mark(kPhRepair);
-    if (tall) apply_q_g<NT>(M, N, qs, main, LDM, sh.tau, sh.scr);
     if (!wide) {
         double* u = P.u + itm.u_off;
         for (int e = tid; e < M * N; e += NT) u[(size_t)(e / N) * N + e % N] = main[(e / N) * LDM + e % N];
@@ -595,13 +621,14 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
 template <int NP>
 __global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __restrict__ items, const int* count_ptr,
                                                                   int* counter, Pools pools, double* gws,
-                                                                  int64_t gws_stride) {
+                                                                  int64_t gws_stride, int lds_doubles) {
     extern __shared__ double dyn_lds[];
     __shared__ Sh2<NP> sh;
     const int count = *count_ptr;
+    const int64_t part = gws_stride / 3;
     double* xs = gws + (size_t)blockIdx.x * gws_stride;
-    double* vs = xs + (size_t)NP * NP;
-    double* ys = vs + (size_t)NP * NP;
+    double* vs = xs + part;
+    double* ys = vs + part;
     for (;;) {
         if (threadIdx.x == 0) sh.item = atomicAdd(counter, 1);
         __syncthreads();
@@ -610,10 +637,12 @@ __global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __
         if (it >= count) break;
         const Item itm = items[it];
         const int M = itm.m > itm.n ? itm.m : itm.n;
-        const int rows = M > NP ? M : NP;
+        const int MP = (M + 15) & ~15;
+        const int rows = MP > NP ? MP : NP;
         double* main = dyn_lds;
         double* panel = dyn_lds + (size_t)rows * Cfg2<NP>::LDM;
-        solve_item2<NP>(itm, pools, main, panel, xs, vs, ys, sh);
+        const int panel_rows = (lds_doubles - rows * Cfg2<NP>::LDM) / Cfg2<NP>::LDM;
+        solve_item2<NP>(itm, pools, main, panel, panel_rows, xs, vs, ys, sh);
     }
 }
 
@@ -625,7 +654,7 @@ static hipError_t launch2(const SmallLaunch& L, hipStream_t st) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((small_svd2_kernel<NP>), dim3(L.grid), dim3(Cfg2<NP>::NT), L.lds_bytes, st, L.items,
-                       L.count_ptr, L.counter, L.pools, L.gws, L.gws_stride);
+                       L.count_ptr, L.counter, L.pools, L.gws, L.gws_stride, (int)(L.lds_bytes / sizeof(double)));
     return hipGetLastError();
 }
 
@@ -650,16 +679,27 @@ int v2_threads(int np) {
     return 0;
 }
 
+// LDS of one workgroup: the matrix area (max(16 ceil(M/16), NP) rows of
+// LDM) and the panel (the larger of a K x 16 operand panel and 16 rows of
+// LDM).
 size_t v2_lds_bytes(int np, int M) {
-    const int rows = M > np ? M : np;
-    switch (np) {
-#define X(np_) \
-    case np_:  \
-        return sizeof(double) * ((size_t)rows * Cfg2<np_>::LDM + Cfg2<np_>::PANEL);
-        LSVD_V2_SIZES(X)
-#undef X
-    }
-    return 0;
+    const int mp = (M + 15) & ~15;
+    const int rows = mp > np ? mp : np;
+    const int ldm = np + 1;
+    const int panel = rows * 17 > 16 * ldm ? rows * 17 : 16 * ldm;
+    const int ypanel = panel > np * ldm ? panel : np * ldm;  // room for Y (NP x LDM)
+    const size_t with_y = sizeof(double) * ((size_t)rows * ldm + ypanel);
+    if (np <= kV2YModeMaxNP && with_y <= kV2LdsBudget) return with_y;
+    return sizeof(double) * ((size_t)rows * ldm + panel);
+}
+
+// Global slot of one workgroup: three parts (parked Q, sorted V, staging)
+// of max(16 ceil(M/16), NP) x NP doubles each, 256-byte aligned.
+int64_t v2_slot_doubles(int np, int M) {
+    const int mp = (M + 15) & ~15;
+    const int rows = mp > np ? mp : np;
+    const int64_t part = ((int64_t)rows * np + 31) / 32 * 32;
+    return 3 * part;
 }
 
 hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st) {

@@ -13,13 +13,23 @@ Also reported: the per-launch roofline of the dominant kernel (algorithmic
 FP64 flops from the oracle's iteration counts / event-timed launch time),
 the sigma error against the imposed spectra, and the CPU oracle (the
 restatement of the reference algorithm) timed on all host cores on the
-same workload.
+same workload.  Further legs, each with its CPU oracle beside it:
+  single   one 64 x 64 N(0,1) LieSvdSmall solve (latency);
+  polar    NS polar of 4096 x 4096 and 8192 x 2048 (FP64 vs BF16/FP32 MFMA
+           iterations with FP64 refinement), polar TFLOP/s;
+  joint    Phase-JADE of 32 families x 64 symmetric 256 x 256 matrices,
+           GB/s of the layer traffic, off-diagonal residual, recovery of
+           the planted basis;
+  vjp      gradient of L = sum w_i s_i + <G, U V^T> for 512 96 x 64
+           matrices with gaps ~1e-3 plus 32 with one 1e-8 gap, against the
+           analytic FP64 gradient and central differences.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f64]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import os
 import sys
@@ -46,6 +56,185 @@ def algorithmic_flops(m: int, n: int, k_ns: int, sweeps: int) -> float:
     if N <= 4 and m == n:
         return 0.0
     return 4.0 * M * N * N * (M > N) + (4.0 * k_ns + 4.0 + 6.0 * sweeps) * N ** 3
+
+
+def kernel_name(variant: int) -> str:
+    if variant >= 100:
+        return f"small_svd2_kernel<{variant - 100}>"
+    return "micro_svd_kernel" if variant < 0 else f"small_svd_kernel variant {variant}"
+
+
+def cpu_threads() -> int:
+    return int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+
+def single_leg(h, dev, args):
+    """One 64 x 64 N(0,1) matrix through LieSvdSmall::solve on device
+    pointers: median latency over repeated calls, sigma vs the oracle."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib, metrics, workloads
+    a = workloads.single_64x64()
+    n = a.shape[0]
+    ad = torch.from_numpy(a).to(dev)
+    u = torch.empty((n, n), dtype=torch.float64, device=dev)
+    s = torch.empty(n, dtype=torch.float64, device=dev)
+    vt = torch.empty((n, n), dtype=torch.float64, device=dev)
+    fn = _lib.lib().lsvd_small_solve_f64
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+
+    def call():
+        h.check(fn(h.raw, ptr(ad), n, ptr(u), ptr(s), ptr(vt), None))
+
+    for _ in range(5):
+        call()
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.single_reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        call()
+        e1.record()
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1))
+    ms = float(np.median(times))
+    uh, sh, vth = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+    ru, rs, rvt, rinfo = oracle.small_solve(a)
+    ct = []
+    for _ in range(6):
+        t0 = time.perf_counter()
+        oracle.small_solve(a)
+        ct.append(time.perf_counter() - t0)
+    cpu_ms = float(np.median(ct[1:])) * 1e3
+    e_rel, e_abs = metrics.sigma_errors(sh, rs)
+    mm = metrics.compute(a, uh, sh, vth)
+    return {"ms_per_matrix": round(ms, 4), "sigma_vs_oracle": {"rel": e_rel, "abs": e_abs},
+            "rel_recon": mm.rel_recon, "orth_u": mm.orth_u, "orth_v": mm.orth_v,
+            "oracle_counts": {"ns_iters": rinfo["ns_iters"], "jacobi_sweeps": rinfo["jacobi_sweeps"]},
+            "cpu_baseline": {"value": round(cpu_ms, 4), "unit": "ms/matrix", "cores": 1, "kind": "port",
+                             "sample": "the same matrix, C restatement of LieSvdSmall::solve, median of 5"}}
+
+
+def joint_leg(h, dev, args):
+    """Phase-JADE (LieSvdJoint::diagonalize_symmetric) of 32 families x K=64
+    symmetric 256 x 256 matrices on device pointers.  Algorithmic traffic:
+    every entry of the K matrices read and written once per layer,
+    16 K n^2 (n - 1) bytes per sweep per family."""
+    import torch
+    from scipy.optimize import linear_sum_assignment
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib, workloads
+    from lie_cliffalg_analog_svd_amd.joint import JointDiagonalizationParams
+    F, K, n = args.joint_families, 64, 256
+    t0 = time.time()
+    fams, qs, _ = workloads.jade_batch(F, K, n)
+    log(f"joint: {F} families generated in {time.time() - t0:.1f}s")
+    md = torch.from_numpy(fams).to(dev)
+    vd = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+    dd = torch.empty((F, K, n), dtype=torch.float64, device=dev)
+    prm = JointDiagonalizationParams.for_n(n)._c()
+    tr = (_lib.LsvdJointTrace * F)()
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+
+    def run():  # the bench handle is in device pointer mode
+        h.check(_lib.lib().lsvd_joint_diag_sym_f64(h.raw, F, K, n, ptr(md), C.byref(prm), ptr(vd), ptr(dd), tr))
+
+    run()  # warm-up (kernel load, graph build)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    run()
+    torch.cuda.synchronize()
+    sec = time.perf_counter() - t1
+    traces = [tr[f].as_dict() for f in range(F)]
+    sweeps = [t["sweeps"] for t in traces]
+    layer_bytes = 16.0 * K * n * n * (n - 1)
+    total_bytes = layer_bytes * float(sum(sweeps))
+    v = vd.cpu().numpy()
+    rec = []
+    for f in range(F):
+        c = np.abs(qs[f].T @ v[f])
+        r, cidx = linear_sum_assignment(-c)
+        rec.append(float(np.max(np.abs(1.0 - c[r, cidx]))))
+    rel_off = [t["final_offdiag"] / max(t["initial_offdiag"], 1e-300) for t in traces]
+    # CPU oracle: one sweep of one family per thread, extrapolated by sweeps
+    threads = min(cpu_threads(), F)
+    one = oracle.JointParams(1, prm.tol, prm.min_pair_scale, prm.line_search_steps)
+    t2 = time.perf_counter()
+    oracle.joint_batch(fams[:threads], one, threads=threads)
+    cpu_sweep = time.perf_counter() - t2
+    cpu_est = cpu_sweep * float(np.mean(sweeps)) * (F / threads)
+    return {"seconds": round(sec, 4), "families": F, "K": K, "n": n,
+            "sweeps": {"min": int(min(sweeps)), "max": int(max(sweeps)), "mean": float(np.mean(sweeps))},
+            "gbs": round(total_bytes / sec / 1e9, 2), "frac_of_hbm": round(total_bytes / sec / 1e9 / HBM_PEAK_GBS, 5),
+            "final_offdiag_rel_max": max(rel_off), "recovery_err_max": max(rec),
+            "rejected_rotations": int(sum(t["rejected_rotations"] for t in traces)),
+            "cpu_baseline": {"value": round(F / cpu_est, 5), "unit": "families/s", "cores": threads, "kind": "port",
+                             "sample": f"one sweep of {threads} families (one per thread) of the C restatement, "
+                                       f"{cpu_sweep:.2f}s, scaled by the GPU run's mean sweep count and "
+                                       f"{F}/{threads} families"},
+            "gpu_families_per_s": round(F / sec, 3)}
+
+
+def vjp_leg(dev, args):
+    """Gradient of L = sum_i w_i s_i + <G, U V^T> for 512 96 x 64 matrices
+    with singular gaps in [1e-3, 2e-3] plus 32 with one gap of 1e-8."""
+    import oracle
+    from lie_cliffalg_analog_svd_amd import adjoint, rng, solvers, workloads
+    from lie_cliffalg_analog_svd_amd.handle import Handle
+    h = Handle(0)  # host pointer mode: these entry points take numpy arrays
+    m, n = 96, 64
+    seed = workloads.SEED_VJP
+    mats, ws, gs = [], [], []
+    for b in range(512 + 32):
+        sb = rng.mix_seed(seed, b)
+        u = rng.haar(rng.mix_seed(sb, 0), m, n)
+        v = rng.haar(rng.mix_seed(sb, 1), n)
+        if b < 512:
+            gaps = 1e-3 + 1e-3 * rng.uniform(rng.mix_seed(sb, 2), n)
+            sig = 1.0 + np.concatenate([np.cumsum(gaps[::-1])[::-1][1:], [0.0]])
+        else:
+            sig = np.linspace(2.0, 1.0, n)
+            sig[n // 2] = sig[n // 2 - 1] - 1e-8
+        mats.append((u * sig) @ v.T)
+        ws.append(rng.uniform(rng.mix_seed(sb, 3), n))
+        gs.append(rng.normal(rng.mix_seed(sb, 4), m * n).reshape(m, n))
+    a = np.stack(mats); w = np.stack(ws); g = np.stack(gs)
+    adjoint.svd_loss_grad(a[:4], w[:4], g[:4], handle=h)  # warm-up
+    t0 = time.perf_counter()
+    grad = adjoint.svd_loss_grad(a, w, g, handle=h)
+    sec = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    ref = np.stack([oracle.svd_loss_grad(a[b], w[b], g[b])[1] for b in range(len(a))])
+    cpu_sec = time.perf_counter() - t1
+    def rel(sl):
+        return float(np.max(np.abs(grad[sl] - ref[sl])) / max(np.max(np.abs(ref[sl])), 1e-300))
+    err_analytic = rel(slice(0, 512))
+    err_analytic_gap = rel(slice(512, None))
+    # central differences along one random direction per matrix (gap-1e-3 set)
+    hstep = 1e-6
+    e = np.stack([rng.normal(rng.mix_seed(seed, 10_000 + b), m * n).reshape(m, n) for b in range(512)])
+    e /= np.linalg.norm(e, axis=(1, 2), keepdims=True)
+
+    def loss(x):
+        outs = solvers.svd_batched(list(x), handle=h)
+        return np.array([float(w[b] @ outs[b][1] + np.sum(g[b] * (outs[b][0] @ outs[b][2])))
+                         for b in range(len(x))])
+
+    fd = (loss(a[:512] + hstep * e) - loss(a[:512] - hstep * e)) / (2 * hstep)
+    an = np.einsum("bij,bij->b", grad[:512], e)
+    h.close()
+    return {"matrices": len(a), "seconds_gpu_host_io": round(sec, 4),
+            "max_rel_err_vs_analytic_fp64": err_analytic,
+            "max_rel_err_vs_analytic_fp64_gap_1e-8_set": err_analytic_gap,
+            "max_abs_err_vs_central_diff": float(np.max(np.abs(fd - an))),
+            "finite_on_gap_1e-8_set": bool(np.all(np.isfinite(grad[512:]))),
+            "cpu_baseline": {"value": round(len(a) / cpu_sec, 1), "unit": "gradients/s", "cores": 1,
+                             "kind": "port", "sample": "analytic FP64 gradient (forward SVD + closed form) "
+                                                       "of all 544 matrices in the C oracle"},
+            "gpu_gradients_per_s_host_io": round(len(a) / sec, 1)}
 
 
 def polar_leg(h, dev, args):
@@ -90,6 +279,19 @@ def polar_leg(h, dev, args):
             }
             log(f"polar {m}x{n} {name}: {entry[name]}")
             del qd
+        # CPU oracle: NS iterations at 1024 wide, scaled by flops (4 m n^2)
+        import oracle
+        threads = cpu_threads()
+        xs = np.ascontiguousarray(np.random.default_rng(0).standard_normal((1024 * m // n, 1024)) / 64.0)
+        t2 = time.perf_counter()
+        oracle.ns_iterations(xs, 2, threads=threads)
+        cpu_it = (time.perf_counter() - t2) / 2 * (m * n * n) / (xs.shape[0] * 1024.0 * 1024.0)
+        f64_iters = entry["f64"]["lowp_iters"] + entry["f64"]["f64_iters"]
+        entry["cpu_baseline"] = {"value": round(1.0 / (cpu_it * f64_iters), 5), "unit": "polar/s", "cores": threads,
+                                 "kind": "port",
+                                 "sample": f"2 NS iterations of a {xs.shape[0]}x1024 matrix in the C restatement "
+                                           f"on {threads} threads, scaled by 4mn^2 to {m}x{n} and by the "
+                                           f"{f64_iters} FP64 iterations"}
         entry["speedup_bf16_vs_f64"] = round(entry["f64"]["seconds"] / entry["bf16"]["seconds"], 3)
         entry["speedup_f32_vs_f64"] = round(entry["f64"]["seconds"] / entry["f32"]["seconds"], 3)
         res[f"{m}x{n}"] = entry
@@ -109,14 +311,32 @@ def main(argv=None) -> int:
     ap.add_argument("--no-tune", action="store_true", help="skip the autotuner")
     ap.add_argument("--no-polar", action="store_true", help="skip the large polar leg")
     ap.add_argument("--polar-reps", type=int, default=2)
+    ap.add_argument("--timing-runs", type=int, default=5, help="eager runs averaged for per-launch times")
+    ap.add_argument("--no-single", action="store_true", help="skip the single 64x64 leg")
+    ap.add_argument("--single-reps", type=int, default=50)
+    ap.add_argument("--no-joint", action="store_true", help="skip the Phase-JADE leg")
+    ap.add_argument("--joint-families", type=int, default=32)
+    ap.add_argument("--no-vjp", action="store_true", help="skip the vjp leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy a MIB-MiB device buffer 3 times (HBM counter calibration) and exit")
     args = ap.parse_args(argv)
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); "
-              "the batch shards by matrix with no collective, see DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for --gpus {args.gpus}); the batch would shard "
+              "by matrix across GPUs with no collective, so one GPU's throughput is the figure", file=sys.stderr)
         return 2
 
     import torch
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}))
+        return 0
 
     import lie_cliffalg_analog_svd_amd as L
     from lie_cliffalg_analog_svd_amd import _lib, metrics, workloads
@@ -176,8 +396,11 @@ def main(argv=None) -> int:
     log(f"timed: {args.steps} steps, {ms_per_step:.3f} ms/step (wall {wall / args.steps * 1e3:.3f} ms/step)")
 
     # ---- per-launch times of each bucket (HIP events on the plan's stream)
+    # each launch timed while the other lanes run beside it, averaged over
+    # several eager runs of the same schedule
     descs, order = plan.describe()
-    bucket_ms = plan.time_buckets(a_pool, u_pool, s_pool, vt_pool)
+    runs = [plan.time_buckets(a_pool, u_pool, s_pool, vt_pool) for _ in range(args.timing_runs)]
+    bucket_ms = [float(np.mean([r[i] for r in runs])) for i in range(len(runs[0]))]
     torch.cuda.synchronize()
 
     # ---- where the cycles go (per-phase clock64 sums over the batch)
@@ -242,16 +465,27 @@ def main(argv=None) -> int:
     tf_path = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tf_path):
         try:
-            traffic = json.load(open(tf_path)).get(f"batch_variant{d['variant']}_maxn{d['max_n']}")
+            traffic = json.load(open(tf_path)).get(f"variant{d['variant']}")
         except Exception:
             traffic = None
     roofline = {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(achieved / FP64_PEAK_TFLOPS, 5), "traffic": traffic,
-                "kernel": f"small_svd_kernel variant {d['variant']} (max N {d['max_n']}, {d['count']} matrices, "
+                "kernel": f"{kernel_name(d['variant'])} (max N {d['max_n']}, {d['count']} matrices, "
                           f"{d['threads']} threads x {d['grid']} workgroups)",
                 "launch_ms": round(bucket_ms[dom], 4)}
 
-    polar_res = None if args.no_polar else polar_leg(h, dev, args)
+    legs = {}
+    if not args.no_single:
+        legs["single"] = single_leg(h, dev, args)
+        log(f"single: {legs['single']}")
+    if not args.no_polar:
+        legs["polar"] = polar_leg(h, dev, args)
+    if not args.no_joint:
+        legs["joint"] = joint_leg(h, dev, args)
+        log(f"joint: {legs['joint']}")
+    if not args.no_vjp:
+        legs["vjp"] = vjp_leg(dev, args)
+        log(f"vjp: {legs['vjp']}")
 
     out = {
         "metric": "batched SVD matrices/s + polar TFLOP/s, chosen vs plain FP64, at stated sigma err",
@@ -277,7 +511,7 @@ def main(argv=None) -> int:
         "phase_split": phase_split,
         "roofline": roofline,
         "cpu_baseline": cpu,
-        "polar": polar_res,
+        "legs": legs,
     }
     print(json.dumps(out), flush=True)
     plan.close()

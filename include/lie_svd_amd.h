@@ -243,16 +243,18 @@ typedef struct {
  * NULL) receives, for every queue entry, the caller's matrix index. */
 int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, int* nbuckets, int32_t* order);
 
-/* Launches every bucket on its own with HIP events around it on the
- * handle's stream and writes each launch's time in ms (same order as
- * describe).  Pool pointers follow the pointer mode of the plan. */
+/* Runs the plan once without a graph, on the same concurrent lanes as the
+ * graph, with HIP events recorded on each lane stream around every launch,
+ * and writes each launch's time in ms (same order as describe): the
+ * duration of a bucket while the others run beside it.  Device pointer
+ * mode only. */
 int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
                                  double* vt_pool, float* ms, int max);
 
 /* Runs the plan once (no graph) with per-matrix phase clocks: `clocks` is
  * device memory of count x 8 int64, zeroed by the caller; entry [b][ph]
- * receives the clock64() cycles matrix b spent in phase ph (0 QR front
- * end, 1 Newton-Schulz, 2 P = Q^T A, 3 Jacobi, 4 U = Q V + check,
+ * receives the clock64() cycles matrix b spent in phase ph (0 front end
+ * (QR for N > 128), 1 Newton-Schulz, 2 P = Q^T A, 3 Jacobi, 4 U = Q V + check,
  * 5 repair, 6 output).  Device pointer mode only. */
 int lsvd_batch_plan_phase_clocks(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
                                  double* vt_pool, int64_t* clocks);

@@ -317,7 +317,7 @@ hipError_t plan_ensure_streams(lsvd_plan_t p) {
 // with the least estimated work, so a lane never queues a long bucket
 // behind another one while the CUs a large-N bucket leaves idle take the
 // small-N buckets.  The micro escalations drain after the join.
-hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
+hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const hipEvent_t* tev) {
     hipError_t e = launch_reset_counters(p->d_counters, p->n_counters, st);
     if (e != hipSuccess) return e;
     const int nb = (int)p->buckets.size();
@@ -353,13 +353,18 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st) {
         L.pools = pools;
         L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
         L.gws_stride = bk.gws_stride;
+        if (tev && (e = hipEventRecord(tev[2 * (i + 1)], p->side[l])) != hipSuccess) return e;
         e = launch_small_variant(bk.variant, L, p->side[l]);
         if (e != hipSuccess) return e;
+        if (tev && (e = hipEventRecord(tev[2 * (i + 1) + 1], p->side[l])) != hipSuccess) return e;
     }
     if (p->n_micro > 0) {
+        const int l = least();
         MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, fb_count};
-        e = launch_micro(ml, p->side[least()]);
+        if (tev && (e = hipEventRecord(tev[0], p->side[l])) != hipSuccess) return e;
+        e = launch_micro(ml, p->side[l]);
         if (e != hipSuccess) return e;
+        if (tev && (e = hipEventRecord(tev[1], p->side[l])) != hipSuccess) return e;
     }
     for (int l = 0; l < lanes; ++l) {  // join
         e = hipEventRecord(p->ev[l + 1], p->side[l]);
@@ -735,47 +740,29 @@ int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_
     if (p->ptr_mode != LSVD_PTR_DEVICE) return set_error(h, LSVD_E_ARG, "time_buckets needs device pointers");
     hipStream_t st = h->stream;
     Pools pools{a_pool, u_pool, s_pool, vt_pool, nullptr};
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
-        return set_error(h, LSVD_E_HIP, "hipEventCreate");
     const int nb = (int)p->buckets.size();
-    int slot = 0;
+    // one eager run of the same lane schedule as the graph, with start/stop
+    // events around every launch: each duration is measured while the other
+    // lanes run beside it, as in the graph
+    std::vector<hipEvent_t> ev(2 * (nb + 1), nullptr);
     int rc = LSVD_OK;
-    auto timed = [&](auto&& launch) {
-        hipError_t e = launch_reset_counters(p->d_counters, p->n_counters, st);
-        if (e == hipSuccess) e = hipEventRecord(e0, st);
-        if (e == hipSuccess) e = launch();
-        if (e == hipSuccess) e = hipEventRecord(e1, st);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float t = 0.f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+    for (auto& x : ev)
+        if (hipEventCreate(&x) != hipSuccess) rc = set_error(h, LSVD_E_HIP, "hipEventCreate");
+    if (rc == LSVD_OK) {
+        hipError_t e = plan_enqueue(p, pools, st, ev.data());
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) rc = hip_fail(h, e, "bucket timing");
+    }
+    int slot = 0;
+    for (int i = (p->n_micro > 0 ? 0 : 1); i <= nb && rc == LSVD_OK; ++i) {
+        float t = 0.f;
+        hipError_t e = hipEventElapsedTime(&t, ev[2 * i], ev[2 * i + 1]);
+        if (e != hipSuccess) rc = hip_fail(h, e, "hipEventElapsedTime");
         if (slot < max) ms[slot] = t;
         ++slot;
-    };
-    if (p->n_micro > 0) {
-        timed([&]() {
-            MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, p->d_counters + nb + 1};
-            return launch_micro(ml, st);
-        });
     }
-    for (int i = 0; i < nb && rc == LSVD_OK; ++i) {
-        const Bucket& bk = p->buckets[i];
-        timed([&]() {
-            SmallLaunch L{};
-            L.grid = bk.grid;
-            L.lds_bytes = bk.lds_bytes;
-            L.items = p->d_items + bk.first;
-            L.count_ptr = p->d_consts + i;
-            L.counter = p->d_counters + i;
-            L.pools = pools;
-            L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
-            L.gws_stride = bk.gws_stride;
-            return launch_small_variant(bk.variant, L, st);
-        });
-    }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    for (auto& x : ev)
+        if (x) hipEventDestroy(x);
     return rc;
 }
 

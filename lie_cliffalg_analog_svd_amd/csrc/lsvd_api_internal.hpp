@@ -77,7 +77,10 @@ struct lsvd_plan_s {
 };
 
 namespace lsvd {
-hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st);
+// `tev`, when given, holds 2 * (buckets + 1) timing events: a start/stop
+// pair recorded on the lane stream around the micro launch (pair 0) and
+// around each bucket's launch (pair 1 + i).
+hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const hipEvent_t* tev = nullptr);
 }
 
 // Frees the handle's polar / joint / vjp state (defined with those paths).

@@ -12,7 +12,7 @@
 // near-degenerate gap; it is guarded as d / (d^2 + eps^2) with
 // eps = guard * s_max (and 1/s as s / (s^2 + eps^2)).  For losses that are
 // invariant to the singular-vector gauge (e.g. <G, U Vt>) Om_U + Om_V = 0
-// exactly and the guard never enters.
+// exactly and the guard never enters; the polar mode drops that term.
 //
 // One workgroup per matrix; the n x n and m x n intermediates live in a
 // per-matrix global workspace (L2 resident) and the products use the
@@ -88,7 +88,10 @@ __global__ __launch_bounds__(NT) void svd_vjp_kernel(VjpArgs a) {
             const double ou = 0.5 * (JU[(size_t)i * n + j] - JU[(size_t)j * n + i]);
             const double ov = 0.5 * (JV[(size_t)i * n + j] - JV[(size_t)j * n + i]);
             const double d = sj - si, su = sj + si;
-            mij = (ou + ov) * d / (d * d + eps2) + (ou - ov) * su / (su * su + eps2);
+            // for the polar factor Om_U + Om_V vanishes identically; its
+            // rounding residue is left out instead of being divided by a gap
+            const double dif = a.qbar ? 0.0 : (ou + ov) * d / (d * d + eps2);
+            mij = dif + (ou - ov) * su / (su * su + eps2);
         }
         Nm[(size_t)i * n + j] = mij - JU[(size_t)i * n + j] * inv_sj;
     }

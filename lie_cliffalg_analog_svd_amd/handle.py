@@ -164,7 +164,8 @@ class BatchPlan:
         return [arr[i].as_dict() for i in range(nb.value)], order
 
     def time_buckets(self, a_pool, u_pool, s_pool, vt_pool):
-        """Event-timed launch of every bucket on its own (device pointers)."""
+        """Per-launch times (ms) of every bucket, each measured with HIP events
+        on its lane while the other lanes run beside it (device pointers)."""
         descs, _ = self.describe()
         ms = (C.c_float * max(1, len(descs)))()
         check(self.handle.raw, lib().lsvd_batch_plan_time_buckets(

@@ -393,3 +393,27 @@ def test_planted_mistake_is_caught():
     _, e_bad, _ = oracle.eigh_jacobi_full(np.triu(p_bad) + np.triu(p_bad, 1).T)  # no symmetrization
     want = np.linalg.svd(a, compute_uv=False)
     assert np.max(np.abs(e_bad - want)) > 1e-9
+
+
+@pytest.mark.parametrize("m,n,seed", [(12, 8, 1), (9, 9, 2), (20, 5, 3)])
+def test_svd_loss_grad_value_and_central_differences(m, n, seed):
+    """The oracle's analytic gradient of L = sum w_i s_i + <G, U V^T>:
+    the loss value against LAPACK (sigma and the polar factor U V^T are
+    unique for distinct, nonzero singular values) and the gradient against
+    central differences of the oracle's own loss along random directions."""
+    rng = np.random.default_rng(seed)
+    u, _ = np.linalg.qr(rng.standard_normal((m, n)))
+    v, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    s = np.linspace(2.0, 1.0, n)
+    a = (u * s) @ v.T
+    w = rng.uniform(size=n)
+    g = rng.standard_normal((m, n))
+    loss, grad = oracle.svd_loss_grad(a, w, g)
+    uu, ss, vvt = np.linalg.svd(a, full_matrices=False)
+    assert abs(loss - (w @ ss + np.sum(g * (uu @ vvt)))) < 1e-12 * (1 + abs(loss))
+    h = 1e-6
+    for _ in range(4):
+        e = rng.standard_normal((m, n))
+        e /= np.linalg.norm(e)
+        fd = (oracle.svd_loss_grad(a + h * e, w, g)[0] - oracle.svd_loss_grad(a - h * e, w, g)[0]) / (2 * h)
+        assert abs(fd - np.sum(grad * e)) < 1e-7 * (1 + np.abs(grad).max())

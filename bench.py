@@ -24,7 +24,11 @@ same workload.  Further legs, each with its CPU oracle beside it:
            matrices with gaps ~1e-3 plus 32 with one 1e-8 gap, against the
            analytic FP64 gradient and central differences.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision f64]
+The batch value is the chosen schedule (FP64 Newton-Schulz with scaled
+steps while small singular values grow, LSVD_PREC_F64_SCALED); the plain
+FP64 reference iteration is timed beside it on the same workload.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision scaled|f64]
 """
 from __future__ import annotations
 
@@ -240,10 +244,13 @@ def vjp_leg(dev, args):
 def polar_leg(h, dev, args):
     """Large NS polar: 4096 x 4096 and 8192 x 2048, A = U diag(s) V^T with Haar
     factors and s log-uniform in [1e-6, 1].  Plain FP64 vs BF16 MFMA
-    iterations + FP64 refinement; polar TFLOP/s = iterations x 4 m n^2 / time."""
+    iterations + FP64 refinement, and FP64 with the scaled step schedule; polar
+    TFLOP/s = (plain FP64 iterations) x 4 m n^2 / time, the same work for
+    every variant, so the ratios are time ratios."""
     import torch
 
-    from lie_cliffalg_analog_svd_amd import LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, polar, workloads
+    from lie_cliffalg_analog_svd_amd import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_SCALED, polar,
+                                             workloads)
     res = {}
     for (m, n) in [(4096, 4096), (8192, 2048)]:
         t0 = time.time()
@@ -253,7 +260,8 @@ def polar_leg(h, dev, args):
         ad = torch.from_numpy(a).to(dev)
         entry = {}
         q64 = None
-        for name, prec in [("f64", LSVD_PREC_F64), ("bf16", LSVD_PREC_BF16), ("f32", LSVD_PREC_F32)]:
+        for name, prec in [("f64", LSVD_PREC_F64), ("f64_scaled", LSVD_PREC_F64_SCALED), ("bf16", LSVD_PREC_BF16),
+                           ("f32", LSVD_PREC_F32)]:
             qd = torch.empty((m, n), dtype=torch.float64, device=dev)
             info = polar.polar_device(h, ad, m, n, qd, prec)  # warm-up + counts
             torch.cuda.synchronize()
@@ -266,7 +274,11 @@ def polar_leg(h, dev, args):
                 times.append(time.perf_counter() - t1)
             sec = min(times)
             iters = info["lowp_iters"] + info["f64_iters"]
-            flops = iters * 4.0 * m * n * n
+            if name == "f64":
+                ref_iters = iters
+            # polar TFLOP/s counts the reference iteration's work (4 m n^2 per
+            # plain FP64 step), so fewer steps at the same limit show as speed
+            flops = ref_iters * 4.0 * m * n * n
             q = qd.cpu().numpy()
             if name == "f64":
                 q64 = q
@@ -292,6 +304,7 @@ def polar_leg(h, dev, args):
                                  "sample": f"2 NS iterations of a {xs.shape[0]}x1024 matrix in the C restatement "
                                            f"on {threads} threads, scaled by 4mn^2 to {m}x{n} and by the "
                                            f"{f64_iters} FP64 iterations"}
+        entry["speedup_f64_scaled_vs_f64"] = round(entry["f64"]["seconds"] / entry["f64_scaled"]["seconds"], 3)
         entry["speedup_bf16_vs_f64"] = round(entry["f64"]["seconds"] / entry["bf16"]["seconds"], 3)
         entry["speedup_f32_vs_f64"] = round(entry["f64"]["seconds"] / entry["f32"]["seconds"], 3)
         res[f"{m}x{n}"] = entry
@@ -304,7 +317,8 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="f64", choices=["f64", "f32", "bf16"])
+    ap.add_argument("--precision", default="scaled", choices=["scaled", "f64"],
+                    help="batch NS schedule: scaled FP64 steps (chosen) or the reference iteration")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--tiny", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -342,7 +356,7 @@ def main(argv=None) -> int:
     from lie_cliffalg_analog_svd_amd import _lib, metrics, workloads
     from lie_cliffalg_analog_svd_amd.handle import BatchPlan, Handle, info_array
 
-    prec = {"f64": L.LSVD_PREC_F64, "f32": L.LSVD_PREC_F32, "bf16": L.LSVD_PREC_BF16}[args.precision]
+    prec = {"f64": L.LSVD_PREC_F64, "scaled": L.LSVD_PREC_F64_SCALED}[args.precision]
     t0 = time.time()
     batch = workloads.ragged_batch(args.count, args.tiny)
     mats = batch.mats
@@ -394,6 +408,26 @@ def main(argv=None) -> int:
     ms_per_step = ev0.elapsed_time(ev1) / args.steps
     value = count / (ms_per_step / 1e3)
     log(f"timed: {args.steps} steps, {ms_per_step:.3f} ms/step (wall {wall / args.steps * 1e3:.3f} ms/step)")
+
+    # ---- the plain FP64 reference iteration on the same workload, same count
+    plain = None
+    if prec != L.LSVD_PREC_F64:
+        plan0 = BatchPlan(h, ms, ns, L.LSVD_PREC_F64)
+        u0 = torch.zeros_like(u_pool); s0 = torch.zeros_like(s_pool); vt0 = torch.zeros_like(vt_pool)
+        for _ in range(args.warmup + 1):
+            plan0.execute(a_pool, u0, s0, vt0)
+        torch.cuda.synchronize()
+        ev0.record()
+        for _ in range(args.steps):
+            plan0.execute(a_pool, u0, s0, vt0)
+        ev1.record()
+        torch.cuda.synchronize()
+        ms0 = ev0.elapsed_time(ev1) / args.steps
+        d_s = float((s0 - s_pool).abs().max() / s_pool.abs().max().clamp_min(1e-300))
+        plain = {"ms_per_step": round(ms0, 4), "value": round(count / (ms0 / 1e3), 1),
+                 "chosen_speedup": round(ms0 / ms_per_step, 3), "max_sigma_diff_vs_chosen_over_max": d_s}
+        log(f"plain FP64 iteration: {ms0:.3f} ms/step, chosen speedup {ms0 / ms_per_step:.3f}")
+        plan0.close()
 
     # ---- per-launch times of each bucket (HIP events on the plan's stream)
     # each launch timed while the other lanes run beside it, averaged over
@@ -498,11 +532,12 @@ def main(argv=None) -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"f64": "f64", "f32": "f32+f64", "bf16": "bf16+f64"}[args.precision],
+        "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "batched uneven-size SVD: 4096 f64 m~U[8,128], n~U[4,m], kappa 10^U[0,12], "
                                "10% clustered + 8192 square 2..4 (LieSvdMicro), one hipGraph launch",
-                   "matrices": count, "precision": args.precision, "parallelism": "single GPU"},
+                   "matrices": count, "schedule": args.precision, "parallelism": "single GPU"},
+        "plain_fp64": plain,
         "sigma_err": {"rel_max_vs_imposed": e_rel_max, "abs_max_vs_imposed": e_abs_max,
                       "rule": "rel over sigma>=1e-3*sigma_max, abs normalised by sigma_max"},
         "residuals_sampled": {"rel_recon_max": worst_recon, "orth_max": worst_orth},

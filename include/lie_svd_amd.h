@@ -48,12 +48,22 @@ enum lsvd_status {
 
 enum lsvd_pointer_mode { LSVD_PTR_HOST = 0, LSVD_PTR_DEVICE = 1 };
 
-/* Arithmetic of the Newton-Schulz polar iterations.  Every mode ends in
- * FP64: FP64 refinement of the polar factor and FP64 Jacobi cleanup. */
+/* Arithmetic and step schedule of the Newton-Schulz polar iterations.
+ * Every mode ends in FP64 with the reference's FP64 stopping rule.
+ *   LSVD_PREC_F64        the reference iteration X <- X (3I - X^T X) / 2.
+ *   LSVD_PREC_F64_SCALED the same limit in fewer FP64 steps: while small
+ *                        singular values are still growing the step is
+ *                        applied to 1.5 X (they grow 2.25x instead of
+ *                        1.5x per step); plain steps finish.  No reference
+ *                        counterpart.
+ *   LSVD_PREC_F32 / BF16 low-precision MFMA iterations (FP32 accumulate,
+ *                        scaled schedule), then FP64 refinement steps; large
+ *                        polar only (lsvd_polar_f64), see there. */
 enum lsvd_precision {
-    LSVD_PREC_F64 = 0,   /* plain FP64 throughout (the reference algorithm) */
-    LSVD_PREC_F32 = 1,   /* FP32 NS iterations, FP64 refinement + cleanup */
-    LSVD_PREC_BF16 = 2   /* BF16 MFMA NS iterations (FP32 accumulate), FP64 refinement */
+    LSVD_PREC_F64 = 0,
+    LSVD_PREC_F32 = 1,
+    LSVD_PREC_BF16 = 2,
+    LSVD_PREC_F64_SCALED = 3
 };
 
 /* Per-matrix diagnostics (fills the reference's solver traces). */
@@ -139,9 +149,11 @@ int lsvd_micro_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, dou
  * reference runs on square matrices, for tall or square m x n (m >= n):
  * q (m x n) gets the orthonormal polar factor.
  *   LSVD_PREC_F64  : the reference iteration in FP64 (FP64 MFMA products).
- *   LSVD_PREC_F32  : FP32 MFMA products until the iteration stalls, then
- *   LSVD_PREC_BF16   (BF16 MFMA, FP32 accumulate) FP64 refinement iterations
- *                    to the reference's FP64 stopping rule.
+ *   LSVD_PREC_F64_SCALED : FP64 with the scaled step schedule (same limit).
+ *   LSVD_PREC_F32  : FP32 MFMA products (scaled schedule) until the
+ *   LSVD_PREC_BF16   iteration stalls, then FP64 refinement iterations
+ *                    (BF16: BF16 MFMA, FP32 accumulate) to the reference's
+ *                    FP64 stopping rule.
  * Low-precision products perturb the singular directions the FP64
  * refinement cannot restore; info->orth_err is ||Q^T Q - I||_F and the
  * caller states the error against the FP64 factor.  The Gram-Schmidt
@@ -207,7 +219,12 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
  * Square matrices with n <= 4 use the LieSvdMicro schedule, as the
  * reference's router does (src/lie_svd_adaptive.rs:93); the rest use
  * LieSvdSmall::solve_rectangular.  The plan sorts the work, picks tile
- * shapes from the autotune cache and captures one hipGraph. */
+ * shapes from the autotune cache and captures one hipGraph.  precision is
+ * LSVD_PREC_F64 (the reference iteration) or LSVD_PREC_F64_SCALED; the
+ * low-precision modes are rejected here (LSVD_E_ARG): at the batch's
+ * accuracy target (relative sigma error 1e-12 up to condition 1e12) a
+ * low-precision iterate's error in the small-sigma directions is not
+ * removed by FP64 refinement. */
 int lsvd_batch_plan_create(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n,
                            const int64_t* a_off, const int64_t* u_off, const int64_t* s_off,
                            const int64_t* vt_off, int precision, lsvd_plan_t* out);

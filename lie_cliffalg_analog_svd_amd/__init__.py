@@ -6,8 +6,8 @@ Hand-written HIP kernels behind a C ABI (``include/lie_svd_amd.h``,
 batch plans.  ``profiles`` and ``metrics`` restate the reference's input
 generators and residual metrics (src/lib.rs:133-388).
 """
-from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PTR_DEVICE, LSVD_PTR_HOST, LsvdError,
-                   ShapeError)
+from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_SCALED, LSVD_PTR_DEVICE,
+                   LSVD_PTR_HOST, LsvdError, ShapeError)
 from .handle import BatchPlan, Handle, default_handle, pack_offsets
 from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall, eigh_jacobi_full, run_small,
                       svd_batched)

@@ -27,6 +27,7 @@ LSVD_PTR_DEVICE = 1
 LSVD_PREC_F64 = 0
 LSVD_PREC_F32 = 1
 LSVD_PREC_BF16 = 2
+LSVD_PREC_F64_SCALED = 3
 
 
 class LsvdInfo(C.Structure):

@@ -116,8 +116,9 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
                       int precision, lsvd_plan_t* out) {
     if (!h || !out) return LSVD_E_ARG;
     if (count < 0) return set_error(h, LSVD_E_ARG, "negative count");
-    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16)
-        return set_error(h, LSVD_E_ARG, "unknown precision mode");
+    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F64_SCALED)
+        return set_error(h, LSVD_E_ARG,
+                         "batch plans compute in FP64: precision must be LSVD_PREC_F64 or LSVD_PREC_F64_SCALED");
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
 
@@ -353,6 +354,7 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const
         L.pools = pools;
         L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
         L.gws_stride = bk.gws_stride;
+        L.ns_scaled = p->precision == LSVD_PREC_F64_SCALED;
         if (tev && (e = hipEventRecord(tev[2 * (i + 1)], p->side[l])) != hipSuccess) return e;
         e = launch_small_variant(bk.variant, L, p->side[l]);
         if (e != hipSuccess) return e;
@@ -692,6 +694,7 @@ int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int3
                 L.pools = pools;
                 L.gws = gws;
                 L.gws_stride = bk.gws_stride;
+                L.ns_scaled = precision == LSVD_PREC_F64_SCALED;
                 TCHECK(launch_small_variant(bk.variant, L, h->stream));
                 TCHECK(hipEventRecord(e1, h->stream));
                 TCHECK(hipEventSynchronize(e1));

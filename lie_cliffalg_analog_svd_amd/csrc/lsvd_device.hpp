@@ -59,6 +59,28 @@ __device__ __forceinline__ double block_sum(double v, double* red) {
     return s;
 }
 
+// Two block-wide sums with the barriers of one; `red` holds >= 2 NT/64.
+template <int NT>
+__device__ __forceinline__ void block_sum2(double& a, double& b, double* red) {
+    a = wave_sum(a);
+    b = wave_sum(b);
+    const int w = tix() / kWave;
+    if ((tix() & (kWave - 1)) == 0) {
+        red[2 * w] = a;
+        red[2 * w + 1] = b;
+    }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / kWave; ++i) {
+        sa += red[2 * i];
+        sb += red[2 * i + 1];
+    }
+    __syncthreads();
+    a = sa;
+    b = sb;
+}
+
 template <int NT>
 __device__ __forceinline__ double block_max(double v, double* red) {
     v = wave_max(v);

@@ -27,6 +27,7 @@ struct SmallLaunch {
     Pools pools;
     double* gws;           // global arena slots (global variants)
     int64_t gws_stride;    // doubles per slot
+    int ns_scaled;         // scaled Newton-Schulz schedule (on-chip kernel)
 };
 
 // Arena rows for a tall-ified problem M x N (M >= N); ld = N | 1.

@@ -29,6 +29,7 @@ extern "C" {
 // NS polar factor of a tall or square m x n matrix (m >= n).
 //   LSVD_PREC_F64 : the reference iteration in FP64 (X0 = A/||A||_F, X <- X(3I - X^T X)/2,
 //                   stop at ||dX||_F < 1e-13 sqrt(n), polar_ok = ||Q^T Q - I||_F <= 1e-8 sqrt(n)).
+//   LSVD_PREC_F64_SCALED : FP64 with the scaled step schedule (NsSchedule), same limit.
 //   LSVD_PREC_F32 / LSVD_PREC_BF16 : the same iteration with FP32 / BF16 MFMA products
 //                   (FP32 accumulation) until it stalls at that precision, then FP64 refinement
 //                   iterations with the FP64 stopping rule.
@@ -36,7 +37,8 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
                    lsvd_polar_info_t* info) {
     if (!h) return LSVD_E_ARG;
     if (m < n || n < 0) return set_error(h, LSVD_E_SHAPE, "lsvd_polar_f64: needs m >= n");
-    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16)
+    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16 &&
+        precision != LSVD_PREC_F64_SCALED)
         return set_error(h, LSVD_E_ARG, "unknown precision");
     lsvd_polar_info_t inf{};
     if (n == 0) {
@@ -50,7 +52,7 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
     hipStream_t st = h->stream;
     const size_t mn = (size_t)m * n, nn = (size_t)n * n;
     const bool host = h->ptr_mode == LSVD_PTR_HOST;
-    const bool lowp = precision != LSVD_PREC_F64;
+    const bool lowp = precision == LSVD_PREC_F32 || precision == LSVD_PREC_BF16;
     const int lp = precision == LSVD_PREC_F32 ? kPolF32 : kPolBF16;
     Buf bA, bX, bX2, bM, bXf, bXf2, bMf, bS;
 #define PCHK(expr)                                          \
@@ -94,10 +96,12 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
         const double floor_tol = (lp == kPolBF16 ? 1e-3 : 1e-6) * rootn;
         double best = 1e300;
         int stall = 0;
+        NsSchedule sched;  // scaled steps first; the stop tests apply to plain steps
+        sched.init(true);
         for (int it = 0; it < 120; ++it) {
             PCHK(polar_zero(S + 1, 3, st));
-            PCHK(ns_gram(lp, m, n, X, bMf.p, S + 2, st));
-            PCHK(ns_update(lp, m, n, X, bMf.p, Xn, S + 1, bad, st));
+            PCHK(ns_gram(lp, m, n, X, bMf.p, S + 2, st, sched.alpha()));
+            PCHK(ns_update(lp, m, n, X, bMf.p, Xn, S + 1, bad, st, sched.c()));
             PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
             PCHK(hipStreamSynchronize(st));
             ++it_low;
@@ -110,6 +114,9 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
             }
             std::swap(X, Xn);
             const double orth = std::sqrt(std::max(hs[2], 0.0));  // of the iterate before this step
+            const bool plain = sched.plain();
+            sched.after(hs[2], delta);
+            if (!plain) continue;
             if (orth < floor_tol) break;
             if (orth < 0.99 * best) {
                 best = orth;
@@ -130,10 +137,12 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
     double* Xn = (double*)bX2.p;
     bool converged = false, bad64 = false;
     const int budget = 120;
+    NsSchedule sched;  // plain steps unless LSVD_PREC_F64_SCALED
+    sched.init(precision == LSVD_PREC_F64_SCALED);
     for (int it = 0; it < budget; ++it) {
         PCHK(polar_zero(S + 1, 3, st));
-        PCHK(ns_gram(kPolF64, m, n, X, bM.p, nullptr, st));
-        PCHK(ns_update(kPolF64, m, n, X, bM.p, Xn, S + 1, bad, st));
+        PCHK(ns_gram(kPolF64, m, n, X, bM.p, sched.enabled ? S + 2 : nullptr, st, sched.alpha()));
+        PCHK(ns_update(kPolF64, m, n, X, bM.p, Xn, S + 1, bad, st, sched.c()));
         PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
         PCHK(hipStreamSynchronize(st));
         ++it64;
@@ -145,7 +154,9 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
             break;
         }
         std::swap(X, Xn);
-        if (delta < 1e-13 * rootn) {
+        const bool plain = sched.plain();
+        sched.after(hs[2], delta);
+        if (plain && delta < 1e-13 * rootn) {
             converged = true;
             break;
         }

@@ -9,12 +9,13 @@ namespace lsvd {
 enum PolarPrec { kPolF64 = 0, kPolF32 = 1, kPolBF16 = 2 };
 enum Epilogue { kEpiStore = 0, kEpi3IMinus = 1, kEpiHalfDelta = 2, kEpiOrth = 3 };
 
-// M = 3I - X^T X (n x n) from X (m x n); X/M are f64 (kPolF64) or f32.
-// *orth_sq += ||X^T X - I||_F^2 when orth_sq is non-null.
-hipError_t ns_gram(int prec, int m, int n, const void* X, void* M, double* orth_sq, hipStream_t st);
-// Xn = 0.5 X M; *dsq += ||Xn - X||_F^2; *bad |= any non-finite entry of Xn.
+// M = alpha I - X^T X (n x n) from X (m x n); X/M are f64 (kPolF64) or
+// f32.  *orth_sq += ||X^T X - I||_F^2 when orth_sq is non-null.
+hipError_t ns_gram(int prec, int m, int n, const void* X, void* M, double* orth_sq, hipStream_t st,
+                   double alpha = 3.0);
+// Xn = c X M; *dsq += ||Xn - X||_F^2; *bad |= any non-finite entry of Xn.
 hipError_t ns_update(int prec, int m, int n, const void* X, const void* M, void* Xn, double* dsq, int* bad,
-                     hipStream_t st);
+                     hipStream_t st, double c = 0.5);
 // *sum += ||X^T X - I||_F^2 in FP64.
 hipError_t ns_orth(int m, int n, const double* X, double* sum, hipStream_t st);
 // C = alpha op(A) B, FP64; op(A) = A^T when trans_a (A stored K x M).

@@ -70,7 +70,7 @@ __device__ __forceinline__ void epi_apply(const EpiArgs& e, int N, int i, int j,
         c[(size_t)i * N + j] = v;
         if (mirror) c[(size_t)j * N + i] = v;
     } else if (EPI == kEpi3IMinus) {
-        const T v = (T)((i == j ? 3.0 : 0.0) - (double)acc);
+        const T v = (T)((i == j ? e.alpha : 0.0) - (double)acc);
         c[(size_t)i * N + j] = v;
         if (mirror) c[(size_t)j * N + i] = v;
         if (e.sum) {  // ||X^T X - I||_F^2 as a by-product
@@ -78,7 +78,7 @@ __device__ __forceinline__ void epi_apply(const EpiArgs& e, int N, int i, int j,
             part += (mirror ? 2.0 : 1.0) * d * d;
         }
     } else if (EPI == kEpiHalfDelta) {
-        const T v = (T)(0.5 * (double)acc);
+        const T v = (T)(e.alpha * (double)acc);
         const T* xo = reinterpret_cast<const T*>(e.xold);
         const double d = (double)v - (double)xo[(size_t)i * N + j];
         c[(size_t)i * N + j] = v;
@@ -310,17 +310,18 @@ static hipError_t launch_bf16(int M, int N, int K, const float* A, const float* 
 }
 
 // M = 3I - X^T X (n x n) from X (m x n); *orth_sq += ||X^T X - I||_F^2 when non-null
-hipError_t ns_gram(int prec, int m, int n, const void* X, void* Mout, double* orth_sq, hipStream_t st) {
-    EpiArgs e{Mout, nullptr, orth_sq, nullptr, 1.0};
+hipError_t ns_gram(int prec, int m, int n, const void* X, void* Mout, double* orth_sq, hipStream_t st,
+                   double alpha) {
+    EpiArgs e{Mout, nullptr, orth_sq, nullptr, alpha};
     if (prec == kPolF64) return launch4<double, true, true, kEpi3IMinus>(n, n, m, (const double*)X, (const double*)X, e, st);
     if (prec == kPolF32) return launch4<float, true, true, kEpi3IMinus>(n, n, m, (const float*)X, (const float*)X, e, st);
     return launch_bf16<true, true, kEpi3IMinus>(n, n, m, (const float*)X, (const float*)X, e, st);
 }
 
-// Xn = 0.5 X M, accumulating ||Xn - X||^2 into *dsq and non-finite into *bad
+// Xn = c X M, accumulating ||Xn - X||^2 into *dsq and non-finite into *bad
 hipError_t ns_update(int prec, int m, int n, const void* X, const void* Mm, void* Xn, double* dsq, int* bad,
-                     hipStream_t st) {
-    EpiArgs e{Xn, X, dsq, bad, 1.0};
+                     hipStream_t st, double c) {
+    EpiArgs e{Xn, X, dsq, bad, c};
     if (prec == kPolF64) return launch4<double, false, false, kEpiHalfDelta>(m, n, n, (const double*)X, (const double*)Mm, e, st);
     if (prec == kPolF32) return launch4<float, false, false, kEpiHalfDelta>(m, n, n, (const float*)X, (const float*)Mm, e, st);
     return launch_bf16<false, false, kEpiHalfDelta>(m, n, n, (const float*)X, (const float*)Mm, e, st);

@@ -53,7 +53,7 @@ struct Cfg2 {
 
 template <int NP>
 struct Sh2 {
-    double red[Cfg2<NP>::NW];
+    double red[2 * Cfg2<NP>::NW];
     double diag[NP];
     double tau[NP];
     double scr[NP];
@@ -165,7 +165,7 @@ struct PhaseClock2 {
 
 template <int NP>
 __device__ void solve_item2(const Item& itm, const Pools& P, double* main, double* panel, int panel_rows,
-                            double* xs, double* vs, double* ys, Sh2<NP>& sh) {
+                            double* xs, double* vs, double* ys, Sh2<NP>& sh, int ns_scaled) {
     using C = Cfg2<NP>;
     constexpr int NT = C::NT, NW = C::NW, LDM = C::LDM, NTILE = C::NTILE;
     constexpr int TPR = C::TPR, CH = C::CH, SLOTS = C::SLOTS, LDP = C::LDP;
@@ -204,11 +204,14 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         }
         __syncthreads();
         int diverged = 0;
-        // Y = 3I - X^T X fits next to X: compute its upper tiles once per step
-        // and update X in place, row tile by row tile (no staging)
+        // Y = alpha I - X^T X fits next to X: compute its upper tiles once per
+        // step and update X in place, row tile by row tile (no staging)
         const bool ymode = NP <= kV2YModeMaxNP && panel_rows >= NP;
+        NsSchedule sched;
+        sched.init(ns_scaled != 0);
         for (int it = 0; it < kPolarMaxIters; ++it) {
-            double dsq = 0.0;
+            const double alpha = sched.alpha(), cstep = sched.c();
+            double dsq = 0.0, gi2 = 0.0;  // ||X_new - X||^2, ||X^T X - I||^2 (N x N part)
             int bad = 0;
             if constexpr (NP <= kV2YModeMaxNP) {
                 if (ymode) {
@@ -225,9 +228,13 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int row = I * 16 + trow(r), col = J * 16 + tcol();
-                            const double y = (row == col ? 3.0 : 0.0) - g[r];
+                            const double y = (row == col ? alpha : 0.0) - g[r];
                             panel[row * LDM + col] = y;
                             panel[col * LDM + row] = y;
+                            if (row < N && col < N) {
+                                const double e = g[r] - (row == col ? 1.0 : 0.0);
+                                gi2 += (I == J ? 1.0 : 2.0) * e * e;
+                            }
                         }
                     }
                     __syncthreads();
@@ -242,7 +249,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
 #pragma unroll
                             for (int r = 0; r < 4; ++r) {
                                 const int o = (R * 16 + trow(r)) * LDM + J * 16 + tcol();
-                                const double xn = 0.5 * acc[J][r];
+                                const double xn = cstep * acc[J][r];
                                 const double d = xn - main[o];
                                 main[o] = xn;
                                 bad |= !isfinite(xn);
@@ -254,25 +261,29 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
             if (!ymode) {
 #pragma unroll 1
                 for (int J = 0; J < NTILE; ++J) {
-                    // panel = (3I - X^T X)[:, J]
+                    // panel = (alpha I - X^T X)[:, J]
                     for (int I = w; I < NTILE; I += NW) {
                         const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
                                               [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int row = I * 16 + trow(r), col = tcol();
-                            panel[row * LDP + col] = (row == J * 16 + col ? 3.0 : 0.0) - g[r];
+                            panel[row * LDP + col] = (row == J * 16 + col ? alpha : 0.0) - g[r];
+                            if (row < N && J * 16 + col < N) {
+                                const double e = g[r] - (row == J * 16 + col ? 1.0 : 0.0);
+                                gi2 += e * e;
+                            }
                         }
                     }
                     __syncthreads();
-                    // X_new[:, J] = 0.5 X panel -> staging slot; |X_new - X|^2 on the fly
+                    // X_new[:, J] = c X panel -> staging slot; |X_new - X|^2 on the fly
                     for (int R = w; R < MT; R += NW) {
                         const d4v x = tile_mm(NP, [&](int i, int k) { return main[(R * 16 + i) * LDM + k]; },
                                               [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
 #pragma unroll
                         for (int r = 0; r < 4; ++r) {
                             const int row = R * 16 + trow(r), col = J * 16 + tcol();
-                            const double xn = 0.5 * x[r];
+                            const double xn = cstep * x[r];
                             const double d = xn - main[row * LDM + col];
                             ys[row * NP + col] = xn;
                             bad |= !isfinite(xn);
@@ -283,7 +294,8 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                 }
             }
             // a non-finite entry makes the step infinite
-            dsq = block_sum<NT>(bad ? INFINITY : dsq, sh.red);
+            dsq = bad ? INFINITY : dsq;
+            block_sum2<NT>(dsq, gi2, sh.red);
             iters = it + 1;
             const double delta = sqrt(dsq);
             if (!(delta <= 1e6 * rootn)) {  // (:169-171): Gram-Schmidt of A
@@ -295,7 +307,9 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                 for (int e = tid; e < MP * NP; e += NT) main[(e / NP) * LDM + e % NP] = ys[e];
                 __syncthreads();
             }
-            if (delta < kPolarTol * rootn) break;
+            const bool plain = sched.plain();
+            sched.after(gi2, delta);
+            if (plain && delta < kPolarTol * rootn) break;
         }
         // orthogonality_err (:146-150) of the final X: X^T X - I, N x N part
         {
@@ -640,7 +654,8 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
 template <int NP>
 __global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __restrict__ items, const int* count_ptr,
                                                                   int* counter, Pools pools, double* gws,
-                                                                  int64_t gws_stride, int lds_doubles) {
+                                                                  int64_t gws_stride, int lds_doubles,
+                                                                  int ns_scaled) {
     extern __shared__ double dyn_lds[];
     __shared__ Sh2<NP> sh;
     const int count = *count_ptr;
@@ -661,7 +676,7 @@ __global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __
         double* main = dyn_lds;
         double* panel = dyn_lds + (size_t)rows * Cfg2<NP>::LDM;
         const int panel_rows = (lds_doubles - rows * Cfg2<NP>::LDM) / Cfg2<NP>::LDM;
-        solve_item2<NP>(itm, pools, main, panel, panel_rows, xs, vs, ys, sh);
+        solve_item2<NP>(itm, pools, main, panel, panel_rows, xs, vs, ys, sh, ns_scaled);
     }
 }
 
@@ -673,7 +688,8 @@ static hipError_t launch2(const SmallLaunch& L, hipStream_t st) {
         if (e != hipSuccess) return e;
     }
     hipLaunchKernelGGL((small_svd2_kernel<NP>), dim3(L.grid), dim3(Cfg2<NP>::NT), L.lds_bytes, st, L.items,
-                       L.count_ptr, L.counter, L.pools, L.gws, L.gws_stride, (int)(L.lds_bytes / sizeof(double)));
+                       L.count_ptr, L.counter, L.pools, L.gws, L.gws_stride, (int)(L.lds_bytes / sizeof(double)),
+                       L.ns_scaled);
     return hipGetLastError();
 }
 

@@ -4,8 +4,10 @@
 ``newton_schulz_polar`` (square) and extends it to tall matrices: it returns
 ``(Q, polar_ok)`` plus the diagnostics in ``last_info``.  ``precision`` picks
 the arithmetic of the iterations (``LSVD_PREC_F64``: the reference
-iteration; ``LSVD_PREC_F32`` / ``LSVD_PREC_BF16``: MFMA products at that
-precision, then FP64 refinement iterations).
+iteration; ``LSVD_PREC_F64_SCALED``: FP64 with scaled steps while small
+singular values are still growing, same limit in fewer steps;
+``LSVD_PREC_F32`` / ``LSVD_PREC_BF16``: MFMA products at that precision,
+then FP64 refinement iterations).
 """
 from __future__ import annotations
 
@@ -42,7 +44,7 @@ def polar_device(handle: Handle, a_dev, m: int, n: int, q_dev, precision: int) -
 
 def gemm_device(handle: Handle, trans_a: bool, m: int, n: int, k: int, a_dev, b_dev, c_dev, precision: int):
     """C = op(A) B on device buffers (FP64, or FP32 data with FP32/BF16 products)."""
-    if precision == _lib.LSVD_PREC_F64:
+    if precision in (_lib.LSVD_PREC_F64, _lib.LSVD_PREC_F64_SCALED):
         st = _lib.lib().lsvd_gemm_f64(handle.raw, int(trans_a), m, n, k, _ptr(a_dev), _ptr(b_dev), _ptr(c_dev))
     else:
         st = _lib.lib().lsvd_gemm_lowp(handle.raw, precision, int(trans_a), m, n, k, _ptr(a_dev), _ptr(b_dev),

@@ -82,6 +82,27 @@ def test_polar_f64_vs_oracle(m, n):
     assert info["orth_err"] < 1e-13 * np.sqrt(n) * 10
 
 
+@pytest.mark.parametrize("m,n,lo", [(256, 256, 1e-6), (384, 128, 1e-10), (200, 200, 1.0)])
+def test_polar_f64_scaled_schedule(m, n, lo):
+    """LSVD_PREC_F64_SCALED reaches the exact polar factor U V^T as closely
+    as the reference iteration does (the FP64 limit: rounding amplified by
+    the condition number in the small-sigma directions), in fewer steps."""
+    from lie_cliffalg_analog_svd_amd import LSVD_PREC_F64_SCALED
+    a, s, u, v = workloads.polar_matrix(m, n, seed=13, lo=lo)
+    print("polar scaled", m, n, lo, flush=True)
+    q, ok = newton_schulz_polar(a, LSVD_PREC_F64_SCALED)
+    info = polar.last_info
+    qo, oko, it_o = oracle.newton_schulz_polar(a)
+    assert ok and oko
+    exact = u @ v.T
+    e_scaled = np.linalg.norm(q - exact) / np.sqrt(n)
+    e_ref = np.linalg.norm(qo - exact) / np.sqrt(n)
+    assert e_scaled <= max(1e-13, 4.0 * e_ref), (e_scaled, e_ref)
+    assert info["orth_err"] < 1e-13 * np.sqrt(n) * 10
+    if lo < 1e-3:
+        assert info["f64_iters"] <= 0.8 * it_o, (info["f64_iters"], it_o)
+
+
 @pytest.mark.parametrize("prec", [LSVD_PREC_F32, LSVD_PREC_BF16])
 def test_polar_mixed_precision(prec):
     """Mixed precision: orthogonal to FP64 precision; error against the

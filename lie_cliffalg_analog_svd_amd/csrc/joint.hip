@@ -215,7 +215,6 @@ __global__ __launch_bounds__(64) void joint_decide_kernel(JointArgs a, int layer
 template <int SPB>
 __global__ __launch_bounds__(256) void joint_apply_kernel(JointArgs a, int layer) {
     extern __shared__ double rows[];  // [2*SPB][n]
-    __shared__ double red[8];
     const int f = blockIdx.y, k = blockIdx.z, chunk = blockIdx.x;
     const int n = a.n, K = a.K;
     if (a.fam[f].done) return;
@@ -311,11 +310,19 @@ __global__ __launch_bounds__(256) void joint_apply_kernel(JointArgs a, int layer
         }
     }
     if (!is_basis) {
+        // row energies: wave sums, then one pass over the four waves' partials
+        __shared__ double part[2 * SPB][4];
+        const int wv = threadIdx.x >> 6;
 #pragma unroll
         for (int e = 0; e < 2 * SPB; ++e) {
-            const double t = block_sum<256>(ro[e], red);
-            if (threadIdx.x == 0 && rid[e] >= 0) a.offrow[((size_t)f * K + k) * n + rid[e]] = t;
+            const double t = wave_sum(ro[e]);
+            if ((threadIdx.x & 63) == 0) part[e][wv] = t;
         }
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < 2 * SPB; ++e)
+            if (threadIdx.x == e && rid[e] >= 0)
+                a.offrow[((size_t)f * K + k) * n + rid[e]] = part[e][0] + part[e][1] + part[e][2] + part[e][3];
     }
 }
 
@@ -349,7 +356,7 @@ __global__ void joint_diag_kernel(JointArgs a, double* diag) {
 }
 
 // ---- launches --------------------------------------------------------------------
-constexpr int kSPB = 4;
+constexpr int kSPB = 2;
 
 hipError_t joint_launch_init(const JointArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(joint_init_kernel, dim3(a.K, a.F), dim3(256), 0, st, a);

@@ -209,6 +209,18 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
                             const lsvd_joint_params_t* params, double* v, double* diag,
                             lsvd_joint_trace_t* trace);
 
+/* Replaces LieSvdJoint::joint_svd_with_params (src/lie_svd_joint.rs:
+ * 170-262) for `families` independent families of k matrices of
+ * rows x cols (mats: families x k x rows x cols, row-major).  Outputs per
+ * family: u (rows x rows), sigmas (k x min(rows, cols), the |diagonal| of
+ * U^T M V), vt (cols x cols, V^T) and the trace (JointSvdTrace,
+ * :53-59, same fields as lsvd_joint_trace_t).  params NULL uses
+ * lsvd_joint_params_for_n(min(rows, cols)).  The two-sided rotations pair
+ * indices below min(rows, cols) only, as the reference does. */
+int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols, const double* mats,
+                       const lsvd_joint_params_t* params, double* u, double* sigmas, double* vt,
+                       lsvd_joint_trace_t* trace);
+
 /* ---- Ragged batch (no reference counterpart: the reference does one
  *      matrix per call; this runs many LieSvd::solve / solve_rectangular
  *      calls, src/lie_svd.rs:49-51 + src/lie_svd_small.rs:340, at once) -- */

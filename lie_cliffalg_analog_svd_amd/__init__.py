@@ -14,7 +14,7 @@ from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall
 from . import metrics, profiles
 from .polar import newton_schulz_polar
 from .adjoint import polar_vjp, svd_loss_grad, svd_vjp
-from .joint import (JointDiagonalizationParams, JointDiagonalizationTrace, LieSvdJoint,
-                    diagonalize_families)
+from .joint import (JointDiagonalizationParams, JointDiagonalizationTrace, JointSvdTrace, LieSvdJoint,
+                    diagonalize_families, svd_families)
 
 __version__ = "0.1.0"

@@ -145,6 +145,8 @@ SIGNATURES = {
     "lsvd_joint_params_for_n": (C.c_int, [C.c_int, C.POINTER(LsvdJointParams)]),
     "lsvd_joint_diag_sym_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdJointParams), _vp,
                                           _vp, _vp]),
+    "lsvd_joint_svd_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdJointParams),
+                                     _vp, _vp, _vp, C.POINTER(LsvdJointTrace)]),
     "lsvd_batch_plan_describe": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int), _vp]),
     "lsvd_batch_plan_time_buckets": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "lsvd_batch_plan_phase_clocks": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),

@@ -7,6 +7,7 @@
 
 #include "lsvd_api_internal.hpp"
 #include "lsvd_joint.hpp"
+#include "lsvd_joint_svd.hpp"
 
 using namespace lsvd;
 
@@ -161,6 +162,137 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
             trace[f].sweeps = fam[f].sweeps;
             trace[f].rotations = fam[f].rotations;
             trace[f].rejected_rotations = fam[f].rejected;
+        }
+    }
+    return LSVD_OK;
+}
+
+// Replaces LieSvdJoint::joint_svd_with_params (src/lie_svd_joint.rs:170-262)
+// for `families` independent families of k rows x cols matrices.
+int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols, const double* mats,
+                       const lsvd_joint_params_t* params, double* u, double* sigmas, double* vt,
+                       lsvd_joint_trace_t* trace) {
+    if (!h) return LSVD_E_ARG;
+    if (families < 0 || rows < 0 || cols < 0) return set_error(h, LSVD_E_SHAPE, "negative size");
+    if (k <= 0) return set_error(h, LSVD_E_SHAPE, "LieSvdJoint: empty matrix set");
+    if (rows == 0 || cols == 0) return set_error(h, LSVD_E_SHAPE, "LieSvdJoint: empty matrix");
+    if (families == 0) return LSVD_OK;
+    if (!mats || !u || !sigmas || !vt) return set_error(h, LSVD_E_ARG, "null pointer");
+    const int kk = std::min(rows, cols);
+    lsvd_joint_params_t prm;
+    if (params) prm = *params;
+    else lsvd_joint_params_for_n(kk, &prm);
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    hipStream_t st = h->stream;
+    const size_t rc = (size_t)rows * cols, uu = (size_t)rows * rows, vv = (size_t)cols * cols;
+    const size_t mats_n = (size_t)families * k * rc;
+    const int slots = (kk + (kk & 1)) / 2;
+    DevBuf bmats, bwork, bu, bv, bsig, bvt, brow, bcol, bpart, brot, bfam;
+#define JCHECK(expr)                                        \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+    const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    const double* d_mats = mats;
+    double* d_sig = sigmas;
+    double* d_vt = vt;
+    if (host) {
+        JCHECK(bmats.ensure(sizeof(double) * mats_n));
+        JCHECK(hipMemcpyAsync(bmats.p, mats, sizeof(double) * mats_n, hipMemcpyHostToDevice, st));
+        d_mats = (const double*)bmats.p;
+        JCHECK(bsig.ensure(sizeof(double) * (size_t)families * k * kk));
+        d_sig = (double*)bsig.p;
+        JCHECK(bvt.ensure(sizeof(double) * families * vv));
+        d_vt = (double*)bvt.p;
+    }
+    JCHECK(bwork.ensure(sizeof(double) * mats_n));
+    JCHECK(bu.ensure(sizeof(double) * families * uu));
+    JCHECK(bv.ensure(sizeof(double) * families * vv));
+    JCHECK(brow.ensure(sizeof(double) * (size_t)families * k * rows));
+    JCHECK(bcol.ensure(sizeof(double) * (size_t)families * k * cols));
+    JCHECK(bpart.ensure(sizeof(double) * (size_t)families * k * 2));
+    JCHECK(brot.ensure(sizeof(JsvdRot) * (size_t)families * std::max(slots, 1)));
+    JCHECK(bfam.ensure(sizeof(JointFamily) * families));
+
+    JsvdArgs a{};
+    a.F = families;
+    a.K = k;
+    a.rows = rows;
+    a.cols = cols;
+    a.kk = kk;
+    a.slots = slots;
+    a.mats = d_mats;
+    a.work = (double*)bwork.p;
+    a.ubasis = (double*)bu.p;
+    a.vbasis = (double*)bv.p;
+    a.rowoff = (double*)brow.p;
+    a.coloff = (double*)bcol.p;
+    a.part = (double*)bpart.p;
+    a.rot = (JsvdRot*)brot.p;
+    a.fam = (JointFamily*)bfam.p;
+    a.tol = prm.tol;
+    a.min_pair_scale = prm.min_pair_scale;
+    a.line_search_steps = prm.line_search_steps;
+    a.max_sweeps = prm.max_sweeps;
+
+    JCHECK(jsvd_launch_init(a, st));
+    std::vector<JointFamily> fam(families);
+    if (kk > 1 && prm.max_sweeps > 0) {
+        // one sweep (2 launches per layer + the stop test) as a graph,
+        // relaunched until every family has stopped
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        JCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t le = jsvd_launch_sweep(a, st);
+        JCHECK(hipStreamEndCapture(st, &graph));
+        if (le != hipSuccess) {
+            hipGraphDestroy(graph);
+            return hip_fail(h, le, "joint svd sweep launch");
+        }
+        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            hipGraphDestroy(graph);
+            return hip_fail(h, e, "hipGraphInstantiate");
+        }
+        int rc2 = LSVD_OK;
+        for (int sweep = 0; sweep < prm.max_sweeps; ++sweep) {
+            e = hipGraphLaunch(exec, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(fam.data(), a.fam, sizeof(JointFamily) * families, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) {
+                rc2 = hip_fail(h, e, "joint svd sweep");
+                break;
+            }
+            bool all = true;
+            for (auto& s : fam) all = all && s.done;
+            if (all) break;
+        }
+        hipGraphExecDestroy(exec);
+        hipGraphDestroy(graph);
+        if (rc2 != LSVD_OK) return rc2;
+    }
+    JCHECK(jsvd_launch_final(a, d_sig, d_vt, st));
+    JCHECK(hipMemcpyAsync(fam.data(), a.fam, sizeof(JointFamily) * families, hipMemcpyDeviceToHost, st));
+    if (host) {
+        JCHECK(hipMemcpyAsync(u, bu.p, sizeof(double) * families * uu, hipMemcpyDeviceToHost, st));
+        JCHECK(hipMemcpyAsync(sigmas, d_sig, sizeof(double) * (size_t)families * k * kk, hipMemcpyDeviceToHost, st));
+        JCHECK(hipMemcpyAsync(vt, d_vt, sizeof(double) * families * vv, hipMemcpyDeviceToHost, st));
+    } else {
+        JCHECK(hipMemcpyAsync(u, bu.p, sizeof(double) * families * uu, hipMemcpyDeviceToDevice, st));
+    }
+    JCHECK(hipStreamSynchronize(st));
+#undef JCHECK
+    if (trace) {
+        for (int f = 0; f < families; ++f) {
+            const bool trivial = kk <= 1;  // (:183-199): identity bases, zero trace
+            trace[f].initial_offdiag = trivial ? 0.0 : fam[f].initial_offdiag;
+            trace[f].final_offdiag = trivial ? 0.0 : fam[f].final_offdiag;
+            trace[f].sweeps = trivial ? 0 : fam[f].sweeps;
+            trace[f].rotations = trivial ? 0 : fam[f].rotations;
+            trace[f].rejected_rotations = trivial ? 0 : fam[f].rejected;
         }
     }
     return LSVD_OK;

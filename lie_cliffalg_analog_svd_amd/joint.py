@@ -68,6 +68,41 @@ def diagonalize_families(families, params: JointDiagonalizationParams | None = N
     return v, d, traces
 
 
+@dataclass
+class JointSvdTrace:
+    """src/lie_svd_joint.rs:53-59."""
+    initial_offdiag: float
+    final_offdiag: float
+    sweeps: int
+    rotations: int
+    rejected_rotations: int
+
+
+def svd_families(families, params: JointDiagonalizationParams | None = None, handle: Handle | None = None):
+    """Two-sided family SVD of independent families.  families: (F, K, rows,
+    cols) -> (U (F, rows, rows), sigmas (F, K, min), Vt (F, cols, cols),
+    traces)."""
+    fam = np.ascontiguousarray(np.asarray(families, dtype=np.float64))
+    if fam.ndim != 4:
+        raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: families must be (F, K, rows, cols)")
+    F, K, rows, cols = fam.shape
+    if K == 0:
+        raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: empty matrix set")
+    if rows == 0 or cols == 0:
+        raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: empty matrix")
+    kk = min(rows, cols)
+    params = params or JointDiagonalizationParams.for_n(kk)
+    h = handle or default_handle()
+    u = np.zeros((F, rows, rows)); sg = np.zeros((F, K, kk)); vt = np.zeros((F, cols, cols))
+    tr = (_lib.LsvdJointTrace * max(F, 1))()
+    cp = params._c()
+    st = _lib.lib().lsvd_joint_svd_f64(h.raw, F, K, rows, cols, _ptr(fam), C.byref(cp), _ptr(u), _ptr(sg), _ptr(vt),
+                                       tr)
+    h.check(st)
+    traces = [JointSvdTrace(**tr[f].as_dict()) for f in range(F)]
+    return u, sg, vt, traces
+
+
 class LieSvdJoint:
     @staticmethod
     def diagonalize_symmetric(matrices, handle: Handle | None = None):
@@ -89,3 +124,27 @@ class LieSvdJoint:
                 raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: size mismatch / not square")
         v, d, tr = diagonalize_families(np.stack(mats)[None], params, handle)
         return v[0], [d[0, k] for k in range(len(mats))], tr[0]
+
+    @staticmethod
+    def joint_svd(matrices, handle: Handle | None = None):
+        """src/lie_svd_joint.rs:160-168."""
+        mats = list(matrices)
+        n = min(mats[0].shape) if mats else 0
+        return LieSvdJoint.joint_svd_with_params(mats, JointDiagonalizationParams.for_n(n), handle)
+
+    @staticmethod
+    def joint_svd_with_params(matrices, params: JointDiagonalizationParams, handle: Handle | None = None):
+        """src/lie_svd_joint.rs:170-262 -> (U, [sigmas_k], Vt, trace)."""
+        mats = [np.asarray(m, dtype=np.float64) for m in matrices]
+        if not mats:
+            raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: empty matrix set")
+        rows, cols = mats[0].shape
+        if rows == 0 or cols == 0:
+            raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: empty matrix")
+        for m in mats:
+            if m.shape[0] != rows:
+                raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: row-size mismatch")
+            if m.shape[1] != cols:
+                raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdJoint: col-size mismatch")
+        u, sg, vt, tr = svd_families(np.stack(mats)[None], params, handle)
+        return u[0], [sg[0, k] for k in range(len(mats))], vt[0], tr[0]

@@ -90,6 +90,7 @@ def lib():
         L.lso_joint_params_for_n.argtypes = [C.c_int]
         L.lso_joint_params_for_n.restype = JointParams
         L.lso_joint_diag_sym.argtypes = [dp, C.c_int, C.c_int, JointParams, dp, dp, C.POINTER(JointTrace)]
+        L.lso_joint_svd.argtypes = [dp, C.c_int, C.c_int, C.c_int, JointParams, dp, dp, dp, C.POINTER(JointTrace)]
         L.lso_layer_count.argtypes = [C.c_int]
         L.lso_layer_count.restype = C.c_int
         L.lso_layer_pairs.argtypes = [C.c_int, C.c_int, ip]
@@ -204,6 +205,24 @@ def joint_diag_sym(mats, params: JointParams | None = None):
     tr = JointTrace()
     lib().lso_joint_diag_sym(_dp(mats), k, n, params, _dp(v), _dp(d), C.byref(tr))
     return v, d, tr.as_dict()
+
+
+def joint_svd(mats, params: JointParams | None = None):
+    """LieSvdJoint::joint_svd_with_params (src/lie_svd_joint.rs:170-262).
+
+    mats: (K, rows, cols).  Returns (U (rows, rows), sigmas (K, min), Vt
+    (cols, cols), trace dict); the reference's panics are assertions."""
+    mats = _c(mats)
+    assert mats.ndim == 3 and mats.shape[0] > 0, "LieSvdJoint: empty matrix set"
+    k, rows, cols = mats.shape
+    assert rows > 0 and cols > 0, "LieSvdJoint: empty matrix"
+    kk = min(rows, cols)
+    if params is None:
+        params = joint_params_for_n(kk)
+    u = np.zeros((rows, rows)); sg = np.zeros((k, kk)); vt = np.zeros((cols, cols))
+    tr = JointTrace()
+    lib().lso_joint_svd(_dp(mats), k, rows, cols, params, _dp(u), _dp(sg), _dp(vt), C.byref(tr))
+    return u, sg, vt, tr.as_dict()
 
 
 def layer_pairs(n: int, layer: int):

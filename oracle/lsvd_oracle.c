@@ -725,6 +725,111 @@ void lso_joint_diag_sym(const double* mats, int k, int n, lso_joint_params param
     free(pairs);
 }
 
+/* pair_block_norm, src/lie_svd_joint.rs:277-280. */
+static double pair_block_norm(const double* m, int cols, int i, int j) {
+    double a = m[(size_t)i * cols + i], b = m[(size_t)i * cols + j];
+    double c = m[(size_t)j * cols + i], d = m[(size_t)j * cols + j];
+    return sqrt(a * a + b * b + c * c + d * d);
+}
+
+/* LieSvdJoint::joint_svd_with_params, src/lie_svd_joint.rs:170-262, with
+ * joint_svd_pair_angles :264-275 (weighted averaged 2x2 block, then the
+ * two-sided angles of local_pair_svd_angles_from_values :282-289 and
+ * wrap_svd_angle :291-299) and accept_joint_svd_rotor :303-336.
+ * mats: k consecutive rows x cols matrices (not modified).  u_out: rows x
+ * rows; sig_out: k x min(rows, cols) (|diagonal|, diagonal_corridor
+ * :501-504); vt_out: cols x cols. */
+void lso_joint_svd(const double* mats, int k, int rows, int cols, lso_joint_params params, double* u_out,
+                   double* sig_out, double* vt_out, lso_joint_trace* trace) {
+    lso_joint_trace tr;
+    memset(&tr, 0, sizeof tr);
+    const int kk = rows < cols ? rows : cols;
+    const size_t rc = (size_t)rows * cols;
+    double* ub = dalloc((size_t)rows * rows);
+    double* vb = dalloc((size_t)cols * cols);
+    for (int i = 0; i < rows; ++i) ub[(size_t)i * rows + i] = 1.0;
+    for (int i = 0; i < cols; ++i) vb[(size_t)i * cols + i] = 1.0;
+    double* w = dalloc(rc * (k > 0 ? k : 1));
+    memcpy(w, mats, sizeof(double) * rc * k);
+    if (kk > 1) {
+        double ref_norm = frob(w, rc * k);
+        if (ref_norm < 1e-300) ref_norm = 1e-300;
+        double pair_tol = params.min_pair_scale * fmax(ref_norm, 1.0);
+        double tol = params.tol * fmax(ref_norm, 1.0);
+        tr.initial_offdiag = joint_offdiag_norm(w, k, rows, cols);
+        int* pairs = (int*)malloc(sizeof(int) * (kk + 1));
+        int lss = params.line_search_steps > 1 ? params.line_search_steps : 1;
+        for (int sw = 0; sw < params.max_sweeps; ++sw) {
+            tr.sweeps = sw + 1;
+            double before = joint_offdiag_norm(w, k, rows, cols);
+            int layers = lso_layer_count(kk);
+            for (int layer = 0; layer < layers; ++layer) {
+                int np = lso_layer_pairs(kk, layer, pairs);
+                for (int pi = 0; pi < np; ++pi) {
+                    int i = pairs[2 * pi], j = pairs[2 * pi + 1];
+                    double po = 0.0; /* joint_pair_offdiag :451-453 */
+                    for (int q = 0; q < k; ++q)
+                        po += fabs(w[(size_t)q * rc + (size_t)i * cols + j]) +
+                              fabs(w[(size_t)q * rc + (size_t)j * cols + i]);
+                    if (po <= pair_tol) continue;
+                    double a = 0.0, b = 0.0, c = 0.0, d = 0.0;
+                    for (int q = 0; q < k; ++q) {
+                        const double* m = w + (size_t)q * rc;
+                        double wt = fmax(pair_block_norm(m, cols, i, j), 1e-300);
+                        a += m[(size_t)i * cols + i] / wt;
+                        b += m[(size_t)i * cols + j] / wt;
+                        c += m[(size_t)j * cols + i] / wt;
+                        d += m[(size_t)j * cols + j] / wt;
+                    }
+                    double tl, trr;
+                    pair_svd_angles(a, b, c, d, &tl, &trr);
+                    if (fabs(tl) + fabs(trr) <= 1e-18) continue;
+                    double bef = 0.0;
+                    for (int q = 0; q < k; ++q) bef += local_offdiag_sq(w + (size_t)q * rc, rows, cols, i, j);
+                    double slack = 1e-14 * fmax(bef, 1.0);
+                    double scale = 1.0;
+                    int accepted = 0;
+                    for (int ls = 0; ls < lss; ++ls) {
+                        double al = tl * scale, ar = trr * scale;
+                        for (int q = 0; q < k; ++q) {
+                            rot_left(w + (size_t)q * rc, rows, cols, i, j, al);
+                            rot_right(w + (size_t)q * rc, rows, cols, i, j, ar);
+                        }
+                        double aft = 0.0;
+                        for (int q = 0; q < k; ++q) aft += local_offdiag_sq(w + (size_t)q * rc, rows, cols, i, j);
+                        if (aft <= bef + slack && isfinite(aft)) {
+                            rot_right(ub, rows, rows, i, j, al); /* apply_basis_rotor */
+                            rot_right(vb, cols, cols, i, j, ar);
+                            accepted = 1;
+                            break;
+                        }
+                        for (int q = 0; q < k; ++q) {
+                            rot_right(w + (size_t)q * rc, rows, cols, i, j, -ar);
+                            rot_left(w + (size_t)q * rc, rows, cols, i, j, -al);
+                        }
+                        scale *= 0.5;
+                    }
+                    if (accepted) ++tr.rotations;
+                    else ++tr.rejected_rotations;
+                }
+            }
+            double after = joint_offdiag_norm(w, k, rows, cols);
+            if (after <= tol || after >= before * (1.0 - 1e-10)) break;
+        }
+        tr.final_offdiag = joint_offdiag_norm(w, k, rows, cols);
+        free(pairs);
+    }
+    for (int q = 0; q < k; ++q)
+        for (int i = 0; i < kk; ++i) sig_out[(size_t)q * kk + i] = fabs(w[(size_t)q * rc + (size_t)i * cols + i]);
+    memcpy(u_out, ub, sizeof(double) * rows * rows);
+    for (int i = 0; i < cols; ++i) /* Vt = V^T */
+        for (int j = 0; j < cols; ++j) vt_out[(size_t)i * cols + j] = vb[(size_t)j * cols + i];
+    if (trace) *trace = tr;
+    free(w);
+    free(ub);
+    free(vb);
+}
+
 /* ------------------------------------------------------------------ */
 /* metrics::compute, src/lib.rs:149-194                                */
 /* ------------------------------------------------------------------ */

@@ -87,3 +87,80 @@ def test_joint_single_matrix_is_eigh():
     p = g + g.T
     v, d, tr = LieSvdJoint.diagonalize_symmetric_with_params([p], JointDiagonalizationParams(60, 1e-14))
     assert np.max(np.abs(np.sort(d[0]) - np.linalg.eigvalsh(p))) < 1e-11
+
+
+# ---- two-sided family SVD (LieSvdJoint::joint_svd_with_params) ------------
+
+def _planted_svd_family(rows, cols, k, seed, step_f=0.2, step_i=0.11):
+    r = np.random.default_rng(seed)
+    u0 = np.linalg.qr(r.standard_normal((rows, rows)))[0]
+    v0 = np.linalg.qr(r.standard_normal((cols, cols)))[0]
+    kk = min(rows, cols)
+    mats = []
+    for f in range(k):
+        sig = np.zeros((rows, cols))
+        sig[np.arange(kk), np.arange(kk)] = 1.0 + f * step_f + np.arange(kk) * step_i
+        mats.append(u0 @ sig @ v0.T)
+    return np.stack(mats), u0, v0
+
+
+@pytest.mark.parametrize("n,k,seed", [(10, 4, 150), (16, 3, 152), (33, 5, 153), (64, 4, 154)])
+def test_joint_svd_nonsymmetric_vs_oracle(n, k, seed):
+    """src/lie_svd_joint.rs:598-626 assertions, the planted factors and
+    singular values, and the oracle's outputs."""
+    from lie_cliffalg_analog_svd_amd import JointDiagonalizationParams as P
+    print("joint svd", n, k, seed, flush=True)
+    mats, u0, v0 = _planted_svd_family(n, n, k, seed)
+    u, sg, vt, tr = LieSvdJoint.joint_svd_with_params(list(mats), P(36 + n))
+    uo, so, vto, tro = oracle.joint_svd(mats, oracle.JointParams(36 + n, 1e-12, 1e-14, 6))
+    assert tr.final_offdiag < tr.initial_offdiag * 1e-8 and tr.rotations > 0
+    assert len(sg) == k
+    assert np.linalg.norm(u.T @ u - np.eye(n)) < 1e-10
+    assert np.linalg.norm(vt @ vt.T - np.eye(n)) < 1e-10
+    for f in range(k):
+        d = u.T @ mats[f] @ vt.T
+        assert np.linalg.norm(d - np.diag(np.diag(d))) < 1e-9 * np.linalg.norm(mats[f])
+        want = np.sort(1.0 + f * 0.2 + np.arange(n) * 0.11)
+        assert np.allclose(np.sort(sg[f]), want, atol=1e-9)
+        assert np.allclose(np.sort(sg[f]), np.sort(so[f]), atol=1e-10)
+    c = np.abs(u0.T @ u)
+    assert np.max(np.abs(1.0 - np.max(c, axis=1))) < 1e-9
+    # same shared bases as the oracle, up to the sign of each column pair
+    # (both follow the same rotation sequence)
+    assert np.max(np.abs(np.abs(u) - np.abs(uo))) < 1e-8
+    assert np.max(np.abs(np.abs(vt) - np.abs(vto))) < 1e-8
+
+
+@pytest.mark.parametrize("rows,cols,k,seed", [(9, 13, 3, 151), (20, 7, 4, 155), (1, 5, 2, 156)])
+def test_joint_svd_rectangular_vs_oracle(rows, cols, k, seed):
+    """src/lie_svd_joint.rs:628-660: the off-diagonal does not grow, U and V
+    stay orthogonal, sigmas have min(rows, cols) entries; oracle parity."""
+    from lie_cliffalg_analog_svd_amd import JointDiagonalizationParams as P
+    print("joint svd rect", rows, cols, k, seed, flush=True)
+    mats, u0, v0 = _planted_svd_family(rows, cols, k, seed, step_i=0.07)
+    kk = min(rows, cols)
+    u, sg, vt, tr = LieSvdJoint.joint_svd_with_params(list(mats), P(32))
+    uo, so, vto, tro = oracle.joint_svd(mats, oracle.JointParams(32, 1e-12, 1e-14, 6))
+    assert tr.final_offdiag <= tr.initial_offdiag + 1e-14
+    assert len(sg) == k and sg[0].shape == (kk,)
+    assert np.linalg.norm(u.T @ u - np.eye(rows)) < 1e-10
+    assert np.linalg.norm(vt @ vt.T - np.eye(cols)) < 1e-10
+    assert abs(tr.final_offdiag - tro["final_offdiag"]) <= 1e-9 * max(1.0, tro["initial_offdiag"])
+    for f in range(k):
+        assert np.allclose(np.sort(sg[f]), np.sort(so[f]), atol=1e-9)
+
+
+def test_joint_svd_batched_families_and_errors():
+    """Several independent families in one call equal one call each; the
+    reference's panics (empty set, size mismatch) are ShapeErrors."""
+    from lie_cliffalg_analog_svd_amd import svd_families
+    fams = np.stack([_planted_svd_family(12, 12, 3, 160 + f)[0] for f in range(5)])
+    u, sg, vt, tr = svd_families(fams)
+    for f in (0, 3):
+        u1, sg1, vt1, tr1 = svd_families(fams[f:f + 1])
+        assert np.allclose(u[f], u1[0], atol=1e-12) and np.allclose(sg[f], sg1[0], atol=1e-12)
+        assert tr[f].sweeps == tr1[0].sweeps
+    with pytest.raises(ShapeError):
+        LieSvdJoint.joint_svd([])
+    with pytest.raises(ShapeError):
+        LieSvdJoint.joint_svd([np.eye(3), np.eye(4)])

@@ -417,3 +417,83 @@ def test_svd_loss_grad_value_and_central_differences(m, n, seed):
         e /= np.linalg.norm(e)
         fd = (oracle.svd_loss_grad(a + h * e, w, g)[0] - oracle.svd_loss_grad(a - h * e, w, g)[0]) / (2 * h)
         assert abs(fd - np.sum(grad * e)) < 1e-7 * (1 + np.abs(grad).max())
+
+
+def _planted_joint_svd(rows, cols, k, seed, step_f=0.2, step_i=0.11):
+    """The reference tests' family: M_f = U0 Sigma_f V0^T with Haar U0, V0
+    and Sigma_f[i, i] = 1 + f step_f + i step_i."""
+    r = np.random.default_rng(seed)
+    u0 = np.linalg.qr(r.standard_normal((rows, rows)))[0]
+    v0 = np.linalg.qr(r.standard_normal((cols, cols)))[0]
+    kk = min(rows, cols)
+    mats = []
+    for f in range(k):
+        sig = np.zeros((rows, cols))
+        sig[np.arange(kk), np.arange(kk)] = 1.0 + f * step_f + np.arange(kk) * step_i
+        mats.append(u0 @ sig @ v0.T)
+    return np.stack(mats), u0, v0
+
+
+def test_joint_svd_nonsymmetric_family():
+    """src/lie_svd_joint.rs:598-626: off-diagonal falls by 1e-8, U and V
+    orthogonal; and the planted factors are recovered up to sign and order
+    with the planted singular values."""
+    mats, u0, v0 = _planted_joint_svd(10, 10, 4, 150)
+    p = oracle.joint_params_for_n(10)
+    p.max_sweeps = 36
+    u, sg, vt, tr = oracle.joint_svd(mats, p)
+    assert tr["final_offdiag"] < tr["initial_offdiag"] * 1e-8
+    assert tr["rotations"] > 0 and sg.shape == (4, 10)
+    assert np.linalg.norm(u.T @ u - np.eye(10)) < 1e-10
+    assert np.linalg.norm(vt @ vt.T - np.eye(10)) < 1e-10
+    for f in range(4):  # U^T M V is diagonal with the planted values
+        d = u.T @ mats[f] @ vt.T
+        assert np.linalg.norm(d - np.diag(np.diag(d))) < 1e-9
+        want = np.sort(1.0 + f * 0.2 + np.arange(10) * 0.11)
+        assert np.allclose(np.sort(sg[f]), want, atol=1e-9)
+    c = np.abs(u0.T @ u)  # recovery up to permutation and sign
+    assert np.max(np.abs(1.0 - np.max(c, axis=1))) < 1e-9
+
+
+def test_joint_svd_rectangular_family():
+    """src/lie_svd_joint.rs:628-660: a 9 x 13 family is accepted, the
+    off-diagonal does not grow, U and V stay orthogonal."""
+    mats, u0, v0 = _planted_joint_svd(9, 13, 3, 151, step_i=0.07)
+    p = oracle.joint_params_for_n(9)
+    p.max_sweeps = 32
+    u, sg, vt, tr = oracle.joint_svd(mats, p)
+    assert tr["final_offdiag"] <= tr["initial_offdiag"]
+    assert sg.shape == (3, 9)
+    assert np.linalg.norm(u.T @ u - np.eye(9)) < 1e-10
+    assert np.linalg.norm(vt @ vt.T - np.eye(13)) < 1e-10
+
+
+def test_joint_svd_single_matrix_singular_values():
+    """K = 1 square: the two-sided sweeps are a Jacobi SVD, |diag| equals
+    LAPACK's singular values (up to order).  Rectangular: only indices below
+    min(rows, cols) are ever paired (:214-216), so the bases stay the
+    identity outside the leading block."""
+    r = np.random.default_rng(9)
+    for n in (2, 8, 11):
+        a = r.standard_normal((1, n, n))
+        u, sg, vt, tr = oracle.joint_svd(a, oracle.joint_params_for_n(n))
+        want = np.linalg.svd(a[0], compute_uv=False)
+        assert np.allclose(np.sort(sg[0])[::-1], want, atol=1e-12 * want[0]), n
+        d = u.T @ a[0] @ vt.T
+        assert np.linalg.norm(d - np.diag(np.diag(d))) <= 1e-12 * np.linalg.norm(a[0]) * 10
+    for rows, cols in ((7, 5), (5, 9)):
+        a = r.standard_normal((2, rows, cols))
+        u, sg, vt, tr = oracle.joint_svd(a)
+        k = min(rows, cols)
+        assert np.array_equal(u[k:, k:], np.eye(rows - k)) and not np.any(u[:k, k:]) and not np.any(u[k:, :k])
+        assert np.array_equal(vt[k:, k:], np.eye(cols - k)) and not np.any(vt[:k, k:]) and not np.any(vt[k:, :k])
+        assert tr["final_offdiag"] <= tr["initial_offdiag"]
+
+
+def test_joint_svd_degenerate_sizes():
+    """min(rows, cols) <= 1: identity bases, |diagonal|, zero trace (:183-199)."""
+    a = np.array([[[-3.0, 1.0, 2.0]], [[4.0, 0.0, 1.0]]])
+    u, sg, vt, tr = oracle.joint_svd(a)
+    assert np.array_equal(u, np.eye(1)) and np.array_equal(vt, np.eye(3))
+    assert np.array_equal(sg, np.array([[3.0], [4.0]]))
+    assert tr["sweeps"] == 0 and tr["rotations"] == 0

@@ -182,6 +182,73 @@ def joint_leg(h, dev, args):
             "gpu_families_per_s": round(F / sec, 3)}
 
 
+def joint_svd_leg(h, dev, args):
+    """LieSvdJoint::joint_svd on the stress smoke families (K = 4 matrices
+    U0 diag(1 + 0.15 k + 0.05 i) V0^T, src/bin/stress_cpu.rs:597-626) at
+    n = 64 and 128, one family per call as the smoke runs it, then many
+    n = 128 families in one call.  Algorithmic traffic per layer: the K
+    matrices and both bases read and written once, 16 (K + 2) n^2 bytes."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib, workloads
+    from lie_cliffalg_analog_svd_amd.joint import JointDiagonalizationParams
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    K = 4
+
+    def gpu(fams):
+        F, _, n, _ = fams.shape
+        md = torch.from_numpy(fams).to(dev)
+        ud = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+        vd = torch.empty((F, n, n), dtype=torch.float64, device=dev)
+        sd = torch.empty((F, K, n), dtype=torch.float64, device=dev)
+        prm = JointDiagonalizationParams.for_n(n)._c()
+        tr = (_lib.LsvdJointTrace * F)()
+
+        def run():
+            h.check(_lib.lib().lsvd_joint_svd_f64(h.raw, F, K, n, n, ptr(md), C.byref(prm), ptr(ud), ptr(sd),
+                                                  ptr(vd), tr))
+        run()  # warm-up (kernel load, graph build)
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            run()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        return float(np.median(ts)), sd.cpu().numpy(), [tr[f].as_dict() for f in range(F)]
+
+    out = {"K": K, "smoke": []}
+    for n in (64, 128):
+        fam, _, _, d = workloads.joint_svd_family(n, K)
+        sec, sg, trs = gpu(fam[None])
+        t = trs[0]
+        serr = max(float(np.max(np.abs(np.sort(sg[0, k]) - np.sort(d[k])))) for k in range(K))
+        t1 = time.perf_counter()
+        _, osg, _, otr = oracle.joint_svd(fam)
+        cpu = time.perf_counter() - t1
+        out["smoke"].append({
+            "n": n, "gpu_s": round(sec, 5), "offdiag": [t["initial_offdiag"], t["final_offdiag"]],
+            "sweeps": t["sweeps"], "rotations": t["rotations"], "rejected": t["rejected_rotations"],
+            "sigma_err_vs_imposed": serr,
+            "sigma_diff_vs_oracle": max(float(np.max(np.abs(np.sort(sg[0, k]) - np.sort(osg[k])))) for k in range(K)),
+            "oracle": {"seconds": round(cpu, 4), "sweeps": otr["sweeps"], "final_offdiag": otr["final_offdiag"]},
+            "cpu_baseline": {"value": round(cpu, 4), "unit": "s/family", "cores": 1, "kind": "port",
+                             "sample": "the same family, C restatement of LieSvdJoint::joint_svd_with_params"}})
+        log(f"joint_svd n={n}: gpu {sec:.4f}s, oracle {cpu:.3f}s, sweeps {t['sweeps']}, "
+            f"offdiag {t['initial_offdiag']:.3e}->{t['final_offdiag']:.3e}")
+    F, n = args.joint_svd_families, 128
+    fams = np.stack([workloads.joint_svd_family(n, K, seed=workloads.SEED_JADE + 7 + f)[0] for f in range(F)])
+    sec, _, trs = gpu(fams)
+    layers = sum(t["sweeps"] for t in trs) * (n - 1)
+    byts = 16.0 * (K + 2) * n * n * layers
+    out["batched"] = {"families": F, "n": n, "seconds": round(sec, 4), "families_per_s": round(F / sec, 2),
+                      "sweeps_mean": float(np.mean([t["sweeps"] for t in trs])),
+                      "gbs": round(byts / sec / 1e9, 2), "frac_of_hbm": round(byts / sec / 1e9 / HBM_PEAK_GBS, 5),
+                      "final_offdiag_max": max(t["final_offdiag"] for t in trs)}
+    return out
+
+
 def vjp_leg(dev, args):
     """Gradient of L = sum_i w_i s_i + <G, U V^T> for 512 96 x 64 matrices
     with singular gaps in [1e-3, 2e-3] plus 32 with one gap of 1e-8."""
@@ -330,6 +397,8 @@ def main(argv=None) -> int:
     ap.add_argument("--single-reps", type=int, default=50)
     ap.add_argument("--no-joint", action="store_true", help="skip the Phase-JADE leg")
     ap.add_argument("--joint-families", type=int, default=32)
+    ap.add_argument("--no-joint-svd", action="store_true", help="skip the two-sided joint SVD leg")
+    ap.add_argument("--joint-svd-families", type=int, default=64)
     ap.add_argument("--no-vjp", action="store_true", help="skip the vjp leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 3 times (HBM counter calibration) and exit")
@@ -517,6 +586,9 @@ def main(argv=None) -> int:
     if not args.no_joint:
         legs["joint"] = joint_leg(h, dev, args)
         log(f"joint: {legs['joint']}")
+    if not args.no_joint_svd:
+        legs["joint_svd"] = joint_svd_leg(h, dev, args)
+        log(f"joint_svd: {legs['joint_svd']}")
     if not args.no_vjp:
         legs["vjp"] = vjp_leg(dev, args)
         log(f"vjp: {legs['vjp']}")

@@ -26,6 +26,7 @@
 #ifndef LIE_SVD_AMD_H
 #define LIE_SVD_AMD_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -220,6 +221,81 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
 int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols, const double* mats,
                        const lsvd_joint_params_t* params, double* u, double* sigmas, double* vt,
                        lsvd_joint_trace_t* trace);
+
+/* ---- Rotor-stream export (src/lie_svd_compiler.rs) ---------------------- */
+
+/* HardwareTarget (src/lie_svd_compiler.rs:12-25). */
+enum lsvd_hw_target { LSVD_TARGET_MZI_MESH = 0, LSVD_TARGET_FPGA_ROTOR_MESH = 1 };
+
+/* HardwarePhaseEvent::source and ::kind (src/lie_svd_compiler.rs:59-124,
+ * 270-290): the event-producing routes and the rotor kinds they name. */
+enum lsvd_event_source {
+    LSVD_SRC_REAL_PHASEFLOW = 0,
+    LSVD_SRC_COMPLEX_PHASEFLOW = 1,
+    LSVD_SRC_ORTHOGONAL_GIVENS = 2
+};
+enum lsvd_event_kind {
+    LSVD_KIND_GOLDEN_PRESPIN = 0,
+    LSVD_KIND_CAUSAL_ANTISPIN = 1,
+    LSVD_KIND_CROSS_PHASE_YINYANG = 2,
+    LSVD_KIND_PHASE_CONJUGATE = 3,
+    LSVD_KIND_BOTTLENECK = 4,
+    LSVD_KIND_PHASE_JUMP = 5,
+    LSVD_KIND_UNWRAP = 6,
+    LSVD_KIND_DIRECTIONAL = 7,
+    LSVD_KIND_SURGERY = 8,
+    LSVD_KIND_COMPLEX_GOLDEN_PRESPIN = 9,
+    LSVD_KIND_COMPLEX_DIAGONAL_PHASE = 10,
+    LSVD_KIND_GIVENS_ELIMINATION = 11
+};
+
+/* HardwarePhaseEvent (src/lie_svd_compiler.rs:27-41).  flags bit 0:
+ * energy_before present, bit 1: energy_after present (the Option fields). */
+typedef struct {
+    int32_t layer, i, j, flags;
+    double phi_l, phi_r, theta, theta_l, theta_r;
+    double energy_before, energy_after;
+    int32_t source; /* enum lsvd_event_source */
+    int32_t kind;   /* enum lsvd_event_kind */
+} lsvd_phase_event_t;
+
+/* Replaces the computation of HardwareSchedule::from_orthogonal_matrix
+ * (src/lie_svd_compiler.rs:144-197) for `count` square matrices at once:
+ * matrix b is d[b] x d[b] at v + v_off[b] (host arrays d and v_off;
+ * transpose != 0 eliminates each matrix's transpose, e.g. V read from a Vt
+ * pool).  Eliminations run in the reference's order (column j = 0..d-1,
+ * rows i = j+1..d-1; entries with |r_ij| <= 1e-300 skipped) with its
+ * arithmetic, so i, j, c, s and the signs equal the reference's bit for
+ * bit; the reference's theta is atan2(s, c).  Outputs, packed in matrix
+ * order (E_b = sum over earlier matrices of d (d - 1) / 2 event slots,
+ * S_b = sum over earlier matrices of d sign slots):
+ *   n_events[b]                         events of matrix b (event e's layer
+ *                                       is e, as in the reference)
+ *   ev_ij[2 (E_b + e)], [.. + 1]        i, j
+ *   ev_cs[2 (E_b + e)], [.. + 1]        c, s
+ *   ev_theta[E_b + e]                   atan2(s, c) by the device's atan2
+ *                                       (NULL: not computed)
+ *   signs[S_b + k]                      diagonal_signs: +-1, NaN for NaN
+ * Slots past n_events[b] are left unwritten.  d[b] <= 1024 (LSVD_E_LIMIT). */
+int lsvd_givens_schedule_f64(lsvd_handle_t h, int count, const int32_t* d, const int64_t* v_off, const double* v,
+                             int transpose, int32_t* n_events, int32_t* ev_ij, double* ev_cs, double* ev_theta,
+                             double* signs);
+
+/* Host memory only.  The events from_orthogonal_matrix records for n
+ * (i, j, c, s) tuples of lsvd_givens_schedule_f64 (src/lie_svd_compiler.rs:
+ * 173-186): layer e, theta = theta_l = atan2(s, c) by the host's libm (the
+ * function the reference calls), phi_l = phi_r = theta_r = 0, no energies. */
+int lsvd_givens_events(int n, const int32_t* ev_ij, const double* ev_cs, lsvd_phase_event_t* out);
+
+/* Host memory only.  HardwareSchedule::to_json_string
+ * (src/lie_svd_compiler.rs:213-268), format "phase-schedule-v1": writes at
+ * most cap - 1 bytes and a terminating NUL into buf (buf may be NULL when
+ * cap is 0) and returns the full length without the NUL.  n_signs = 0
+ * omits "diagonal_signs", as the reference does for an empty vector.
+ * Numbers are printed as Rust's {:.17e} and the signs as Rust's {}.
+ * Returns (size_t)-1 for an unknown target, source or kind. */
+size_t lsvd_schedule_json(int target, int channels, int n_events, const lsvd_phase_event_t* events, int n_signs,
+                          const double* signs, char* buf, size_t cap);
 
 /* ---- Ragged batch (no reference counterpart: the reference does one
  *      matrix per call; this runs many LieSvd::solve / solve_rectangular

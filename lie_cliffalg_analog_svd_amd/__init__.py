@@ -2,8 +2,9 @@
 
 Hand-written HIP kernels behind a C ABI (``include/lie_svd_amd.h``,
 ``liblsvd_amd.so``), with a Python mirror of the reference's Rust API:
-``LieSvdSmall``, ``LieSvdMicro``, ``LieSvd``, ``eigh_jacobi_full`` and the
-batch plans.  ``profiles`` and ``metrics`` restate the reference's input
+``LieSvdSmall``, ``LieSvdMicro``, ``LieSvd``, ``eigh_jacobi_full``, the
+batch plans, ``LieSvdJoint`` and the rotor-stream export
+(``HardwareSchedule``).  ``profiles`` and ``metrics`` restate the reference's input
 generators and residual metrics (src/lib.rs:133-388).
 """
 from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_SCALED, LSVD_PTR_DEVICE,
@@ -16,5 +17,7 @@ from .polar import newton_schulz_polar
 from .adjoint import polar_vjp, svd_loss_grad, svd_vjp
 from .joint import (JointDiagonalizationParams, JointDiagonalizationTrace, JointSvdTrace, LieSvdJoint,
                     diagonalize_families, svd_families)
+from .schedule import (ComplexMziPhase, ComplexPhaseEventKind, HardwarePhaseEvent, HardwareSchedule, HardwareTarget,
+                       MziPhase, PhaseRotorKind, givens_batch, schedules_from_orthogonal)
 
 __version__ = "0.1.0"

@@ -29,6 +29,9 @@ LSVD_PREC_F32 = 1
 LSVD_PREC_BF16 = 2
 LSVD_PREC_F64_SCALED = 3
 
+LSVD_TARGET_MZI_MESH = 0
+LSVD_TARGET_FPGA_ROTOR_MESH = 1
+
 
 class LsvdInfo(C.Structure):
     _fields_ = [
@@ -80,6 +83,17 @@ class LsvdPolarInfo(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class LsvdPhaseEvent(C.Structure):
+    """lsvd_phase_event_t (HardwarePhaseEvent, src/lie_svd_compiler.rs:27-41)."""
+    _fields_ = [
+        ("layer", C.c_int32), ("i", C.c_int32), ("j", C.c_int32), ("flags", C.c_int32),
+        ("phi_l", C.c_double), ("phi_r", C.c_double), ("theta", C.c_double),
+        ("theta_l", C.c_double), ("theta_r", C.c_double),
+        ("energy_before", C.c_double), ("energy_after", C.c_double),
+        ("source", C.c_int32), ("kind", C.c_int32),
+    ]
 
 
 class LsvdBucketDesc(C.Structure):
@@ -147,6 +161,10 @@ SIGNATURES = {
                                           _vp, _vp]),
     "lsvd_joint_svd_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdJointParams),
                                      _vp, _vp, _vp, C.POINTER(LsvdJointTrace)]),
+    "lsvd_givens_schedule_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
+    "lsvd_givens_events": (C.c_int, [C.c_int, _vp, _vp, C.POINTER(LsvdPhaseEvent)]),
+    "lsvd_schedule_json": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.POINTER(LsvdPhaseEvent), C.c_int, _vp,
+                                        C.c_char_p, C.c_size_t]),
     "lsvd_batch_plan_describe": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int), _vp]),
     "lsvd_batch_plan_time_buckets": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "lsvd_batch_plan_phase_clocks": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),

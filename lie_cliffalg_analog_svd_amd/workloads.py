@@ -105,6 +105,18 @@ def jade_batch(families: int = 32, K: int = 64, n: int = 256, eps: float = 1e-6,
     return np.stack(fams), qs, ds
 
 
+def joint_svd_family(n: int, K: int = 4, seed: int = SEED_JADE + 1):
+    """The two-sided family of the joint-SVD stress smoke
+    (src/bin/stress_cpu.rs:597-626): A_k = U0 diag(1 + 0.15 k + 0.05 i) V0^T
+    with Haar U0, V0 (n x n).  Returns (mats (K, n, n), U0, V0, d (K, n))."""
+    u0 = rng.haar(rng.mix_seed(seed, 2 * n), n)
+    v0 = rng.haar(rng.mix_seed(seed, 2 * n + 1), n)
+    i = np.arange(n, dtype=np.float64)
+    d = np.stack([1.0 + 0.15 * k + 0.05 * i for k in range(K)])
+    mats = np.einsum("ij,kj,lj->kil", u0, d, v0, optimize=True)
+    return np.ascontiguousarray(mats), u0, v0, d
+
+
 def recovery_error(q, v):
     """Distance between planted Q and recovered V up to column permutation
     and sign: match columns greedily by |Q^T V| (a permutation when the

@@ -8,6 +8,7 @@ skovnats/lie_cliffalg_analog_svd:
 - ``src/lie_svd_micro.rs:15-214``  (LieSvdMicro: n <= 4)
 - ``src/lie_svd_joint.rs:18-539``  (LieSvdJoint: Phase-JADE)
 - ``src/lib.rs:149-194``           (metrics::compute)
+- ``src/lie_svd_compiler.rs:144-268`` (HardwareSchedule: Givens export, JSON)
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
 ``cpu_baseline`` leg may import this package, and only as the checker or as
@@ -62,6 +63,18 @@ class JointTrace(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class PhaseEvent(C.Structure):
+    """HardwarePhaseEvent (src/lie_svd_compiler.rs:27-41)."""
+    _fields_ = [
+        ("layer", C.c_int), ("i", C.c_int), ("j", C.c_int),
+        ("phi_l", C.c_double), ("phi_r", C.c_double), ("theta", C.c_double),
+        ("theta_l", C.c_double), ("theta_r", C.c_double),
+        ("has_energy_before", C.c_int), ("has_energy_after", C.c_int),
+        ("energy_before", C.c_double), ("energy_after", C.c_double),
+        ("source", C.c_char_p), ("kind", C.c_char_p),
+    ]
+
+
 def build() -> None:
     """Compile the oracle with its Makefile (gcc, no GPU needed)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -103,6 +116,11 @@ def lib():
         L.lso_ns_iterations.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.lso_svd_loss_grad.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp]
         L.lso_svd_loss_grad.restype = C.c_double
+        L.lso_givens_schedule.argtypes = [dp, C.c_int, ip, ip, dp, dp, dp, dp]
+        L.lso_givens_schedule.restype = C.c_int
+        L.lso_schedule_json.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(PhaseEvent), C.c_int, dp,
+                                        C.c_char_p, C.c_size_t]
+        L.lso_schedule_json.restype = C.c_size_t
         _lib = L
     return _lib
 
@@ -310,3 +328,55 @@ def svd_loss_grad(a, w, g):
     grad = np.zeros((m, n))
     loss = lib().lso_svd_loss_grad(_dp(a), m, n, _dp(w), _dp(g), _dp(grad))
     return loss, grad
+
+
+def givens_schedule(v):
+    """HardwareSchedule::from_orthogonal_matrix (src/lie_svd_compiler.rs:
+    144-197) -> (events (n, 5): columns i, j, c, s, theta; diagonal signs).
+    Event e's layer is e."""
+    v = _c(v)
+    d = v.shape[0]
+    assert v.shape == (d, d), f"HardwareSchedule::from_orthogonal_matrix: matrix must be square, got {v.shape}"
+    cap = max(d * (d - 1) // 2, 1)
+    ei = np.zeros(cap, dtype=np.int32); ej = np.zeros(cap, dtype=np.int32)
+    ec = np.zeros(cap); es = np.zeros(cap); et = np.zeros(cap); sg = np.zeros(max(d, 1))
+    ip = C.POINTER(C.c_int)
+    n = lib().lso_givens_schedule(_dp(v), d, ei.ctypes.data_as(ip), ej.ctypes.data_as(ip), _dp(ec), _dp(es),
+                                  _dp(et), _dp(sg))
+    ev = np.zeros((n, 5))
+    ev[:, 0] = ei[:n]; ev[:, 1] = ej[:n]; ev[:, 2] = ec[:n]; ev[:, 3] = es[:n]; ev[:, 4] = et[:n]
+    return ev, sg[:d].copy()
+
+
+def schedule_json(target: str, channels: int, events, signs=()) -> str:
+    """HardwareSchedule::to_json_string (src/lie_svd_compiler.rs:213-268).
+    events: dicts with the HardwarePhaseEvent fields (energy_before /
+    energy_after optional)."""
+    arr = (PhaseEvent * max(len(events), 1))()
+    keep = []
+    for k, e in enumerate(events):
+        src = e["source"].encode(); kind = e["kind"].encode()
+        keep += [src, kind]
+        arr[k] = PhaseEvent(int(e["layer"]), int(e["i"]), int(e["j"]), float(e["phi_l"]), float(e["phi_r"]),
+                            float(e["theta"]), float(e["theta_l"]), float(e["theta_r"]),
+                            int(e.get("energy_before") is not None), int(e.get("energy_after") is not None),
+                            float(e.get("energy_before") or 0.0), float(e.get("energy_after") or 0.0), src, kind)
+    sg = _c(np.asarray(signs, dtype=np.float64).reshape(-1))
+    if sg.size == 0:
+        sg = np.zeros(1)
+        ns = 0
+    else:
+        ns = sg.size
+    L = lib()
+    need = L.lso_schedule_json(target.encode(), channels, len(events), arr, ns, _dp(sg), None, 0)
+    buf = C.create_string_buffer(need + 1)
+    L.lso_schedule_json(target.encode(), channels, len(events), arr, ns, _dp(sg), buf, need + 1)
+    return buf.value.decode()
+
+
+def givens_events(ev):
+    """The HardwarePhaseEvent dicts from_orthogonal_matrix records
+    (src/lie_svd_compiler.rs:172-186) for givens_schedule's events."""
+    return [{"layer": k, "i": int(r[0]), "j": int(r[1]), "phi_l": 0.0, "phi_r": 0.0, "theta": r[4],
+             "theta_l": r[4], "theta_r": 0.0, "source": "orthogonal_givens", "kind": "givens_elimination"}
+            for k, r in enumerate(ev)]

@@ -21,6 +21,7 @@
  * Array2<f64> layout.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1004,4 +1005,150 @@ double lso_svd_loss_grad(const double* a, int m, int n, const double* w, const d
     gemm(0, 0, m, n, n, t, vt, grad);
     free(u); free(s); free(vt); free(q); free(gv); free(x); free(mid); free(t); free(ux);
     return loss;
+}
+
+/* ---- HardwareSchedule (src/lie_svd_compiler.rs) -------------------------- */
+
+/* HardwareSchedule::from_orthogonal_matrix (src/lie_svd_compiler.rs:144-197).
+ * Givens elimination of the strict lower triangle of a d x d row-major
+ * matrix: columns j = 0..d-1 in order, rows i = j+1..d-1 in order, each
+ * rotation acting on rows j and i over all d columns; an entry with
+ * |r_ij| <= 1e-300 is skipped (:158-160).  Event e records (i, j, c, s,
+ * theta = atan2(s, c)); its layer is e (:175-187).  signs[k] = signum(r_kk)
+ * after the sweep (:189), Rust's f64::signum: +-1 from the sign bit, NaN for
+ * NaN.  Returns the event count. */
+int lso_givens_schedule(const double* v, int d, int* ev_i, int* ev_j, double* ev_c, double* ev_s,
+                        double* ev_theta, double* signs) {
+    double* r = dalloc((size_t)d * d);
+    memcpy(r, v, sizeof(double) * (size_t)d * d);
+    int ne = 0;
+    for (int j = 0; j < d; ++j) {
+        for (int i = j + 1; i < d; ++i) {
+            double a = r[(size_t)j * d + j];
+            double b = r[(size_t)i * d + j];
+            if (fabs(b) <= 1e-300) continue;
+            double norm = sqrt(a * a + b * b);
+            double c = a / norm;
+            double s = b / norm;
+            for (int k = 0; k < d; ++k) {
+                double rj = r[(size_t)j * d + k];
+                double ri = r[(size_t)i * d + k];
+                r[(size_t)j * d + k] = c * rj + s * ri;
+                r[(size_t)i * d + k] = -s * rj + c * ri;
+            }
+            ev_i[ne] = i;
+            ev_j[ne] = j;
+            ev_c[ne] = c;
+            ev_s[ne] = s;
+            ev_theta[ne] = atan2(s, c);
+            ++ne;
+        }
+    }
+    for (int k = 0; k < d; ++k) {
+        double x = r[(size_t)k * d + k];
+        signs[k] = isnan(x) ? x : copysign(1.0, x);
+    }
+    free(r);
+    return ne;
+}
+
+/* One event of HardwarePhaseEvent (src/lie_svd_compiler.rs:27-41). */
+typedef struct {
+    int layer, i, j;
+    double phi_l, phi_r, theta, theta_l, theta_r;
+    int has_energy_before, has_energy_after;
+    double energy_before, energy_after;
+    const char* source;
+    const char* kind;
+} lso_phase_event;
+
+typedef struct {
+    char* buf;
+    size_t cap, len;
+} lso_sbuf;
+
+static void sb_put(lso_sbuf* b, const char* s, size_t n) {
+    if (b->len + n < b->cap) memcpy(b->buf + b->len, s, n);
+    else if (b->len < b->cap) memcpy(b->buf + b->len, s, b->cap - 1 - b->len);
+    b->len += n;
+}
+static void sb_str(lso_sbuf* b, const char* s) { sb_put(b, s, strlen(s)); }
+
+/* Rust's `{:.17e}` (core::fmt::LowerExp with precision 17): the exact
+ * decimal rounded to 18 significant digits (as C's "%.17e"), then 'e' and
+ * the exponent with a '-' only when negative and no zero padding; the
+ * sign of -0.0 is kept; NaN -> "NaN", +-inf -> "inf" / "-inf". */
+static void sb_exp17(lso_sbuf* b, double x) {
+    char t[64];
+    if (isnan(x)) { sb_str(b, "NaN"); return; }
+    if (isinf(x)) { sb_str(b, x < 0 ? "-inf" : "inf"); return; }
+    snprintf(t, sizeof t, "%.17e", x);
+    char* e = strchr(t, 'e');
+    int ex = atoi(e + 1);
+    *e = 0;
+    sb_str(b, t);
+    snprintf(t, sizeof t, "e%d", ex);
+    sb_str(b, t);
+}
+
+/* Rust's Display of the signs (`{sign}`, :255): +-1 print as "1" / "-1",
+ * NaN as "NaN"; other values do not occur in a sign vector. */
+static void sb_sign(lso_sbuf* b, double x) {
+    char t[64];
+    if (isnan(x)) { sb_str(b, "NaN"); return; }
+    snprintf(t, sizeof t, "%.17g", x);
+    sb_str(b, t);
+}
+
+/* HardwareSchedule::to_json_string (src/lie_svd_compiler.rs:213-268).
+ * Writes at most cap - 1 bytes plus a NUL into buf; returns the full
+ * length. */
+size_t lso_schedule_json(const char* target, int channels, int n, const lso_phase_event* ev, int n_signs,
+                         const double* signs, char* buf, size_t cap) {
+    lso_sbuf b = {buf, cap, 0};
+    char t[128];
+    sb_str(&b, "{\n  \"format_version\": \"phase-schedule-v1\",\n  \"target\": \"");
+    sb_str(&b, target);
+    snprintf(t, sizeof t, "\",\n  \"channels\": %d,\n  \"total_events\": %d,\n  \"events\": [\n", channels, n);
+    sb_str(&b, t);
+    for (int e = 0; e < n; ++e) {
+        const lso_phase_event* p = ev + e;
+        snprintf(t, sizeof t, "    {\"layer\": %d, \"i\": %d, \"j\": %d, \"phi_l\": ", p->layer, p->i, p->j);
+        sb_str(&b, t);
+        sb_exp17(&b, p->phi_l);
+        sb_str(&b, ", \"phi_r\": ");
+        sb_exp17(&b, p->phi_r);
+        sb_str(&b, ", \"theta\": ");
+        sb_exp17(&b, p->theta);
+        sb_str(&b, ", \"theta_l\": ");
+        sb_exp17(&b, p->theta_l);
+        sb_str(&b, ", \"theta_r\": ");
+        sb_exp17(&b, p->theta_r);
+        sb_str(&b, ", \"source\": \"");
+        sb_str(&b, p->source);
+        sb_str(&b, "\", \"kind\": \"");
+        sb_str(&b, p->kind);
+        sb_str(&b, "\"");
+        if (p->has_energy_before) {
+            sb_str(&b, ", \"energy_before\": ");
+            sb_exp17(&b, p->energy_before);
+        }
+        if (p->has_energy_after) {
+            sb_str(&b, ", \"energy_after\": ");
+            sb_exp17(&b, p->energy_after);
+        }
+        sb_str(&b, e + 1 != n ? "},\n" : "}\n");
+    }
+    sb_str(&b, "  ]");
+    if (n_signs > 0) {
+        sb_str(&b, ",\n  \"diagonal_signs\": [");
+        for (int k = 0; k < n_signs; ++k) {
+            if (k) sb_str(&b, ", ");
+            sb_sign(&b, signs[k]);
+        }
+        sb_str(&b, "]");
+    }
+    sb_str(&b, "\n}\n");
+    if (cap) buf[b.len < cap ? b.len : cap - 1] = 0;
+    return b.len;
 }

@@ -217,7 +217,8 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
  * U^T M V), vt (cols x cols, V^T) and the trace (JointSvdTrace,
  * :53-59, same fields as lsvd_joint_trace_t).  params NULL uses
  * lsvd_joint_params_for_n(min(rows, cols)).  The two-sided rotations pair
- * indices below min(rows, cols) only, as the reference does. */
+ * indices below min(rows, cols) only, as the reference does.  cols <= 4096
+ * (LSVD_E_LIMIT beyond). */
 int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols, const double* mats,
                        const lsvd_joint_params_t* params, double* u, double* sigmas, double* vt,
                        lsvd_joint_trace_t* trace);

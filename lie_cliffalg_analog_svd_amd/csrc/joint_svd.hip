@@ -16,8 +16,11 @@
 //     new(M_ij^2 + M_ji^2) - old(M_ij^2 + M_ji^2) <= slack, with `before`
 //     assembled from per-row and per-column off-diagonal energies.
 // So a layer is `decide` (one wavefront per pair, lanes over K) and `apply`
-// (one workgroup per matrix or basis: every 2x2 block of the layer's row
-// and column groups rotated in place, then the row and column energies).
+// (one wavefront per row group of a matrix or basis -- a row pair, or two
+// rows that take no left rotor -- with its lanes over the layer's column
+// groups: every 2x2 block rotated in place, the two rows' off-diagonal
+// energies by wave sums, and column sums of squares per workgroup, which
+// `decide` adds up over the row blocks).
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -80,7 +83,7 @@ __global__ __launch_bounds__(256) void jsvd_init_kernel(JsvdArgs a) {
     }
     // row and column off-diagonal energies
     double* ro = a.rowoff + ((size_t)f * a.K + k) * R;
-    double* co = a.coloff + ((size_t)f * a.K + k) * Cc;
+    double* co = a.colpart + ((size_t)f * a.K + k) * a.rb * Cc;
     for (int r = threadIdx.x; r < R; r += blockDim.x) {
         double s = 0.0;
         for (int c = 0; c < Cc; ++c)
@@ -89,9 +92,9 @@ __global__ __launch_bounds__(256) void jsvd_init_kernel(JsvdArgs a) {
     }
     for (int c = threadIdx.x; c < Cc; c += blockDim.x) {
         double s = 0.0;
-        for (int r = 0; r < R; ++r)
-            if (r != c) s += m[(size_t)r * Cc + c] * m[(size_t)r * Cc + c];
+        for (int r = 0; r < R; ++r) s += m[(size_t)r * Cc + c] * m[(size_t)r * Cc + c];
         co[c] = s;
+        for (int b = 1; b < a.rb; ++b) co[(size_t)b * Cc + c] = 0.0;
     }
     if (k == 0) {
         double* u = a.ubasis + (size_t)f * R * R;
@@ -155,9 +158,13 @@ __global__ __launch_bounds__(64) void jsvd_decide_kernel(JsvdArgs a, int layer) 
         sc += cc / wt;
         sd += dd / wt;
         const double* ro = a.rowoff + ((size_t)f * K + k) * R;
-        const double* co = a.coloff + ((size_t)f * K + k) * Cc;
-        loc += ro[i] + ro[j] + (co[i] - cc * cc) + (co[j] - bb * bb);
+        // coloff_i = S_i - M_ii^2 with S the column sums of squares
+        loc += ro[i] + ro[j] - (aa * aa + cc * cc) - (dd * dd + bb * bb);
         old2 += bb * bb + cc * cc;
+    }
+    for (int t = lane; t < K * a.rb; t += kWave) {
+        const double* co = a.colpart + ((size_t)f * K * a.rb + t) * Cc;
+        loc += co[i] + co[j];
     }
     poff = wave_sum(poff);
     sa = wave_sum(sa);
@@ -219,86 +226,111 @@ __global__ __launch_bounds__(64) void jsvd_decide_kernel(JsvdArgs a, int layer) 
     }
 }
 
-// ---- apply: one workgroup per matrix (x < K), U basis (x == K) or V basis
-// (x == K + 1).  Rows (resp. columns) below kk pair as the layer says;
-// every other index is its own group.  Each (row group, column group)
-// block is read, rotated and written by one thread, in place.
+// ---- apply: blockIdx.x < K rb: matrix blockIdx.x / rb, row block
+// blockIdx.x % rb; then the U basis' rbu blocks, then the V basis' rbv.
+// One wavefront per row task (see jsvd_matrix_row_tasks); lanes over the
+// column groups: the layer's slots (a pair, or the odd index alone), then
+// the columns >= kk alone.  U's columns take the left angle, V's and the
+// matrices' columns the right angle; only the matrices' rows take the left
+// angle.  Every block is read, rotated and written by one lane, in place.
 __global__ __launch_bounds__(256) void jsvd_apply_kernel(JsvdArgs a, int layer) {
-    const int f = blockIdx.y, x = blockIdx.x;
+    extern __shared__ double colp[];  // kJsvdTasks x cols column sums
+    const int f = blockIdx.y;
     if (a.fam[f].done) return;
     const int K = a.K, kk = a.kk;
-    const bool is_u = x == K, is_v = x == K + 1;
-    const int R = is_u ? a.rows : (is_v ? a.cols : a.rows);
-    const int Cc = is_u ? a.rows : (is_v ? a.cols : a.cols);
-    double* w = is_u ? a.ubasis + (size_t)f * a.rows * a.rows
-                     : (is_v ? a.vbasis + (size_t)f * a.cols * a.cols : a.work + ((size_t)f * K + x) * a.rows * a.cols);
+    int bx = blockIdx.x;
+    int kind, x = 0;  // 0 matrix x, 1 U, 2 V
+    if (bx < K * a.rb) {
+        kind = 0;
+        x = bx / a.rb;
+        bx -= x * a.rb;
+    } else if (bx < K * a.rb + a.rbu) {
+        kind = 1;
+        bx -= K * a.rb;
+    } else {
+        kind = 2;
+        bx -= K * a.rb + a.rbu;
+    }
+    const int R = kind == 2 ? a.cols : a.rows;
+    const int Cc = kind == 1 ? a.rows : a.cols;
+    double* w = kind == 1 ? a.ubasis + (size_t)f * a.rows * a.rows
+                          : (kind == 2 ? a.vbasis + (size_t)f * a.cols * a.cols
+                                       : a.work + ((size_t)f * K + x) * a.rows * a.cols);
     const JsvdRot* rt = a.rot + (size_t)f * a.slots;
-    // groups: index g is the first of its group when it is unpaired or
-    // smaller than its partner
-    const int ng_r = R, ng_c = Cc;
-    for (int e = threadIdx.x; e < ng_r * ng_c; e += blockDim.x) {
-        const int r0 = e / ng_c, c0 = e % ng_c;
-        // row group of r0 (left rotor; bases take none)
-        int r1 = -1;
-        double cl = 1.0, sl = 0.0;
-        if (!is_u && !is_v && r0 < kk) {
-            const int p = jsvd_partner(kk, layer, r0);
-            if (p >= 0) {
-                if (p < r0) continue;  // the group's first index owns it
-                r1 = p;
-                const JsvdRot q = rt[jsvd_slot(kk, layer, r0)];
-                cl = q.cl;
-                sl = q.sl;
-            }
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int task = bx * kJsvdTasks + wv;
+    // the wave's rows r0, r1 (-1: none) and their left rotor
+    int r0 = -1, r1 = -1;
+    double cl = 1.0, sl = 0.0;
+    if (kind == 0 && task < a.slots) {
+        int p, q;
+        rr_pair(kk, layer, task, p, q);
+        r0 = p;
+        if (q >= 0) {
+            r1 = q;
+            const JsvdRot t = rt[task];
+            cl = t.cl;
+            sl = t.sl;
         }
-        // column group of c0 (right rotor: U takes the left angle, V the right)
-        int c1 = -1;
+    } else {
+        const int base = kind == 0 ? kk + 2 * (task - a.slots) : 2 * task;
+        if (base < R) r0 = base;
+        if (base + 1 < R) r1 = base + 1;
+    }
+    double e0 = 0.0, e1 = 0.0;  // off-diagonal energies of rows r0, r1
+    const int ctasks = a.slots + (Cc - kk);
+    for (int t = lane; t < ctasks; t += kWave) {
+        int c0, c1 = -1;
         double cr = 1.0, sr = 0.0;
-        if (c0 < kk) {
-            const int p = jsvd_partner(kk, layer, c0);
-            if (p >= 0) {
-                if (p < c0) continue;
-                c1 = p;
-                const JsvdRot q = rt[jsvd_slot(kk, layer, c0)];
-                cr = is_u ? q.cl : q.cr;
-                sr = is_u ? q.sl : q.sr;
+        if (t < a.slots) {
+            int p, q;
+            rr_pair(kk, layer, t, p, q);
+            c0 = p;
+            if (q >= 0) {
+                c1 = q;
+                const JsvdRot g = rt[t];
+                cr = kind == 1 ? g.cl : g.cr;
+                sr = kind == 1 ? g.sl : g.sr;
             }
+        } else {
+            c0 = kk + (t - a.slots);
         }
-        double* p00 = w + (size_t)r0 * Cc + c0;
-        const double b00 = *p00;
-        const double b01 = c1 >= 0 ? w[(size_t)r0 * Cc + c1] : 0.0;
-        const double b10 = r1 >= 0 ? w[(size_t)r1 * Cc + c0] : 0.0;
-        const double b11 = (r1 >= 0 && c1 >= 0) ? w[(size_t)r1 * Cc + c1] : 0.0;
+        const bool h0 = r0 >= 0, h1 = r1 >= 0, k1 = c1 >= 0;
+        const double b00 = h0 ? w[(size_t)r0 * Cc + c0] : 0.0;
+        const double b01 = (h0 && k1) ? w[(size_t)r0 * Cc + c1] : 0.0;
+        const double b10 = h1 ? w[(size_t)r1 * Cc + c0] : 0.0;
+        const double b11 = (h1 && k1) ? w[(size_t)r1 * Cc + c1] : 0.0;
         // left: row_i = c row_i - s row_j, row_j = s row_i + c row_j
         const double l00 = cl * b00 - sl * b10, l01 = cl * b01 - sl * b11;
         const double l10 = sl * b00 + cl * b10, l11 = sl * b01 + cl * b11;
         // right: col_i = c col_i - s col_j, col_j = s col_i + c col_j
-        *p00 = cr * l00 - sr * l01;
-        if (c1 >= 0) w[(size_t)r0 * Cc + c1] = sr * l00 + cr * l01;
-        if (r1 >= 0) w[(size_t)r1 * Cc + c0] = cr * l10 - sr * l11;
-        if (r1 >= 0 && c1 >= 0) w[(size_t)r1 * Cc + c1] = sr * l10 + cr * l11;
+        const double y00 = cr * l00 - sr * l01, y01 = sr * l00 + cr * l01;
+        const double y10 = cr * l10 - sr * l11, y11 = sr * l10 + cr * l11;
+        if (h0) w[(size_t)r0 * Cc + c0] = y00;
+        if (h0 && k1) w[(size_t)r0 * Cc + c1] = y01;
+        if (h1) w[(size_t)r1 * Cc + c0] = y10;
+        if (h1 && k1) w[(size_t)r1 * Cc + c1] = y11;
+        if (kind == 0) {
+            e0 += (c0 != r0 ? y00 * y00 : 0.0) + (k1 && c1 != r0 ? y01 * y01 : 0.0);
+            e1 += (c0 != r1 ? y10 * y10 : 0.0) + (k1 && c1 != r1 ? y11 * y11 : 0.0);
+            colp[wv * Cc + c0] = y00 * y00 + y10 * y10;
+            if (k1) colp[wv * Cc + c1] = y01 * y01 + y11 * y11;
+        }
     }
-    if (is_u || is_v) return;
+    if (kind != 0) return;
+    e0 = wave_sum(e0);
+    e1 = wave_sum(e1);
+    double* ro = a.rowoff + ((size_t)f * K + x) * a.rows;
+    if (lane == 0) {
+        if (r0 >= 0) ro[r0] = e0;
+        if (r1 >= 0) ro[r1] = e1;
+    }
     __syncthreads();
-    // row and column off-diagonal energies of the rotated matrix
-    double* ro = a.rowoff + ((size_t)f * K + x) * R;
-    double* co = a.coloff + ((size_t)f * K + x) * Cc;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int r = wv; r < R; r += 4) {  // a wavefront per row
+    double* co = a.colpart + (((size_t)f * K + x) * a.rb + bx) * Cc;
+    for (int c = threadIdx.x; c < Cc; c += blockDim.x) {
         double s = 0.0;
-        for (int c = lane; c < Cc; c += 64) {
-            const double y = w[(size_t)r * Cc + c];
-            if (c != r) s += y * y;
-        }
-        s = wave_sum(s);
-        if (lane == 0) ro[r] = s;
-    }
-    for (int c = threadIdx.x; c < Cc; c += blockDim.x) {  // a thread per column
-        double s = 0.0;
-        for (int r = 0; r < R; ++r) {
-            const double y = w[(size_t)r * Cc + c];
-            if (r != c) s += y * y;
-        }
+#pragma unroll
+        for (int v = 0; v < kJsvdTasks; ++v) s += colp[v * Cc + c];
         co[c] = s;
     }
 }
@@ -341,6 +373,12 @@ __global__ void jsvd_final_kernel(JsvdArgs a, double* sig, double* vt) {
 
 // ---- launches ---------------------------------------------------------------
 hipError_t jsvd_launch_init(const JsvdArgs& a, hipStream_t st) {
+    const size_t lds = sizeof(double) * kJsvdTasks * a.cols;  // the apply launches of the sweep graph
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&jsvd_apply_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(jsvd_init_kernel, dim3(a.K, a.F), dim3(256), 0, st, a);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -354,7 +392,8 @@ hipError_t jsvd_launch_sweep(const JsvdArgs& a, hipStream_t st) {
         hipLaunchKernelGGL(jsvd_decide_kernel, dim3(a.slots, a.F), dim3(64), 0, st, a, layer);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
-        hipLaunchKernelGGL(jsvd_apply_kernel, dim3(a.K + 2, a.F), dim3(256), 0, st, a, layer);
+        hipLaunchKernelGGL(jsvd_apply_kernel, dim3(a.K * a.rb + a.rbu + a.rbv, a.F), dim3(64 * kJsvdTasks),
+                           sizeof(double) * kJsvdTasks * a.cols, st, a, layer);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

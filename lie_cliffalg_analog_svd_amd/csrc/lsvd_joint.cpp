@@ -178,6 +178,7 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
     if (rows == 0 || cols == 0) return set_error(h, LSVD_E_SHAPE, "LieSvdJoint: empty matrix");
     if (families == 0) return LSVD_OK;
     if (!mats || !u || !sigmas || !vt) return set_error(h, LSVD_E_ARG, "null pointer");
+    if (cols > 4096) return set_error(h, LSVD_E_LIMIT, "LieSvdJoint::joint_svd: cols > 4096 (LDS column sums)");
     const int kk = std::min(rows, cols);
     lsvd_joint_params_t prm;
     if (params) prm = *params;
@@ -211,7 +212,8 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
     JCHECK(bu.ensure(sizeof(double) * families * uu));
     JCHECK(bv.ensure(sizeof(double) * families * vv));
     JCHECK(brow.ensure(sizeof(double) * (size_t)families * k * rows));
-    JCHECK(bcol.ensure(sizeof(double) * (size_t)families * k * cols));
+    const int rb = jsvd_blocks(jsvd_matrix_row_tasks(slots, rows, kk));
+    JCHECK(bcol.ensure(sizeof(double) * (size_t)families * k * rb * cols));
     JCHECK(bpart.ensure(sizeof(double) * (size_t)families * k * 2));
     JCHECK(brot.ensure(sizeof(JsvdRot) * (size_t)families * std::max(slots, 1)));
     JCHECK(bfam.ensure(sizeof(JointFamily) * families));
@@ -228,7 +230,10 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
     a.ubasis = (double*)bu.p;
     a.vbasis = (double*)bv.p;
     a.rowoff = (double*)brow.p;
-    a.coloff = (double*)bcol.p;
+    a.colpart = (double*)bcol.p;
+    a.rb = rb;
+    a.rbu = jsvd_blocks(jsvd_basis_row_tasks(rows));
+    a.rbv = jsvd_blocks(jsvd_basis_row_tasks(cols));
     a.part = (double*)bpart.p;
     a.rot = (JsvdRot*)brot.p;
     a.fam = (JointFamily*)bfam.p;

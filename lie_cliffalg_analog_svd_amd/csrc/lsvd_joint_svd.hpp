@@ -24,13 +24,25 @@ struct JsvdArgs {
     double* ubasis;      // F x rows x rows
     double* vbasis;      // F x cols x cols
     double* rowoff;      // F x K x rows off-diagonal row energies
-    double* coloff;      // F x K x cols off-diagonal column energies
+    double* colpart;     // F x K x rb x cols column sums of squares (diagonal
+                         // included), one partial per row block
+    int rb;              // row blocks per matrix in the apply launch
+    int rbu, rbv;        // row blocks of the U and V bases
     double* part;        // F x K x 2 (Frobenius^2, off-diagonal^2) partials
     JsvdRot* rot;        // F x slots, current layer
     JointFamily* fam;    // F
     double tol, min_pair_scale;
     int line_search_steps, max_sweeps;
 };
+
+// Apply launch geometry: one wavefront per row task, kJsvdTasks tasks per
+// workgroup.  A matrix has the layer's slots (a row pair, or the odd index
+// alone) and then its rows >= kk two at a time; a basis has its rows two
+// at a time.
+constexpr int kJsvdTasks = 4;
+inline int jsvd_blocks(int tasks) { return (tasks + kJsvdTasks - 1) / kJsvdTasks; }
+inline int jsvd_matrix_row_tasks(int slots, int rows, int kk) { return slots + (rows - kk + 1) / 2; }
+inline int jsvd_basis_row_tasks(int n) { return (n + 1) / 2; }
 
 hipError_t jsvd_launch_init(const JsvdArgs& a, hipStream_t st);
 hipError_t jsvd_launch_sweep(const JsvdArgs& a, hipStream_t st);

@@ -249,6 +249,67 @@ def joint_svd_leg(h, dev, args):
     return out
 
 
+def givens_leg(h, vt_pool, lay, ms, ns, outs):
+    """Rotor-stream export (HardwareSchedule::from_orthogonal_matrix,
+    src/lie_svd_compiler.rs:144-197) of every V the batch returned: N x N,
+    read transposed from the Vt pool in HBM (a wide input's thin Vt is not
+    square and is skipped).  One call for the whole set, device pointers;
+    the call includes its item upload and synchronizes.  Parity: i, j, c, s
+    and the signs bit-exact to the oracle on every 40th matrix."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib
+    sel = np.nonzero(ms >= ns)[0]
+    d = np.ascontiguousarray(ns[sel].astype(np.int32))
+    v_off = np.ascontiguousarray(np.asarray(lay["vt_off"], dtype=np.int64)[sel])
+    E = int(np.sum(d.astype(np.int64) * (d - 1) // 2))
+    S = int(np.sum(d))
+    dev = vt_pool.device
+    n_ev = torch.zeros(len(sel), dtype=torch.int32, device=dev)
+    ij = torch.zeros(2 * max(E, 1), dtype=torch.int32, device=dev)
+    cs = torch.zeros(2 * max(E, 1), dtype=torch.float64, device=dev)
+    th = torch.zeros(max(E, 1), dtype=torch.float64, device=dev)
+    sg = torch.zeros(max(S, 1), dtype=torch.float64, device=dev)
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+
+    def run():
+        h.check(_lib.lib().lsvd_givens_schedule_f64(h.raw, len(sel), d.ctypes.data_as(_lib._i32p),
+                                                    v_off.ctypes.data_as(_lib._i64p), ptr(vt_pool), 1, ptr(n_ev),
+                                                    ptr(ij), ptr(cs), ptr(th), ptr(sg)))
+    run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    sec = float(np.median(ts))
+    nev = n_ev.cpu().numpy()
+    ij_h, cs_h, sg_h = ij.cpu().numpy().reshape(-1, 2), cs.cpu().numpy().reshape(-1, 2), sg.cpu().numpy()
+    ev_off = np.concatenate([[0], np.cumsum(d.astype(np.int64) * (d - 1) // 2)])
+    sg_off = np.concatenate([[0], np.cumsum(d.astype(np.int64))])
+    sample = list(range(0, len(sel), 40))
+    mism = 0
+    t1 = time.perf_counter()
+    ref = [oracle.givens_schedule(np.ascontiguousarray(outs[sel[b]][2].T)) for b in sample]
+    cpu = time.perf_counter() - t1
+    for b, (g, so) in zip(sample, ref):
+        e0, n = int(ev_off[b]), int(nev[b])
+        ok = (n == len(g) and np.array_equal(ij_h[e0:e0 + n], g[:, :2].astype(np.int32))
+              and np.array_equal(cs_h[e0:e0 + n], g[:, 2:4])
+              and np.array_equal(sg_h[int(sg_off[b]):int(sg_off[b]) + int(d[b])], so))
+        mism += 0 if ok else 1
+    cpu_rate = len(sample) / cpu
+    return {"matrices": int(len(sel)), "rotations": int(nev.sum()), "d_range": [int(d.min()), int(d.max())],
+            "seconds": round(sec, 5), "matrices_per_s": round(len(sel) / sec, 1),
+            "rotations_per_s": round(float(nev.sum()) / sec, 1),
+            "bit_exact_sample": f"{len(sample) - mism}/{len(sample)}",
+            "cpu_baseline": {"value": round(cpu_rate, 2), "unit": "matrices/s", "cores": 1, "kind": "port",
+                             "sample": f"every 40th of the same V set ({len(sample)} matrices), C restatement of "
+                                       f"from_orthogonal_matrix, {cpu:.2f}s"}}
+
+
 def vjp_leg(dev, args):
     """Gradient of L = sum_i w_i s_i + <G, U V^T> for 512 96 x 64 matrices
     with singular gaps in [1e-3, 2e-3] plus 32 with one gap of 1e-8."""
@@ -399,6 +460,7 @@ def main(argv=None) -> int:
     ap.add_argument("--joint-families", type=int, default=32)
     ap.add_argument("--no-joint-svd", action="store_true", help="skip the two-sided joint SVD leg")
     ap.add_argument("--joint-svd-families", type=int, default=64)
+    ap.add_argument("--no-givens", action="store_true", help="skip the rotor-stream export leg")
     ap.add_argument("--no-vjp", action="store_true", help="skip the vjp leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 3 times (HBM counter calibration) and exit")
@@ -589,6 +651,9 @@ def main(argv=None) -> int:
     if not args.no_joint_svd:
         legs["joint_svd"] = joint_svd_leg(h, dev, args)
         log(f"joint_svd: {legs['joint_svd']}")
+    if not args.no_givens:
+        legs["givens"] = givens_leg(h, vt_pool, lay, ms, ns, outs)
+        log(f"givens: {legs['givens']}")
     if not args.no_vjp:
         legs["vjp"] = vjp_leg(dev, args)
         log(f"vjp: {legs['vjp']}")

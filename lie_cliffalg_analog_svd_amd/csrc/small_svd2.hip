@@ -47,6 +47,11 @@ struct Cfg2 {
     static constexpr int NTILE = NP / 16;
     static constexpr int LDM = NP + 1;
     static constexpr int SLOTS = NP / 2;
+    // (c, s) of the layer's slots, grouped by the lanes of a V row (CSG
+    // slots each) with an odd group stride in 16-byte units, so the 8
+    // groups a wave reads fall in distinct LDS banks
+    static constexpr int CSG = CH / 2;
+    static constexpr int CSP = (CSG % 2 == 0) ? CSG + 1 : CSG;
     static constexpr int LDP = 17;  // panel leading dimension (16 columns)
     static constexpr int PANEL = (NP * LDP > 16 * LDM) ? NP * LDP : 16 * LDM;
 };
@@ -57,12 +62,18 @@ struct Sh2 {
     double diag[NP];
     double tau[NP];
     double scr[NP];
-    double2 cs[2][NP / 2];  // per layer parity: (c, s oriented to the slot order)
+    double2 cs[2][Cfg2<NP>::TPR * Cfg2<NP>::CSP];  // per layer parity: (c, s oriented to the slot order),
+                                                   // slot k at csi(k)
     double tv[2][NP / 2];   // oriented t of the slot's diagonal update
     int rf[2][NP / 2];      // rotation applied (|a_pq| >= 1e-300)
     int rank[NP];
     int item;
 };
+
+template <int NP>
+__device__ __forceinline__ int csi(int k) {
+    return (k / Cfg2<NP>::CSG) * Cfg2<NP>::CSP + k % Cfg2<NP>::CSG;
+}
 
 __device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
@@ -383,11 +394,17 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     // when the move crosses the diagonal): one read, four writes per block
     // and two barriers per layer.  V's update for a layer is deferred to the
     // next layer's first phase, where it overlaps the angle computation.
+    // Within a row, even positions are stored first and odd ones after
+    // them (pcol), so the lanes of a wave, which hold consecutive k_b, read
+    // and write consecutive doubles (a stride of two doubles would put two
+    // lanes of each half-wave on one LDS bank).
+    constexpr int HP = NP / 2;
+    auto pcol = [](int c) { return (c & 1) * HP + (c >> 1); };
     for (int e = tid; e < NP * NP; e += NT) ys[e] = main[(e / NP) * LDM + e % NP];
     __syncthreads();
     for (int e = tid; e < NP * NP; e += NT) {
         const int g = e / NP, hh = e % NP;
-        main[g * LDM + hh] = ys[ring_idx0(NP, g) * NP + ring_idx0(NP, hh)];
+        main[g * LDM + pcol(hh)] = ys[ring_idx0(NP, g) * NP + ring_idx0(NP, hh)];
     }
     __syncthreads();
     const int vrow = tid / TPR, vch = tid % TPR;
@@ -405,7 +422,8 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     ref = fmax(sqrt(block_sum<NT>(ref, sh.red)), 1e-300);
     constexpr int NBLK = SLOTS * (SLOTS + 1) / 2;
     constexpr int MB = (NBLK + NT - 1) / NT;
-    // per block: packed (k_a, k_b) and the four upper-triangle write offsets
+    // per block: packed (k_a, k_b, csi(k_a), csi(k_b)) and the four
+    // upper-triangle write offsets
     int bk[MB], bw01[MB], bw23[MB];
 #pragma unroll
     for (int i = 0; i < MB; ++i) {
@@ -418,8 +436,8 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         const int kb = ka + rem;
         const int da0 = ring_dst(NP, 2 * ka), da1 = ring_dst(NP, 2 * ka + 1);
         const int db0 = ring_dst(NP, 2 * kb), db1 = ring_dst(NP, 2 * kb + 1);
-        auto up = [](int r, int c) { return r <= c ? r * LDM + c : c * LDM + r; };
-        bk[i] = bI < NBLK ? (ka | (kb << 8)) : -1;
+        auto up = [&](int r, int c) { return r <= c ? r * LDM + pcol(c) : c * LDM + pcol(r); };
+        bk[i] = bI < NBLK ? (ka | (kb << 8) | (csi<NP>(ka) << 16) | (csi<NP>(kb) << 24)) : -1;
         bw01[i] = up(da0, db0) | (up(da0, db1) << 16);
         bw23[i] = up(da1, db0) | (up(da1, db1) << 16);
     }
@@ -432,7 +450,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     auto v_update = [&](int buf) {
 #pragma unroll
         for (int j = 0; j < CH / 2; ++j) {
-            const double2 r = sh.cs[buf][vch * (CH / 2) + j];
+            const double2 r = sh.cs[buf][vch * Cfg2<NP>::CSP + j];
             const double x = v[2 * j], y = v[2 * j + 1];
             v[2 * j] = r.x * x - r.y * y;
             v[2 * j + 1] = r.y * x + r.x * y;
@@ -453,7 +471,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         double off = 0.0;
         for (int e = tid; e < NP * NP; e += NT) {
             const int g = e / NP, hh = e % NP;
-            const double x = main[g * LDM + hh];
+            const double x = main[g * LDM + pcol(hh)];
             if (g < hh) off += x * x;
         }
         off = block_sum<NT>(off, sh.red);
@@ -468,10 +486,10 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                 int a, b;
                 ring_pair(NP, L, k, a, b);
                 double c, sv, tv;
-                const int rot = jacobi_angle(main[2 * k * LDM + 2 * k], main[(2 * k + 1) * LDM + 2 * k + 1],
-                                             main[2 * k * LDM + 2 * k + 1], a < b, c, sv, tv);
+                const int rot = jacobi_angle(main[2 * k * LDM + k], main[(2 * k + 1) * LDM + HP + k],
+                                             main[2 * k * LDM + HP + k], a < b, c, sv, tv);
                 rots += rot;
-                sh.cs[buf][k] = make_double2(c, sv);
+                sh.cs[buf][csi<NP>(k)] = make_double2(c, sv);
                 sh.tv[buf][k] = tv;
                 sh.rf[buf][k] = rot;
             }
@@ -479,12 +497,12 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
 #pragma unroll
             for (int i = 0; i < MB; ++i) {
                 if (bk[i] >= 0) {
-                    const int ka = bk[i] & 255, kb = bk[i] >> 8;
-                    const double* r0 = main + 2 * ka * LDM + 2 * kb;
+                    const int ka = bk[i] & 255, kb = (bk[i] >> 8) & 255;
+                    const double* r0 = main + 2 * ka * LDM + kb;
                     blk[i][0] = r0[0];
-                    blk[i][1] = r0[1];
-                    blk[i][2] = ka == kb ? r0[1] : r0[LDM];
-                    blk[i][3] = r0[LDM + 1];
+                    blk[i][1] = r0[HP];
+                    blk[i][2] = ka == kb ? r0[HP] : r0[LDM];
+                    blk[i][3] = r0[LDM + HP];
                 }
             }
             if (L > 0) v_update(buf ^ 1);
@@ -493,7 +511,8 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
 #pragma unroll
             for (int i = 0; i < MB; ++i) {
                 if (bk[i] < 0) continue;
-                const int ka = bk[i] & 255, kb = bk[i] >> 8;
+                const int ka = bk[i] & 255, kb = (bk[i] >> 8) & 255;
+                const int ca = (bk[i] >> 16) & 255, cb = bk[i] >> 24;
                 const int w0 = bw01[i] & 0xffff, w1 = bw01[i] >> 16, w2 = bw23[i] & 0xffff, w3 = bw23[i] >> 16;
                 const double b00 = blk[i][0], b01 = blk[i][1], b10 = blk[i][2], b11 = blk[i][3];
                 if (ka == kb) {
@@ -509,7 +528,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                     main[w3] = n11;
                     continue;
                 }
-                const double2 ra = sh.cs[buf][ka], rb = sh.cs[buf][kb];
+                const double2 ra = sh.cs[buf][ca], rb = sh.cs[buf][cb];
                 const double l00 = ra.x * b00 - ra.y * b10, l01 = ra.x * b01 - ra.y * b11;
                 const double l10 = ra.y * b00 + ra.x * b10, l11 = ra.y * b01 + ra.x * b11;
                 main[w0] = rb.x * l00 - rb.y * l01;
@@ -522,7 +541,10 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         v_update((NP - 2) & 1);  // the sweep's last layer
     }
     clk.mark(kPhJacobi);
-    for (int i = tid; i < N; i += NT) sh.diag[i] = main[ring_pos0(NP, i) * (LDM + 1)];
+    for (int i = tid; i < N; i += NT) {
+        const int q = ring_pos0(NP, i);
+        sh.diag[i] = main[q * LDM + pcol(q)];
+    }
     __syncthreads();
     for (int i = tid; i < N; i += NT) {  // rank sort, eigenvalues descending
         const double di = isnan(sh.diag[i]) ? -INFINITY : sh.diag[i];

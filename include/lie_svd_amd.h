@@ -307,8 +307,9 @@ size_t lsvd_schedule_json(int target, int channels, int n_events, const lsvd_pha
  * and its outputs at u_off[b] (m x k), s_off[b] (k) and vt_off[b] (k x n).
  * Square matrices with n <= 4 use the LieSvdMicro schedule, as the
  * reference's router does (src/lie_svd_adaptive.rs:93); the rest use
- * LieSvdSmall::solve_rectangular.  The plan sorts the work, picks tile
- * shapes from the autotune cache and captures one hipGraph.  precision is
+ * LieSvdSmall::solve_rectangular.  The plan sorts the work into buckets of
+ * one persistent kernel launch each, picks tile shapes from the autotune
+ * cache and spreads the buckets over up to four concurrent lanes.  precision is
  * LSVD_PREC_F64 (the reference iteration) or LSVD_PREC_F64_SCALED; the
  * low-precision modes are rejected here (LSVD_E_ARG): at the batch's
  * accuracy target (relative sigma error 1e-12 up to condition 1e12) a
@@ -317,8 +318,12 @@ size_t lsvd_schedule_json(int target, int channels, int n_events, const lsvd_pha
 int lsvd_batch_plan_create(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n,
                            const int64_t* a_off, const int64_t* u_off, const int64_t* s_off,
                            const int64_t* vt_off, int precision, lsvd_plan_t* out);
-/* Runs the plan: one hipGraph launch.  Pool pointers follow the pointer
- * mode; info (count entries) may be NULL. */
+/* Runs the plan.  Pool pointers follow the pointer mode; info (count
+ * entries) may be NULL.  A plan with one lane replays a captured hipGraph;
+ * a plan with concurrent lanes is enqueued on the handle's lane streams,
+ * each with a hardware queue of its own, forked from and joined back to the
+ * handle's stream with events.  Both forms may be captured into a caller's
+ * graph (no memset nodes; counters are cleared by a kernel). */
 int lsvd_batch_plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool,
                             double* vt_pool, lsvd_info_t* info);
 int lsvd_batch_plan_destroy(lsvd_plan_t p);

@@ -286,9 +286,13 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
 
 namespace lsvd {
 
-// Side streams and fork/join events for the concurrent lanes.  Created
-// outside any graph capture (plan_run_device calls this first).
-constexpr int kPlanLanes = 4;  // one per hardware queue HIP gives a process
+// Lanes (streams) and fork/join events for the concurrent buckets.
+// Created outside any graph capture (plan_run_device calls this first).
+// The lanes belong to the handle and are created with a CU mask covering
+// every CU: such a stream gets a hardware queue of its own, whereas plain
+// streams share the process's few queues in creation order, and two lanes
+// on one queue run their buckets one after the other.
+constexpr int kPlanLanes = 4;
 
 static int plan_lanes(lsvd_plan_t p) {
     return std::min(kPlanLanes, (int)p->buckets.size() + (p->n_micro > 0 ? 1 : 0));
@@ -296,12 +300,18 @@ static int plan_lanes(lsvd_plan_t p) {
 
 hipError_t plan_ensure_streams(lsvd_plan_t p) {
     const int lanes = plan_lanes(p);
-    for (int i = (int)p->side.size(); i < lanes; ++i) {
+    lsvd_handle_t h = p->h;
+    while ((int)h->lanes.size() < lanes) {
+        uint32_t mask[32];
+        for (uint32_t& w : mask) w = 0xffffffffu;
+        const uint32_t words = (uint32_t)std::min((h->num_cus + 31) / 32, 32);
         hipStream_t s2;
-        hipError_t e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
+        hipError_t e = hipExtStreamCreateWithCUMask(&s2, words, mask);
+        if (e != hipSuccess) e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
         if (e != hipSuccess) return e;
-        p->side.push_back(s2);
+        h->lanes.push_back(s2);
     }
+    p->side.assign(h->lanes.begin(), h->lanes.begin() + lanes);
     while ((int)p->ev.size() < lanes + 1) {
         hipEvent_t ev;
         hipError_t e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
@@ -534,6 +544,7 @@ int lsvd_destroy(lsvd_handle_t h) {
     h->plan_cache.clear();
     lsvd_release_extras(h);
     if (h->own_stream) hipStreamDestroy(h->own_stream);
+    for (hipStream_t s2 : h->lanes) hipStreamDestroy(s2);
     delete h;
     return LSVD_OK;
 }
@@ -576,7 +587,11 @@ int lsvd_batch_plan_create(lsvd_handle_t h, int count, const int32_t* m, const i
 
 int lsvd_batch_plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, double* s_pool, double* vt_pool,
                             lsvd_info_t* info) {
-    return plan_execute(p, a_pool, u_pool, s_pool, vt_pool, info, true);
+    // a plan with concurrent lanes is enqueued on the handle's lanes (graph
+    // replays map their parallel branches onto hardware queues of the
+    // runtime's choosing, measured 12.4 or 14.6 ms for the same 12288-matrix
+    // batch from one process to the next); a one-lane plan replays its graph
+    return plan_execute(p, a_pool, u_pool, s_pool, vt_pool, info, plan_lanes(p) <= 1);
 }
 
 int lsvd_batch_plan_destroy(lsvd_plan_t p) {
@@ -584,7 +599,6 @@ int lsvd_batch_plan_destroy(lsvd_plan_t p) {
     hipSetDevice(p->h->device);
     if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
     if (p->graph) hipGraphDestroy(p->graph);
-    for (hipStream_t s2 : p->side) hipStreamDestroy(s2);
     for (hipEvent_t ev : p->ev) hipEventDestroy(ev);
     void* ptrs[] = {p->d_items, p->d_counters, p->d_consts, p->d_gws, p->d_a, p->d_u, p->d_s, p->d_vt, p->d_info};
     for (void* q : ptrs)

@@ -42,6 +42,7 @@ struct lsvd_handle_s {
     int num_cus = 1;
     std::string arch;
     hipStream_t own_stream = nullptr;
+    std::vector<hipStream_t> lanes;              // concurrent bucket lanes, one hardware queue each
     hipStream_t stream = nullptr;
     int ptr_mode = LSVD_PTR_HOST;
     std::string last_error;
@@ -72,7 +73,7 @@ struct lsvd_plan_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
     lsvd::Pools graph_pools = {};
-    std::vector<hipStream_t> side;     // side streams for concurrent buckets
+    std::vector<hipStream_t> side;     // the handle's lanes this plan uses (not owned)
     std::vector<hipEvent_t> ev;        // fork/join events
 };
 

@@ -223,6 +223,24 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
                        const lsvd_joint_params_t* params, double* u, double* sigmas, double* vt,
                        lsvd_joint_trace_t* trace);
 
+/* ---- Digital polish (batched consumer of LieSvdSmall) ---------------- */
+
+/* Replaces the digital-polish step of the PhaseFlow, Analog and Hybrid
+ * routes (src/lie_svd_phaseflow.rs:643-658, src/lie_svd_analog.rs:170-184,
+ * src/lie_svd_hybrid.rs:108-116) for `count` square matrices at once:
+ *   core = U0^T A V0 (V0 = Vt0^T), (Ur, sigma, Vrt) = LieSvdSmall::solve(core),
+ *   U = U0 Ur, Vt = Vrt Vt0.
+ * Matrix b is n[b] x n[b] (1 <= n[b] <= 1024); its A, U0, Vt0, U and Vt
+ * blocks start at mat_off[b] of their pools and sigma at s_off[b] (host
+ * arrays).  Every core takes the LieSvdSmall kernels, n <= 4 included, as
+ * the reference calls LieSvdSmall::solve there.  precision: LSVD_PREC_F64
+ * or LSVD_PREC_F64_SCALED for the cores' Newton-Schulz steps.  info (count
+ * entries, the cores' diagnostics) may be NULL.  Synchronous in both
+ * pointer modes. */
+int lsvd_polish_batched_f64(lsvd_handle_t h, int count, const int32_t* n, const int64_t* mat_off,
+                            const int64_t* s_off, const double* a, const double* u0, const double* vt0, int precision,
+                            double* u, double* s, double* vt, lsvd_info_t* info);
+
 /* ---- Rotor-stream export (src/lie_svd_compiler.rs) ---------------------- */
 
 /* HardwareTarget (src/lie_svd_compiler.rs:12-25). */

@@ -10,8 +10,8 @@ generators and residual metrics (src/lib.rs:133-388).
 from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_SCALED, LSVD_PTR_DEVICE,
                    LSVD_PTR_HOST, LsvdError, ShapeError)
 from .handle import BatchPlan, Handle, default_handle, pack_offsets
-from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall, eigh_jacobi_full, run_small,
-                      svd_batched)
+from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall, digital_polish,
+                      digital_polish_batched, eigh_jacobi_full, run_small, svd_batched)
 from . import metrics, profiles
 from .polar import newton_schulz_polar
 from .adjoint import polar_vjp, svd_loss_grad, svd_vjp

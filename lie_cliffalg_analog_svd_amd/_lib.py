@@ -161,6 +161,8 @@ SIGNATURES = {
                                           _vp, _vp]),
     "lsvd_joint_svd_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdJointParams),
                                      _vp, _vp, _vp, C.POINTER(LsvdJointTrace)]),
+    "lsvd_polish_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _i64p, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
+                                          _vp]),
     "lsvd_givens_schedule_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "lsvd_givens_events": (C.c_int, [C.c_int, _vp, _vp, C.POINTER(LsvdPhaseEvent)]),
     "lsvd_schedule_json": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.POINTER(LsvdPhaseEvent), C.c_int, _vp,

@@ -465,6 +465,24 @@ static int plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, dou
 }
 
 namespace lsvd {
+int svd_device_small(lsvd_handle_t h, int count, const int32_t* n, const int64_t* mat_off, const int64_t* s_off,
+                     int precision, const double* a, double* u, double* s, double* vt, lsvd_info_t* info) {
+    std::vector<int32_t> modes(count, kModeSvd);
+    const int saved = h->ptr_mode;
+    h->ptr_mode = LSVD_PTR_DEVICE;
+    lsvd_plan_t p = nullptr;
+    int rc = plan_build(h, count, n, n, mat_off, mat_off, s_off, mat_off, modes.data(), precision, &p);
+    h->ptr_mode = saved;
+    if (rc != LSVD_OK) return rc;
+    rc = plan_execute(p, a, u, s, vt, info, plan_lanes(p) <= 1);
+    if (rc == LSVD_OK) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) rc = hip_fail(h, e, "hipStreamSynchronize");
+    }
+    lsvd_batch_plan_destroy(p);
+    return rc;
+}
+
 int svd_device_uniform(lsvd_handle_t h, int count, int m, int n, const double* a, double* u, double* s, double* vt) {
     std::vector<int32_t> mm(count, m), nn(count, n), modes(count, kModeSvd);
     std::vector<int64_t> ao(count), uo(count), so(count), vo(count);

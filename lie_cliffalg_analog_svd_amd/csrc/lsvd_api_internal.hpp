@@ -33,6 +33,11 @@ inline uint64_t tune_key(int maxN, size_t arena_bytes, int precision) {
 struct Extras;  // lazily created state of the polar / joint / vjp paths
 
 int set_error(struct lsvd_handle_s* h, int code, const std::string& msg);
+// LieSvdSmall::solve of `count` square n[b] x n[b] matrices on device pools
+// (every n through the small kernels, n <= 4 included); synchronous.
+int svd_device_small(struct lsvd_handle_s* h, int count, const int32_t* n, const int64_t* mat_off,
+                     const int64_t* s_off, int precision, const double* a, double* u, double* s, double* vt,
+                     lsvd_info_t* info);
 int hip_fail(struct lsvd_handle_s* h, hipError_t e, const char* what);
 
 }  // namespace lsvd

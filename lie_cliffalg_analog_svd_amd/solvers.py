@@ -9,6 +9,7 @@ reference crate (skovnats/lie_cliffalg_analog_svd v0.44.0):
 - ``LieSvdMicro.solve``             src/lie_svd_micro.rs:21-52
 - ``LieSvd.solve``                  src/lie_svd.rs:49-51 (Micro/Small routing)
 - ``run_small``                     src/lib.rs:56-60
+- ``digital_polish[_batched]``      src/lie_svd_phaseflow.rs:643-658 (finishing step)
 
 Every function returns ``(U, sigma, Vt)`` as numpy float64 arrays with
 sigma descending; shape misuse raises ``ShapeError`` (an AssertionError),
@@ -155,5 +156,52 @@ def svd_batched(mats, precision: int = _lib.LSVD_PREC_F64, handle: Handle | None
     return out
 
 
+def digital_polish_batched(mats, u0s, vt0s, precision: int = _lib.LSVD_PREC_F64, handle: Handle | None = None,
+                           with_info=False):
+    """The digital-polish finishing step of the PhaseFlow, Analog and Hybrid
+    routes (src/lie_svd_phaseflow.rs:643-658, src/lie_svd_analog.rs:170-184,
+    src/lie_svd_hybrid.rs:108-116) for a batch of square matrices in one
+    call: core = U0^T A V0, LieSvdSmall.solve(core), U = U0 Ur,
+    Vt = Vrt Vt0.  Returns a list of (U, sigma, Vt)."""
+    mats = [_f64(a) for a in mats]
+    u0s = [_f64(a) for a in u0s]
+    vt0s = [_f64(a) for a in vt0s]
+    if not (len(mats) == len(u0s) == len(vt0s)):
+        raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "polish: as many U0 and Vt0 as matrices")
+    for a, u0, vt0 in zip(mats, u0s, vt0s):
+        n = a.shape[0]
+        if a.ndim != 2 or a.shape != (n, n) or u0.shape != (n, n) or vt0.shape != (n, n) or n == 0:
+            raise _lib.ShapeError(_lib.LSVD_E_SHAPE, "LieSvdSmall::solve: matrix must be square N x N")
+    h = handle or default_handle()
+    B = len(mats)
+    if B == 0:
+        return ([], []) if with_info else []
+    n = np.array([a.shape[0] for a in mats], dtype=np.int32)
+    mo = np.zeros(B, dtype=np.int64)
+    mo[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    so = np.zeros(B, dtype=np.int64)
+    so[1:] = np.cumsum(n.astype(np.int64))[:-1]
+    pa = np.concatenate([a.ravel() for a in mats])
+    pu0 = np.concatenate([a.ravel() for a in u0s])
+    pvt0 = np.concatenate([a.ravel() for a in vt0s])
+    pu = np.zeros_like(pa); pvt = np.zeros_like(pa); ps = np.zeros(int(n.sum()))
+    info = info_array(B)
+    st = _lib.lib().lsvd_polish_batched_f64(h.raw, B, n.ctypes.data_as(_lib._i32p), mo.ctypes.data_as(_lib._i64p),
+                                            so.ctypes.data_as(_lib._i64p), _ptr(pa), _ptr(pu0), _ptr(pvt0),
+                                            int(precision), _ptr(pu), _ptr(ps), _ptr(pvt), _ptr(info))
+    h.check(st)
+    out = []
+    for b in range(B):
+        k = int(n[b]); o = int(mo[b]); q = int(so[b])
+        out.append((pu[o:o + k * k].reshape(k, k).copy(), ps[q:q + k].copy(), pvt[o:o + k * k].reshape(k, k).copy()))
+    return (out, info_dicts(info, B)) if with_info else out
+
+
+def digital_polish(mat, u0, vt0, handle: Handle | None = None):
+    """One matrix of digital_polish_batched: (U, sigma, Vt)."""
+    return digital_polish_batched([mat], [u0], [vt0], handle=handle)[0]
+
+
 __all__ = ["LieSvdSmall", "LieSvdMicro", "LieSvd", "eigh_jacobi_full", "run_small", "svd_batched",
+           "digital_polish", "digital_polish_batched",
            "pack_offsets", "MICRO_MAX_N", "SMALL_MAX_N"]

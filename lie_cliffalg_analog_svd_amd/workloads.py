@@ -117,6 +117,33 @@ def joint_svd_family(n: int, K: int = 4, seed: int = SEED_JADE + 1):
     return np.ascontiguousarray(mats), u0, v0, d
 
 
+def polish_batch(count: int = 2048, seed: int = SEED_BATCH + 11, eps: float = 1e-2):
+    """Digital-polish inputs: square A = U diag(s) V^T with n log-uniform in
+    [8, 128], s log-uniform over kappa up to 1e8, and bases U0, V0 that are
+    U and V perturbed by eps (Gaussian) and re-orthonormalised (QR with the
+    sign of diag(R) folded in) -- what an analog or phase-flow preconditioner
+    hands to the polish.  Returns (mats, U0s, Vt0s, sigmas)."""
+    mats, u0s, vt0s, sig = [], [], [], []
+    for b in range(count):
+        s0 = rng.mix_seed(seed, b)
+        n = int(np.exp(np.log(8.0) + (np.log(128.0) - np.log(8.0)) * rng.uniform(rng.mix_seed(s0, 0), 1)[0]))
+        n = max(8, min(128, n))
+        kap = np.exp(np.log(1e8) * rng.uniform(rng.mix_seed(s0, 1), 1)[0])
+        s = np.sort(np.exp(-np.log(kap) * rng.uniform(rng.mix_seed(s0, 2), n)))[::-1]
+        u = rng.haar(rng.mix_seed(s0, 3), n)
+        v = rng.haar(rng.mix_seed(s0, 4), n)
+        a = (u * s[None, :]) @ v.T
+        gu = rng.normal(rng.mix_seed(s0, 5), n * n).reshape(n, n)
+        gv = rng.normal(rng.mix_seed(s0, 6), n * n).reshape(n, n)
+        qu, ru = np.linalg.qr(u + eps * gu)
+        qv, rv = np.linalg.qr(v + eps * gv)
+        mats.append(np.ascontiguousarray(a))
+        u0s.append(np.ascontiguousarray(qu * np.sign(np.diag(ru))[None, :]))
+        vt0s.append(np.ascontiguousarray((qv * np.sign(np.diag(rv))[None, :]).T))
+        sig.append(s)
+    return mats, u0s, vt0s, sig
+
+
 def recovery_error(q, v):
     """Distance between planted Q and recovered V up to column permutation
     and sign: match columns greedily by |Q^T V| (a permutation when the

@@ -9,6 +9,7 @@ skovnats/lie_cliffalg_analog_svd:
 - ``src/lie_svd_joint.rs:18-539``  (LieSvdJoint: Phase-JADE)
 - ``src/lib.rs:149-194``           (metrics::compute)
 - ``src/lie_svd_compiler.rs:144-268`` (HardwareSchedule: Givens export, JSON)
+- ``src/lie_svd_phaseflow.rs:643-658`` (the digital-polish finishing step)
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
 ``cpu_baseline`` leg may import this package, and only as the checker or as
@@ -116,6 +117,7 @@ def lib():
         L.lso_ns_iterations.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.lso_svd_loss_grad.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp]
         L.lso_svd_loss_grad.restype = C.c_double
+        L.lso_polish.argtypes = [dp, dp, dp, C.c_int, dp, dp, dp, C.POINTER(Info)]
         L.lso_givens_schedule.argtypes = [dp, C.c_int, ip, ip, dp, dp, dp, dp]
         L.lso_givens_schedule.restype = C.c_int
         L.lso_schedule_json.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(PhaseEvent), C.c_int, dp,
@@ -380,3 +382,16 @@ def givens_events(ev):
     return [{"layer": k, "i": int(r[0]), "j": int(r[1]), "phi_l": 0.0, "phi_r": 0.0, "theta": r[4],
              "theta_l": r[4], "theta_r": 0.0, "source": "orthogonal_givens", "kind": "givens_elimination"}
             for k, r in enumerate(ev)]
+
+
+def polish(a, u0, vt0):
+    """solve_with_digital_polish's finishing step (src/lie_svd_phaseflow.rs:
+    651-657): core = U0^T A Vt0^T, LieSvdSmall::solve(core), U = U0 Ur,
+    Vt = Vrt Vt0 -> (U, s, Vt, info of the core's solve)."""
+    a, u0, vt0 = _c(a), _c(u0), _c(vt0)
+    n = a.shape[0]
+    assert a.shape == (n, n) and u0.shape == (n, n) and vt0.shape == (n, n), "polish: square n x n inputs"
+    u = np.zeros((n, n)); s = np.zeros(n); vt = np.zeros((n, n))
+    info = Info()
+    lib().lso_polish(_dp(a), _dp(u0), _dp(vt0), n, _dp(u), _dp(s), _dp(vt), C.byref(info))
+    return u, s, vt, info.as_dict()

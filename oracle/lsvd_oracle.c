@@ -1152,3 +1152,24 @@ size_t lso_schedule_json(const char* target, int channels, int n, const lso_phas
     if (cap) buf[b.len < cap ? b.len : cap - 1] = 0;
     return b.len;
 }
+
+/* ---- Digital polish (PhaseFlow / Analog / Hybrid finishing step) -------- */
+
+/* solve_with_digital_polish's last step (src/lie_svd_phaseflow.rs:651-657,
+ * src/lie_svd_analog.rs:177-183, src/lie_svd_hybrid.rs:110-115): core =
+ * U0^T A V0 with V0 = Vt0^T, (Ur, sigma, Vrt) = LieSvdSmall::solve(core),
+ * U = U0 Ur, Vt = Vrt Vt0.  All n x n. */
+void lso_polish(const double* a, const double* u0, const double* vt0, int n, double* u, double* s, double* vt,
+                lso_info* info) {
+    const size_t nn = (size_t)n * n;
+    double* t = dalloc(nn);
+    double* core = dalloc(nn);
+    double* ur = dalloc(nn);
+    double* vrt = dalloc(nn);
+    gemm(1, 0, n, n, n, u0, a, t);     /* U0^T A */
+    gemm(0, 1, n, n, n, t, vt0, core); /* (U0^T A) Vt0^T */
+    lso_small_solve(core, n, ur, s, vrt, info);
+    gemm(0, 0, n, n, n, u0, ur, u);
+    gemm(0, 0, n, n, n, vrt, vt0, vt);
+    free(t); free(core); free(ur); free(vrt);
+}

@@ -61,3 +61,24 @@ def max_subspace_angle(u_got, u_ref, sigma_ref, gap_rtol: float = 1e-6, min_rel_
         if s[i:j].min() >= min_rel_sigma * smax:
             worst = max(worst, ang)
     return worst
+
+
+def vector_parity(u_got, u_ref, s_ref, n, what):
+    """Per-cluster principal angles between computed and oracle singular
+    subspaces, bounded by the FP64 perturbation bound eps*||A||/gap."""
+    s = np.asarray(s_ref)
+    smax = max(float(s.max()), 1e-300)
+    # clusters: consecutive sigma within 1e-9 sigma_max
+    i = 0
+    while i < s.size:
+        j = i + 1
+        while j < s.size and s[j - 1] - s[j] <= 1e-9 * smax:
+            j += 1
+        lo = s[i - 1] - s[i] if i > 0 else np.inf
+        hi = s[j - 1] - s[j] if j < s.size else np.inf
+        gap = min(lo, hi)
+        if s[j - 1] >= 1e-8 * smax and np.isfinite(gap):
+            ang = subspace_sine(u_got[:, i:j], u_ref[:, i:j])
+            bound = max(1e3 * max(n, 1) * EPS * smax / gap, 1e-10)
+            assert ang <= bound, f"{what}: cluster [{i},{j}) angle {ang:.3e} > {bound:.3e}"
+        i = j

@@ -18,6 +18,7 @@ import oracle
 from lie_cliffalg_analog_svd_amd import (LieSvd, LieSvdMicro, LieSvdSmall, ShapeError, eigh_jacobi_full,
                                          metrics, profiles, rng, svd_batched)
 from tests._util import EPS, assert_factors, assert_sigma_close, imposed_spectrum, subspace_sine, uniform_matrix
+from tests._util import vector_parity as _vector_parity
 
 pytestmark = pytest.mark.gpu
 
@@ -25,27 +26,6 @@ pytestmark = pytest.mark.gpu
 def _say(*a):
     print(*a, flush=True)
     sys.stdout.flush()
-
-
-def _vector_parity(u_got, u_ref, s_ref, n, what):
-    """Per-cluster principal angles between computed and oracle singular
-    subspaces, bounded by the FP64 perturbation bound eps*||A||/gap."""
-    s = np.asarray(s_ref)
-    smax = max(float(s.max()), 1e-300)
-    # clusters: consecutive sigma within 1e-9 sigma_max
-    i = 0
-    while i < s.size:
-        j = i + 1
-        while j < s.size and s[j - 1] - s[j] <= 1e-9 * smax:
-            j += 1
-        lo = s[i - 1] - s[i] if i > 0 else np.inf
-        hi = s[j - 1] - s[j] if j < s.size else np.inf
-        gap = min(lo, hi)
-        if s[j - 1] >= 1e-8 * smax and np.isfinite(gap):
-            ang = subspace_sine(u_got[:, i:j], u_ref[:, i:j])
-            bound = max(1e3 * max(n, 1) * EPS * smax / gap, 1e-10)
-            assert ang <= bound, f"{what}: cluster [{i},{j}) angle {ang:.3e} > {bound:.3e}"
-        i = j
 
 
 def _check_vs_oracle(a, what, rel_tol=1e-12, resid=1e-11):

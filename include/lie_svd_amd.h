@@ -223,6 +223,53 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
                        const lsvd_joint_params_t* params, double* u, double* sigmas, double* vt,
                        lsvd_joint_trace_t* trace);
 
+/* ---- Adaptive triage and router (src/lie_svd_adaptive.rs) ------------- */
+
+/* AdaptiveRoute (src/lie_svd_adaptive.rs:8-15). */
+enum lsvd_route {
+    LSVD_ROUTE_MICRO = 0,
+    LSVD_ROUTE_SMALL = 1,
+    LSVD_ROUTE_PHASEFLOW = 2,
+    LSVD_ROUTE_COREFLOW_TOPO = 3,
+    LSVD_ROUTE_HYBRID = 4
+};
+
+/* LieSvdAdaptiveParams (src/lie_svd_adaptive.rs:59-78). */
+typedef struct {
+    int32_t small_max_n, coreflow_topo_max_n, phaseflow_max_n, reserved;
+    double nearly_diagonal_ratio, suspicious_threshold;
+} lsvd_adaptive_params_t;
+
+/* AdaptiveTriage (src/lie_svd_adaptive.rs:18-50) and the route
+ * choose_route picks from it (enum lsvd_route). */
+typedef struct {
+    int32_t n, route;
+    double offdiag_ratio, diagonal_dominance, row_cv, col_cv, row_col_mismatch, symmetry_ratio,
+        transpose_torsion_ratio, diagonal_entropy, row_mass_entropy, col_mass_entropy, phase_mean_stress,
+        phase_max_twist, phase_causal_disbalance, phase_entropy_gap, phase_torsion_energy, phase_chirality_balance,
+        phase_entropy, suspicious_score;
+} lsvd_triage_t;
+
+/* LieSvdAdaptiveParams::default (src/lie_svd_adaptive.rs:68-78). */
+int lsvd_adaptive_params_default(lsvd_adaptive_params_t* out);
+
+/* Replaces choose_route / analyze_matrix (src/lie_svd_adaptive.rs:103-119,
+ * 186-274, with phase_signature src/lie_svd_phasehealth.rs:129-140 and the
+ * Block4 dual balance src/lie_svd_block4.rs:210-262) for `count` square
+ * matrices at once: matrix b is n[b] x n[b] at a + a_off[b] (host arrays;
+ * 0 <= n[b] <= 1024, LSVD_E_LIMIT beyond).  out[b] receives the triage and
+ * the route (params NULL: the defaults).  Sums run in another order than the
+ * reference's loops and the device's log / atan2 / sin / cos differ from the
+ * host's in the last bits, so a statistic sitting exactly on a threshold can
+ * route differently. */
+int lsvd_analyze_batched_f64(lsvd_handle_t h, int count, const int32_t* n, const int64_t* a_off, const double* a,
+                             const lsvd_adaptive_params_t* params, lsvd_triage_t* out);
+
+/* Host memory only.  choose_route's decision (src/lie_svd_adaptive.rs:
+ * 103-184) on a triage (its route field is ignored); params NULL: the
+ * defaults.  Returns the lsvd_route (-1 for a NULL triage). */
+int lsvd_choose_route(const lsvd_triage_t* triage, const lsvd_adaptive_params_t* params);
+
 /* ---- Digital polish (batched consumer of LieSvdSmall) ---------------- */
 
 /* Replaces the digital-polish step of the PhaseFlow, Analog and Hybrid

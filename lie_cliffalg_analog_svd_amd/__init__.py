@@ -17,6 +17,8 @@ from .polar import newton_schulz_polar
 from .adjoint import polar_vjp, svd_loss_grad, svd_vjp
 from .joint import (JointDiagonalizationParams, JointDiagonalizationTrace, JointSvdTrace, LieSvdJoint,
                     diagonalize_families, svd_families)
+from .adaptive import (AdaptiveRoute, AdaptiveTrace, AdaptiveTriage, LieSvdAdaptive, LieSvdAdaptiveParams,
+                       analyze_batch, analyze_matrix, choose_route)
 from .schedule import (ComplexMziPhase, ComplexPhaseEventKind, HardwarePhaseEvent, HardwareSchedule, HardwareTarget,
                        MziPhase, PhaseRotorKind, givens_batch, schedules_from_orthogonal)
 

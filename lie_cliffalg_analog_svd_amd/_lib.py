@@ -96,6 +96,23 @@ class LsvdPhaseEvent(C.Structure):
     ]
 
 
+TRIAGE_FIELDS = ["offdiag_ratio", "diagonal_dominance", "row_cv", "col_cv", "row_col_mismatch", "symmetry_ratio",
+                 "transpose_torsion_ratio", "diagonal_entropy", "row_mass_entropy", "col_mass_entropy",
+                 "phase_mean_stress", "phase_max_twist", "phase_causal_disbalance", "phase_entropy_gap",
+                 "phase_torsion_energy", "phase_chirality_balance", "phase_entropy", "suspicious_score"]
+
+
+class LsvdTriage(C.Structure):
+    """lsvd_triage_t (AdaptiveTriage, src/lie_svd_adaptive.rs:18-50, and the route)."""
+    _fields_ = [("n", C.c_int32), ("route", C.c_int32)] + [(f, C.c_double) for f in TRIAGE_FIELDS]
+
+
+class LsvdAdaptiveParams(C.Structure):
+    """lsvd_adaptive_params_t (LieSvdAdaptiveParams, src/lie_svd_adaptive.rs:59-78)."""
+    _fields_ = [("small_max_n", C.c_int32), ("coreflow_topo_max_n", C.c_int32), ("phaseflow_max_n", C.c_int32),
+                ("reserved", C.c_int32), ("nearly_diagonal_ratio", C.c_double), ("suspicious_threshold", C.c_double)]
+
+
 class LsvdBucketDesc(C.Structure):
     _fields_ = [
         ("first", C.c_int32),
@@ -161,6 +178,10 @@ SIGNATURES = {
                                           _vp, _vp]),
     "lsvd_joint_svd_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, C.POINTER(LsvdJointParams),
                                      _vp, _vp, _vp, C.POINTER(LsvdJointTrace)]),
+    "lsvd_adaptive_params_default": (C.c_int, [C.POINTER(LsvdAdaptiveParams)]),
+    "lsvd_analyze_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, C.POINTER(LsvdAdaptiveParams),
+                                           C.POINTER(LsvdTriage)]),
+    "lsvd_choose_route": (C.c_int, [C.POINTER(LsvdTriage), C.POINTER(LsvdAdaptiveParams)]),
     "lsvd_polish_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _i64p, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                                           _vp]),
     "lsvd_givens_schedule_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),

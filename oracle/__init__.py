@@ -10,6 +10,8 @@ skovnats/lie_cliffalg_analog_svd:
 - ``src/lib.rs:149-194``           (metrics::compute)
 - ``src/lie_svd_compiler.rs:144-268`` (HardwareSchedule: Givens export, JSON)
 - ``src/lie_svd_phaseflow.rs:643-658`` (the digital-polish finishing step)
+- ``src/lie_svd_adaptive.rs:59-392``, ``src/lie_svd_phasehealth.rs:100-345``,
+  ``src/lie_svd_block4.rs:210-266`` (adaptive triage and router)
 
 Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
 ``cpu_baseline`` leg may import this package, and only as the checker or as
@@ -76,6 +78,27 @@ class PhaseEvent(C.Structure):
     ]
 
 
+class Triage(C.Structure):
+    """AdaptiveTriage (src/lie_svd_adaptive.rs:18-50) plus the route."""
+    _fields_ = [("n", C.c_int), ("route", C.c_int)] + [(f, C.c_double) for f in (
+        "offdiag_ratio", "diagonal_dominance", "row_cv", "col_cv", "row_col_mismatch", "symmetry_ratio",
+        "transpose_torsion_ratio", "diagonal_entropy", "row_mass_entropy", "col_mass_entropy", "phase_mean_stress",
+        "phase_max_twist", "phase_causal_disbalance", "phase_entropy_gap", "phase_torsion_energy",
+        "phase_chirality_balance", "phase_entropy", "suspicious_score")]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class AdaptiveParams(C.Structure):
+    """LieSvdAdaptiveParams (src/lie_svd_adaptive.rs:59-78)."""
+    _fields_ = [("small_max_n", C.c_int), ("coreflow_topo_max_n", C.c_int), ("phaseflow_max_n", C.c_int),
+                ("nearly_diagonal_ratio", C.c_double), ("suspicious_threshold", C.c_double)]
+
+
+ROUTES = ["Micro", "Small", "PhaseFlow", "CoreFlowTopo", "Hybrid"]
+
+
 def build() -> None:
     """Compile the oracle with its Makefile (gcc, no GPU needed)."""
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -117,6 +140,12 @@ def lib():
         L.lso_ns_iterations.argtypes = [dp, C.c_int, C.c_int, C.c_int, C.c_int]
         L.lso_svd_loss_grad.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp]
         L.lso_svd_loss_grad.restype = C.c_double
+        L.lso_adaptive_params_default.restype = AdaptiveParams
+        L.lso_analyze_matrix.argtypes = [dp, C.c_int, C.POINTER(Triage)]
+        for fn in ("lso_choose_route", "lso_should_use_phaseflow", "lso_should_use_coreflow_topo",
+                   "lso_is_balanced_degenerate"):
+            getattr(L, fn).argtypes = [C.POINTER(Triage), C.POINTER(AdaptiveParams)]
+            getattr(L, fn).restype = C.c_int
         L.lso_polish.argtypes = [dp, dp, dp, C.c_int, dp, dp, dp, C.POINTER(Info)]
         L.lso_givens_schedule.argtypes = [dp, C.c_int, ip, ip, dp, dp, dp, dp]
         L.lso_givens_schedule.restype = C.c_int
@@ -395,3 +424,43 @@ def polish(a, u0, vt0):
     info = Info()
     lib().lso_polish(_dp(a), _dp(u0), _dp(vt0), n, _dp(u), _dp(s), _dp(vt), C.byref(info))
     return u, s, vt, info.as_dict()
+
+
+def adaptive_params_default() -> AdaptiveParams:
+    return lib().lso_adaptive_params_default()
+
+
+def analyze_matrix(a) -> dict:
+    """analyze_matrix (src/lie_svd_adaptive.rs:186-274) -> triage dict."""
+    a = _c(a)
+    n = a.shape[0]
+    assert a.shape == (n, n), "adaptive SVD expects square matrices"
+    t = Triage()
+    lib().lso_analyze_matrix(_dp(a), n, C.byref(t))
+    return t.as_dict()
+
+
+def _triage(d: dict) -> Triage:
+    t = Triage()
+    for k, _ in Triage._fields_:
+        if k in d:
+            setattr(t, k, d[k])
+    return t
+
+
+def choose_route(a, params: AdaptiveParams | None = None) -> tuple:
+    """choose_route (src/lie_svd_adaptive.rs:103-119) -> (route name, triage)."""
+    d = analyze_matrix(a)
+    p = params or adaptive_params_default()
+    r = lib().lso_choose_route(C.byref(_triage(d)), C.byref(p))
+    d["route"] = r
+    return ROUTES[r], d
+
+
+def route_predicate(name: str, triage: dict, params: AdaptiveParams | None = None) -> bool:
+    """should_use_phaseflow / should_use_coreflow_topo /
+    is_balanced_degenerate_case on a triage dict."""
+    p = params or adaptive_params_default()
+    fn = {"phaseflow": "lso_should_use_phaseflow", "coreflow_topo": "lso_should_use_coreflow_topo",
+          "balanced_degenerate": "lso_is_balanced_degenerate"}[name]
+    return bool(getattr(lib(), fn)(C.byref(_triage(triage)), C.byref(p)))

@@ -1173,3 +1173,292 @@ void lso_polish(const double* a, const double* u0, const double* vt0, int n, dou
     gemm(0, 0, n, n, n, vrt, vt0, vt);
     free(t); free(core); free(ur); free(vrt);
 }
+
+/* ---- Adaptive triage and router (src/lie_svd_adaptive.rs) ------------- */
+
+typedef struct {
+    int n;
+    int route;
+    double offdiag_ratio, diagonal_dominance, row_cv, col_cv, row_col_mismatch, symmetry_ratio,
+        transpose_torsion_ratio, diagonal_entropy, row_mass_entropy, col_mass_entropy, phase_mean_stress,
+        phase_max_twist, phase_causal_disbalance, phase_entropy_gap, phase_torsion_energy, phase_chirality_balance,
+        phase_entropy, suspicious_score;
+} lso_triage;
+
+typedef struct {
+    int small_max_n, coreflow_topo_max_n, phaseflow_max_n;
+    double nearly_diagonal_ratio, suspicious_threshold;
+} lso_adaptive_params;
+
+/* LieSvdAdaptiveParams::default (src/lie_svd_adaptive.rs:68-78). */
+lso_adaptive_params lso_adaptive_params_default(void) {
+    lso_adaptive_params p = {512, 64, 64, 1e-5, 1.15};
+    return p;
+}
+
+typedef struct {
+    double scalar_mean, scalar_sq, vector_sq, delay_bivector_sq, gradient_sq, energy_entropy, twist_ratio, phase;
+} lso_vphase;
+
+/* analyze_vector_phase_by + normalized_energy_entropy_by
+ * (src/lie_svd_phasehealth.rs:146-211, 313-329): x[k * stride], k < n. */
+static lso_vphase vector_phase(const double* x, int n, int stride) {
+    lso_vphase h = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (n == 0) return h;
+    const double nf = (double)n;
+    double sum = 0.0;
+    for (int i = 0; i < n; ++i) sum += x[(size_t)i * stride];
+    const double mean = sum / nf;
+    h.scalar_mean = mean;
+    h.scalar_sq = nf * mean * mean;
+    double vector_sq = 0.0, norm_sq = 0.0, delay_dot = 0.0, gradient_sq = 0.0, energy_sum = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double xi = x[(size_t)i * stride];
+        const double xn = x[(size_t)((i + 1) % n) * stride];
+        const double c = xi - mean;
+        vector_sq += c * c;
+        norm_sq += xi * xi;
+        delay_dot += xi * xn;
+        const double d = xn - xi;
+        gradient_sq += d * d;
+        energy_sum += xi * xi;
+    }
+    h.vector_sq = vector_sq;
+    h.gradient_sq = gradient_sq;
+    h.delay_bivector_sq = fmax(norm_sq * norm_sq - delay_dot * delay_dot, 0.0);
+    double ent = 0.0;
+    if (n > 1 && energy_sum > 1e-300) {
+        for (int i = 0; i < n; ++i) {
+            const double v = x[(size_t)i * stride];
+            const double p = (v * v) / energy_sum;
+            if (p > 0.0) ent -= p * log(p);
+        }
+        ent /= log(nf);
+    }
+    h.energy_entropy = ent;
+    h.twist_ratio = sqrt(h.delay_bivector_sq) / fmax(norm_sq, 1e-300);
+    const double orient = delay_dot >= 0.0 ? 1.0 : -1.0;
+    h.phase = orient * atan2(sqrt(gradient_sq), fabs(mean) + sqrt(vector_sq) + 1e-300);
+    return h;
+}
+
+/* coeff_var, relative_l2_mismatch, entropy01 (src/lie_svd_adaptive.rs:348-392). */
+static double coeff_var(const double* v, int n) {
+    double mean = 0.0;
+    for (int i = 0; i < n; ++i) mean += v[i];
+    mean /= (double)(n > 1 ? n : 1);
+    if (fabs(mean) <= 1e-300) return 0.0;
+    double var = 0.0;
+    for (int i = 0; i < n; ++i) var += (v[i] - mean) * (v[i] - mean);
+    var /= (double)(n > 1 ? n : 1);
+    return sqrt(var) / fabs(mean);
+}
+static double rel_l2_mismatch(const double* a, const double* b, int n) {
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double d = a[i] - b[i];
+        num += d * d;
+        den += a[i] * a[i] + b[i] * b[i];
+    }
+    return sqrt(num) / fmax(sqrt(den), 1e-300);
+}
+static double entropy01(const double* v, int n) {
+    double sum = 0.0;
+    for (int i = 0; i < n; ++i) sum += v[i];
+    if (sum <= 1e-300 || n <= 1) return 1.0;
+    double h = 0.0;
+    for (int i = 0; i < n; ++i) {
+        const double p = v[i] / sum;
+        if (p > 0.0) h -= p * log(p);
+    }
+    return h / fmax(log((double)n), 1e-300);
+}
+
+/* analyze_block4_signature's dual_balance (src/lie_svd_block4.rs:210-266). */
+static double block4_dual_balance(const double* a, int rows, int cols) {
+    const int n = rows < cols ? rows : cols;
+    const int blocks = n / 4;
+    if (blocks == 0) return 0.0;
+    double sd2 = 0.0, asd2 = 0.0;
+    const double r2 = 0.70710678118654752440; /* FRAC_1_SQRT_2 */
+#define SKW(p, q) (0.5 * (a[(size_t)(st + (q)) * cols + st + (p)] - a[(size_t)(st + (p)) * cols + st + (q)]))
+    for (int b = 0; b < blocks; ++b) {
+        const int st = 4 * b;
+        const double w12 = SKW(0, 1), w13 = SKW(0, 2), w14 = SKW(0, 3), w23 = SKW(1, 2), w24 = SKW(1, 3),
+                     w34 = SKW(2, 3);
+        const double sd[3] = {(w12 + w34) * r2, (w13 - w24) * r2, (w14 + w23) * r2};
+        const double asd[3] = {(w12 - w34) * r2, (w13 + w24) * r2, (w14 - w23) * r2};
+        sd2 += sd[0] * sd[0] + sd[1] * sd[1] + sd[2] * sd[2];
+        asd2 += asd[0] * asd[0] + asd[1] * asd[1] + asd[2] * asd[2];
+    }
+#undef SKW
+    const double s = sqrt(sd2), t = sqrt(asd2);
+    return fabs(s - t) / fmax(s + t, 1e-300);
+}
+
+/* analyze_matrix (src/lie_svd_adaptive.rs:186-274) with phase_signature,
+ * analyze_fractal_phase_health, summarize, triangular_disbalance and
+ * global_phase_invariants (src/lie_svd_phasehealth.rs:100-140, 213-345);
+ * square n x n. */
+void lso_analyze_matrix(const double* a, int n, lso_triage* t) {
+    double all_sq = 0.0, diag_sq = 0.0, off_sq = 0.0, skew_sq = 0.0, sym_sq = 0.0;
+    double* diag_abs = dalloc(n > 0 ? n : 1);
+    double* row_l1 = dalloc(n > 0 ? n : 1);
+    double* col_l1 = dalloc(n > 0 ? n : 1);
+    double* row_l2 = dalloc(n > 0 ? n : 1);
+    double* col_l2 = dalloc(n > 0 ? n : 1);
+    for (int i = 0; i < n; ++i) diag_abs[i] = row_l1[i] = col_l1[i] = row_l2[i] = col_l2[i] = 0.0;
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) {
+            const double v = a[(size_t)i * n + j], av = fabs(v);
+            all_sq += v * v;
+            row_l1[i] += av;
+            col_l1[j] += av;
+            row_l2[i] += v * v;
+            col_l2[j] += v * v;
+            if (i == j) {
+                diag_sq += v * v;
+                diag_abs[i] = av;
+            } else {
+                off_sq += v * v;
+            }
+        }
+    for (int i = 0; i < n; ++i) {
+        row_l2[i] = sqrt(row_l2[i]);
+        col_l2[i] = sqrt(col_l2[i]);
+        for (int j = i + 1; j < n; ++j) {
+            const double x = a[(size_t)i * n + j], y = a[(size_t)j * n + i];
+            const double sym = 0.5 * (x + y), skew = 0.5 * (x - y);
+            sym_sq += 2.0 * sym * sym;
+            skew_sq += 2.0 * skew * skew;
+        }
+    }
+    const double norm = fmax(sqrt(all_sq), 1e-300);
+    t->n = n;
+    t->offdiag_ratio = sqrt(off_sq) / norm;
+    t->diagonal_dominance = sqrt(diag_sq) / norm;
+    t->row_cv = coeff_var(row_l2, n);
+    t->col_cv = coeff_var(col_l2, n);
+    t->row_col_mismatch = rel_l2_mismatch(row_l1, col_l1, n);
+    t->symmetry_ratio = sqrt(sym_sq) / fmax(sqrt(sym_sq + skew_sq), 1e-300);
+    t->transpose_torsion_ratio = sqrt(skew_sq) / fmax(sqrt(sym_sq + skew_sq), 1e-300);
+    t->diagonal_entropy = entropy01(diag_abs, n);
+    t->row_mass_entropy = entropy01(row_l1, n);
+    t->col_mass_entropy = entropy01(col_l1, n);
+    /* phase_signature: row and column phase health, summarised */
+    double d_sum = 0.0, g_sum = 0.0, max_tw = 0.0, row_tw = 0.0, col_tw = 0.0, row_en = 0.0, col_en = 0.0;
+    double sin_sum = 0.0, cos_sum = 0.0, w_sum = 0.0;
+    for (int i = 0; i < n; ++i) { /* rows: :106-108, :219-225 */
+        const lso_vphase h = vector_phase(a + (size_t)i * n, n, 1);
+        d_sum += h.delay_bivector_sq;
+        g_sum += h.gradient_sq;
+        max_tw = fmax(max_tw, h.twist_ratio);
+        row_tw += h.twist_ratio;
+        row_en += h.energy_entropy;
+        const double w = sqrt(h.scalar_sq + h.vector_sq);
+        sin_sum += w * sin(h.phase);
+        cos_sum += w * cos(h.phase);
+        w_sum += w;
+    }
+    for (int j = 0; j < n; ++j) { /* columns: :109-111, :226-232 */
+        const lso_vphase h = vector_phase(a + j, n, n);
+        d_sum += h.delay_bivector_sq;
+        g_sum += h.gradient_sq;
+        max_tw = fmax(max_tw, h.twist_ratio);
+        col_tw += h.twist_ratio;
+        col_en += h.energy_entropy;
+        const double w = sqrt(h.scalar_sq + h.vector_sq);
+        sin_sum += w * sin(h.phase);
+        cos_sum += w * cos(h.phase);
+        w_sum += w;
+    }
+    (void)row_tw; (void)col_tw; (void)sin_sum; (void)cos_sum; (void)w_sum;
+    const double cnt = (double)(2 * n > 1 ? 2 * n : 1);
+    t->phase_mean_stress = (sqrt(d_sum) + sqrt(g_sum)) / cnt;
+    t->phase_max_twist = max_tw;
+    {
+        double up = 0.0, lo = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                up += a[(size_t)i * n + j] * a[(size_t)i * n + j];
+                lo += a[(size_t)j * n + i] * a[(size_t)j * n + i];
+            }
+        t->phase_causal_disbalance = fabs(up - lo) / fmax(up + lo, 1e-300);
+    }
+    t->phase_entropy_gap = n > 0 ? fabs(row_en / n - col_en / n) : 0.0;
+    {
+        double tor = 0.0;
+        for (int i = 0; i < n; ++i)
+            for (int j = i + 1; j < n; ++j) {
+                const double d = (a[(size_t)i * n + j] - a[(size_t)j * n + i]) * 0.5;
+                tor += 2.0 * d * d;
+            }
+        t->phase_torsion_energy = sqrt(tor);
+    }
+    t->phase_chirality_balance = block4_dual_balance(a, n, n);
+    {
+        const size_t count = (size_t)n * n;
+        double ent = 0.0;
+        if (count > 1 && all_sq > 1e-300) {
+            double tot = 0.0;
+            for (size_t e = 0; e < count; ++e) tot += a[e] * a[e];
+            for (size_t e = 0; e < count; ++e) {
+                const double p = (a[e] * a[e]) / tot;
+                if (p > 0.0) ent -= p * log(p);
+            }
+            ent /= log((double)count);
+        }
+        t->phase_entropy = ent;
+    }
+    /* suspicious_score (:319-339) */
+    double sc = 0.0;
+    sc += fmax(t->offdiag_ratio - 0.55, 0.0) * 0.9;
+    sc += fmax(0.35 - t->diagonal_dominance, 0.0) * 0.8;
+    sc += fmax(t->row_cv, t->col_cv) * 1.4;
+    sc += t->row_col_mismatch * 1.6;
+    sc += t->transpose_torsion_ratio * 0.45;
+    sc += t->offdiag_ratio * fmax(t->symmetry_ratio - 0.82, 0.0) * 1.8;
+    sc += fmax(0.78 - t->diagonal_entropy, 0.0) * 1.2;
+    t->suspicious_score = sc;
+    free(diag_abs); free(row_l1); free(col_l1); free(row_l2); free(col_l2);
+}
+
+static int balanced_degenerate(const lso_triage* t, const lso_adaptive_params* p) { /* :175-184 */
+    return t->suspicious_score >= p->suspicious_threshold + 0.25 && t->offdiag_ratio > 0.90 &&
+           t->diagonal_dominance < 0.25 && t->row_cv >= 0.20 && t->col_cv >= 0.20 && t->row_cv < 0.45 &&
+           t->col_cv < 0.45 && t->row_col_mismatch < 0.45;
+}
+
+int lso_should_use_phaseflow(const lso_triage* t, const lso_adaptive_params* p) { /* :121-150 */
+    if (balanced_degenerate(t, p)) return 1;
+    const int causal = t->phase_causal_disbalance > 0.88 && t->offdiag_ratio > 0.20 && t->phase_mean_stress > 1.0;
+    const int locking = t->phase_mean_stress > 50.0 && t->phase_max_twist > 0.92 && t->row_cv < 0.42 &&
+                        t->col_cv < 0.42 && t->row_col_mismatch < 0.30 && t->offdiag_ratio > 0.85 &&
+                        t->diagonal_dominance < 0.35;
+    const int chiral = t->phase_chirality_balance > 0.30 && t->offdiag_ratio > 0.20 &&
+                       t->phase_torsion_energy > 1.0 && t->diagonal_dominance < 0.5;
+    return causal || locking || chiral;
+}
+
+int lso_should_use_coreflow_topo(const lso_triage* t, const lso_adaptive_params* p) { /* :152-173 */
+    if (t->symmetry_ratio > 0.96 && t->offdiag_ratio > 0.35 && t->diagonal_dominance < 0.95) return 1;
+    if (balanced_degenerate(t, p)) return 1;
+    const int mass = t->row_mass_entropy < 0.92 || t->col_mass_entropy < 0.92;
+    const int balanced = t->row_col_mismatch < 0.22;
+    const int avoids = t->transpose_torsion_ratio < 0.62 || t->diagonal_entropy < 0.62;
+    const int not_rank1 = t->row_cv < 0.42 && t->col_cv < 0.42;
+    return t->suspicious_score >= p->suspicious_threshold && mass && balanced && avoids && not_rank1;
+}
+
+int lso_is_balanced_degenerate(const lso_triage* t, const lso_adaptive_params* p) { return balanced_degenerate(t, p); }
+
+/* choose_route (src/lie_svd_adaptive.rs:103-119): 0 Micro, 1 Small,
+ * 2 PhaseFlow, 3 CoreFlowTopo, 4 Hybrid (MICRO_MAX_N = 4, src/lie_svd.rs:31). */
+int lso_choose_route(const lso_triage* t, const lso_adaptive_params* p) {
+    if (t->n <= 4) return 0;
+    if (t->offdiag_ratio <= p->nearly_diagonal_ratio) return 1;
+    if (t->n <= p->phaseflow_max_n && lso_should_use_phaseflow(t, p)) return 2;
+    if (t->n <= p->coreflow_topo_max_n && lso_should_use_coreflow_topo(t, p)) return 3;
+    if (t->n <= p->small_max_n) return 1;
+    return 4;
+}

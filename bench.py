@@ -1,7 +1,8 @@
 #!/usr/bin/env python3
 """Benchmark: batched uneven-size SVD on one MI355X.
 
-Workload (one "step"): one hipGraph launch of the batch plan over
+Workload (one "step"): one execution of the batch plan (its buckets on four
+concurrent lanes) over
   - 4096 f64 matrices, m ~ U{8..128}, n ~ U{4..m}, A = U diag(s) V^T with
     Haar U, V, s log-spaced to condition 10^k, k ~ U[0, 12], 10% with a
     cluster of 2-4 equal singular values;
@@ -146,7 +147,7 @@ def joint_leg(h, dev, args):
     def run():  # the bench handle is in device pointer mode
         h.check(_lib.lib().lsvd_joint_diag_sym_f64(h.raw, F, K, n, ptr(md), C.byref(prm), ptr(vd), ptr(dd), tr))
 
-    run()  # warm-up (kernel load, graph build)
+    run()  # warm-up (kernel load, graph build where one is used)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     run()
@@ -208,7 +209,7 @@ def joint_svd_leg(h, dev, args):
         def run():
             h.check(_lib.lib().lsvd_joint_svd_f64(h.raw, F, K, n, n, ptr(md), C.byref(prm), ptr(ud), ptr(sd),
                                                   ptr(vd), tr))
-        run()  # warm-up (kernel load, graph build)
+        run()  # warm-up (kernel load, graph build where one is used)
         torch.cuda.synchronize()
         ts = []
         for _ in range(3):
@@ -374,6 +375,58 @@ def polish_leg(h, dev, args):
                                        f"{cpu:.2f}s"}}
 
 
+def triage_leg(h, args):
+    """The adaptive router's triage and route decision (analyze_matrix +
+    choose_route, src/lie_svd_adaptive.rs:103-274) for the polish leg's
+    square matrices in one call on device pointers.  Algorithmic traffic:
+    each matrix read once, 8 n^2 bytes (the kernel reads it four times:
+    rows, columns and two entry sweeps, mostly from L2)."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib, workloads
+    from lie_cliffalg_analog_svd_amd.adaptive import LieSvdAdaptiveParams
+    mats, _, _, _ = workloads.polish_batch(args.polish_count)
+    count = len(mats)
+    n = np.array([a.shape[0] for a in mats], dtype=np.int32)
+    off = np.zeros(count, dtype=np.int64)
+    off[1:] = np.cumsum(n.astype(np.int64) ** 2)[:-1]
+    pool = torch.from_numpy(np.concatenate([a.ravel() for a in mats])).to("cuda:0")
+    out = torch.zeros((count, C.sizeof(_lib.LsvdTriage) // 8), dtype=torch.float64, device="cuda:0")
+    prm = LieSvdAdaptiveParams()._c()
+
+    def run():
+        h.check(_lib.lib().lsvd_analyze_batched_f64(h.raw, count, n.ctypes.data_as(_lib._i32p),
+                                                    off.ctypes.data_as(_lib._i64p), C.c_void_p(pool.data_ptr()),
+                                                    C.byref(prm), C.cast(C.c_void_p(out.data_ptr()),
+                                                                         C.POINTER(_lib.LsvdTriage))))
+    run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        run()
+        ts.append(time.perf_counter() - t0)
+    sec = float(np.median(ts))
+    raw = out.cpu().numpy().tobytes()
+    tri = (_lib.LsvdTriage * count).from_buffer_copy(raw)
+    sample = list(range(0, count, 16))
+    t1 = time.perf_counter()
+    ref = [oracle.choose_route(mats[b]) for b in sample]
+    cpu = time.perf_counter() - t1
+    agree = sum(1 for b, (r, _) in zip(sample, ref) if oracle.ROUTES[tri[b].route] == r)
+    routes = {}
+    for b in range(count):
+        routes[oracle.ROUTES[tri[b].route]] = routes.get(oracle.ROUTES[tri[b].route], 0) + 1
+    byts = 8.0 * float(np.sum(n.astype(np.float64) ** 2))
+    return {"matrices": count, "seconds": round(sec, 6), "matrices_per_s": round(count / sec, 1),
+            "gbs_algorithmic": round(byts / sec / 1e9, 2), "routes": routes,
+            "route_agreement_sample": f"{agree}/{len(sample)}",
+            "cpu_baseline": {"value": round(len(sample) / cpu, 1), "unit": "matrices/s", "cores": 1, "kind": "port",
+                             "sample": f"every 16th matrix ({len(sample)}), C restatement of analyze_matrix + "
+                                       f"choose_route, {cpu:.2f}s"}}
+
+
 def vjp_leg(dev, args):
     """Gradient of L = sum_i w_i s_i + <G, U V^T> for 512 96 x 64 matrices
     with singular gaps in [1e-3, 2e-3] plus 32 with one gap of 1e-8."""
@@ -526,6 +579,7 @@ def main(argv=None) -> int:
     ap.add_argument("--joint-svd-families", type=int, default=64)
     ap.add_argument("--no-givens", action="store_true", help="skip the rotor-stream export leg")
     ap.add_argument("--no-polish", action="store_true", help="skip the digital-polish leg")
+    ap.add_argument("--no-triage", action="store_true", help="skip the adaptive triage leg")
     ap.add_argument("--polish-count", type=int, default=2048)
     ap.add_argument("--no-vjp", action="store_true", help="skip the vjp leg")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -591,7 +645,7 @@ def main(argv=None) -> int:
         plan.execute(a_pool, u_pool, s_pool, vt_pool)
     torch.cuda.synchronize()
 
-    # ---- timed region: exactly K graph launches
+    # ---- timed region: exactly K plan executions
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -630,7 +684,11 @@ def main(argv=None) -> int:
     # each launch timed while the other lanes run beside it, averaged over
     # several eager runs of the same schedule
     descs, order = plan.describe()
+    # a one-cycle spin kernel on either side marks these runs in a kernel
+    # trace, so a profile can average the same launches
+    torch.cuda._sleep(1)
     runs = [plan.time_buckets(a_pool, u_pool, s_pool, vt_pool) for _ in range(args.timing_runs)]
+    torch.cuda._sleep(1)
     bucket_ms = [float(np.mean([r[i] for r in runs])) for i in range(len(runs[0]))]
     torch.cuda.synchronize()
 
@@ -723,6 +781,9 @@ def main(argv=None) -> int:
     if not args.no_polish:
         legs["polish"] = polish_leg(h, dev, args)
         log(f"polish: {legs['polish']}")
+    if not args.no_triage:
+        legs["triage"] = triage_leg(h, args)
+        log(f"triage: {legs['triage']}")
     if not args.no_vjp:
         legs["vjp"] = vjp_leg(dev, args)
         log(f"vjp: {legs['vjp']}")
@@ -741,7 +802,7 @@ def main(argv=None) -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": "batched uneven-size SVD: 4096 f64 m~U[8,128], n~U[4,m], kappa 10^U[0,12], "
-                               "10% clustered + 8192 square 2..4 (LieSvdMicro), one hipGraph launch",
+                               "10% clustered + 8192 square 2..4 (LieSvdMicro), one plan execution per step",
                    "matrices": count, "schedule": args.precision, "parallelism": "single GPU"},
         "plain_fp64": plain,
         "sigma_err": {"rel_max_vs_imposed": e_rel_max, "abs_max_vs_imposed": e_abs_max,

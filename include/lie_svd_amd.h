@@ -419,8 +419,9 @@ typedef struct {
  * NULL) receives, for every queue entry, the caller's matrix index. */
 int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, int* nbuckets, int32_t* order);
 
-/* Runs the plan once without a graph, on the same concurrent lanes as the
- * graph, with HIP events recorded on each lane stream around every launch,
+/* Runs the plan once without a graph, on the same concurrent lanes as
+ * lsvd_batch_plan_execute, with HIP events recorded on each lane stream
+ * around every launch,
  * and writes each launch's time in ms (same order as describe): the
  * duration of a bucket while the others run beside it.  Device pointer
  * mode only. */

@@ -776,9 +776,8 @@ int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_
     hipStream_t st = h->stream;
     Pools pools{a_pool, u_pool, s_pool, vt_pool, nullptr};
     const int nb = (int)p->buckets.size();
-    // one eager run of the same lane schedule as the graph, with start/stop
-    // events around every launch: each duration is measured while the other
-    // lanes run beside it, as in the graph
+    // one run of the execute schedule with start/stop events around every
+    // launch: each duration is measured while the other lanes run beside it
     std::vector<hipEvent_t> ev(2 * (nb + 1), nullptr);
     int rc = LSVD_OK;
     for (auto& x : ev)

@@ -120,7 +120,7 @@ def pack_offsets(ms, ns, eigh: bool = False):
 
 
 class BatchPlan:
-    """lsvd_batch_plan_t: a sorted, bucketed, graph-captured ragged batch."""
+    """lsvd_batch_plan_t: a sorted, bucketed ragged batch on concurrent lanes."""
 
     def __init__(self, handle: Handle, ms, ns, precision: int = _lib.LSVD_PREC_F64):
         self.handle = handle

@@ -134,7 +134,7 @@ def run_small(a, handle: Handle | None = None):
 
 
 def svd_batched(mats, precision: int = _lib.LSVD_PREC_F64, handle: Handle | None = None, with_info=False):
-    """Ragged batch in one graph launch (host arrays in, host arrays out).
+    """Ragged batch in one plan execution (host arrays in, host arrays out).
 
     Square n <= 4 matrices take the LieSvdMicro schedule; the rest take
     LieSvdSmall.solve_rectangular.  Returns a list of (U, s, Vt)."""

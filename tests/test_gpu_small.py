@@ -50,6 +50,13 @@ def test_square_uniform_vs_oracle(n):
     _check_vs_oracle(uniform_matrix(100 + n, n, n), f"uniform n={n}")
 
 
+@pytest.mark.parametrize("n", [129, 150, 200, 256])
+def test_square_above_on_chip_sizes_vs_oracle(n):
+    """N > 128: the workgroup-per-matrix kernels with global slots
+    (LieSvd routes these to LieSvdSmall up to SMALL_MAX_N = 512)."""
+    _check_vs_oracle(uniform_matrix(300 + n, n, n), f"uniform n={n}")
+
+
 def test_single_64x64_normal():
     """The 64 x 64 N(0,1) single-matrix workload."""
     a = rng.normal_matrix(0x5EED0001, 64, 64)

@@ -171,6 +171,7 @@ def joint_leg(h, dev, args):
     oracle.joint_batch(fams[:threads], one, threads=threads)
     cpu_sweep = time.perf_counter() - t2
     cpu_est = cpu_sweep * float(np.mean(sweeps)) * (F / threads)
+    smoke = jade_smoke(h, dev)
     return {"seconds": round(sec, 4), "families": F, "K": K, "n": n,
             "sweeps": {"min": int(min(sweeps)), "max": int(max(sweeps)), "mean": float(np.mean(sweeps))},
             "gbs": round(total_bytes / sec / 1e9, 2), "frac_of_hbm": round(total_bytes / sec / 1e9 / HBM_PEAK_GBS, 5),
@@ -180,12 +181,73 @@ def joint_leg(h, dev, args):
                              "sample": f"one sweep of {threads} families (one per thread) of the C restatement, "
                                        f"{cpu_sweep:.2f}s, scaled by the GPU run's mean sweep count and "
                                        f"{F}/{threads} families"},
-            "gpu_families_per_s": round(F / sec, 3)}
+            "gpu_families_per_s": round(F / sec, 3), "smoke": smoke}
+
+
+# `stress_cpu --joint` lines the reference publishes (README.md:1345-1348):
+# n -> (initial, final, sweeps, rotations, seconds)
+JADE_SMOKE_PUBLISHED = {16: (2.537e0, 9.848e-14, 6, 681, 0.002), 64: (5.835e0, 1.398e-12, 8, 14917, 0.078),
+                        128: (8.353e0, 2.695e-12, 10, 63520, 0.66), 256: (1.198e1, 8.459e-12, 11, 270435, 11.24)}
+
+
+def jade_smoke(h, dev):
+    """The reference's Phase-JADE smoke (family of 6 symmetric matrices
+    Q diag(1 + 0.2 k + |sin(i + 1)|) Q^T, src/bin/stress_cpu.rs:468-495) at
+    n = 16, 64, 128, 256, one family per call as the smoke runs it: GPU
+    seconds beside the reference's published seconds and the C restatement
+    timed on this host (single thread, as the reference runs it)."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib, workloads
+    from lie_cliffalg_analog_svd_amd.joint import JointDiagonalizationParams
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    out = []
+    for n, (p_init, p_final, p_sweeps, p_rot, p_sec) in JADE_SMOKE_PUBLISHED.items():
+        fam, q = workloads.jade_stress_family(n)
+        K = fam.shape[0]
+        md = torch.from_numpy(fam[None]).to(dev)
+        vd = torch.empty((1, n, n), dtype=torch.float64, device=dev)
+        dd = torch.empty((1, K, n), dtype=torch.float64, device=dev)
+        prm = JointDiagonalizationParams.for_n(n)._c()
+        tr = (_lib.LsvdJointTrace * 1)()
+
+        def run():
+            h.check(_lib.lib().lsvd_joint_diag_sym_f64(h.raw, 1, K, n, ptr(md), C.byref(prm), ptr(vd), ptr(dd), tr))
+        run()
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            run()
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        sec = float(np.median(ts))
+        t = tr[0].as_dict()
+        t1 = time.perf_counter()
+        _, _, otr = oracle.joint_diag_sym(fam, oracle.joint_params_for_n(n))
+        cpu = time.perf_counter() - t1
+        out.append({"n": n, "family": K, "gpu_s": round(sec, 5),
+                    "offdiag": [t["initial_offdiag"], t["final_offdiag"]], "sweeps": t["sweeps"],
+                    "recovery_err": workloads.recovery_error(q, vd[0].cpu().numpy()),
+                    "reference_published": {"offdiag": [p_init, p_final], "sweeps": p_sweeps, "rotations": p_rot,
+                                            "seconds": p_sec},
+                    "oracle": {"seconds": round(cpu, 4), "sweeps": otr["sweeps"], "rotations": otr["rotations"],
+                               "final_offdiag": otr["final_offdiag"]},
+                    "speedup_vs_published": round(p_sec / sec, 1)})
+        log(f"jade smoke n={n}: gpu {sec:.4f}s, oracle {cpu:.3f}s, published {p_sec}s, sweeps {t['sweeps']} "
+            f"(oracle {otr['sweeps']}), offdiag {t['initial_offdiag']:.3e}->{t['final_offdiag']:.3e}")
+    return out
+
+
+# `stress_cpu --joint-svd` lines the reference publishes (README.md:1366-1367)
+JOINT_SVD_SMOKE_PUBLISHED = {64: (4.688e1, 4.582e-12, 12, 23384, 0.071), 128: (1.076e2, 6.987e-12, 16, 118627, 1.06)}
 
 
 def joint_svd_leg(h, dev, args):
-    """LieSvdJoint::joint_svd on the stress smoke families (K = 4 matrices
-    U0 diag(1 + 0.15 k + 0.05 i) V0^T, src/bin/stress_cpu.rs:597-626) at
+    """LieSvdJoint::joint_svd on the reference's stress smoke families (K = 4
+    matrices U0 diag(1 + 0.15 k + 0.05 i) V0^T with its deterministic U0, V0,
+    src/bin/stress_cpu.rs:597-626) at
     n = 64 and 128, one family per call as the smoke runs it, then many
     n = 128 families in one call.  Algorithmic traffic per layer: the K
     matrices and both bases read and written once, 16 (K + 2) n^2 bytes."""
@@ -221,18 +283,24 @@ def joint_svd_leg(h, dev, args):
 
     out = {"K": K, "smoke": []}
     for n in (64, 128):
-        fam, _, _, d = workloads.joint_svd_family(n, K)
+        fam, _, _ = workloads.joint_svd_stress_family(n, K)
+        i = np.arange(n, dtype=np.float64)
+        d = [1.0 + 0.15 * k + 0.05 * i for k in range(K)]
         sec, sg, trs = gpu(fam[None])
         t = trs[0]
         serr = max(float(np.max(np.abs(np.sort(sg[0, k]) - np.sort(d[k])))) for k in range(K))
         t1 = time.perf_counter()
         _, osg, _, otr = oracle.joint_svd(fam)
         cpu = time.perf_counter() - t1
+        p_init, p_final, p_sweeps, p_rot, p_sec = JOINT_SVD_SMOKE_PUBLISHED[n]
         out["smoke"].append({
             "n": n, "gpu_s": round(sec, 5), "offdiag": [t["initial_offdiag"], t["final_offdiag"]],
             "sweeps": t["sweeps"], "rotations": t["rotations"], "rejected": t["rejected_rotations"],
             "sigma_err_vs_imposed": serr,
             "sigma_diff_vs_oracle": max(float(np.max(np.abs(np.sort(sg[0, k]) - np.sort(osg[k])))) for k in range(K)),
+            "reference_published": {"offdiag": [p_init, p_final], "sweeps": p_sweeps, "rotations": p_rot,
+                                    "seconds": p_sec},
+            "speedup_vs_published": round(p_sec / sec, 1),
             "oracle": {"seconds": round(cpu, 4), "sweeps": otr["sweeps"], "final_offdiag": otr["final_offdiag"]},
             "cpu_baseline": {"value": round(cpu, 4), "unit": "s/family", "cores": 1, "kind": "port",
                              "sample": "the same family, C restatement of LieSvdJoint::joint_svd_with_params"}})

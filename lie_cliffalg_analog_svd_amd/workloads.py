@@ -117,6 +117,62 @@ def joint_svd_family(n: int, K: int = 4, seed: int = SEED_JADE + 1):
     return np.ascontiguousarray(mats), u0, v0, d
 
 
+STRESS_SEED_JADE = 0x9E3779B97F4A7C15
+STRESS_SEED_JOINT_SVD = 0xA24BAED4963EE407
+_M64 = (1 << 64) - 1
+
+
+def _seq_sum(x: np.ndarray) -> float:
+    # left-to-right accumulation, one rounding per add (np.sum is pairwise)
+    return float(np.cumsum(x)[-1]) if x.size else 0.0
+
+
+def stress_orthogonal(n: int, state: int):
+    """The deterministic orthogonal factor of the reference's stress smokes
+    (src/bin/stress_cpu.rs:805-838): n*n draws of the 64-bit LCG
+    s <- s * 6364136223846793005 + 1, x = (s >> 11) / 2^53 - 0.5, filled
+    row-major, then classical Gram-Schmidt column by column against the
+    already normalised columns (dot products accumulated left to right) and
+    division by max(norm, 1e-300).  Returns (Q, next LCG state)."""
+    vals = np.empty(n * n)
+    s = state
+    for k in range(n * n):
+        s = (s * 6364136223846793005 + 1) & _M64
+        vals[k] = (s >> 11) / float(1 << 53) - 0.5
+    q = vals.reshape(n, n).copy()
+    for j in range(n):
+        for k in range(j):
+            dot = _seq_sum(q[:, j] * q[:, k])
+            q[:, j] -= dot * q[:, k]
+        nrm = max(float(np.sqrt(_seq_sum(q[:, j] * q[:, j]))), 1e-300)
+        q[:, j] /= nrm
+    return q, s
+
+
+def jade_stress_family(n: int, family: int = 6):
+    """The Phase-JADE smoke family of `stress_cpu --joint`
+    (src/bin/stress_cpu.rs:474-486): M_k = Q diag(1 + 0.2 k + |sin(i + 1)|) Q^T,
+    k < 6, Q = stress_orthogonal(n, 0x9e3779b97f4a7c15).  The reference clamps
+    n to [4, 256].  Returns (mats (family, n, n), Q)."""
+    q, _ = stress_orthogonal(n, STRESS_SEED_JADE)
+    i = np.arange(n, dtype=np.float64)
+    mats = np.stack([(q * (1.0 + 0.2 * k + np.abs(np.sin(i + 1.0)))[None, :]) @ q.T for k in range(family)])
+    return np.ascontiguousarray(mats), q
+
+
+def joint_svd_stress_family(n: int, family: int = 4):
+    """The two-sided family of `stress_cpu --joint-svd`
+    (src/bin/stress_cpu.rs:597-612): A_k = U0 diag(1 + 0.15 k + 0.05 i) V0^T,
+    k < 4, U0 then V0 drawn in turn from one stress_orthogonal stream seeded
+    0xa24baed4963ee407.  The reference clamps n to [4, 128].  Returns
+    (mats (family, n, n), U0, V0)."""
+    u0, s = stress_orthogonal(n, STRESS_SEED_JOINT_SVD)
+    v0, _ = stress_orthogonal(n, s)
+    i = np.arange(n, dtype=np.float64)
+    mats = np.stack([(u0 * (1.0 + 0.15 * k + 0.05 * i)[None, :]) @ v0.T for k in range(family)])
+    return np.ascontiguousarray(mats), u0, v0
+
+
 def polish_batch(count: int = 2048, seed: int = SEED_BATCH + 11, eps: float = 1e-2):
     """Digital-polish inputs: square A = U diag(s) V^T with n log-uniform in
     [8, 128], s log-uniform over kappa up to 1e8, and bases U0, V0 that are

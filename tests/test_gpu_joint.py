@@ -164,3 +164,58 @@ def test_joint_svd_batched_families_and_errors():
         LieSvdJoint.joint_svd([])
     with pytest.raises(ShapeError):
         LieSvdJoint.joint_svd([np.eye(3), np.eye(4)])
+
+
+# `stress_cpu --joint` / `--joint-svd` lines the reference publishes
+# (README.md:1345-1348, 1366-1367; N=16 full-suite rows README.md:2131-2132):
+# n -> (initial, final, sweeps)
+JADE_SMOKE = {16: (2.537e0, 9.848e-14, 6), 64: (5.835e0, 1.398e-12, 8), 128: (8.353e0, 2.695e-12, 10),
+              256: (1.198e1, 8.459e-12, 11)}
+JOINT_SVD_SMOKE = {16: (1.267e1, 9.231e-14, None), 64: (4.688e1, 4.582e-12, 12), 128: (1.076e2, 6.987e-12, 16)}
+
+
+@pytest.mark.parametrize("n", sorted(JADE_SMOKE))
+def test_jade_reference_smoke(n):
+    """The reference's Phase-JADE smoke family (src/bin/stress_cpu.rs:468-495)
+    on the GPU: the initial residual equals the published one to its 4
+    printed digits; the GPU's round-robin layers reach the same rounding
+    floor (within 4x the published final residual) in no more sweeps than
+    the published count plus 2, and recover the planted Q and the imposed
+    diagonals to relative error 1e-12."""
+    print("jade smoke", n, flush=True)
+    p_init, p_final, p_sweeps = JADE_SMOKE[n]
+    mats, q = workloads.jade_stress_family(n)
+    v, diags, tr = LieSvdJoint.diagonalize_symmetric(list(mats))
+    print("  gpu", tr, flush=True)
+    assert "%.3e" % tr.initial_offdiag == "%.3e" % p_init
+    assert tr.final_offdiag < 4.0 * p_final
+    assert tr.sweeps <= p_sweeps + 2 and tr.rejected_rotations == 0
+    assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-12 * n
+    assert workloads.recovery_error(q, v) < 1e-10
+    i = np.arange(n, dtype=np.float64)
+    for k in range(len(mats)):
+        want = np.sort(1.0 + 0.2 * k + np.abs(np.sin(i + 1.0)))
+        assert np.max(np.abs(np.sort(diags[k]) - want) / want) < 1e-12
+
+
+@pytest.mark.parametrize("n", sorted(JOINT_SVD_SMOKE))
+def test_joint_svd_reference_smoke(n):
+    """The reference's two-sided joint SVD smoke (src/bin/stress_cpu.rs:
+    597-626) on the GPU: published initial residual to 4 digits, the same
+    rounding floor (within 4x the published final residual), sweeps within
+    the published count plus 2, the planted U0, V0, and the imposed singular
+    values to relative error 1e-12."""
+    print("joint svd smoke", n, flush=True)
+    p_init, p_final, p_sweeps = JOINT_SVD_SMOKE[n]
+    mats, u0, v0 = workloads.joint_svd_stress_family(n)
+    u, sg, vt, tr = LieSvdJoint.joint_svd(list(mats))
+    print("  gpu", tr, flush=True)
+    assert "%.3e" % tr.initial_offdiag == "%.3e" % p_init
+    assert tr.final_offdiag < 4.0 * p_final
+    if p_sweeps is not None:
+        assert tr.sweeps <= p_sweeps + 2
+    assert workloads.recovery_error(u0, u) < 1e-10 and workloads.recovery_error(v0, vt.T) < 1e-10
+    i = np.arange(n, dtype=np.float64)
+    for k in range(len(mats)):
+        want = np.sort(1.0 + 0.15 * k + 0.05 * i)
+        assert np.max(np.abs(np.sort(sg[k]) - want) / want) < 1e-12

@@ -122,6 +122,61 @@ def single_leg(h, dev, args):
                              "sample": "the same matrix, C restatement of LieSvdSmall::solve, median of 5"}}
 
 
+# LieSvdSmall::solve timings the reference states (src/lie_svd_small.rs:51-54)
+LARGE_PUBLISHED_S = {384: 2.3, 512: 7.0, 1024: 77.0}
+
+
+def large_leg(h, dev, args):
+    """One large N(0,1) matrix through LieSvdSmall::solve on device
+    pointers (the chip-wide route: polar GEMMs, Jacobi over pair layers on
+    all CUs, sort, U = Q V) at n = 384, 512, 1024, beside the reference's
+    stated CPU timings; the C restatement timed on this host at n = 384."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib, metrics, rng
+    fn = _lib.lib().lsvd_small_solve_f64
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    out = []
+    for n, pub in LARGE_PUBLISHED_S.items():
+        a = rng.normal_matrix(0x5EED0100 + n, n, n)
+        ad = torch.from_numpy(a).to(dev)
+        u = torch.empty((n, n), dtype=torch.float64, device=dev)
+        s = torch.empty(n, dtype=torch.float64, device=dev)
+        vt = torch.empty((n, n), dtype=torch.float64, device=dev)
+        info = torch.zeros(8, dtype=torch.int32, device=dev)
+        h.check(fn(h.raw, ptr(ad), n, ptr(u), ptr(s), ptr(vt), ptr(info)))  # warm-up
+        torch.cuda.synchronize()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            h.check(fn(h.raw, ptr(ad), n, ptr(u), ptr(s), ptr(vt), ptr(info)))
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        sec = float(np.median(ts))
+        inf = info.cpu().numpy()
+        uh, sh, vth = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+        mm = metrics.compute(a, uh, sh, vth)
+        ref = np.linalg.svd(a, compute_uv=False)
+        top = ref >= 1e-2 * ref[0]
+        row = {"n": n, "gpu_s": round(sec, 4), "path": int(inf[6]), "ns_iters": int(inf[1]),
+               "jacobi_sweeps": int(inf[2]), "rel_recon": mm.rel_recon, "orth_u": mm.orth_u, "orth_v": mm.orth_v,
+               "sigma_rel_vs_lapack_top": float(np.max(np.abs(sh[top] - ref[top]) / ref[top])),
+               "reference_stated_s": pub, "speedup_vs_stated": round(pub / sec, 1)}
+        if n == 384:
+            t0 = time.perf_counter()
+            _, so, _, _ = oracle.small_solve(a)
+            cpu = time.perf_counter() - t0
+            e_rel, e_abs = metrics.sigma_errors(sh, so)
+            row["sigma_vs_oracle"] = {"rel": e_rel, "abs": e_abs}
+            row["cpu_baseline"] = {"value": round(cpu, 3), "unit": "s/matrix", "cores": 1, "kind": "port",
+                                   "sample": "the same matrix, C restatement of LieSvdSmall::solve"}
+        out.append(row)
+        log(f"large n={n}: gpu {sec:.4f}s (stated reference {pub}s), path {int(inf[6])}, "
+            f"recon {mm.rel_recon:.2e}")
+    return out
+
+
 def joint_leg(h, dev, args):
     """Phase-JADE (LieSvdJoint::diagonalize_symmetric) of 32 families x K=64
     symmetric 256 x 256 matrices on device pointers.  Algorithmic traffic:
@@ -641,6 +696,7 @@ def main(argv=None) -> int:
     ap.add_argument("--timing-runs", type=int, default=5, help="eager runs averaged for per-launch times")
     ap.add_argument("--no-single", action="store_true", help="skip the single 64x64 leg")
     ap.add_argument("--single-reps", type=int, default=50)
+    ap.add_argument("--no-large", action="store_true", help="skip the large single-matrix leg")
     ap.add_argument("--no-joint", action="store_true", help="skip the Phase-JADE leg")
     ap.add_argument("--joint-families", type=int, default=32)
     ap.add_argument("--no-joint-svd", action="store_true", help="skip the two-sided joint SVD leg")
@@ -835,6 +891,8 @@ def main(argv=None) -> int:
     if not args.no_single:
         legs["single"] = single_leg(h, dev, args)
         log(f"single: {legs['single']}")
+    if not args.no_large:
+        legs["large"] = large_leg(h, dev, args)
     if not args.no_polar:
         legs["polar"] = polar_leg(h, dev, args)
     if not args.no_joint:

@@ -75,7 +75,8 @@ typedef struct {
     int32_t rotations;      /* rotations applied */
     int32_t repaired;       /* left basis repaired from the right one */
     int32_t micro_fallback; /* LieSvdMicro escalated to LieSvdSmall */
-    int32_t path;           /* 0 small, 1 micro, 2 micro->small fallback */
+    int32_t path;           /* 0 small, 1 micro, 2 micro->small fallback,
+                               3 chip-wide route of a large single matrix */
     int32_t refine_iters;   /* FP64 refinement iterations (mixed precision) */
 } lsvd_info_t;
 
@@ -125,7 +126,15 @@ const char* lsvd_version(void);
 /* ---- LieSvdSmall ----------------------------------------------------- */
 
 /* Replaces LieSvdSmall::solve (src/lie_svd_small.rs:296-323).
- * a: n x n.  u: n x n, s: n, vt: n x n.  info may be NULL. */
+ * a: n x n (n <= 1024).  u: n x n, s: n, vt: n x n.  info may be NULL.
+ * n <= 128 runs on one workgroup (LDS-resident kernels).  Larger n takes
+ * the chip-wide route (info.path = 3): the polar iteration on the MFMA
+ * GEMMs, P = sym(Q^T A), cyclic Jacobi over round-robin pair layers spread
+ * over all CUs, the stable descending sort and U = Q V.  When the polar
+ * factor is not trusted or Q V fails the orthogonality test, the solve
+ * takes the reference's repair branch on the workgroup kernels (path 0).
+ * The environment variable LSVD_LARGE_MIN_N, read when a handle is
+ * created, moves the route's first n (values above 1024 disable it). */
 int lsvd_small_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
                          lsvd_info_t* info);
 

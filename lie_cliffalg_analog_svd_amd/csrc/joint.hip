@@ -89,8 +89,9 @@ __global__ void joint_setup_kernel(JointArgs a) {
     if (threadIdx.x == 0) {
         JointFamily& s = a.fam[f];
         const double ref = fmax(sqrt(fro), 1e-300);
-        s.pair_tol = a.min_pair_scale * fmax(ref, 1.0);
-        s.tol = a.tol * fmax(ref, 1.0);
+        const double scale = a.rel_tol ? ref : fmax(ref, 1.0);
+        s.pair_tol = a.min_pair_scale * scale;
+        s.tol = a.tol * scale;
         s.initial_offdiag = sqrt(off);
         s.before = sqrt(off);
         s.final_offdiag = sqrt(off);

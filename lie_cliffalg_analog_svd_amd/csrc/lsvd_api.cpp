@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <mutex>
@@ -18,6 +19,7 @@
 #include <vector>
 
 #include "lsvd_api_internal.hpp"
+#include "lsvd_large.hpp"
 
 using namespace lsvd;
 
@@ -551,6 +553,12 @@ int lsvd_create(int device, lsvd_handle_t* out) {
         return LSVD_E_HIP;
     }
     h->stream = h->own_stream;
+    h->large_min_n = kLargeMinNDefault;
+    if (const char* env = std::getenv("LSVD_LARGE_MIN_N")) {
+        char* end = nullptr;
+        const long v = std::strtol(env, &end, 10);
+        if (end != env && v >= 2) h->large_min_n = (int)std::min<long>(v, kLargeMaxN + 1);
+    }
     *out = h;
     return LSVD_OK;
 }
@@ -825,6 +833,13 @@ static int single(lsvd_handle_t h, int mode, const double* a, int m, int n, doub
         return LSVD_OK;
     }
     if (!a || !u || !s || (mode != kModeEigh && !vt)) return set_error(h, LSVD_E_ARG, "null pointer");
+    if (mode == kModeSvd && m == n && n >= h->large_min_n && n <= kLargeMaxN) {
+        hipError_t e = hipSetDevice(h->device);
+        if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+        bool done = false;
+        int rc = svd_large_square(h, a, n, u, s, vt, info, &done);
+        if (rc != LSVD_OK || done) return rc;
+    }
     const uint64_t key = ((uint64_t)mode << 60) | ((uint64_t)(uint32_t)m << 30) | (uint64_t)(uint32_t)n;
     lsvd_plan_t p = nullptr;
     auto it = h->plan_cache.find(key);

@@ -32,6 +32,46 @@ struct DevBuf {
 
 }  // namespace
 
+namespace lsvd {
+
+int joint_run_sweeps(lsvd_handle_t h, const JointArgs& a, int max_sweeps, JointFamily* fam) {
+    hipStream_t st = h->stream;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return hip_fail(h, e, "hipStreamBeginCapture");
+    hipError_t le = joint_launch_sweep(a, st);
+    e = hipStreamEndCapture(st, &graph);
+    if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
+    if (le != hipSuccess) {
+        hipGraphDestroy(graph);
+        return hip_fail(h, le, "joint sweep launch");
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(graph);
+        return hip_fail(h, e, "hipGraphInstantiate");
+    }
+    int rc = LSVD_OK;
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        e = hipGraphLaunch(exec, st);
+        if (e == hipSuccess) e = hipMemcpyAsync(fam, a.fam, sizeof(JointFamily) * a.F, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            rc = hip_fail(h, e, "joint sweep");
+            break;
+        }
+        bool all = true;
+        for (int f = 0; f < a.F; ++f) all = all && fam[f].done;
+        if (all) break;
+    }
+    hipGraphExecDestroy(exec);
+    hipGraphDestroy(graph);
+    return rc;
+}
+
+}  // namespace lsvd
+
 extern "C" {
 
 // JointDiagonalizationParams::for_n (src/lie_svd_joint.rs:26-35).
@@ -113,38 +153,7 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
     JCHECK(joint_launch_init(a, st));
     std::vector<JointFamily> fam(families);
     if (n > 1 && prm.max_sweeps > 0) {
-        // One sweep = 2 launches per layer + the stop test, captured once as a
-        // graph and relaunched until every family has stopped.
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        JCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t le = joint_launch_sweep(a, st);
-        JCHECK(hipStreamEndCapture(st, &graph));
-        if (le != hipSuccess) {
-            hipGraphDestroy(graph);
-            return hip_fail(h, le, "joint sweep launch");
-        }
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            hipGraphDestroy(graph);
-            return hip_fail(h, e, "hipGraphInstantiate");
-        }
-        int rc = LSVD_OK;
-        for (int sweep = 0; sweep < prm.max_sweeps; ++sweep) {
-            e = hipGraphLaunch(exec, st);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(fam.data(), a.fam, sizeof(JointFamily) * families, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) {
-                rc = hip_fail(h, e, "joint sweep");
-                break;
-            }
-            bool all = true;
-            for (auto& s : fam) all = all && s.done;
-            if (all) break;
-        }
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
+        int rc = joint_run_sweeps(h, a, prm.max_sweeps, fam.data());
         if (rc != LSVD_OK) return rc;
     }
     JCHECK(joint_launch_diag(a, d_diag, st));

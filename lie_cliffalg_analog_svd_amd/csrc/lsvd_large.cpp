@@ -1,0 +1,150 @@
+// lsvd_large.cpp -- LieSvdSmall::solve (src/lie_svd_small.rs:296-323) for one
+// large square matrix with every step spread over the chip:
+//   Q = newton_schulz_polar(A)            polar_run (MFMA GEMMs, all CUs)
+//   P = sym(Q^T A)                        gemm_f64 + large_symmetrize
+//   (V, eig) = eigh_jacobi_full(P)        Phase-JADE layer kernels, one
+//                                         matrix per family (cyclic Jacobi
+//                                         over the round-robin pair layers)
+//   sort, sigma = max(eig, 0), U = Q V    large_rank / large_permute + gemm_f64
+// The workgroup-per-matrix kernels run each step on one CU; for a single
+// matrix above a couple of hundred rows this route is the faster one.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+
+#include "lsvd_api_internal.hpp"
+#include "lsvd_joint.hpp"
+#include "lsvd_large.hpp"
+#include "lsvd_polar.hpp"
+
+namespace lsvd {
+
+namespace {
+
+struct Mem {
+    void* p = nullptr;
+    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
+    ~Mem() {
+        if (p) hipFree(p);
+    }
+};
+
+}  // namespace
+
+int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
+                     bool* done) {
+    *done = false;
+    if (n < 2 || n > kLargeMaxN) return LSVD_OK;
+    hipStream_t st = h->stream;
+    const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    const size_t nn = (size_t)n * n;
+    const int slots = (n + (n & 1)) / 2;
+#define LCHK(expr)                                          \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+    Mem bA, bQ, bP, bW, bV, bVs, bOff, bPart, bRot, bFam, bDiag, bPerm, bScal, bU, bS, bVt;
+    const double* dA = a;
+    double *dU = u, *dS = s, *dVt = vt;
+    if (host) {
+        LCHK(bA.alloc(sizeof(double) * nn));
+        LCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+        dA = (const double*)bA.p;
+        LCHK(bU.alloc(sizeof(double) * nn));
+        LCHK(bS.alloc(sizeof(double) * n));
+        LCHK(bVt.alloc(sizeof(double) * nn));
+        dU = (double*)bU.p;
+        dS = (double*)bS.p;
+        dVt = (double*)bVt.p;
+    }
+    LCHK(bQ.alloc(sizeof(double) * nn));
+    lsvd_polar_info_t pinf{};
+    int rc = polar_run(h, dA, n, n, LSVD_PREC_F64, (double*)bQ.p, &pinf);
+    if (rc != LSVD_OK) return rc;
+    if (!pinf.polar_ok) return LSVD_OK;  // the repair branch (:316-318)
+    const double* dQ = (const double*)bQ.p;
+
+    LCHK(bP.alloc(sizeof(double) * nn));
+    double* dP = (double*)bP.p;
+    LCHK(gemm_f64(true, n, n, n, dQ, dA, dP, 1.0, st));
+    LCHK(large_symmetrize(dP, n, st));
+
+    // eigh_jacobi_full (:212-285): stop once the upper-triangle off-diagonal
+    // norm is below JACOBI_TOL ||P||_F; the layer kernels measure the full
+    // off-diagonal norm (sqrt 2 times the upper one) and scale by ||P||_F.
+    LCHK(bW.alloc(sizeof(double) * nn));
+    LCHK(bV.alloc(sizeof(double) * nn));
+    LCHK(bOff.alloc(sizeof(double) * n));
+    LCHK(bPart.alloc(sizeof(double) * 2));
+    LCHK(bRot.alloc(sizeof(JointRot) * slots));
+    LCHK(bFam.alloc(sizeof(JointFamily)));
+    JointArgs ja{};
+    ja.F = 1;
+    ja.K = 1;
+    ja.n = n;
+    ja.slots = slots;
+    ja.mats = dP;
+    ja.work = (double*)bW.p;
+    ja.basis = (double*)bV.p;
+    ja.offrow = (double*)bOff.p;
+    ja.part = (double*)bPart.p;
+    ja.rot = (JointRot*)bRot.p;
+    ja.fam = (JointFamily*)bFam.p;
+    ja.tol = std::sqrt(2.0) * 1e-13;
+    ja.min_pair_scale = 0.0;
+    ja.line_search_steps = 1;
+    ja.max_sweeps = 60 + 20 * n;
+    ja.rel_tol = 1;
+    LCHK(joint_launch_init(ja, st));
+    JointFamily fam{};
+    rc = joint_run_sweeps(h, ja, ja.max_sweeps, &fam);
+    if (rc != LSVD_OK) return rc;
+
+    LCHK(bDiag.alloc(sizeof(double) * n));
+    LCHK(bPerm.alloc(sizeof(int) * n));
+    LCHK(bScal.alloc(sizeof(double) * 2));
+    LCHK(bVs.alloc(sizeof(double) * nn));
+    double* dDiag = (double*)bDiag.p;
+    double* scal = (double*)bScal.p;  // [0] ||U^T U - I||_F^2, [1] non-finite flag (as int)
+    LCHK(joint_launch_diag(ja, dDiag, st));
+    LCHK(polar_zero(scal, 2, st));
+    LCHK(large_rank(dDiag, n, (int*)bPerm.p, (int*)(scal + 1), st));
+    LCHK(large_permute((const double*)bV.p, dDiag, (const int*)bPerm.p, n, (double*)bVs.p, dVt, dS, st));
+    LCHK(gemm_f64(false, n, n, n, dQ, (const double*)bVs.p, dU, 1.0, st));
+    LCHK(ns_orth(n, n, dU, scal, st));
+    double hs[2];
+    LCHK(hipMemcpyAsync(hs, scal, sizeof(hs), hipMemcpyDeviceToHost, st));
+    LCHK(hipStreamSynchronize(st));
+    int bad = 0;
+    std::memcpy(&bad, &hs[1], sizeof(int));
+    const double orth = std::sqrt(std::max(hs[0], 0.0));
+    if (bad || !(orth <= 1e-8 * std::sqrt((double)n))) return LSVD_OK;  // the repair branch
+    if (host) {
+        LCHK(hipMemcpyAsync(u, dU, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+        LCHK(hipMemcpyAsync(s, dS, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        LCHK(hipMemcpyAsync(vt, dVt, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+        LCHK(hipStreamSynchronize(st));
+    }
+#undef LCHK
+    if (info) {
+        lsvd_info_t inf{};
+        inf.polar_ok = 1;
+        inf.ns_iters = pinf.f64_iters;
+        inf.jacobi_sweeps = fam.sweeps;
+        inf.rotations = fam.rotations;
+        inf.path = 3;
+        if (host) {
+            *info = inf;
+        } else {
+            hipError_t e = hipMemcpyAsync(info, &inf, sizeof(inf), hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+            if (e != hipSuccess) return hip_fail(h, e, "info copy");
+        }
+    }
+    *done = true;
+    return LSVD_OK;
+}
+
+}  // namespace lsvd

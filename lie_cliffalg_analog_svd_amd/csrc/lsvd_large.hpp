@@ -1,0 +1,27 @@
+// lsvd_large.hpp -- the chip-wide route of LieSvdSmall::solve for one large
+// square matrix (large.hip kernels, lsvd_large.cpp driver).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/lie_svd_amd.h"
+
+namespace lsvd {
+
+constexpr int kLargeMaxN = 1024;          // the single-matrix API's limit
+constexpr int kLargeMinNDefault = 129;    // smallest n routed chip-wide by default (above the on-chip sizes)
+
+hipError_t large_symmetrize(double* P, int n, hipStream_t st);
+hipError_t large_rank(const double* d, int n, int* perm, int* bad, hipStream_t st);
+hipError_t large_permute(const double* V, const double* d, const int* perm, int n, double* Vs, double* Vt,
+                         double* sigma, hipStream_t st);
+
+// LieSvdSmall::solve of one n x n matrix with every step spread over the
+// chip.  Pointers follow the handle's pointer mode.  Sets *done = false
+// (and returns LSVD_OK) when the polar factor is not trusted or U = Q V
+// fails the orthogonality test: those cases take the reference's repair
+// branch, which the caller runs through the workgroup-per-matrix kernels.
+int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
+                     bool* done);
+
+}  // namespace lsvd

@@ -24,52 +24,26 @@ struct Buf {
 
 }  // namespace
 
-extern "C" {
+namespace lsvd {
 
-// NS polar factor of a tall or square m x n matrix (m >= n).
-//   LSVD_PREC_F64 : the reference iteration in FP64 (X0 = A/||A||_F, X <- X(3I - X^T X)/2,
-//                   stop at ||dX||_F < 1e-13 sqrt(n), polar_ok = ||Q^T Q - I||_F <= 1e-8 sqrt(n)).
-//   LSVD_PREC_F64_SCALED : FP64 with the scaled step schedule (NsSchedule), same limit.
-//   LSVD_PREC_F32 / LSVD_PREC_BF16 : the same iteration with FP32 / BF16 MFMA products
-//                   (FP32 accumulation) until it stalls at that precision, then FP64 refinement
-//                   iterations with the FP64 stopping rule.
-int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision, double* q,
-                   lsvd_polar_info_t* info) {
-    if (!h) return LSVD_E_ARG;
-    if (m < n || n < 0) return set_error(h, LSVD_E_SHAPE, "lsvd_polar_f64: needs m >= n");
-    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16 &&
-        precision != LSVD_PREC_F64_SCALED)
-        return set_error(h, LSVD_E_ARG, "unknown precision");
-    lsvd_polar_info_t inf{};
-    if (n == 0) {
-        inf.polar_ok = 1;
-        if (info) *info = inf;
-        return LSVD_OK;
-    }
-    if (!a || !q) return set_error(h, LSVD_E_ARG, "null pointer");
-    hipError_t e = hipSetDevice(h->device);
-    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+// The iteration on device buffers: dA (m x n) in, dQ (m x n) out.
+int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, double* dQ, lsvd_polar_info_t* info) {
     hipStream_t st = h->stream;
     const size_t mn = (size_t)m * n, nn = (size_t)n * n;
-    const bool host = h->ptr_mode == LSVD_PTR_HOST;
     const bool lowp = precision == LSVD_PREC_F32 || precision == LSVD_PREC_BF16;
     const int lp = precision == LSVD_PREC_F32 ? kPolF32 : kPolBF16;
-    Buf bA, bX, bX2, bM, bXf, bXf2, bMf, bS;
+    lsvd_polar_info_t inf{};
+    Buf bX, bX2, bM, bXf, bXf2, bMf, bS;
 #define PCHK(expr)                                          \
     do {                                                    \
         hipError_t e_ = (expr);                             \
         if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
     } while (0)
-    const double* dA = a;
-    if (host) {
-        PCHK(bA.alloc(sizeof(double) * mn));
-        PCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * mn, hipMemcpyHostToDevice, st));
-        dA = (const double*)bA.p;
-    }
+
     PCHK(bX.alloc(sizeof(double) * mn));
     PCHK(bX2.alloc(sizeof(double) * mn));
     PCHK(bM.alloc(sizeof(double) * nn));
-    PCHK(bS.alloc(sizeof(double) * 4));
+    PCHK(bS.alloc(sizeof(double) * (4 + kSumsqBlocks)));
     if (lowp) {
         PCHK(bXf.alloc(sizeof(float) * mn));
         PCHK(bXf2.alloc(sizeof(float) * mn));
@@ -79,7 +53,7 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
     int* bad = (int*)(S + 3);
     double hs[4];
     PCHK(polar_zero(S, 4, st));
-    PCHK(polar_sumsq(dA, mn, S, st));
+    PCHK(polar_sumsq(dA, mn, S, S + 4, st));
     const double rootn = std::sqrt((double)n);
     double delta = 0.0;
     int it_low = 0, it64 = 0;
@@ -171,15 +145,60 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
     inf.lowp_iters = it_low;
     inf.f64_iters = it64;
     inf.last_delta = delta;
-    if (host)
-        PCHK(hipMemcpyAsync(q, X, sizeof(double) * mn, hipMemcpyDeviceToHost, st));
-    else
-        PCHK(hipMemcpyAsync(q, X, sizeof(double) * mn, hipMemcpyDeviceToDevice, st));
+    PCHK(hipMemcpyAsync(dQ, X, sizeof(double) * mn, hipMemcpyDeviceToDevice, st));
     PCHK(hipStreamSynchronize(st));
 #undef PCHK
     if (info) *info = inf;
     return LSVD_OK;
 }
+
+}  // namespace lsvd
+
+extern "C" {
+
+// NS polar factor of a tall or square m x n matrix (m >= n).
+//   LSVD_PREC_F64 : the reference iteration in FP64 (X0 = A/||A||_F, X <- X(3I - X^T X)/2,
+//                   stop at ||dX||_F < 1e-13 sqrt(n), polar_ok = ||Q^T Q - I||_F <= 1e-8 sqrt(n)).
+//   LSVD_PREC_F64_SCALED : FP64 with the scaled step schedule (NsSchedule), same limit.
+//   LSVD_PREC_F32 / LSVD_PREC_BF16 : the same iteration with FP32 / BF16 MFMA products
+//                   (FP32 accumulation) until it stalls at that precision, then FP64 refinement
+//                   iterations with the FP64 stopping rule.
+int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision, double* q,
+                   lsvd_polar_info_t* info) {
+    if (!h) return LSVD_E_ARG;
+    if (m < n || n < 0) return set_error(h, LSVD_E_SHAPE, "lsvd_polar_f64: needs m >= n");
+    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F32 && precision != LSVD_PREC_BF16 &&
+        precision != LSVD_PREC_F64_SCALED)
+        return set_error(h, LSVD_E_ARG, "unknown precision");
+    lsvd_polar_info_t inf{};
+    if (n == 0) {
+        inf.polar_ok = 1;
+        if (info) *info = inf;
+        return LSVD_OK;
+    }
+    if (!a || !q) return set_error(h, LSVD_E_ARG, "null pointer");
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    hipStream_t st = h->stream;
+    const size_t mn = (size_t)m * n;
+#define PCHK(expr)                                          \
+    do {                                                    \
+        hipError_t e_ = (expr);                             \
+        if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
+    } while (0)
+    if (h->ptr_mode != LSVD_PTR_HOST) return polar_run(h, a, m, n, precision, q, info);
+    Buf bA, bQ;
+    PCHK(bA.alloc(sizeof(double) * mn));
+    PCHK(bQ.alloc(sizeof(double) * mn));
+    PCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * mn, hipMemcpyHostToDevice, st));
+    int rc = polar_run(h, (const double*)bA.p, m, n, precision, (double*)bQ.p, info);
+    if (rc != LSVD_OK) return rc;
+    PCHK(hipMemcpyAsync(q, bQ.p, sizeof(double) * mn, hipMemcpyDeviceToHost, st));
+    PCHK(hipStreamSynchronize(st));
+#undef PCHK
+    return LSVD_OK;
+}
+
 
 // Plain FP64 product C = op(A) B (diagnostic; used by the GEMM parity tests).
 int lsvd_gemm_f64(lsvd_handle_t h, int trans_a, int m, int n, int k, const double* a, const double* b, double* c) {

@@ -4,7 +4,13 @@
 #include <hip/hip_runtime.h>
 #include <stddef.h>
 
+#include "../../include/lie_svd_amd.h"
+
 namespace lsvd {
+
+// newton_schulz_polar on device buffers (dA, dQ: m x n, m >= n >= 1) on the
+// handle's stream; synchronous.  Precision as lsvd_polar_f64.
+int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, double* dQ, lsvd_polar_info_t* info);
 
 enum PolarPrec { kPolF64 = 0, kPolF32 = 1, kPolBF16 = 2 };
 enum Epilogue { kEpiStore = 0, kEpi3IMinus = 1, kEpiHalfDelta = 2, kEpiOrth = 3 };
@@ -24,7 +30,9 @@ hipError_t gemm_f64(bool trans_a, int M, int N, int K, const double* A, const do
 // C = op(A) B with FP32 (kPolF32) or BF16-rounded (kPolBF16) operands, FP32 out.
 hipError_t gemm_lowp(int prec, bool trans_a, int M, int N, int K, const float* A, const float* B, float* C,
                      hipStream_t st);
-hipError_t polar_sumsq(const double* x, size_t n, double* out, hipStream_t st);
+// *out = ||x||^2 (overwritten), deterministic; partials: kSumsqBlocks doubles.
+constexpr int kSumsqBlocks = 256;
+hipError_t polar_sumsq(const double* x, size_t n, double* out, double* partials, hipStream_t st);
 hipError_t polar_scale(const double* x, size_t n, const double* sumsq, void* y, bool f32, hipStream_t st);
 hipError_t polar_widen(const float* x, size_t n, double* y, hipStream_t st);
 hipError_t polar_zero(double* p, int n, hipStream_t st);

@@ -256,13 +256,23 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
 }
 
 // ---- helpers --------------------------------------------------------------------
-__global__ void sumsq_kernel(const double* x, size_t n, double* out) {
+// ||x||^2 in two fixed-shape passes (per-block partials, then one block
+// sums them in a fixed order), so the norm that scales X0 is the same bits
+// on every run.
+__global__ void sumsq_kernel(const double* x, size_t n, double* partials) {
     __shared__ double red[4];
     double s = 0.0;
     for (size_t e = blockIdx.x * (size_t)blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
         s += x[e] * x[e];
     s = block_sum<256>(s, red);
-    if (threadIdx.x == 0) atomicAdd(out, s);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
+__global__ void sumsq_final_kernel(const double* partials, double* out) {
+    __shared__ double red[4];
+    double s = partials[threadIdx.x];
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) *out = s;
 }
 
 // y = x / max(sqrt(*sumsq), 1e-300), into f64 or f32 storage
@@ -352,8 +362,11 @@ hipError_t gemm_lowp(int prec, bool trans_a, int M, int N, int K, const float* A
     return launch_bf16<false, false, kEpiStore>(M, N, K, A, B, e, st);
 }
 
-hipError_t polar_sumsq(const double* x, size_t n, double* out, hipStream_t st) {
-    hipLaunchKernelGGL(sumsq_kernel, dim3(1024), dim3(256), 0, st, x, n, out);
+hipError_t polar_sumsq(const double* x, size_t n, double* out, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(sumsq_kernel, dim3(kSumsqBlocks), dim3(256), 0, st, x, n, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(kSumsqBlocks), 0, st, partials, out);
     return hipGetLastError();
 }
 

@@ -1,0 +1,169 @@
+"""LieSvdSmall::solve of one large square matrix on the chip-wide route
+(src/lie_svd_small.rs:296-323 with every step spread over the GPU: the
+polar GEMMs, P = sym(Q^T A), cyclic Jacobi over the round-robin pair layers
+with one matrix per family, the stable descending sort and U = Q V) against
+the CPU oracle, against the workgroup-per-matrix kernels, and at the sizes
+the oracle is slow on through LAPACK's singular values and the residuals.
+
+Tolerances as for the other square solves: sigma two-tier (relative 1e-12
+above 1e-3 sigma_max, absolute 64 n eps sigma_max), singular vectors per
+cluster by principal angle, residuals at the FP64 level scaled by n / 32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from lie_cliffalg_analog_svd_amd import Handle, LieSvdSmall, _lib, rng
+from tests._util import assert_factors, assert_sigma_close, imposed_spectrum, uniform_matrix
+from tests._util import vector_parity
+
+pytestmark = pytest.mark.gpu
+
+LARGE_PATH = 3
+
+
+def _say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _handle(min_n: int) -> Handle:
+    """A handle whose chip-wide route starts at min_n (LSVD_LARGE_MIN_N is
+    read when the handle is created)."""
+    old = os.environ.get("LSVD_LARGE_MIN_N")
+    os.environ["LSVD_LARGE_MIN_N"] = str(min_n)
+    try:
+        return Handle()
+    finally:
+        if old is None:
+            del os.environ["LSVD_LARGE_MIN_N"]
+        else:
+            os.environ["LSVD_LARGE_MIN_N"] = old
+
+
+def _vs_oracle(a, what, h=None):
+    n = a.shape[0]
+    _say("case", what, a.shape)
+    u, s, vt = LieSvdSmall.solve(a, h)
+    info = dict(LieSvdSmall.last_info)
+    uo, so, vto, _ = oracle.small_solve(a)
+    assert_factors(a, u, s, vt, 1e-11 * max(1.0, n / 32), what)
+    assert_sigma_close(s, so, n, what=what)
+    vector_parity(u, uo, so, n, what + " U")
+    vector_parity(vt.T, vto.T, so, n, what + " V")
+    return info
+
+
+@pytest.mark.parametrize("n", [192, 255, 256, 320, 384])
+def test_large_route_uniform_vs_oracle(n):
+    info = _vs_oracle(uniform_matrix(700 + n, n, n), f"large uniform n={n}")
+    assert info["path"] == LARGE_PATH and info["polar_ok"] == 1, info
+    assert 3 <= info["jacobi_sweeps"] <= 20 and info["rotations"] > 0
+
+
+def test_large_route_matches_workgroup_kernels():
+    """The same matrix through both routes of the single-matrix API."""
+    n = 224
+    a = rng.normal_matrix(0x5EED0300, n, n)
+    h_large, h_wg = _handle(2), _handle(100000)
+    u1, s1, vt1 = LieSvdSmall.solve(a, h_large)
+    assert LieSvdSmall.last_info["path"] == LARGE_PATH
+    u2, s2, vt2 = LieSvdSmall.solve(a, h_wg)
+    assert LieSvdSmall.last_info["path"] == 0
+    assert_sigma_close(s1, s2, n, what="large vs workgroup")
+    vector_parity(u1, u2, s2, n, "large vs workgroup U")
+    vector_parity(vt1.T, vt2.T, s2, n, "large vs workgroup V")
+    # small n on the chip-wide route (forced) against the oracle
+    for m in (2, 3, 17, 64):
+        info = _vs_oracle(uniform_matrix(800 + m, m, m), f"forced large n={m}", h_large)
+        assert info["path"] == LARGE_PATH
+    h_large.close()
+    h_wg.close()
+
+
+@pytest.mark.parametrize("kind", ["clusters", "kappa1e10", "graded"])
+def test_large_route_imposed_spectra(kind):
+    """Imposed spectra at n = 256: repeated singular values (per-cluster
+    subspaces), condition 1e10, and a graded spectrum; sigma against both
+    the oracle and the imposed values."""
+    n = 256
+    if kind == "clusters":
+        sig = np.repeat([8.0, 4.0, 2.0, 1.0], n // 4)
+    elif kind == "kappa1e10":
+        sig = np.logspace(0, -10, n)
+    else:
+        sig = 1.0 + np.arange(n)[::-1] * 0.01
+    a = imposed_spectrum(0x5EED0400 + len(kind), n, n, sig)
+    info = _vs_oracle(a, f"large {kind}")
+    assert info["path"] == LARGE_PATH
+    s = LieSvdSmall.solve(a)[1]
+    assert_sigma_close(s, np.sort(sig)[::-1], n, what=kind + " vs imposed")
+
+
+def test_large_route_rank_deficient_takes_repair_branch():
+    """A rank-150 matrix of order 200: Newton-Schulz cannot make Q
+    orthogonal on the null space, so the polar factor is not trusted and the
+    solve takes the reference's repair branch (U from A V / sigma, then
+    Gram-Schmidt) on the workgroup-per-matrix kernels."""
+    n, r = 200, 150
+    sig = np.concatenate([np.linspace(2.0, 1.0, r), np.zeros(n - r)])
+    a = imposed_spectrum(0x5EED0500, n, n, sig)
+    _say("case rank deficient", n, r)
+    u, s, vt = LieSvdSmall.solve(a)
+    info = dict(LieSvdSmall.last_info)
+    uo, so, vto, oinfo = oracle.small_solve(a)
+    assert info["path"] == 0 and info["polar_ok"] == oinfo["polar_ok"] == 0, (info, oinfo)
+    assert_factors(a, u, s, vt, 1e-10, "rank deficient")
+    assert_sigma_close(s, so, n, what="rank deficient")
+
+
+@pytest.mark.parametrize("n", [512, 1024])
+def test_large_route_full_sizes(n):
+    """The reference times N = 512 at ~7 s and N = 1024 at ~77 s
+    (src/lie_svd_small.rs:51-54); here against LAPACK's singular values
+    and the residual checks (the oracle takes minutes at 1024)."""
+    a = rng.normal_matrix(0x5EED0600 + n, n, n)
+    _say("case large full", n)
+    u, s, vt = LieSvdSmall.solve(a)
+    info = dict(LieSvdSmall.last_info)
+    assert info["path"] == LARGE_PATH, info
+    assert_factors(a, u, s, vt, 1e-11 * n / 32, f"n={n}")
+    ref = np.linalg.svd(a, compute_uv=False)
+    # LAPACK's own error on the smallest values is ~ kappa eps, so the
+    # relative tier starts at 1e-2 sigma_max here
+    assert_sigma_close(s, ref, n, rel_tol=1e-12, rel_floor=1e-2, what=f"n={n} vs LAPACK")
+
+
+def test_large_route_device_pointers_match_host_mode():
+    """Device pointer mode (torch buffers) gives the host mode's factors.
+    Compared to rounding: the iteration's stop-test norms are workgroup
+    partials added with atomics, so near a threshold two runs can stop one
+    step apart."""
+    import torch
+    n = 256
+    a = uniform_matrix(0x5EED0700, n, n)
+    u1, s1, vt1 = LieSvdSmall.solve(a)
+    h = Handle()
+    h.set_pointer_mode(_lib.LSVD_PTR_DEVICE)
+    dev = torch.device("cuda:0")
+    da = torch.from_numpy(a).to(dev)
+    du = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    ds = torch.zeros(n, dtype=torch.float64, device=dev)
+    dvt = torch.zeros((n, n), dtype=torch.float64, device=dev)
+    dinfo = torch.zeros(8, dtype=torch.int32, device=dev)
+    p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    torch.cuda.synchronize()
+    h.check(_lib.lib().lsvd_small_solve_f64(h.raw, p(da), n, p(du), p(ds), p(dvt), p(dinfo)))
+    torch.cuda.synchronize()
+    assert int(dinfo[6].item()) == LARGE_PATH
+    s2 = ds.cpu().numpy()
+    assert_sigma_close(s2, s1, n, what="device vs host mode")
+    vector_parity(du.cpu().numpy(), u1, s1, n, "device vs host mode U")
+    vector_parity(dvt.cpu().numpy().T, vt1.T, s1, n, "device vs host mode V")
+    h.close()

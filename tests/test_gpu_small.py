@@ -53,8 +53,30 @@ def test_square_uniform_vs_oracle(n):
 @pytest.mark.parametrize("n", [129, 150, 200, 256])
 def test_square_above_on_chip_sizes_vs_oracle(n):
     """N > 128: the workgroup-per-matrix kernels with global slots
-    (LieSvd routes these to LieSvdSmall up to SMALL_MAX_N = 512)."""
-    _check_vs_oracle(uniform_matrix(300 + n, n, n), f"uniform n={n}")
+    (LieSvd routes these to LieSvdSmall up to SMALL_MAX_N = 512).  The
+    handle keeps single solves off the chip-wide route (LSVD_LARGE_MIN_N
+    above every n), which has its own tests."""
+    import os
+    from lie_cliffalg_analog_svd_amd import Handle
+    old = os.environ.get("LSVD_LARGE_MIN_N")
+    os.environ["LSVD_LARGE_MIN_N"] = "100000"
+    try:
+        h = Handle()
+    finally:
+        if old is None:
+            del os.environ["LSVD_LARGE_MIN_N"]
+        else:
+            os.environ["LSVD_LARGE_MIN_N"] = old
+    a = uniform_matrix(300 + n, n, n)
+    _say("case wg", n)
+    u, s, vt = LieSvdSmall.solve(a, h)
+    assert LieSvdSmall.last_info["path"] == 0
+    uo, so, vto, _ = oracle.small_solve(a)
+    assert_factors(a, u, s, vt, 1e-11 * max(1.0, n / 32), f"wg n={n}")
+    assert_sigma_close(s, so, n, what=f"wg n={n}")
+    _vector_parity(u, uo, so, n, f"wg n={n} U")
+    _vector_parity(vt.T, vto.T, so, n, f"wg n={n} V")
+    h.close()
 
 
 def test_single_64x64_normal():

@@ -2,9 +2,10 @@
 // large square matrix with every step spread over the chip:
 //   Q = newton_schulz_polar(A)            polar_run (MFMA GEMMs, all CUs)
 //   P = sym(Q^T A)                        gemm_f64 + large_symmetrize
-//   (V, eig) = eigh_jacobi_full(P)        Phase-JADE layer kernels, one
-//                                         matrix per family (cyclic Jacobi
-//                                         over the round-robin pair layers)
+//   (V, eig) = eigh_jacobi_full(P)        block Jacobi (block_jacobi.hip):
+//                                         Jacobi sweeps on block-pair
+//                                         subproblems in LDS, their rotations
+//                                         applied as MFMA tile products
 //   sort, sigma = max(eig, 0), U = Q V    large_rank / large_permute + gemm_f64
 // The workgroup-per-matrix kernels run each step on one CU; for a single
 // matrix above a couple of hundred rows this route is the faster one.
@@ -14,7 +15,6 @@
 #include <cstring>
 
 #include "lsvd_api_internal.hpp"
-#include "lsvd_joint.hpp"
 #include "lsvd_large.hpp"
 #include "lsvd_polar.hpp"
 
@@ -39,13 +39,12 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
     hipStream_t st = h->stream;
     const bool host = h->ptr_mode == LSVD_PTR_HOST;
     const size_t nn = (size_t)n * n;
-    const int slots = (n + (n & 1)) / 2;
 #define LCHK(expr)                                          \
     do {                                                    \
         hipError_t e_ = (expr);                             \
         if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
     } while (0)
-    Mem bA, bQ, bP, bW, bV, bVs, bOff, bPart, bRot, bFam, bDiag, bPerm, bScal, bU, bS, bVt;
+    Mem bA, bQ, bP, bApad, bVpad, bVs, bWrot, bPart, bRot, bVc, bDiag, bPerm, bScal, bU, bS, bVt;
     const double* dA = a;
     double *dU = u, *dS = s, *dVt = vt;
     if (host) {
@@ -71,36 +70,47 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
     LCHK(gemm_f64(true, n, n, n, dQ, dA, dP, 1.0, st));
     LCHK(large_symmetrize(dP, n, st));
 
-    // eigh_jacobi_full (:212-285): stop once the upper-triangle off-diagonal
-    // norm is below JACOBI_TOL ||P||_F; the layer kernels measure the full
-    // off-diagonal norm (sqrt 2 times the upper one) and scale by ||P||_F.
-    LCHK(bW.alloc(sizeof(double) * nn));
-    LCHK(bV.alloc(sizeof(double) * nn));
-    LCHK(bOff.alloc(sizeof(double) * n));
-    LCHK(bPart.alloc(sizeof(double) * 2));
-    LCHK(bRot.alloc(sizeof(JointRot) * slots));
-    LCHK(bFam.alloc(sizeof(JointFamily)));
-    JointArgs ja{};
-    ja.F = 1;
-    ja.K = 1;
-    ja.n = n;
-    ja.slots = slots;
-    ja.mats = dP;
-    ja.work = (double*)bW.p;
-    ja.basis = (double*)bV.p;
-    ja.offrow = (double*)bOff.p;
-    ja.part = (double*)bPart.p;
-    ja.rot = (JointRot*)bRot.p;
-    ja.fam = (JointFamily*)bFam.p;
-    ja.tol = std::sqrt(2.0) * 1e-13;
-    ja.min_pair_scale = 0.0;
-    ja.line_search_steps = 1;
-    ja.max_sweeps = 60 + 20 * n;
-    ja.rel_tol = 1;
-    LCHK(joint_launch_init(ja, st));
-    JointFamily fam{};
-    rc = joint_run_sweeps(h, ja, ja.max_sweeps, &fam);
-    if (rc != LSVD_OK) return rc;
+    // eigh_jacobi_full (:212-285) as a block Jacobi method over the whole
+    // chip (block_jacobi.hip), with the reference's stop test before every
+    // sweep: upper off-diagonal norm below JACOBI_TOL ||P||_F.  A sweep that
+    // no longer lowers the off-diagonal norm (the rounding floor) also ends
+    // the iteration.
+    const int bb = bj_tile_size(n);
+    const int nbp = (n + bb - 1) / bb;
+    const int ld = nbp * bb;
+    const size_t ll = (size_t)ld * ld;
+    LCHK(bApad.alloc(sizeof(double) * ll));
+    LCHK(bVpad.alloc(sizeof(double) * ll));
+    LCHK(bWrot.alloc(sizeof(double) * (size_t)nbp * bb * bb));
+    LCHK(bPart.alloc(sizeof(double) * (kBjOffBlocks + 2)));
+    LCHK(bRot.alloc(sizeof(int)));
+    BlockJacobi bj{n, ld, 2 * nbp, bb, (double*)bApad.p, (double*)bVpad.p, (double*)bWrot.p, (double*)bPart.p,
+                   (int*)bRot.p};
+    double* dsq = (double*)bPart.p + kBjOffBlocks;  // [0] ||P||_F^2, [1] off-diagonal^2
+    LCHK(hipMemsetAsync(bRot.p, 0, sizeof(int), st));
+    LCHK(bj_setup(bj, dP, st));
+    LCHK(polar_sumsq(dP, nn, dsq, bj.partials, st));
+    LCHK(bj_offnorm(bj, dsq + 1, st));
+    double hq[2];
+    LCHK(hipMemcpyAsync(hq, dsq, sizeof(hq), hipMemcpyDeviceToHost, st));
+    LCHK(hipStreamSynchronize(st));
+    const double jtol = 1e-13 * std::max(std::sqrt(hq[0]), 1e-300);
+    double off = std::sqrt(hq[1]);
+    const int max_sweeps = 60 + 20 * n;
+    int sweeps = 0;
+    while (sweeps < max_sweeps && off >= jtol) {
+        for (int layer = 0; layer < bj.nb - 1; ++layer) LCHK(bj_layer(bj, layer, st));
+        LCHK(bj_offnorm(bj, dsq + 1, st));
+        LCHK(hipMemcpyAsync(hq, dsq, sizeof(hq), hipMemcpyDeviceToHost, st));
+        LCHK(hipStreamSynchronize(st));
+        ++sweeps;
+        const double after = std::sqrt(hq[1]);
+        const bool stalled = !(after < off * (1.0 - 1e-10));
+        off = after;
+        if (stalled) break;
+    }
+    int rotations = 0;
+    LCHK(hipMemcpyAsync(&rotations, bRot.p, sizeof(int), hipMemcpyDeviceToHost, st));
 
     LCHK(bDiag.alloc(sizeof(double) * n));
     LCHK(bPerm.alloc(sizeof(int) * n));
@@ -108,10 +118,12 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
     LCHK(bVs.alloc(sizeof(double) * nn));
     double* dDiag = (double*)bDiag.p;
     double* scal = (double*)bScal.p;  // [0] ||U^T U - I||_F^2, [1] non-finite flag (as int)
-    LCHK(joint_launch_diag(ja, dDiag, st));
+    LCHK(bVc.alloc(sizeof(double) * nn));
+    double* dVc = (double*)bVc.p;
+    LCHK(bj_extract(bj, dDiag, dVc, st));
     LCHK(polar_zero(scal, 2, st));
     LCHK(large_rank(dDiag, n, (int*)bPerm.p, (int*)(scal + 1), st));
-    LCHK(large_permute((const double*)bV.p, dDiag, (const int*)bPerm.p, n, (double*)bVs.p, dVt, dS, st));
+    LCHK(large_permute(dVc, dDiag, (const int*)bPerm.p, n, (double*)bVs.p, dVt, dS, st));
     LCHK(gemm_f64(false, n, n, n, dQ, (const double*)bVs.p, dU, 1.0, st));
     LCHK(ns_orth(n, n, dU, scal, st));
     double hs[2];
@@ -132,8 +144,8 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
         lsvd_info_t inf{};
         inf.polar_ok = 1;
         inf.ns_iters = pinf.f64_iters;
-        inf.jacobi_sweeps = fam.sweeps;
-        inf.rotations = fam.rotations;
+        inf.jacobi_sweeps = sweeps;
+        inf.rotations = rotations;
         inf.path = 3;
         if (host) {
             *info = inf;

@@ -11,6 +11,25 @@ namespace lsvd {
 constexpr int kLargeMaxN = 1024;          // the single-matrix API's limit
 constexpr int kLargeMinNDefault = 129;    // smallest n routed chip-wide by default (above the on-chip sizes)
 
+// Block Jacobi eigen-decomposition of one symmetric n x n matrix
+// (block_jacobi.hip).  A and V are ld x ld (ld = n padded to whole tiles);
+// W holds (nb / 2) bb x bb block rotations.
+struct BlockJacobi {
+    int n, ld, nb, bb;
+    double* A;
+    double* V;
+    double* W;
+    double* partials;  // kBjOffBlocks doubles
+    int* rotations;
+};
+constexpr int kBjOffBlocks = 256;
+int bj_tile_size(int n);
+hipError_t bj_setup(const BlockJacobi& j, const double* P, hipStream_t st);
+hipError_t bj_layer(const BlockJacobi& j, int layer, hipStream_t st);
+// *out = sum of squares of the upper off-diagonal entries (overwritten).
+hipError_t bj_offnorm(const BlockJacobi& j, double* out, hipStream_t st);
+hipError_t bj_extract(const BlockJacobi& j, double* d, double* Vc, hipStream_t st);
+
 hipError_t large_symmetrize(double* P, int n, hipStream_t st);
 hipError_t large_rank(const double* d, int n, int* perm, int* bad, hipStream_t st);
 hipError_t large_permute(const double* V, const double* d, const int* perm, int n, double* Vs, double* Vt,

@@ -97,40 +97,42 @@ __global__ __launch_bounds__(16 * BB) void bj_inner_kernel(const double* A, int 
             rf[tid] = f;
         }
         __syncthreads();
-        // rows p, q of every pair: row_p <- c row_p - s row_q, row_q <- s row_p + c row_q
-        for (int e = tid; e < NP * BB; e += NT) {
-            const int k = e / BB, col = e % BB;
-            if (!rf[k]) continue;
-            const double c = rc[k], s = rs[k];
-            double* xp = S + rp[k] * L + col;
-            double* xq = S + rq[k] * L + col;
-            const double vp = *xp, vq = *xq;
-            *xp = c * vp - s * vq;
-            *xq = s * vp + c * vq;
+        // S <- J^T S J on every 2 x 2 block (rows of pair k1, columns of pair
+        // k2) in one pass: the row rotation of k1, then the column rotation
+        // of k2 (:242-257); the rotated pair's own block takes the
+        // reference's exact values (a_pq = a_qp = 0).  W <- W J on columns.
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int k1 = e / NP, k2 = e % NP;
+            const int f1 = rf[k1], f2 = rf[k2];
+            const int p1 = rp[k1], q1 = rq[k1];
+            if (k1 == k2) {
+                if (f1) {
+                    S[p1 * L + p1] = rdp[k1];
+                    S[p1 * L + q1] = 0.0;
+                    S[q1 * L + p1] = 0.0;
+                    S[q1 * L + q1] = rdq[k1];
+                }
+                continue;
+            }
+            if (!f1 && !f2) continue;
+            const int p2 = rp[k2], q2 = rq[k2];
+            const double c1 = rc[k1], s1 = rs[k1], c2 = rc[k2], s2 = rs[k2];
+            double* xp = S + p1 * L;
+            double* xq = S + q1 * L;
+            const double xpp = xp[p2], xpq = xp[q2], xqp = xq[p2], xqq = xq[q2];
+            const double rpp = c1 * xpp - s1 * xqp, rpq = c1 * xpq - s1 * xqq;
+            const double rqp = s1 * xpp + c1 * xqp, rqq = s1 * xpq + c1 * xqq;
+            xp[p2] = c2 * rpp - s2 * rpq;
+            xp[q2] = s2 * rpp + c2 * rpq;
+            xq[p2] = c2 * rqp - s2 * rqq;
+            xq[q2] = s2 * rqp + c2 * rqq;
         }
-        __syncthreads();
-        // columns p, q of S and of W (:242-257); the rotated 2 x 2 block
-        // itself takes the reference's exact values (a_pq = a_qp = 0)
         for (int e = tid; e < NP * BB; e += NT) {
             const int k = e / BB, row = e % BB;
             if (!rf[k]) continue;
             const double c = rc[k], s = rs[k];
-            const int p = rp[k], q = rq[k];
-            double* sp = S + row * L + p;
-            double* sq = S + row * L + q;
-            if (row == p) {
-                *sp = rdp[k];
-                *sq = 0.0;
-            } else if (row == q) {
-                *sp = 0.0;
-                *sq = rdq[k];
-            } else {
-                const double vp = *sp, vq = *sq;
-                *sp = c * vp - s * vq;
-                *sq = s * vp + c * vq;
-            }
-            double* wp = W + row * L + p;
-            double* wq = W + row * L + q;
+            double* wp = W + row * L + rp[k];
+            double* wq = W + row * L + rq[k];
             const double up = *wp, uq = *wq;
             *wp = c * up - s * uq;
             *wq = s * up + c * uq;
@@ -169,21 +171,23 @@ __device__ __forceinline__ void bj_tile_gemm(const double* X, const double* Y, d
     }
 }
 
-// Grid: B (B + 1) / 2 A tiles (P <= Q), then (n_pad / BB) * B V tiles.
+// vmode 0: grid of B (B + 1) / 2 A tiles (P <= Q); vmode 1: grid of
+// (n_pad / BB) * B V tiles.
 template <int BB>
 __global__ __launch_bounds__(256) void bj_update_kernel(double* A, double* V, int ld, int nb, int layer,
-                                                        const double* Wbuf) {
+                                                        const double* Wbuf, int vmode) {
     constexpr int b = BB / 2, L = BB + 1;
     extern __shared__ double lds[];
     double* X = lds;            // tile of A or V
     double* WP = X + BB * L;
-    double* WQ = WP + BB * L;   // A tiles only
-    double* T = WQ + BB * L;
+    double* T = WP + BB * L;
+    double* WQ = T + BB * L;    // A tiles only (the V launch allocates three buffers)
     const int B = nb / 2;
     const int nA = B * (B + 1) / 2;
     const int tid = threadIdx.x;
     int t = blockIdx.x;
-    if (t < nA) {
+    if (!vmode) {
+        if (t >= nA) return;
         // upper-triangle tile t -> (P, Q), P <= Q
         int P = 0;
         while (t >= B - P) {
@@ -218,8 +222,8 @@ __global__ __launch_bounds__(256) void bj_update_kernel(double* A, double* V, in
         }
         return;
     }
-    t -= nA;
     const int P = t % B, R = t / B;  // V rows R*BB .. R*BB + BB - 1, pair P's columns
+    if (R >= ld / BB) return;
     for (int e = tid; e < BB * BB; e += 256) {
         const int r = e / BB, c = e % BB;
         const int gc = bj_index(nb, b, layer, P, c);
@@ -280,7 +284,7 @@ __global__ void bj_extract_kernel(const double* A, const double* V, int n, int l
 template <int BB>
 constexpr size_t bj_inner_lds() { return sizeof(double) * 2 * BB * (BB + 1); }
 template <int BB>
-constexpr size_t bj_update_lds() { return sizeof(double) * 4 * BB * (BB + 1); }
+constexpr size_t bj_update_lds(bool vmode) { return sizeof(double) * (vmode ? 3 : 4) * BB * (BB + 1); }
 
 template <int BB>
 hipError_t bj_prepare_bb() {
@@ -288,20 +292,32 @@ hipError_t bj_prepare_bb() {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bj_inner_lds<BB>());
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&bj_update_kernel<BB>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bj_update_lds<BB>());
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bj_update_lds<BB>(false));
 }
 
 template <int BB>
-hipError_t bj_layer_bb(const BlockJacobi& j, int layer, hipStream_t st) {
+hipError_t bj_launch_bb(const BlockJacobi& j, int what, int layer, const double* W, hipStream_t st) {
     const int B = j.nb / 2;
-    hipLaunchKernelGGL(bj_inner_kernel<BB>, dim3(B), dim3(bj_inner_threads<BB>()), bj_inner_lds<BB>(), st, j.A, j.ld,
-                       j.nb, layer, j.W, j.rotations);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    const int tiles = B * (B + 1) / 2 + (j.ld / BB) * B;
-    hipLaunchKernelGGL(bj_update_kernel<BB>, dim3(tiles), dim3(256), bj_update_lds<BB>(), st, j.A, j.V, j.ld, j.nb,
-                       layer, j.W);
+    if (what == 0) {
+        hipLaunchKernelGGL(bj_inner_kernel<BB>, dim3(B), dim3(bj_inner_threads<BB>()), bj_inner_lds<BB>(), st, j.A,
+                           j.ld, j.nb, layer, const_cast<double*>(W), j.rotations);
+    } else if (what == 1) {
+        hipLaunchKernelGGL(bj_update_kernel<BB>, dim3(B * (B + 1) / 2), dim3(256), bj_update_lds<BB>(false), st, j.A,
+                           j.V, j.ld, j.nb, layer, W, 0);
+    } else {
+        hipLaunchKernelGGL(bj_update_kernel<BB>, dim3((j.ld / BB) * B), dim3(256), bj_update_lds<BB>(true), st, j.A,
+                           j.V, j.ld, j.nb, layer, W, 1);
+    }
     return hipGetLastError();
+}
+
+hipError_t bj_launch(const BlockJacobi& j, int what, int layer, const double* W, hipStream_t st) {
+    switch (j.bb) {
+        case 16: return bj_launch_bb<16>(j, what, layer, W, st);
+        case 32: return bj_launch_bb<32>(j, what, layer, W, st);
+        case 64: return bj_launch_bb<64>(j, what, layer, W, st);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace
@@ -315,13 +331,12 @@ hipError_t bj_setup(const BlockJacobi& j, const double* P, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t bj_layer(const BlockJacobi& j, int layer, hipStream_t st) {
-    switch (j.bb) {
-        case 16: return bj_layer_bb<16>(j, layer, st);
-        case 32: return bj_layer_bb<32>(j, layer, st);
-        case 64: return bj_layer_bb<64>(j, layer, st);
-        default: return hipErrorInvalidValue;
-    }
+hipError_t bj_inner(const BlockJacobi& j, int layer, double* W, hipStream_t st) { return bj_launch(j, 0, layer, W, st); }
+hipError_t bj_update_a(const BlockJacobi& j, int layer, const double* W, hipStream_t st) {
+    return bj_launch(j, 1, layer, W, st);
+}
+hipError_t bj_update_v(const BlockJacobi& j, int layer, const double* W, hipStream_t st) {
+    return bj_launch(j, 2, layer, W, st);
 }
 
 hipError_t bj_offnorm(const BlockJacobi& j, double* out, hipStream_t st) {

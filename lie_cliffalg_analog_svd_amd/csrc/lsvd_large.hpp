@@ -13,19 +13,23 @@ constexpr int kLargeMinNDefault = 129;    // smallest n routed chip-wide by defa
 
 // Block Jacobi eigen-decomposition of one symmetric n x n matrix
 // (block_jacobi.hip).  A and V are ld x ld (ld = n padded to whole tiles);
-// W holds (nb / 2) bb x bb block rotations.
+// a layer's (nb / 2) bb x bb block rotations go to a W buffer of
+// (nb / 2) bb^2 doubles.
 struct BlockJacobi {
     int n, ld, nb, bb;
     double* A;
     double* V;
-    double* W;
     double* partials;  // kBjOffBlocks doubles
     int* rotations;
 };
 constexpr int kBjOffBlocks = 256;
 int bj_tile_size(int n);
 hipError_t bj_setup(const BlockJacobi& j, const double* P, hipStream_t st);
-hipError_t bj_layer(const BlockJacobi& j, int layer, hipStream_t st);
+// One layer: inner (A -> W), then A <- G^T A G (reads W) and V <- V G
+// (reads W; independent of A, so it may run on another stream).
+hipError_t bj_inner(const BlockJacobi& j, int layer, double* W, hipStream_t st);
+hipError_t bj_update_a(const BlockJacobi& j, int layer, const double* W, hipStream_t st);
+hipError_t bj_update_v(const BlockJacobi& j, int layer, const double* W, hipStream_t st);
 // *out = sum of squares of the upper off-diagonal entries (overwritten).
 hipError_t bj_offnorm(const BlockJacobi& j, double* out, hipStream_t st);
 hipError_t bj_extract(const BlockJacobi& j, double* d, double* Vc, hipStream_t st);
@@ -40,6 +44,8 @@ hipError_t large_permute(const double* V, const double* d, const int* perm, int 
 // (and returns LSVD_OK) when the polar factor is not trusted or U = Q V
 // fails the orthogonality test: those cases take the reference's repair
 // branch, which the caller runs through the workgroup-per-matrix kernels.
+hipError_t large_aux_stream(lsvd_handle_t h, hipStream_t* out);
+
 int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
                      bool* done);
 

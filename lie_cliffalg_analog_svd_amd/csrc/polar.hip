@@ -136,24 +136,41 @@ __global__ __launch_bounds__(256) void gemm_mfma4_kernel(int M, int N, int K, co
     for (int a = 0; a < 2; ++a)
 #pragma unroll
         for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
-    for (int k0 = 0; k0 < K; k0 += BK) {
+    // The next K tile is read from global memory into registers while the
+    // current one is multiplied out of LDS.
+    T ra[4], rb[4];
+    auto gload = [&](int k0) {
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             const int e_ = t + 256 * r;
             if (TRANS_A) {  // op(A)[i][k] = A[k][i], rows of A contiguous in i
                 const int kk = e_ / BM, ii = e_ % BM;
                 const int gk = k0 + kk, gi = i0 + ii;
-                As[kk * LDA + ii] = (gk < K && gi < M) ? A[(size_t)gk * M + gi] : (T)0;
+                ra[r] = (gk < K && gi < M) ? A[(size_t)gk * M + gi] : (T)0;
             } else {
                 const int ii = e_ / BK, kk = e_ % BK;
                 const int gk = k0 + kk, gi = i0 + ii;
-                As[kk * LDA + ii] = (gk < K && gi < M) ? A[(size_t)gi * K + gk] : (T)0;
+                ra[r] = (gk < K && gi < M) ? A[(size_t)gi * K + gk] : (T)0;
             }
             const int kk = e_ / BN, jj = e_ % BN;
             const int gk = k0 + kk, gj = j0 + jj;
-            Bs[kk * LDB + jj] = (gk < K && gj < N) ? B[(size_t)gk * N + gj] : (T)0;
+            rb[r] = (gk < K && gj < N) ? B[(size_t)gk * N + gj] : (T)0;
+        }
+    };
+    gload(0);
+    for (int k0 = 0; k0 < K; k0 += BK) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int e_ = t + 256 * r;
+            if (TRANS_A) {
+                As[(e_ / BM) * LDA + e_ % BM] = ra[r];
+            } else {
+                As[(e_ % BK) * LDA + e_ / BK] = ra[r];
+            }
+            Bs[(e_ / BN) * LDB + e_ % BN] = rb[r];
         }
         __syncthreads();
+        if (k0 + BK < K) gload(k0 + BK);
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4), c16 = lane & 15;

@@ -650,8 +650,11 @@ def polar_leg(h, dev, args):
             q = qd.cpu().numpy()
             if name == "f64":
                 q64 = q
+            executed = iters * 4.0 * m * n * n  # the steps this variant ran (low-precision ones included)
             entry[name] = {
                 "seconds": round(sec, 4), "tflops": round(flops / sec / 1e12, 3),
+                "tflops_basis": "plain FP64 iteration count x 4 m n^2 (equal work for every variant)",
+                "tflops_executed": round(executed / sec / 1e12, 3),
                 "lowp_iters": info["lowp_iters"], "f64_iters": info["f64_iters"], "polar_ok": info["polar_ok"],
                 "orth_err_fro": info["orth_err"],
                 "err_vs_f64_fro_over_sqrt_n": float(np.linalg.norm(q - q64) / np.sqrt(n)),

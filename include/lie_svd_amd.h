@@ -144,7 +144,9 @@ int lsvd_small_solve_rectangular_f64(lsvd_handle_t h, const double* a, int m, in
                                      double* vt, lsvd_info_t* info);
 
 /* Replaces eigh_jacobi_full (src/lie_svd_small.rs:212-285): symmetric p
- * (n x n) -> v (n x n, columns are eigenvectors), eig (n, descending). */
+ * (n x n, n <= 1024) -> v (n x n, columns are eigenvectors), eig (n,
+ * descending).  Above 128 (LSVD_LARGE_MIN_N) the Jacobi sweeps run as the
+ * chip-wide block Jacobi method (info.path = 3). */
 int lsvd_eigh_jacobi_f64(lsvd_handle_t h, const double* p, int n, double* v, double* eig, lsvd_info_t* info);
 
 /* ---- LieSvdMicro ----------------------------------------------------- */

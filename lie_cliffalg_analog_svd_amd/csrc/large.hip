@@ -2,9 +2,8 @@
 // square matrix run across the whole chip (src/lie_svd_small.rs:296-323):
 // symmetrising P = Q^T A (:304-308), the descending stable order of the
 // Jacobi diagonal with the column permutation of V (eigh_jacobi_full's sort,
-// :268-284), V^T, and sigma = max(eig, 0) (:312-315).  The products and the
-// Jacobi sweeps themselves are the polar GEMMs and the Phase-JADE layer
-// kernels with one matrix per family.
+// :268-284), V^T, and sigma = max(eig, 0) (:312-315).  The products are the
+// polar GEMMs; the Jacobi sweeps are the block Jacobi of block_jacobi.hip.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -62,11 +61,12 @@ __global__ __launch_bounds__(1024) void large_rank_kernel(const double* d, int n
     if (threadIdx.x == 0) *bad = flag;
 }
 
-// Vs[:, k] = V[:, perm[k]], Vt = Vs^T (through an LDS tile), and
-// sigma[k] = max(d[perm[k]], 0).  Tile (bi, bj): rows bi of V, sorted
-// columns bj.
+// Vs[:, k] = V[:, perm[k]], Vt = Vs^T (through an LDS tile; skipped when
+// Vt is null), and sigma[k] = d[perm[k]], clamped to max(., 0) for the SVD.
+// Tile (bi, bj): rows bi of V, sorted columns bj.
 __global__ __launch_bounds__(kT * kRows) void large_permute_kernel(const double* V, const double* d, const int* perm,
-                                                                  int n, double* Vs, double* Vt, double* sigma) {
+                                                                  int n, double* Vs, double* Vt, double* sigma,
+                                                                  int clamp) {
     __shared__ double t[kT][kT + 1];
     const int bi = blockIdx.y, bj = blockIdx.x, tx = threadIdx.x;
     const int col = bj * kT + tx;
@@ -81,11 +81,13 @@ __global__ __launch_bounds__(kT * kRows) void large_permute_kernel(const double*
         t[r][tx] = v;
     }
     __syncthreads();
-    for (int r = threadIdx.y; r < kT; r += kRows) {
-        const int k = bj * kT + r, i = bi * kT + tx;
-        if (k < n && i < n) Vt[(size_t)k * n + i] = t[tx][r];
+    if (Vt) {
+        for (int r = threadIdx.y; r < kT; r += kRows) {
+            const int k = bj * kT + r, i = bi * kT + tx;
+            if (k < n && i < n) Vt[(size_t)k * n + i] = t[tx][r];
+        }
     }
-    if (bi == 0 && threadIdx.y == 0 && col < n) sigma[col] = fmax(d[src], 0.0);
+    if (bi == 0 && threadIdx.y == 0 && col < n) sigma[col] = clamp ? fmax(d[src], 0.0) : d[src];
 }
 
 }  // namespace
@@ -103,9 +105,10 @@ hipError_t large_rank(const double* d, int n, int* perm, int* bad, hipStream_t s
 }
 
 hipError_t large_permute(const double* V, const double* d, const int* perm, int n, double* Vs, double* Vt,
-                         double* sigma, hipStream_t st) {
+                         double* sigma, bool clamp, hipStream_t st) {
     const int nt = (n + kT - 1) / kT;
-    hipLaunchKernelGGL(large_permute_kernel, dim3(nt, nt), dim3(kT, kRows), 0, st, V, d, perm, n, Vs, Vt, sigma);
+    hipLaunchKernelGGL(large_permute_kernel, dim3(nt, nt), dim3(kT, kRows), 0, st, V, d, perm, n, Vs, Vt, sigma,
+                       clamp ? 1 : 0);
     return hipGetLastError();
 }
 

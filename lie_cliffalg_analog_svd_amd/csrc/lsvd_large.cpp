@@ -61,61 +61,38 @@ hipError_t large_aux_stream(lsvd_handle_t h, hipStream_t* out) {
     return hipSuccess;
 }
 
-int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
-                     bool* done) {
-    *done = false;
-    if (n < 2 || n > kLargeMaxN) return LSVD_OK;
-    hipStream_t st = h->stream;
-    const bool host = h->ptr_mode == LSVD_PTR_HOST;
-    const size_t nn = (size_t)n * n;
 #define LCHK(expr)                                          \
     do {                                                    \
         hipError_t e_ = (expr);                             \
         if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
     } while (0)
-    Mem bA, bQ, bP, bApad, bVpad, bVs, bWrot, bPart, bRot, bVc, bDiag, bPerm, bScal, bU, bS, bVt;
-    const double* dA = a;
-    double *dU = u, *dS = s, *dVt = vt;
-    if (host) {
-        LCHK(bA.alloc(sizeof(double) * nn));
-        LCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * nn, hipMemcpyHostToDevice, st));
-        dA = (const double*)bA.p;
-        LCHK(bU.alloc(sizeof(double) * nn));
-        LCHK(bS.alloc(sizeof(double) * n));
-        LCHK(bVt.alloc(sizeof(double) * nn));
-        dU = (double*)bU.p;
-        dS = (double*)bS.p;
-        dVt = (double*)bVt.p;
-    }
-    LCHK(bQ.alloc(sizeof(double) * nn));
-    lsvd_polar_info_t pinf{};
-    int rc = polar_run(h, dA, n, n, LSVD_PREC_F64, (double*)bQ.p, &pinf);
-    if (rc != LSVD_OK) return rc;
-    if (!pinf.polar_ok) return LSVD_OK;  // the repair branch (:316-318)
-    const double* dQ = (const double*)bQ.p;
 
-    LCHK(bP.alloc(sizeof(double) * nn));
-    double* dP = (double*)bP.p;
-    LCHK(gemm_f64(true, n, n, n, dQ, dA, dP, 1.0, st));
-    LCHK(large_symmetrize(dP, n, st));
+namespace {
 
-    // eigh_jacobi_full (:212-285) as a block Jacobi method over the whole
-    // chip (block_jacobi.hip), with the reference's stop test before every
-    // sweep: upper off-diagonal norm below JACOBI_TOL ||P||_F.  A sweep that
-    // no longer lowers the off-diagonal norm (the rounding floor) also ends
-    // the iteration.
+// eigh_jacobi_full (:212-285) of the symmetric n x n dP as a block Jacobi
+// method over the whole chip (block_jacobi.hip), with the reference's stop
+// test before every sweep: upper off-diagonal norm below JACOBI_TOL
+// ||P||_F.  A sweep that no longer lowers the off-diagonal norm (the
+// rounding floor) also ends the iteration.  Writes the unsorted diagonal
+// dDiag (n) and the eigenvector columns dVc (n x n); synchronous.
+int run_block_jacobi(lsvd_handle_t h, const double* dP, int n, double* dDiag, double* dVc, int* sweeps_out,
+                     int* rotations_out) {
+    hipStream_t st = h->stream;
+    const size_t nn = (size_t)n * n;
     const int bb = bj_tile_size(n);
     const int nbp = (n + bb - 1) / bb;
     const int ld = nbp * bb;
     const size_t ll = (size_t)ld * ld;
+    const size_t wsz = (size_t)nbp * bb * bb;  // one layer's block rotations
+    Mem bApad, bVpad, bWrot, bPart, bRot;
     LCHK(bApad.alloc(sizeof(double) * ll));
     LCHK(bVpad.alloc(sizeof(double) * ll));
-    const size_t wsz = (size_t)nbp * bb * bb;  // one layer's block rotations
     LCHK(bWrot.alloc(sizeof(double) * 2 * wsz));
     LCHK(bPart.alloc(sizeof(double) * (kBjOffBlocks + 2)));
     LCHK(bRot.alloc(sizeof(int)));
     BlockJacobi bj{n, ld, 2 * nbp, bb, (double*)bApad.p, (double*)bVpad.p, (double*)bPart.p, (int*)bRot.p};
     double* wbuf[2] = {(double*)bWrot.p, (double*)bWrot.p + wsz};
+    double* dsq = (double*)bPart.p + kBjOffBlocks;  // [0] ||P||_F^2, [1] off-diagonal^2
     // With 64-wide tiles (n > 512) V <- V G runs on a second stream,
     // overlapping the next layer's inner step; layer g uses W buffer g % 2,
     // so the inner step of layer g + 2 waits for the V update of layer g.
@@ -128,7 +105,6 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
         LCHK(large_aux_stream(h, &vs));
         LCHK(ev.create());
     }
-    double* dsq = (double*)bPart.p + kBjOffBlocks;  // [0] ||P||_F^2, [1] off-diagonal^2
     LCHK(hipMemsetAsync(bRot.p, 0, sizeof(int), st));
     LCHK(bj_setup(bj, dP, st));
     LCHK(polar_sumsq(dP, nn, dsq, bj.partials, st));
@@ -170,19 +146,74 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
     if (overlap && g > 0) LCHK(hipStreamWaitEvent(st, ev.vdone[(g - 1) & 1], 0));  // V complete
     int rotations = 0;
     LCHK(hipMemcpyAsync(&rotations, bRot.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    LCHK(bj_extract(bj, dDiag, dVc, st));
+    LCHK(hipStreamSynchronize(st));
+    *sweeps_out = sweeps;
+    *rotations_out = rotations;
+    return LSVD_OK;
+}
+
+int put_info(lsvd_handle_t h, lsvd_info_t* info, const lsvd_info_t& inf) {
+    if (!info) return LSVD_OK;
+    if (h->ptr_mode == LSVD_PTR_HOST) {
+        *info = inf;
+        return LSVD_OK;
+    }
+    LCHK(hipMemcpyAsync(info, &inf, sizeof(inf), hipMemcpyHostToDevice, h->stream));
+    LCHK(hipStreamSynchronize(h->stream));
+    return LSVD_OK;
+}
+
+}  // namespace
+
+int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
+                     bool* done) {
+    *done = false;
+    if (n < 2 || n > kLargeMaxN) return LSVD_OK;
+    hipStream_t st = h->stream;
+    const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    const size_t nn = (size_t)n * n;
+    Mem bA, bQ, bP, bVs, bVc, bDiag, bPerm, bScal, bU, bS, bVt;
+    const double* dA = a;
+    double *dU = u, *dS = s, *dVt = vt;
+    if (host) {
+        LCHK(bA.alloc(sizeof(double) * nn));
+        LCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+        dA = (const double*)bA.p;
+        LCHK(bU.alloc(sizeof(double) * nn));
+        LCHK(bS.alloc(sizeof(double) * n));
+        LCHK(bVt.alloc(sizeof(double) * nn));
+        dU = (double*)bU.p;
+        dS = (double*)bS.p;
+        dVt = (double*)bVt.p;
+    }
+    LCHK(bQ.alloc(sizeof(double) * nn));
+    lsvd_polar_info_t pinf{};
+    int rc = polar_run(h, dA, n, n, LSVD_PREC_F64, (double*)bQ.p, &pinf);
+    if (rc != LSVD_OK) return rc;
+    if (!pinf.polar_ok) return LSVD_OK;  // the repair branch (:316-318)
+    const double* dQ = (const double*)bQ.p;
+
+    LCHK(bP.alloc(sizeof(double) * nn));
+    double* dP = (double*)bP.p;
+    LCHK(gemm_f64(true, n, n, n, dQ, dA, dP, 1.0, st));
+    LCHK(large_symmetrize(dP, n, st));
 
     LCHK(bDiag.alloc(sizeof(double) * n));
+    LCHK(bVc.alloc(sizeof(double) * nn));
+    double* dDiag = (double*)bDiag.p;
+    double* dVc = (double*)bVc.p;
+    int sweeps = 0, rotations = 0;
+    rc = run_block_jacobi(h, dP, n, dDiag, dVc, &sweeps, &rotations);
+    if (rc != LSVD_OK) return rc;
+
     LCHK(bPerm.alloc(sizeof(int) * n));
     LCHK(bScal.alloc(sizeof(double) * 2));
     LCHK(bVs.alloc(sizeof(double) * nn));
-    double* dDiag = (double*)bDiag.p;
     double* scal = (double*)bScal.p;  // [0] ||U^T U - I||_F^2, [1] non-finite flag (as int)
-    LCHK(bVc.alloc(sizeof(double) * nn));
-    double* dVc = (double*)bVc.p;
-    LCHK(bj_extract(bj, dDiag, dVc, st));
     LCHK(polar_zero(scal, 2, st));
     LCHK(large_rank(dDiag, n, (int*)bPerm.p, (int*)(scal + 1), st));
-    LCHK(large_permute(dVc, dDiag, (const int*)bPerm.p, n, (double*)bVs.p, dVt, dS, st));
+    LCHK(large_permute(dVc, dDiag, (const int*)bPerm.p, n, (double*)bVs.p, dVt, dS, true, st));
     LCHK(gemm_f64(false, n, n, n, dQ, (const double*)bVs.p, dU, 1.0, st));
     LCHK(ns_orth(n, n, dU, scal, st));
     double hs[2];
@@ -198,24 +229,64 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
         LCHK(hipMemcpyAsync(vt, dVt, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
         LCHK(hipStreamSynchronize(st));
     }
-#undef LCHK
-    if (info) {
-        lsvd_info_t inf{};
-        inf.polar_ok = 1;
-        inf.ns_iters = pinf.f64_iters;
-        inf.jacobi_sweeps = sweeps;
-        inf.rotations = rotations;
-        inf.path = 3;
-        if (host) {
-            *info = inf;
-        } else {
-            hipError_t e = hipMemcpyAsync(info, &inf, sizeof(inf), hipMemcpyHostToDevice, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) return hip_fail(h, e, "info copy");
-        }
-    }
-    *done = true;
-    return LSVD_OK;
+    lsvd_info_t inf{};
+    inf.polar_ok = 1;
+    inf.ns_iters = pinf.f64_iters;
+    inf.jacobi_sweeps = sweeps;
+    inf.rotations = rotations;
+    inf.path = 3;
+    rc = put_info(h, info, inf);
+    if (rc == LSVD_OK) *done = true;
+    return rc;
 }
+
+int eigh_large(lsvd_handle_t h, const double* p, int n, double* v, double* eig, lsvd_info_t* info, bool* done) {
+    *done = false;
+    if (n < 2 || n > kLargeMaxN) return LSVD_OK;
+    hipStream_t st = h->stream;
+    const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    const size_t nn = (size_t)n * n;
+    Mem bP, bVc, bDiag, bPerm, bFlag, bV, bE;
+    const double* dP = p;
+    double *dV = v, *dE = eig;
+    if (host) {
+        LCHK(bP.alloc(sizeof(double) * nn));
+        LCHK(hipMemcpyAsync(bP.p, p, sizeof(double) * nn, hipMemcpyHostToDevice, st));
+        dP = (const double*)bP.p;
+        LCHK(bV.alloc(sizeof(double) * nn));
+        LCHK(bE.alloc(sizeof(double) * n));
+        dV = (double*)bV.p;
+        dE = (double*)bE.p;
+    }
+    LCHK(bDiag.alloc(sizeof(double) * n));
+    LCHK(bVc.alloc(sizeof(double) * nn));
+    int sweeps = 0, rotations = 0;
+    int rc = run_block_jacobi(h, dP, n, (double*)bDiag.p, (double*)bVc.p, &sweeps, &rotations);
+    if (rc != LSVD_OK) return rc;
+    LCHK(bPerm.alloc(sizeof(int) * n));
+    LCHK(bFlag.alloc(sizeof(double)));
+    LCHK(polar_zero((double*)bFlag.p, 1, st));
+    LCHK(large_rank((const double*)bDiag.p, n, (int*)bPerm.p, (int*)bFlag.p, st));
+    LCHK(large_permute((const double*)bVc.p, (const double*)bDiag.p, (const int*)bPerm.p, n, dV, nullptr, dE, false,
+                       st));
+    int bad = 0;
+    LCHK(hipMemcpyAsync(&bad, bFlag.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    LCHK(hipStreamSynchronize(st));
+    if (bad) return LSVD_OK;  // non-finite input: the workgroup kernels report it as the reference does
+    if (host) {
+        LCHK(hipMemcpyAsync(v, dV, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
+        LCHK(hipMemcpyAsync(eig, dE, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        LCHK(hipStreamSynchronize(st));
+    }
+    lsvd_info_t inf{};  // no polar step: polar_ok stays 0, as in the workgroup kernels' eigh mode
+    inf.jacobi_sweeps = sweeps;
+    inf.rotations = rotations;
+    inf.path = 3;
+    rc = put_info(h, info, inf);
+    if (rc == LSVD_OK) *done = true;
+    return rc;
+}
+
+#undef LCHK
 
 }  // namespace lsvd

@@ -36,8 +36,9 @@ hipError_t bj_extract(const BlockJacobi& j, double* d, double* Vc, hipStream_t s
 
 hipError_t large_symmetrize(double* P, int n, hipStream_t st);
 hipError_t large_rank(const double* d, int n, int* perm, int* bad, hipStream_t st);
+// Vt may be null; clamp: sigma = max(d, 0) (the SVD) instead of d (eigh).
 hipError_t large_permute(const double* V, const double* d, const int* perm, int n, double* Vs, double* Vt,
-                         double* sigma, hipStream_t st);
+                         double* sigma, bool clamp, hipStream_t st);
 
 // LieSvdSmall::solve of one n x n matrix with every step spread over the
 // chip.  Pointers follow the handle's pointer mode.  Sets *done = false
@@ -48,5 +49,12 @@ hipError_t large_aux_stream(lsvd_handle_t h, hipStream_t* out);
 
 int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
                      bool* done);
+
+// eigh_jacobi_full (src/lie_svd_small.rs:212-285) of one symmetric n x n
+// matrix by the block Jacobi method: v (n x n, eigenvectors as columns)
+// and eig (descending).  Pointers follow the handle's pointer mode; sets
+// *done = false when the diagonal holds a non-finite value (the caller then
+// runs the workgroup kernels).
+int eigh_large(lsvd_handle_t h, const double* p, int n, double* v, double* eig, lsvd_info_t* info, bool* done);
 
 }  // namespace lsvd

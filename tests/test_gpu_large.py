@@ -167,3 +167,49 @@ def test_large_route_device_pointers_match_host_mode():
     vector_parity(du.cpu().numpy(), u1, s1, n, "device vs host mode U")
     vector_parity(dvt.cpu().numpy().T, vt1.T, s1, n, "device vs host mode V")
     h.close()
+
+
+@pytest.mark.parametrize("n", [160, 300])
+def test_large_eigh_vs_oracle(n):
+    """eigh_jacobi_full (src/lie_svd_small.rs:212-285) above 128 on the block
+    Jacobi: eigenvalues to 64 n eps of the largest against the oracle's, the
+    eigen-residual and orthogonality, per-cluster eigenvector parity."""
+    from lie_cliffalg_analog_svd_amd import eigh_jacobi_full
+    _say("case large eigh", n)
+    g = rng.normal_matrix(0x5EED0800 + n, n, n)
+    p = g + g.T
+    v, e = eigh_jacobi_full(p)
+    info = dict(eigh_jacobi_full.last_info)
+    assert info["path"] == LARGE_PATH and info["jacobi_sweeps"] >= 3, info
+    vo, eo, _ = oracle.eigh_jacobi_full(p)
+    eps = np.finfo(float).eps
+    assert np.all(np.diff(e) <= 0)
+    assert np.max(np.abs(e - eo)) < 64 * n * eps * np.abs(eo).max()
+    assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-11 * n / 32
+    assert np.linalg.norm(v.T @ p @ v - np.diag(e)) < 1e-11 * n / 32 * np.linalg.norm(p)
+    # eigenvectors up to sign (distinct eigenvalues of a random symmetric matrix)
+    c = np.abs(np.sum(v * vo, axis=0))
+    gaps = np.minimum(np.abs(np.diff(eo, prepend=np.inf)), np.abs(np.diff(eo, append=-np.inf)))
+    ok = gaps > 1e-6 * np.abs(eo).max()
+    assert np.max(np.abs(1.0 - c[ok])) < 1e-9
+
+
+def test_large_eigh_psd_and_degenerate():
+    """A PSD matrix with repeated eigenvalues (clusters of 32) at n = 256:
+    eigenvalues against the imposed ones, the invariant subspaces by angle,
+    and the same answer from the workgroup kernels."""
+    from lie_cliffalg_analog_svd_amd import eigh_jacobi_full
+    n = 256
+    lam = np.repeat(np.linspace(5.0, 0.5, n // 32), 32)
+    q = rng.haar(0x5EED0900, n)
+    p = (q * lam[None, :]) @ q.T
+    p = 0.5 * (p + p.T)
+    v, e = eigh_jacobi_full(p)
+    assert eigh_jacobi_full.last_info["path"] == LARGE_PATH
+    assert np.max(np.abs(e - np.sort(lam)[::-1])) < 1e-12 * 5.0 * n
+    h_wg = _handle(100000)
+    v2, e2 = eigh_jacobi_full(p, h_wg)
+    assert eigh_jacobi_full.last_info["path"] == 0
+    assert np.max(np.abs(e - e2)) < 64 * n * np.finfo(float).eps * 5.0
+    vector_parity(v, v2, e2, n, "eigh clusters")
+    h_wg.close()

@@ -16,11 +16,17 @@ the sigma error against the imposed spectra, and the CPU oracle (the
 restatement of the reference algorithm) timed on all host cores on the
 same workload.  Further legs, each with its CPU oracle beside it:
   single   one 64 x 64 N(0,1) LieSvdSmall solve (latency);
+  large    one N(0,1) matrix of 384, 512, 1024 through LieSvdSmall (the
+           chip-wide route) beside the reference's stated CPU times;
   polar    NS polar of 4096 x 4096 and 8192 x 2048 (FP64 vs BF16/FP32 MFMA
            iterations with FP64 refinement), polar TFLOP/s;
   joint    Phase-JADE of 32 families x 64 symmetric 256 x 256 matrices,
            GB/s of the layer traffic, off-diagonal residual, recovery of
-           the planted basis;
+           the planted basis; the reference's published smoke families;
+  joint_svd, givens, polish, triage, hosvd
+           the two-sided joint SVD, the rotor-stream export of the batch's
+           V factors, the batched digital polish, the adaptive triage, and
+           the tensor HO-SVD of 4096 tensors of 16^3;
   vjp      gradient of L = sum w_i s_i + <G, U V^T> for 512 96 x 64
            matrices with gaps ~1e-3 plus 32 with one 1e-8 gap, against the
            analytic FP64 gradient and central differences.
@@ -609,6 +615,58 @@ def vjp_leg(dev, args):
             "gpu_gradients_per_s_host_io": round(len(a) / sec, 1)}
 
 
+def hosvd_leg(h, dev, args):
+    """LieSvdTensor::hosvd3 (src/lie_svd_tensor.rs:31-62) of a batch of
+    4096 tensors of 16^3 -- the shape of the reference's published smoke --
+    on device pointers: 12288 mode Grams solved by one batched LieSvdSmall
+    call, then the core rotations.  Each tensor is the smoke tensor with
+    its off-superdiagonal part redrawn N(0, 1e-3)."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import _lib
+    count, n = args.hosvd_count, 16
+    rng = np.random.default_rng(0x5EED0006)
+    i, j, k = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    sup = (i == j) & (j == k)
+    xs = 1e-3 * rng.standard_normal((count, n, n, n))
+    xs[:, sup] = (5.0 / (1.0 + np.arange(n)))[None, :]
+    xd = torch.from_numpy(xs).to(dev)
+    cd = torch.empty_like(xd)
+    ud = [torch.empty((count, n, n), dtype=torch.float64, device=dev) for _ in range(3)]
+    tr = (_lib.LsvdTensorTrace * count)()
+    ptr = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+
+    def run():
+        h.check(_lib.lib().lsvd_hosvd3_f64(h.raw, count, n, n, n, ptr(xd), ptr(cd), ptr(ud[0]), ptr(ud[1]),
+                                           ptr(ud[2]), tr))
+    run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    sec = float(np.median(ts))
+    core = cd.cpu().numpy()
+    sample = list(range(0, count, max(1, count // 64)))
+    t1 = time.perf_counter()
+    worst = 0.0
+    for t in sample:
+        oc = oracle.hosvd3(xs[t])[0]
+        worst = max(worst, float(np.max(np.abs(np.abs(core[t]) - np.abs(oc))) / np.abs(oc).max()))
+    cpu = (time.perf_counter() - t1) / len(sample)
+    out = {"tensors": count, "shape": [n, n, n], "seconds": round(sec, 5), "tensors_per_s": round(count / sec, 1),
+           "superdiag_mass_min": min(tr[t].superdiag_mass_ratio for t in range(count)),
+           "core_abs_diff_vs_oracle_sample": worst,
+           "cpu_baseline": {"value": round(1.0 / cpu, 2), "unit": "tensors/s", "cores": 1, "kind": "port",
+                            "sample": f"{len(sample)} of the tensors, C restatement of LieSvdTensor::hosvd3 "
+                                      f"(six-deep core sum as the reference), {cpu * len(sample):.2f}s"}}
+    log(f"hosvd: {out}")
+    return out
+
+
 def polar_leg(h, dev, args):
     """Large NS polar: 4096 x 4096 and 8192 x 2048, A = U diag(s) V^T with Haar
     factors and s log-uniform in [1e-6, 1].  Plain FP64 vs BF16 MFMA
@@ -709,6 +767,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-triage", action="store_true", help="skip the adaptive triage leg")
     ap.add_argument("--polish-count", type=int, default=2048)
     ap.add_argument("--no-vjp", action="store_true", help="skip the vjp leg")
+    ap.add_argument("--no-hosvd", action="store_true", help="skip the tensor HO-SVD leg")
+    ap.add_argument("--hosvd-count", type=int, default=4096)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 3 times (HBM counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -913,6 +973,8 @@ def main(argv=None) -> int:
     if not args.no_triage:
         legs["triage"] = triage_leg(h, args)
         log(f"triage: {legs['triage']}")
+    if not args.no_hosvd:
+        legs["hosvd"] = hosvd_leg(h, dev, args)
     if not args.no_vjp:
         legs["vjp"] = vjp_leg(dev, args)
         log(f"vjp: {legs['vjp']}")

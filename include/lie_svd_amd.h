@@ -281,6 +281,31 @@ int lsvd_analyze_batched_f64(lsvd_handle_t h, int count, const int32_t* n, const
  * defaults.  Returns the lsvd_route (-1 for a NULL triage). */
 int lsvd_choose_route(const lsvd_triage_t* triage, const lsvd_adaptive_params_t* params);
 
+/* ---- Tensor HO-SVD (batched consumer of LieSvdSmall) ------------------ */
+
+/* TensorPhaseTrace (src/lie_svd_tensor.rs:11-17) without the shape. */
+typedef struct {
+    double initial_offdiag, final_offdiag, superdiag_mass_ratio;
+} lsvd_tensor_trace_t;
+
+/* Replaces LieSvdTensor::hosvd3 (src/lie_svd_tensor.rs:31-62) for `count`
+ * tensors of one shape n1 x n2 x n3 (row-major, the last index fastest;
+ * 1 <= n_i <= 64, LSVD_E_LIMIT beyond, LSVD_E_SHAPE for an empty one):
+ * G_m = X_(m) X_(m)^T (the reference's summation order, no FMA), U_m from
+ * LieSvdSmall::solve(G_m) -- all 3 count Grams in one batched call --, and
+ * core = X x1 U1^T x2 U2^T x3 U3^T.  x, core: count n1 n2 n3 doubles; u1, u2,
+ * u3: count n_m^2 each.  trace (host memory in both pointer modes, may be
+ * NULL) receives count entries. */
+int lsvd_hosvd3_f64(lsvd_handle_t h, int count, int n1, int n2, int n3, const double* x, double* core, double* u1,
+                    double* u2, double* u3, lsvd_tensor_trace_t* trace);
+
+/* Replaces rotate_tensor3 (src/lie_svd_tensor.rs:121-145), as used by
+ * reconstruct_hosvd3 (:64-66): out[a][b][c] = sum q1[a][i] q2[b][j] q3[c][k]
+ * x[i][j][k] for `count` tensors; q_m is r_m x n_m per tensor (count r_m n_m
+ * doubles), out r1 x r2 x r3 per tensor.  Extents 1..64. */
+int lsvd_tensor_rotate3_f64(lsvd_handle_t h, int count, int n1, int n2, int n3, int r1, int r2, int r3,
+                            const double* x, const double* q1, const double* q2, const double* q3, double* out);
+
 /* ---- Digital polish (batched consumer of LieSvdSmall) ---------------- */
 
 /* Replaces the digital-polish step of the PhaseFlow, Analog and Hybrid

@@ -4,7 +4,7 @@ Hand-written HIP kernels behind a C ABI (``include/lie_svd_amd.h``,
 ``liblsvd_amd.so``), with a Python mirror of the reference's Rust API:
 ``LieSvdSmall``, ``LieSvdMicro``, ``LieSvd``, ``eigh_jacobi_full``, the
 batch plans, ``LieSvdJoint`` and the rotor-stream export
-(``HardwareSchedule``).  ``profiles`` and ``metrics`` restate the reference's input
+(``HardwareSchedule``), tensor HO-SVD (``LieSvdTensor``).  ``profiles`` and ``metrics`` restate the reference's input
 generators and residual metrics (src/lib.rs:133-388).
 """
 from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_SCALED, LSVD_PTR_DEVICE,
@@ -19,6 +19,8 @@ from .joint import (JointDiagonalizationParams, JointDiagonalizationTrace, Joint
                     diagonalize_families, svd_families)
 from .adaptive import (AdaptiveRoute, AdaptiveTrace, AdaptiveTriage, LieSvdAdaptive, LieSvdAdaptiveParams,
                        analyze_batch, analyze_matrix, choose_route)
+from .tensor import (LieSvdTensor, TensorPhaseSvd3, TensorPhaseTrace, hosvd3_batched, reconstruct_hosvd3,
+                     rotate_tensor3_batched, tensor_relative_error)
 from .schedule import (ComplexMziPhase, ComplexPhaseEventKind, HardwarePhaseEvent, HardwareSchedule, HardwareTarget,
                        MziPhase, PhaseRotorKind, givens_batch, schedules_from_orthogonal)
 

@@ -71,6 +71,14 @@ class LsvdJointTrace(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LsvdTensorTrace(C.Structure):
+    _fields_ = [
+        ("initial_offdiag", C.c_double),
+        ("final_offdiag", C.c_double),
+        ("superdiag_mass_ratio", C.c_double),
+    ]
+
+
 class LsvdPolarInfo(C.Structure):
     _fields_ = [
         ("polar_ok", C.c_int32),
@@ -184,6 +192,10 @@ SIGNATURES = {
     "lsvd_choose_route": (C.c_int, [C.POINTER(LsvdTriage), C.POINTER(LsvdAdaptiveParams)]),
     "lsvd_polish_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _i64p, _vp, _vp, _vp, C.c_int, _vp, _vp, _vp,
                                           _vp]),
+    "lsvd_hosvd3_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp,
+                                  C.POINTER(LsvdTensorTrace)]),
+    "lsvd_tensor_rotate3_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp,
+                                          _vp, _vp, _vp, _vp]),
     "lsvd_givens_schedule_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "lsvd_givens_events": (C.c_int, [C.c_int, _vp, _vp, C.POINTER(LsvdPhaseEvent)]),
     "lsvd_schedule_json": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.POINTER(LsvdPhaseEvent), C.c_int, _vp,

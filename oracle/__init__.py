@@ -147,6 +147,10 @@ def lib():
             getattr(L, fn).argtypes = [C.POINTER(Triage), C.POINTER(AdaptiveParams)]
             getattr(L, fn).restype = C.c_int
         L.lso_polish.argtypes = [dp, dp, dp, C.c_int, dp, dp, dp, C.POINTER(Info)]
+        L.lso_hosvd3.argtypes = [dp, C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, dp]
+        L.lso_hosvd3.restype = None
+        L.lso_rotate_tensor3.argtypes = [dp, C.c_int, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int, dp, C.c_int, dp]
+        L.lso_rotate_tensor3.restype = None
         L.lso_givens_schedule.argtypes = [dp, C.c_int, ip, ip, dp, dp, dp, dp]
         L.lso_givens_schedule.restype = C.c_int
         L.lso_schedule_json.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(PhaseEvent), C.c_int, dp,
@@ -464,3 +468,27 @@ def route_predicate(name: str, triage: dict, params: AdaptiveParams | None = Non
     fn = {"phaseflow": "lso_should_use_phaseflow", "coreflow_topo": "lso_should_use_coreflow_topo",
           "balanced_degenerate": "lso_is_balanced_degenerate"}[name]
     return bool(getattr(lib(), fn)(C.byref(_triage(triage)), C.byref(p)))
+
+
+def hosvd3(x):
+    """LieSvdTensor::hosvd3 (src/lie_svd_tensor.rs:31-62).  Returns (core, U1,
+    U2, U3, trace dict)."""
+    x = _c(x)
+    n1, n2, n3 = x.shape
+    core = np.zeros_like(x)
+    us = [np.zeros((n, n)) for n in (n1, n2, n3)]
+    tr = np.zeros(3)
+    lib().lso_hosvd3(_dp(x), n1, n2, n3, _dp(core), _dp(us[0]), _dp(us[1]), _dp(us[2]), _dp(tr))
+    return core, us[0], us[1], us[2], {"initial_offdiag": tr[0], "final_offdiag": tr[1],
+                                       "superdiag_mass_ratio": tr[2]}
+
+
+def rotate_tensor3(x, q1, q2, q3):
+    """rotate_tensor3 (src/lie_svd_tensor.rs:121-145): q_m is r_m x n_m."""
+    x = _c(x)
+    q1, q2, q3 = _c(q1), _c(q2), _c(q3)
+    n1, n2, n3 = x.shape
+    out = np.zeros((q1.shape[0], q2.shape[0], q3.shape[0]))
+    lib().lso_rotate_tensor3(_dp(x), n1, n2, n3, _dp(q1), q1.shape[0], _dp(q2), q2.shape[0], _dp(q3), q3.shape[0],
+                             _dp(out))
+    return out

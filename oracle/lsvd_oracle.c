@@ -1462,3 +1462,112 @@ int lso_choose_route(const lso_triage* t, const lso_adaptive_params* p) {
     if (t->n <= p->small_max_n) return 1;
     return 4;
 }
+
+/* ===================== Tensor HO-SVD (src/lie_svd_tensor.rs) ============== */
+
+/* mode_gram (src/lie_svd_tensor.rs:73-119): G[a][b] = sum over the other two
+ * indices of x[..a..] x[..b..], in the reference's loop order. */
+static void tensor_mode_gram(const double* x, int n1, int n2, int n3, int mode, double* g) {
+    if (mode == 0) {
+        for (int a = 0; a < n1; ++a)
+            for (int b = 0; b < n1; ++b) {
+                double s = 0.0;
+                for (int j = 0; j < n2; ++j)
+                    for (int k = 0; k < n3; ++k)
+                        s += x[((size_t)a * n2 + j) * n3 + k] * x[((size_t)b * n2 + j) * n3 + k];
+                g[(size_t)a * n1 + b] = s;
+            }
+    } else if (mode == 1) {
+        for (int a = 0; a < n2; ++a)
+            for (int b = 0; b < n2; ++b) {
+                double s = 0.0;
+                for (int i = 0; i < n1; ++i)
+                    for (int k = 0; k < n3; ++k)
+                        s += x[((size_t)i * n2 + a) * n3 + k] * x[((size_t)i * n2 + b) * n3 + k];
+                g[(size_t)a * n2 + b] = s;
+            }
+    } else {
+        for (int a = 0; a < n3; ++a)
+            for (int b = 0; b < n3; ++b) {
+                double s = 0.0;
+                for (int i = 0; i < n1; ++i)
+                    for (int j = 0; j < n2; ++j)
+                        s += x[((size_t)i * n2 + j) * n3 + a] * x[((size_t)i * n2 + j) * n3 + b];
+                g[(size_t)a * n3 + b] = s;
+            }
+    }
+}
+
+/* rotate_tensor3 (src/lie_svd_tensor.rs:121-145): the reference's six-deep
+ * sum, out[a][b][c] = sum_ijk q1[a][i] q2[b][j] q3[c][k] x[i][j][k]; q_m is
+ * r_m x n_m. */
+void lso_rotate_tensor3(const double* x, int n1, int n2, int n3, const double* q1, int r1, const double* q2, int r2,
+                        const double* q3, int r3, double* out) {
+    for (int a = 0; a < r1; ++a)
+        for (int b = 0; b < r2; ++b)
+            for (int c = 0; c < r3; ++c) {
+                double s = 0.0;
+                for (int i = 0; i < n1; ++i)
+                    for (int j = 0; j < n2; ++j)
+                        for (int k = 0; k < n3; ++k)
+                            s += q1[(size_t)a * n1 + i] * q2[(size_t)b * n2 + j] * q3[(size_t)c * n3 + k] *
+                                 x[((size_t)i * n2 + j) * n3 + k];
+                out[((size_t)a * r2 + b) * r3 + c] = s;
+            }
+}
+
+/* tensor_offdiag_norm (:147-160) and superdiag_mass_ratio (:162-183). */
+static double tensor_offdiag_norm(const double* x, int n1, int n2, int n3) {
+    double s = 0.0;
+    for (int i = 0; i < n1; ++i)
+        for (int j = 0; j < n2; ++j)
+            for (int k = 0; k < n3; ++k)
+                if (!(i == j && j == k)) {
+                    const double v = x[((size_t)i * n2 + j) * n3 + k];
+                    s += v * v;
+                }
+    return sqrt(s);
+}
+
+static double tensor_superdiag_ratio(const double* x, int n1, int n2, int n3) {
+    const int n = n1 < n2 ? (n1 < n3 ? n1 : n3) : (n2 < n3 ? n2 : n3);
+    double diag = 0.0, all = 0.0;
+    for (size_t e = 0; e < (size_t)n1 * n2 * n3; ++e) all += x[e] * x[e];
+    for (int i = 0; i < n; ++i) {
+        const double v = x[((size_t)i * n2 + i) * n3 + i];
+        diag += v * v;
+    }
+    return diag / (all > 1e-300 ? all : 1e-300);
+}
+
+/* LieSvdTensor::hosvd3 (src/lie_svd_tensor.rs:31-62).  trace[3] =
+ * {initial_offdiag, final_offdiag, superdiag_mass_ratio}. */
+void lso_hosvd3(const double* x, int n1, int n2, int n3, double* core, double* u1, double* u2, double* u3,
+                double* trace) {
+    const int dims[3] = {n1, n2, n3};
+    double* us[3] = {u1, u2, u3};
+    for (int m = 0; m < 3; ++m) {
+        const int n = dims[m];
+        double* g = (double*)malloc(sizeof(double) * n * n);
+        double* s = (double*)malloc(sizeof(double) * n);
+        double* vt = (double*)malloc(sizeof(double) * n * n);
+        lso_info info;
+        tensor_mode_gram(x, n1, n2, n3, m, g);
+        lso_small_solve(g, n, us[m], s, vt, &info);
+        free(g);
+        free(s);
+        free(vt);
+    }
+    double* q[3];
+    for (int m = 0; m < 3; ++m) {  /* U_m^T */
+        const int n = dims[m];
+        q[m] = (double*)malloc(sizeof(double) * n * n);
+        for (int r = 0; r < n; ++r)
+            for (int c = 0; c < n; ++c) q[m][(size_t)r * n + c] = us[m][(size_t)c * n + r];
+    }
+    lso_rotate_tensor3(x, n1, n2, n3, q[0], n1, q[1], n2, q[2], n3, core);
+    for (int m = 0; m < 3; ++m) free(q[m]);
+    trace[0] = tensor_offdiag_norm(x, n1, n2, n3);
+    trace[1] = tensor_offdiag_norm(core, n1, n2, n3);
+    trace[2] = tensor_superdiag_ratio(core, n1, n2, n3);
+}

@@ -139,7 +139,11 @@ int lsvd_small_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, dou
                          lsvd_info_t* info);
 
 /* Replaces LieSvdSmall::solve_rectangular (src/lie_svd_small.rs:340-360).
- * a: m x n; k = min(m, n); u: m x k, s: k, vt: k x n. */
+ * a: m x n; k = min(m, n) <= 1024; u: m x k, s: k, vt: k x n.  k > 128
+ * takes the chip-wide route (info.path = 3): the polar iteration on the
+ * tall matrix itself (a wide one transposed), then as lsvd_small_solve_f64;
+ * a rank-deficient input, whose polar factor is not trusted, goes to the
+ * workgroup kernels' QR front end and repair (path 0). */
 int lsvd_small_solve_rectangular_f64(lsvd_handle_t h, const double* a, int m, int n, double* u, double* s,
                                      double* vt, lsvd_info_t* info);
 

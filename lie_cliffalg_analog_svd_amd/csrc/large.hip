@@ -90,7 +90,29 @@ __global__ __launch_bounds__(kT * kRows) void large_permute_kernel(const double*
     if (bi == 0 && threadIdx.y == 0 && col < n) sigma[col] = clamp ? fmax(d[src], 0.0) : d[src];
 }
 
+// out[c][r] = in[r][c] for one 32 x 32 tile.
+__global__ __launch_bounds__(kT * kRows) void large_transpose_kernel(const double* in, int rows, int cols,
+                                                                    double* out) {
+    __shared__ double t[kT][kT + 1];
+    const int br = blockIdx.y, bc = blockIdx.x, tx = threadIdx.x;
+    for (int r = threadIdx.y; r < kT; r += kRows) {
+        const int i = br * kT + r, j = bc * kT + tx;
+        t[r][tx] = (i < rows && j < cols) ? in[(size_t)i * cols + j] : 0.0;
+    }
+    __syncthreads();
+    for (int r = threadIdx.y; r < kT; r += kRows) {
+        const int j = bc * kT + r, i = br * kT + tx;
+        if (i < rows && j < cols) out[(size_t)j * rows + i] = t[tx][r];
+    }
+}
+
 }  // namespace
+
+hipError_t large_transpose(const double* in, int rows, int cols, double* out, hipStream_t st) {
+    hipLaunchKernelGGL(large_transpose_kernel, dim3((cols + kT - 1) / kT, (rows + kT - 1) / kT), dim3(kT, kRows), 0,
+                       st, in, rows, cols, out);
+    return hipGetLastError();
+}
 
 hipError_t large_symmetrize(double* P, int n, hipStream_t st) {
     const int nt = (n + kT - 1) / kT;

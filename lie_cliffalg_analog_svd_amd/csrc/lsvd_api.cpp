@@ -834,11 +834,12 @@ static int single(lsvd_handle_t h, int mode, const double* a, int m, int n, doub
         return LSVD_OK;
     }
     if (!a || !u || !s || (mode != kModeEigh && !vt)) return set_error(h, LSVD_E_ARG, "null pointer");
-    if ((mode == kModeSvd || mode == kModeEigh) && m == n && n >= h->large_min_n && n <= kLargeMaxN) {
+    if (((mode == kModeSvd && k >= h->large_min_n) || (mode == kModeEigh && m == n && n >= h->large_min_n)) &&
+        k <= kLargeMaxN) {
         hipError_t e = hipSetDevice(h->device);
         if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
         bool done = false;
-        int rc = mode == kModeSvd ? svd_large_square(h, a, n, u, s, vt, info, &done)
+        int rc = mode == kModeSvd ? svd_large(h, a, m, n, u, s, vt, info, &done)
                                   : eigh_large(h, a, n, u, s, info, &done);
         if (rc != LSVD_OK || done) return rc;
     }

@@ -166,37 +166,27 @@ int put_info(lsvd_handle_t h, lsvd_info_t* info, const lsvd_info_t& inf) {
 
 }  // namespace
 
-int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
-                     bool* done) {
-    *done = false;
-    if (n < 2 || n > kLargeMaxN) return LSVD_OK;
+namespace {
+
+// The route on device buffers for m >= n: dA m x n in; dU m x n, dS n,
+// dVt n x n out.  *ok = false when the reference's repair branch applies
+// (polar factor not trusted, or Q V not orthogonal to 1e-8 sqrt(n)).
+int svd_large_tall(lsvd_handle_t h, const double* dA, int m, int n, double* dU, double* dS, double* dVt,
+                   lsvd_info_t* inf, bool* ok) {
+    *ok = false;
     hipStream_t st = h->stream;
-    const bool host = h->ptr_mode == LSVD_PTR_HOST;
-    const size_t nn = (size_t)n * n;
-    Mem bA, bQ, bP, bVs, bVc, bDiag, bPerm, bScal, bU, bS, bVt;
-    const double* dA = a;
-    double *dU = u, *dS = s, *dVt = vt;
-    if (host) {
-        LCHK(bA.alloc(sizeof(double) * nn));
-        LCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * nn, hipMemcpyHostToDevice, st));
-        dA = (const double*)bA.p;
-        LCHK(bU.alloc(sizeof(double) * nn));
-        LCHK(bS.alloc(sizeof(double) * n));
-        LCHK(bVt.alloc(sizeof(double) * nn));
-        dU = (double*)bU.p;
-        dS = (double*)bS.p;
-        dVt = (double*)bVt.p;
-    }
-    LCHK(bQ.alloc(sizeof(double) * nn));
+    const size_t mn = (size_t)m * n, nn = (size_t)n * n;
+    Mem bQ, bP, bVs, bVc, bDiag, bPerm, bScal;
+    LCHK(bQ.alloc(sizeof(double) * mn));
     lsvd_polar_info_t pinf{};
-    int rc = polar_run(h, dA, n, n, LSVD_PREC_F64, (double*)bQ.p, &pinf);
+    int rc = polar_run(h, dA, m, n, LSVD_PREC_F64, (double*)bQ.p, &pinf);
     if (rc != LSVD_OK) return rc;
     if (!pinf.polar_ok) return LSVD_OK;  // the repair branch (:316-318)
     const double* dQ = (const double*)bQ.p;
 
     LCHK(bP.alloc(sizeof(double) * nn));
     double* dP = (double*)bP.p;
-    LCHK(gemm_f64(true, n, n, n, dQ, dA, dP, 1.0, st));
+    LCHK(gemm_f64(true, n, n, m, dQ, dA, dP, 1.0, st));  // P = Q^T A
     LCHK(large_symmetrize(dP, n, st));
 
     LCHK(bDiag.alloc(sizeof(double) * n));
@@ -214,8 +204,8 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
     LCHK(polar_zero(scal, 2, st));
     LCHK(large_rank(dDiag, n, (int*)bPerm.p, (int*)(scal + 1), st));
     LCHK(large_permute(dVc, dDiag, (const int*)bPerm.p, n, (double*)bVs.p, dVt, dS, true, st));
-    LCHK(gemm_f64(false, n, n, n, dQ, (const double*)bVs.p, dU, 1.0, st));
-    LCHK(ns_orth(n, n, dU, scal, st));
+    LCHK(gemm_f64(false, m, n, n, dQ, (const double*)bVs.p, dU, 1.0, st));  // U = Q V
+    LCHK(ns_orth(m, n, dU, scal, st));
     double hs[2];
     LCHK(hipMemcpyAsync(hs, scal, sizeof(hs), hipMemcpyDeviceToHost, st));
     LCHK(hipStreamSynchronize(st));
@@ -223,18 +213,66 @@ int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double*
     std::memcpy(&bad, &hs[1], sizeof(int));
     const double orth = std::sqrt(std::max(hs[0], 0.0));
     if (bad || !(orth <= 1e-8 * std::sqrt((double)n))) return LSVD_OK;  // the repair branch
+    *inf = lsvd_info_t{};
+    inf->polar_ok = 1;
+    inf->ns_iters = pinf.f64_iters;
+    inf->jacobi_sweeps = sweeps;
+    inf->rotations = rotations;
+    inf->path = 3;
+    *ok = true;
+    return LSVD_OK;
+}
+
+}  // namespace
+
+int svd_large(lsvd_handle_t h, const double* a, int m, int n, double* u, double* s, double* vt, lsvd_info_t* info,
+              bool* done) {
+    *done = false;
+    const int k = std::min(m, n);
+    if (k < 2 || k > kLargeMaxN) return LSVD_OK;
+    hipStream_t st = h->stream;
+    const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    const bool wide = n > m;  // solved transposed (:348-352): SVD(A^T) = (V, sigma, U^T)
+    const int tm = wide ? n : m, tn = k;  // the tall problem is tm x tn
+    const size_t mn = (size_t)m * n, kk = (size_t)k * k;
+    Mem bA, bAt, bU, bS, bVt, bTu, bTvt;
+    const double* dA = a;
+    double *dU = u, *dS = s, *dVt = vt;
     if (host) {
-        LCHK(hipMemcpyAsync(u, dU, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
-        LCHK(hipMemcpyAsync(s, dS, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-        LCHK(hipMemcpyAsync(vt, dVt, sizeof(double) * nn, hipMemcpyDeviceToHost, st));
-        LCHK(hipStreamSynchronize(st));
+        LCHK(bA.alloc(sizeof(double) * mn));
+        LCHK(hipMemcpyAsync(bA.p, a, sizeof(double) * mn, hipMemcpyHostToDevice, st));
+        dA = (const double*)bA.p;
+        LCHK(bU.alloc(sizeof(double) * (size_t)m * k));
+        LCHK(bS.alloc(sizeof(double) * k));
+        LCHK(bVt.alloc(sizeof(double) * (size_t)k * n));
+        dU = (double*)bU.p;
+        dS = (double*)bS.p;
+        dVt = (double*)bVt.p;
     }
     lsvd_info_t inf{};
-    inf.polar_ok = 1;
-    inf.ns_iters = pinf.f64_iters;
-    inf.jacobi_sweeps = sweeps;
-    inf.rotations = rotations;
-    inf.path = 3;
+    bool ok = false;
+    int rc;
+    if (!wide) {
+        rc = svd_large_tall(h, dA, m, n, dU, dS, dVt, &inf, &ok);
+    } else {
+        // A^T (n x m) = U' S Vt'  =>  A = Vt'^T S U'^T: U = Vt'^T (m x m), Vt = U'^T (m x n)
+        LCHK(bAt.alloc(sizeof(double) * mn));
+        LCHK(bTu.alloc(sizeof(double) * (size_t)n * m));
+        LCHK(bTvt.alloc(sizeof(double) * kk));
+        LCHK(large_transpose(dA, m, n, (double*)bAt.p, st));
+        rc = svd_large_tall(h, (const double*)bAt.p, tm, tn, (double*)bTu.p, dS, (double*)bTvt.p, &inf, &ok);
+        if (rc == LSVD_OK && ok) {
+            LCHK(large_transpose((const double*)bTvt.p, m, m, dU, st));
+            LCHK(large_transpose((const double*)bTu.p, n, m, dVt, st));
+        }
+    }
+    if (rc != LSVD_OK || !ok) return rc;
+    if (host) {
+        LCHK(hipMemcpyAsync(u, dU, sizeof(double) * (size_t)m * k, hipMemcpyDeviceToHost, st));
+        LCHK(hipMemcpyAsync(s, dS, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+        LCHK(hipMemcpyAsync(vt, dVt, sizeof(double) * (size_t)k * n, hipMemcpyDeviceToHost, st));
+    }
+    LCHK(hipStreamSynchronize(st));
     rc = put_info(h, info, inf);
     if (rc == LSVD_OK) *done = true;
     return rc;

@@ -40,15 +40,22 @@ hipError_t large_rank(const double* d, int n, int* perm, int* bad, hipStream_t s
 hipError_t large_permute(const double* V, const double* d, const int* perm, int n, double* Vs, double* Vt,
                          double* sigma, bool clamp, hipStream_t st);
 
-// LieSvdSmall::solve of one n x n matrix with every step spread over the
-// chip.  Pointers follow the handle's pointer mode.  Sets *done = false
-// (and returns LSVD_OK) when the polar factor is not trusted or U = Q V
-// fails the orthogonality test: those cases take the reference's repair
-// branch, which the caller runs through the workgroup-per-matrix kernels.
+// The handle's second stream of the route (created on first use).
 hipError_t large_aux_stream(lsvd_handle_t h, hipStream_t* out);
 
-int svd_large_square(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt, lsvd_info_t* info,
-                     bool* done);
+// out (cols x rows) = in^T (in: rows x cols), through LDS tiles.
+hipError_t large_transpose(const double* in, int rows, int cols, double* out, hipStream_t st);
+
+// LieSvdSmall::solve (m == n) or solve_rectangular of one m x n matrix with
+// every step spread over the chip: u m x k, s k, vt k x n (k = min(m, n));
+// a tall matrix runs the polar route on itself (U = Q V is m x k), a wide
+// one is solved transposed.  Pointers follow the handle's pointer mode.
+// Sets *done = false (and returns LSVD_OK) when the polar factor is not
+// trusted or U = Q V fails the orthogonality test: those cases take the
+// reference's repair branch, which the caller runs through the
+// workgroup-per-matrix kernels.
+int svd_large(lsvd_handle_t h, const double* a, int m, int n, double* u, double* s, double* vt, lsvd_info_t* info,
+              bool* done);
 
 // eigh_jacobi_full (src/lie_svd_small.rs:212-285) of one symmetric n x n
 // matrix by the block Jacobi method: v (n x n, eigenvectors as columns)

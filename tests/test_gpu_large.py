@@ -213,3 +213,40 @@ def test_large_eigh_psd_and_degenerate():
     assert np.max(np.abs(e - e2)) < 64 * n * np.finfo(float).eps * 5.0
     vector_parity(v, v2, e2, n, "eigh clusters")
     h_wg.close()
+
+
+@pytest.mark.parametrize("m,n", [(400, 200), (1500, 300), (200, 450), (129, 1000)])
+def test_large_rectangular_vs_oracle(m, n):
+    """solve_rectangular (src/lie_svd_small.rs:340-360) with min(m, n) > 128
+    on the chip-wide route: a tall matrix runs the polar route on itself (the
+    reference reduces it by MGS QR first; both give the thin SVD), a wide one
+    is solved transposed.  Oracle parity with the two-tier sigma bar and
+    per-cluster subspaces, thin shapes, residuals."""
+    a = uniform_matrix(0x5EED0A00 + m + n, m, n)
+    _say("case large rect", m, n)
+    u, s, vt = LieSvdSmall.solve_rectangular(a)
+    info = dict(LieSvdSmall.last_info)
+    k = min(m, n)
+    assert info["path"] == LARGE_PATH, info
+    assert u.shape == (m, k) and s.shape == (k,) and vt.shape == (k, n)
+    uo, so, vto, _ = oracle.small_solve_rectangular(a)
+    assert_factors(a, u, s, vt, 1e-11 * max(1.0, max(m, n) / 32), f"rect {m}x{n}")
+    assert_sigma_close(s, so, max(m, n), what=f"rect {m}x{n}")
+    vector_parity(u, uo, so, k, f"rect {m}x{n} U")
+    vector_parity(vt.T, vto.T, so, k, f"rect {m}x{n} V")
+
+
+def test_large_rectangular_rank_deficient_takes_workgroup_path():
+    """A tall 300 x 150 matrix of rank 100: the polar factor of A itself is
+    not orthogonal on A's null space, so the solve goes to the workgroup
+    kernels (the reference's MGS QR front end and repair)."""
+    m, n, r = 300, 150, 100
+    rng_ = np.random.default_rng(0x5EED0B00)
+    a = rng_.standard_normal((m, r)) @ rng_.standard_normal((r, n))
+    _say("case large rect rank deficient", m, n, r)
+    u, s, vt = LieSvdSmall.solve_rectangular(a)
+    assert LieSvdSmall.last_info["path"] == 0
+    uo, so, vto, _ = oracle.small_solve_rectangular(a)
+    assert_sigma_close(s, so, m, what="rect rank deficient")
+    m_ = np.linalg.norm(u * s[None, :] @ vt - a) / np.linalg.norm(a)
+    assert m_ < 1e-11

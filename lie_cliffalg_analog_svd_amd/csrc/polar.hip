@@ -226,24 +226,61 @@ __global__ __launch_bounds__(256) void gemm_bf16_kernel(int M, int N, int K, con
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
+    // Each thread owns two (row, 8-wide k chunk) items of each operand tile:
+    // it reads the chunk from global memory (float4 pairs where k is
+    // contiguous, otherwise one float per k with the lanes of a wave on
+    // consecutive rows), and stores it to LDS as one packed 16-byte write.
+    // The next K tile is read into registers while the current one is
+    // multiplied out of LDS.
+    float ra[2][8], rb[2][8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int id = t + 256 * h;
+            if (TRANS_A) {  // A stored K x M: i contiguous
+                const int ii = id % BM, c = id / BM, gi = i0 + ii;
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int gk = k0 + 8 * c + q;
+                    ra[h][q] = (gk < K && gi < M) ? A[(size_t)gk * M + gi] : 0.0f;
+                }
+            } else {  // A stored M x K: k contiguous
+                const int ii = id / 4, c = id % 4, gi = i0 + ii, gk = k0 + 8 * c;
+                if (gi < M && gk + 8 <= K && (K & 3) == 0) {
+                    const float4 v0 = *reinterpret_cast<const float4*>(&A[(size_t)gi * K + gk]);
+                    const float4 v1 = *reinterpret_cast<const float4*>(&A[(size_t)gi * K + gk + 4]);
+                    ra[h][0] = v0.x; ra[h][1] = v0.y; ra[h][2] = v0.z; ra[h][3] = v0.w;
+                    ra[h][4] = v1.x; ra[h][5] = v1.y; ra[h][6] = v1.z; ra[h][7] = v1.w;
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 8; ++q) ra[h][q] = (gi < M && gk + q < K) ? A[(size_t)gi * K + gk + q] : 0.0f;
+                }
+            }
+            const int jj = id % BN, c = id / BN, gj = j0 + jj;  // B stored K x N: j contiguous
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                const int gk = k0 + 8 * c + q;
+                rb[h][q] = (gk < K && gj < N) ? B[(size_t)gk * N + gj] : 0.0f;
+            }
+        }
+    };
+    gload(0);
     for (int k0 = 0; k0 < K; k0 += BK) {
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-            const int e_ = t + 256 * r;
-            if (TRANS_A) {
-                const int kk = e_ / BM, ii = e_ % BM;
-                const int gk = k0 + kk, gi = i0 + ii;
-                As[ii * LDK + kk] = (__bf16)((gk < K && gi < M) ? A[(size_t)gk * M + gi] : 0.0f);
-            } else {
-                const int ii = e_ / BK, kk = e_ % BK;
-                const int gk = k0 + kk, gi = i0 + ii;
-                As[ii * LDK + kk] = (__bf16)((gk < K && gi < M) ? A[(size_t)gi * K + gk] : 0.0f);
+        for (int h = 0; h < 2; ++h) {
+            const int id = t + 256 * h;
+            const int ar = TRANS_A ? id % BM : id / 4, ac = TRANS_A ? id / BM : id % 4;
+            bf8 pa, pb;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                pa[q] = (__bf16)ra[h][q];
+                pb[q] = (__bf16)rb[h][q];
             }
-            const int kk = e_ / BN, jj = e_ % BN;
-            const int gk = k0 + kk, gj = j0 + jj;
-            Bs[jj * LDK + kk] = (__bf16)((gk < K && gj < N) ? B[(size_t)gk * N + gj] : 0.0f);
+            *reinterpret_cast<bf8*>(&As[ar * LDK + 8 * ac]) = pa;
+            *reinterpret_cast<bf8*>(&Bs[(id % BN) * LDK + 8 * (id / BN)]) = pb;
         }
         __syncthreads();
+        if (k0 + BK < K) gload(k0 + BK);
         const int kq = 8 * (lane >> 4), c16 = lane & 15;
         bf8 av[4], bv[4];
 #pragma unroll

@@ -97,3 +97,34 @@ def test_product_does_not_import_oracle():
         assert "import oracle" not in text and "from oracle" not in text, f
     for f in glob.glob(os.path.join(pkg, "csrc", "*")):
         assert "lsvd_oracle" not in open(f).read(), f
+
+
+def test_missing_library_fails_loudly():
+    """With the shared library absent the binding raises ImportError naming
+    the build command; there is no CPU path to fall back to."""
+    import subprocess
+    import sys
+    code = ("import sys; sys.path.insert(0, %r)\n"
+            "from lie_cliffalg_analog_svd_amd import _lib\n"
+            "_lib.LIB_PATH = '/nonexistent/liblsvd_amd.so'\n"
+            "try:\n    _lib.lib()\nexcept ImportError as e:\n    print('IMPORT_ERROR', 'no CPU fallback' in str(e))\n"
+            % ROOT)
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=120)
+    assert "IMPORT_ERROR True" in out.stdout, out.stdout + out.stderr
+
+
+def test_solvers_raise_without_a_gpu():
+    """Where no GPU is visible the solvers raise (handle creation fails)
+    instead of computing anything on the host."""
+    import ctypes
+    h = ctypes.c_void_p()
+    if _lib.lib().lsvd_create(0, ctypes.byref(h)) == _lib.LSVD_OK:
+        _lib.lib().lsvd_destroy(h)
+        pytest.skip("a GPU is visible")
+    from lie_cliffalg_analog_svd_amd import LieSvdSmall, LieSvdTensor, Handle
+    with pytest.raises(_lib.LsvdError):
+        Handle()
+    with pytest.raises(_lib.LsvdError):
+        LieSvdSmall.solve(np.eye(3))
+    with pytest.raises(_lib.LsvdError):
+        LieSvdTensor.hosvd3(np.ones((2, 2, 2)))

@@ -74,9 +74,12 @@ namespace {
 // test before every sweep: upper off-diagonal norm below JACOBI_TOL
 // ||P||_F.  A sweep that no longer lowers the off-diagonal norm (the
 // rounding floor) also ends the iteration.  Writes the unsorted diagonal
-// dDiag (n) and the eigenvector columns dVc (n x n); synchronous.
+// dDiag (n) and the eigenvector columns dVc (n x n); synchronous.  A
+// non-finite dP sets *finite = false and writes nothing (the caller hands
+// such input to the workgroup kernels).
 int run_block_jacobi(lsvd_handle_t h, const double* dP, int n, double* dDiag, double* dVc, int* sweeps_out,
-                     int* rotations_out) {
+                     int* rotations_out, bool* finite) {
+    *finite = false;
     hipStream_t st = h->stream;
     const size_t nn = (size_t)n * n;
     const int bb = bj_tile_size(n);
@@ -112,6 +115,8 @@ int run_block_jacobi(lsvd_handle_t h, const double* dP, int n, double* dDiag, do
     double hq[2];
     LCHK(hipMemcpyAsync(hq, dsq, sizeof(hq), hipMemcpyDeviceToHost, st));
     LCHK(hipStreamSynchronize(st));
+    if (!std::isfinite(hq[0]) || !std::isfinite(hq[1])) return LSVD_OK;
+    *finite = true;
     const double jtol = 1e-13 * std::max(std::sqrt(hq[0]), 1e-300);
     double off = std::sqrt(hq[1]);
     const int max_sweeps = 60 + 20 * n;
@@ -194,8 +199,9 @@ int svd_large_tall(lsvd_handle_t h, const double* dA, int m, int n, double* dU, 
     double* dDiag = (double*)bDiag.p;
     double* dVc = (double*)bVc.p;
     int sweeps = 0, rotations = 0;
-    rc = run_block_jacobi(h, dP, n, dDiag, dVc, &sweeps, &rotations);
-    if (rc != LSVD_OK) return rc;
+    bool finite = false;
+    rc = run_block_jacobi(h, dP, n, dDiag, dVc, &sweeps, &rotations, &finite);
+    if (rc != LSVD_OK || !finite) return rc;
 
     LCHK(bPerm.alloc(sizeof(int) * n));
     LCHK(bScal.alloc(sizeof(double) * 2));
@@ -299,8 +305,9 @@ int eigh_large(lsvd_handle_t h, const double* p, int n, double* v, double* eig, 
     LCHK(bDiag.alloc(sizeof(double) * n));
     LCHK(bVc.alloc(sizeof(double) * nn));
     int sweeps = 0, rotations = 0;
-    int rc = run_block_jacobi(h, dP, n, (double*)bDiag.p, (double*)bVc.p, &sweeps, &rotations);
-    if (rc != LSVD_OK) return rc;
+    bool finite = false;
+    int rc = run_block_jacobi(h, dP, n, (double*)bDiag.p, (double*)bVc.p, &sweeps, &rotations, &finite);
+    if (rc != LSVD_OK || !finite) return rc;  // non-finite input: the workgroup kernels take it
     LCHK(bPerm.alloc(sizeof(int) * n));
     LCHK(bFlag.alloc(sizeof(double)));
     LCHK(polar_zero((double*)bFlag.p, 1, st));

@@ -250,3 +250,25 @@ def test_large_rectangular_rank_deficient_takes_workgroup_path():
     assert_sigma_close(s, so, m, what="rect rank deficient")
     m_ = np.linalg.norm(u * s[None, :] @ vt - a) / np.linalg.norm(a)
     assert m_ < 1e-11
+
+
+def test_large_route_nonfinite_and_zero_inputs():
+    """Inputs the polar step cannot trust take the workgroup kernels' path
+    (path 0), as the reference's Gram-Schmidt fallback and repair do: a NaN
+    entry (the polar iteration flags the non-finite iterate) and the zero
+    matrix (X stays 0, so ||Q^T Q - I|| = sqrt(n)).  The eigh route hands a
+    NaN diagonal to the workgroup kernels the same way."""
+    from lie_cliffalg_analog_svd_amd import eigh_jacobi_full
+    n = 200
+    _say("case large nonfinite/zero", n)
+    a = uniform_matrix(0x5EED0C00, n, n)
+    a[3, 7] = np.nan
+    u, s, vt = LieSvdSmall.solve(a)
+    assert LieSvdSmall.last_info["path"] == 0
+    z = np.zeros((n, n))
+    u, s, vt = LieSvdSmall.solve(z)
+    assert LieSvdSmall.last_info["path"] == 0 and np.all(s == 0)
+    assert np.all(np.isfinite(u)) and np.all(np.isfinite(vt))
+    p = a + a.T
+    v, e = eigh_jacobi_full(p)
+    assert eigh_jacobi_full.last_info["path"] == 0

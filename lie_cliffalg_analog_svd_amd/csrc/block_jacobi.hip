@@ -77,17 +77,11 @@ __global__ __launch_bounds__(16 * BB) void bj_inner_kernel(const double* A, int 
                 q = b + (tid + il) % b;
             }
             const double apq = S[p * L + q], app = S[p * L + p], aqq = S[q * L + q];
-            double c = 1.0, s = 0.0;
-            int f = 0;
-            if (fabs(apq) >= 1e-300) {  // (:229-231)
-                const double theta = (aqq - app) / (2.0 * apq);
-                const double t = theta >= 0.0 ? 1.0 / (theta + sqrt(theta * theta + 1.0))
-                                              : -1.0 / (-theta + sqrt(theta * theta + 1.0));
-                c = 1.0 / sqrt(t * t + 1.0);
-                s = t * c;
+            double c, s, t;
+            const int f = jacobi_angle(app, aqq, apq, true, c, s, t);  // (:229-235)
+            if (f) {
                 rdp[tid] = app - t * apq;  // (:236-241)
                 rdq[tid] = aqq + t * apq;
-                f = 1;
                 ++mine;
             }
             rc[tid] = c;

@@ -203,4 +203,34 @@ __device__ __forceinline__ void rr_pair(int n, int layer, int k, int& p, int& q)
 
 __device__ __forceinline__ int rr_layers(int n) { return (n & 1) ? n : n - 1; }
 
+// Jacobi rotation of one pair (src/lie_svd_small.rs:217-243) from its 2x2
+// diagonal block in slot order (x00 at the slot's first index; fwd: that
+// index is the reference's p < q); c, s and t are oriented to the slot
+// order.  The reference's theta = d / (2 a_pq), t = sgn(theta) /
+// (|theta| + sqrt(theta^2 + 1)), c = 1 / sqrt(t^2 + 1), s = t c are
+// evaluated as t = sgn |2 a_pq| / (|d| + r), c = (|d| + r) / h,
+// s = sgn |2 a_pq| / h with r = sqrt(d^2 + 4 a_pq^2), h = sqrt((|d| + r)^2
+// + 4 a_pq^2): the same quantities (within a few ulps) with a shorter
+// dependent chain (one sqrt, then a division and a reciprocal square root
+// side by side).  d and 2 a_pq are first scaled by the same power of two, so
+// the squares cannot overflow or underflow.
+__device__ __forceinline__ int jacobi_angle(double x00, double x11, double apq, bool fwd, double& c, double& sv,
+                                            double& tv) {
+    const bool rot = fabs(apq) >= 1e-300;
+    const double app = fwd ? x00 : x11, aqq = fwd ? x11 : x00;
+    const double d = aqq - app, a2 = 2.0 * apq;
+    const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(d), fabs(a2)));
+    const double ds = ldexp(d, -ex), as = fabs(ldexp(a2, -ex));
+    const double r = sqrt(fma(ds, ds, as * as));
+    const double den = fabs(ds) + r;
+    const double sgn = (d == 0.0 || ((d > 0.0) == (a2 > 0.0))) ? 1.0 : -1.0;  // sgn(theta), theta = +-0 -> +
+    const double tt = sgn * as / den;
+    const double ih = rsqrt(fma(den, den, as * as));
+    c = rot ? den * ih : 1.0;
+    const double sn = rot ? sgn * as * ih : 0.0;
+    sv = fwd ? sn : -sn;
+    tv = rot ? (fwd ? tt : -tt) : 0.0;
+    return rot ? 1 : 0;
+}
+
 }  // namespace lsvd

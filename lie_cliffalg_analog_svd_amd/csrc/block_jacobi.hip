@@ -15,7 +15,8 @@
 //          pairs inside each block); in the other layers only the pairs
 //          across I x J (b inner layers, pair (i, b + (i + l) mod b)), since
 //          the pairs inside a block were taken in the first layer;
-//   update (one workgroup per tile): A <- G^T A G and V <- V G, where G is
+//   update (one launch, one workgroup per tile): A <- G^T A G and V <- V G,
+//          where G is
 //          W of every block pair on its index set.  Tile (P, Q) of A (rows
 //          I u J of pair P, columns of pair Q) becomes W_P^T A_PQ W_Q, two
 //          BB^3 products on v_mfma_f64_16x16x4f64 with the operands in LDS;
@@ -58,10 +59,19 @@ __global__ __launch_bounds__(16 * BB) void bj_inner_kernel(const double* A, int 
     __shared__ int rp[NP], rq[NP], rf[NP], nrot;
     const int slot = blockIdx.x, tid = threadIdx.x;
     if (tid == 0) nrot = 0;
-    for (int e = tid; e < BB * BB; e += NT) {
-        const int r = e / BB, c = e % BB;
+    constexpr int PER = BB * BB / NT;
+    static_assert(BB * BB % NT == 0, "tile entries per thread");
+    double ra[PER];  // all loads in flight before the first LDS store
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * NT, r = e / BB, c = e % BB;
         const int gr = bj_index(nb, b, layer, slot, r), gc = bj_index(nb, b, layer, slot, c);
-        S[r * L + c] = A[(size_t)gr * ld + gc];
+        ra[i] = A[(size_t)gr * ld + gc];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * NT, r = e / BB, c = e % BB;
+        S[r * L + c] = ra[i];
         W[r * L + c] = r == c ? 1.0 : 0.0;
     }
     __syncthreads();
@@ -165,23 +175,23 @@ __device__ __forceinline__ void bj_tile_gemm(const double* X, const double* Y, d
     }
 }
 
-// vmode 0: grid of B (B + 1) / 2 A tiles (P <= Q); vmode 1: grid of
-// (n_pad / BB) * B V tiles.
+// One launch per layer: workgroups 0 .. B (B + 1) / 2 - 1 take the A tiles
+// (P <= Q), the next (n_pad / BB) B the V tiles (rows R, pair P's columns).
 template <int BB>
 __global__ __launch_bounds__(256) void bj_update_kernel(double* A, double* V, int ld, int nb, int layer,
-                                                        const double* Wbuf, int vmode) {
-    constexpr int b = BB / 2, L = BB + 1;
+                                                        const double* Wbuf) {
+    constexpr int b = BB / 2, L = BB + 1, UT = 256, PER = BB * BB / UT;
+    static_assert(BB * BB % UT == 0, "tile entries per thread");
     extern __shared__ double lds[];
     double* X = lds;            // tile of A or V
     double* WP = X + BB * L;
     double* T = WP + BB * L;
-    double* WQ = T + BB * L;    // A tiles only (the V launch allocates three buffers)
+    double* WQ = T + BB * L;    // A tiles only
     const int B = nb / 2;
     const int nA = B * (B + 1) / 2;
     const int tid = threadIdx.x;
     int t = blockIdx.x;
-    if (!vmode) {
-        if (t >= nA) return;
+    if (t < nA) {
         // upper-triangle tile t -> (P, Q), P <= Q
         int P = 0;
         while (t >= B - P) {
@@ -189,19 +199,29 @@ __global__ __launch_bounds__(256) void bj_update_kernel(double* A, double* V, in
             ++P;
         }
         const int Q = P + t;
-        for (int e = tid; e < BB * BB; e += 256) {
-            const int r = e / BB, c = e % BB;
+        // all loads issued before the first LDS store, so their latencies overlap
+        double ra[PER], rp[PER], rq[PER];
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + i * UT, r = e / BB, c = e % BB;
             const int gr = bj_index(nb, b, layer, P, r), gc = bj_index(nb, b, layer, Q, c);
-            X[r * L + c] = A[(size_t)gr * ld + gc];
-            WP[r * L + c] = Wbuf[((size_t)P * BB + r) * BB + c];
-            WQ[r * L + c] = Wbuf[((size_t)Q * BB + r) * BB + c];
+            ra[i] = A[(size_t)gr * ld + gc];
+            rp[i] = Wbuf[((size_t)P * BB + r) * BB + c];
+            rq[i] = Wbuf[((size_t)Q * BB + r) * BB + c];
+        }
+#pragma unroll
+        for (int i = 0; i < PER; ++i) {
+            const int e = tid + i * UT, r = e / BB, c = e % BB;
+            X[r * L + c] = ra[i];
+            WP[r * L + c] = rp[i];
+            WQ[r * L + c] = rq[i];
         }
         __syncthreads();
         bj_tile_gemm<BB, false>(X, WQ, T, L);  // T = A_PQ W_Q
         __syncthreads();
         bj_tile_gemm<BB, true>(WP, T, X, L);   // X = W_P^T T
         __syncthreads();
-        for (int e = tid; e < BB * BB; e += 256) {
+        for (int e = tid; e < BB * BB; e += UT) {
             const int r = e / BB, c = e % BB;
             const int gr = bj_index(nb, b, layer, P, r), gc = bj_index(nb, b, layer, Q, c);
             double v;
@@ -216,18 +236,27 @@ __global__ __launch_bounds__(256) void bj_update_kernel(double* A, double* V, in
         }
         return;
     }
+    t -= nA;
     const int P = t % B, R = t / B;  // V rows R*BB .. R*BB + BB - 1, pair P's columns
     if (R >= ld / BB) return;
-    for (int e = tid; e < BB * BB; e += 256) {
-        const int r = e / BB, c = e % BB;
+    double rv[PER], rp[PER];
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * UT, r = e / BB, c = e % BB;
         const int gc = bj_index(nb, b, layer, P, c);
-        X[r * L + c] = V[(size_t)(R * BB + r) * ld + gc];
-        WP[r * L + c] = Wbuf[((size_t)P * BB + r) * BB + c];
+        rv[i] = V[(size_t)(R * BB + r) * ld + gc];
+        rp[i] = Wbuf[((size_t)P * BB + r) * BB + c];
+    }
+#pragma unroll
+    for (int i = 0; i < PER; ++i) {
+        const int e = tid + i * UT, r = e / BB, c = e % BB;
+        X[r * L + c] = rv[i];
+        WP[r * L + c] = rp[i];
     }
     __syncthreads();
     bj_tile_gemm<BB, false>(X, WP, T, L);  // T = V_RP W_P
     __syncthreads();
-    for (int e = tid; e < BB * BB; e += 256) {
+    for (int e = tid; e < BB * BB; e += UT) {
         const int r = e / BB, c = e % BB;
         const int gc = bj_index(nb, b, layer, P, c);
         V[(size_t)(R * BB + r) * ld + gc] = T[r * L + c];
@@ -278,7 +307,7 @@ __global__ void bj_extract_kernel(const double* A, const double* V, int n, int l
 template <int BB>
 constexpr size_t bj_inner_lds() { return sizeof(double) * 2 * BB * (BB + 1); }
 template <int BB>
-constexpr size_t bj_update_lds(bool vmode) { return sizeof(double) * (vmode ? 3 : 4) * BB * (BB + 1); }
+constexpr size_t bj_update_lds() { return sizeof(double) * 4 * BB * (BB + 1); }
 
 template <int BB>
 hipError_t bj_prepare_bb() {
@@ -286,51 +315,47 @@ hipError_t bj_prepare_bb() {
                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)bj_inner_lds<BB>());
     if (e != hipSuccess) return e;
     return hipFuncSetAttribute(reinterpret_cast<const void*>(&bj_update_kernel<BB>),
-                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bj_update_lds<BB>(false));
+                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bj_update_lds<BB>());
 }
 
 template <int BB>
-hipError_t bj_launch_bb(const BlockJacobi& j, int what, int layer, const double* W, hipStream_t st) {
+hipError_t bj_launch_bb(const BlockJacobi& j, bool inner, int layer, double* W, hipStream_t st) {
     const int B = j.nb / 2;
-    if (what == 0) {
+    if (inner) {
         hipLaunchKernelGGL(bj_inner_kernel<BB>, dim3(B), dim3(bj_inner_threads<BB>()), bj_inner_lds<BB>(), st, j.A,
-                           j.ld, j.nb, layer, const_cast<double*>(W), j.rotations);
-    } else if (what == 1) {
-        hipLaunchKernelGGL(bj_update_kernel<BB>, dim3(B * (B + 1) / 2), dim3(256), bj_update_lds<BB>(false), st, j.A,
-                           j.V, j.ld, j.nb, layer, W, 0);
+                           j.ld, j.nb, layer, W, j.rotations);
     } else {
-        hipLaunchKernelGGL(bj_update_kernel<BB>, dim3((j.ld / BB) * B), dim3(256), bj_update_lds<BB>(true), st, j.A,
-                           j.V, j.ld, j.nb, layer, W, 1);
+        hipLaunchKernelGGL(bj_update_kernel<BB>, dim3(B * (B + 1) / 2 + (j.ld / BB) * B), dim3(256),
+                           bj_update_lds<BB>(), st, j.A, j.V, j.ld, j.nb, layer, W);
     }
     return hipGetLastError();
 }
 
-hipError_t bj_launch(const BlockJacobi& j, int what, int layer, const double* W, hipStream_t st) {
+hipError_t bj_launch(const BlockJacobi& j, bool inner, int layer, double* W, hipStream_t st) {
     switch (j.bb) {
-        case 16: return bj_launch_bb<16>(j, what, layer, W, st);
-        case 32: return bj_launch_bb<32>(j, what, layer, W, st);
-        case 64: return bj_launch_bb<64>(j, what, layer, W, st);
+        case 16: return bj_launch_bb<16>(j, inner, layer, W, st);
+        case 32: return bj_launch_bb<32>(j, inner, layer, W, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-int bj_tile_size(int n) { return n <= 256 ? 16 : (n <= 512 ? 32 : 64); }
+int bj_tile_size(int n) { return n <= 256 ? 16 : 32; }
 
 hipError_t bj_setup(const BlockJacobi& j, const double* P, hipStream_t st) {
-    hipError_t e = j.bb == 16 ? bj_prepare_bb<16>() : (j.bb == 32 ? bj_prepare_bb<32>() : bj_prepare_bb<64>());
+    if (j.bb != 16 && j.bb != 32) return hipErrorInvalidValue;
+    hipError_t e = j.bb == 16 ? bj_prepare_bb<16>() : bj_prepare_bb<32>();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(bj_setup_kernel, dim3(512), dim3(256), 0, st, P, j.n, j.ld, j.A, j.V);
     return hipGetLastError();
 }
 
-hipError_t bj_inner(const BlockJacobi& j, int layer, double* W, hipStream_t st) { return bj_launch(j, 0, layer, W, st); }
-hipError_t bj_update_a(const BlockJacobi& j, int layer, const double* W, hipStream_t st) {
-    return bj_launch(j, 1, layer, W, st);
+hipError_t bj_inner(const BlockJacobi& j, int layer, double* W, hipStream_t st) {
+    return bj_launch(j, true, layer, W, st);
 }
-hipError_t bj_update_v(const BlockJacobi& j, int layer, const double* W, hipStream_t st) {
-    return bj_launch(j, 2, layer, W, st);
+hipError_t bj_update(const BlockJacobi& j, int layer, const double* W, hipStream_t st) {
+    return bj_launch(j, false, layer, const_cast<double*>(W), st);
 }
 
 hipError_t bj_offnorm(const BlockJacobi& j, double* out, hipStream_t st) {

@@ -571,7 +571,6 @@ int lsvd_destroy(lsvd_handle_t h) {
     lsvd_release_extras(h);
     if (h->own_stream) hipStreamDestroy(h->own_stream);
     for (hipStream_t s2 : h->lanes) hipStreamDestroy(s2);
-    if (h->aux_stream) hipStreamDestroy(h->aux_stream);
     delete h;
     return LSVD_OK;
 }

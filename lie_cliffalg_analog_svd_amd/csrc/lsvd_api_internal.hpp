@@ -54,7 +54,6 @@ struct lsvd_handle_s {
     std::map<uint64_t, int> tune;                 // autotune winners: bucket key -> variant
     std::map<uint64_t, lsvd_plan_s*> plan_cache;  // one-item plans of the single-matrix API
     int large_min_n = 129;                        // square single solves from this n run chip-wide
-    hipStream_t aux_stream = nullptr;             // second stream of the chip-wide route
     lsvd::Extras* extras = nullptr;
 };
 

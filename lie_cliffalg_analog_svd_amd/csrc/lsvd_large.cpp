@@ -30,36 +30,7 @@ struct Mem {
     }
 };
 
-struct Events {
-    hipEvent_t wready[2] = {nullptr, nullptr}, vdone[2] = {nullptr, nullptr};
-    hipError_t create() {
-        for (int i = 0; i < 2; ++i) {
-            hipError_t e = hipEventCreateWithFlags(&wready[i], hipEventDisableTiming);
-            if (e == hipSuccess) e = hipEventCreateWithFlags(&vdone[i], hipEventDisableTiming);
-            if (e != hipSuccess) return e;
-        }
-        return hipSuccess;
-    }
-    ~Events() {
-        for (int i = 0; i < 2; ++i) {
-            if (wready[i]) hipEventDestroy(wready[i]);
-            if (vdone[i]) hipEventDestroy(vdone[i]);
-        }
-    }
-};
-
 }  // namespace
-
-// The handle's second stream for this route (created on first use, default
-// priority, destroyed with the handle).
-hipError_t large_aux_stream(lsvd_handle_t h, hipStream_t* out) {
-    if (!h->aux_stream) {
-        hipError_t e = hipStreamCreateWithFlags(&h->aux_stream, hipStreamNonBlocking);
-        if (e != hipSuccess) return e;
-    }
-    *out = h->aux_stream;
-    return hipSuccess;
-}
 
 #define LCHK(expr)                                          \
     do {                                                    \
@@ -90,24 +61,12 @@ int run_block_jacobi(lsvd_handle_t h, const double* dP, int n, double* dDiag, do
     Mem bApad, bVpad, bWrot, bPart, bRot;
     LCHK(bApad.alloc(sizeof(double) * ll));
     LCHK(bVpad.alloc(sizeof(double) * ll));
-    LCHK(bWrot.alloc(sizeof(double) * 2 * wsz));
+    LCHK(bWrot.alloc(sizeof(double) * wsz));
     LCHK(bPart.alloc(sizeof(double) * (kBjOffBlocks + 2)));
     LCHK(bRot.alloc(sizeof(int)));
     BlockJacobi bj{n, ld, 2 * nbp, bb, (double*)bApad.p, (double*)bVpad.p, (double*)bPart.p, (int*)bRot.p};
-    double* wbuf[2] = {(double*)bWrot.p, (double*)bWrot.p + wsz};
+    double* wrot = (double*)bWrot.p;
     double* dsq = (double*)bPart.p + kBjOffBlocks;  // [0] ||P||_F^2, [1] off-diagonal^2
-    // With 64-wide tiles (n > 512) V <- V G runs on a second stream,
-    // overlapping the next layer's inner step; layer g uses W buffer g % 2,
-    // so the inner step of layer g + 2 waits for the V update of layer g.
-    // Smaller tiles keep one stream: there the events cost more host time
-    // per layer than the overlap saves.
-    const bool overlap = bb >= 64;
-    hipStream_t vs = nullptr;
-    Events ev;
-    if (overlap) {
-        LCHK(large_aux_stream(h, &vs));
-        LCHK(ev.create());
-    }
     LCHK(hipMemsetAsync(bRot.p, 0, sizeof(int), st));
     LCHK(bj_setup(bj, dP, st));
     LCHK(polar_sumsq(dP, nn, dsq, bj.partials, st));
@@ -121,23 +80,10 @@ int run_block_jacobi(lsvd_handle_t h, const double* dP, int n, double* dDiag, do
     double off = std::sqrt(hq[1]);
     const int max_sweeps = 60 + 20 * n;
     int sweeps = 0;
-    long g = 0;
     while (sweeps < max_sweeps && off >= jtol) {
-        for (int layer = 0; layer < bj.nb - 1; ++layer, ++g) {
-            if (!overlap) {
-                LCHK(bj_inner(bj, layer, wbuf[0], st));
-                LCHK(bj_update_a(bj, layer, wbuf[0], st));
-                LCHK(bj_update_v(bj, layer, wbuf[0], st));
-                continue;
-            }
-            const int w = (int)(g & 1);
-            if (g >= 2) LCHK(hipStreamWaitEvent(st, ev.vdone[w], 0));
-            LCHK(bj_inner(bj, layer, wbuf[w], st));
-            LCHK(hipEventRecord(ev.wready[w], st));
-            LCHK(bj_update_a(bj, layer, wbuf[w], st));
-            LCHK(hipStreamWaitEvent(vs, ev.wready[w], 0));
-            LCHK(bj_update_v(bj, layer, wbuf[w], vs));
-            LCHK(hipEventRecord(ev.vdone[w], vs));
+        for (int layer = 0; layer < bj.nb - 1; ++layer) {
+            LCHK(bj_inner(bj, layer, wrot, st));
+            LCHK(bj_update(bj, layer, wrot, st));
         }
         LCHK(bj_offnorm(bj, dsq + 1, st));
         LCHK(hipMemcpyAsync(hq, dsq, sizeof(hq), hipMemcpyDeviceToHost, st));
@@ -148,7 +94,6 @@ int run_block_jacobi(lsvd_handle_t h, const double* dP, int n, double* dDiag, do
         off = after;
         if (stalled) break;
     }
-    if (overlap && g > 0) LCHK(hipStreamWaitEvent(st, ev.vdone[(g - 1) & 1], 0));  // V complete
     int rotations = 0;
     LCHK(hipMemcpyAsync(&rotations, bRot.p, sizeof(int), hipMemcpyDeviceToHost, st));
     LCHK(bj_extract(bj, dDiag, dVc, st));

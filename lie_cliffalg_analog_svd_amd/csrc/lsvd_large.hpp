@@ -25,11 +25,10 @@ struct BlockJacobi {
 constexpr int kBjOffBlocks = 256;
 int bj_tile_size(int n);
 hipError_t bj_setup(const BlockJacobi& j, const double* P, hipStream_t st);
-// One layer: inner (A -> W), then A <- G^T A G (reads W) and V <- V G
-// (reads W; independent of A, so it may run on another stream).
+// One layer: inner (A -> W), then A <- G^T A G and V <- V G in one launch
+// (reads W).
 hipError_t bj_inner(const BlockJacobi& j, int layer, double* W, hipStream_t st);
-hipError_t bj_update_a(const BlockJacobi& j, int layer, const double* W, hipStream_t st);
-hipError_t bj_update_v(const BlockJacobi& j, int layer, const double* W, hipStream_t st);
+hipError_t bj_update(const BlockJacobi& j, int layer, const double* W, hipStream_t st);
 // *out = sum of squares of the upper off-diagonal entries (overwritten).
 hipError_t bj_offnorm(const BlockJacobi& j, double* out, hipStream_t st);
 hipError_t bj_extract(const BlockJacobi& j, double* d, double* Vc, hipStream_t st);
@@ -39,9 +38,6 @@ hipError_t large_rank(const double* d, int n, int* perm, int* bad, hipStream_t s
 // Vt may be null; clamp: sigma = max(d, 0) (the SVD) instead of d (eigh).
 hipError_t large_permute(const double* V, const double* d, const int* perm, int n, double* Vs, double* Vt,
                          double* sigma, bool clamp, hipStream_t st);
-
-// The handle's second stream of the route (created on first use).
-hipError_t large_aux_stream(lsvd_handle_t h, hipStream_t* out);
 
 // out (cols x rows) = in^T (in: rows x cols), through LDS tiles.
 hipError_t large_transpose(const double* in, int rows, int cols, double* out, hipStream_t st);

@@ -22,6 +22,23 @@ struct Buf {
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
 };
 
+// Pinned host slots and one event per slot for the stop-test reads of the
+// plain FP64 steps.
+struct StepSlots {
+    double* host = nullptr;  // 2 slots x 4 doubles: ||dX||^2, unused, bad flag (as int), unused
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    hipError_t create() {
+        hipError_t e = hipHostMalloc((void**)&host, sizeof(double) * 8, hipHostMallocDefault);
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipEventCreateWithFlags(&ev[i], hipEventDisableTiming);
+        return e;
+    }
+    ~StepSlots() {
+        for (hipEvent_t e : ev)
+            if (e) hipEventDestroy(e);
+        if (host) hipHostFree(host);
+    }
+};
+
 }  // namespace
 
 namespace lsvd {
@@ -33,7 +50,7 @@ int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, do
     const bool lowp = precision == LSVD_PREC_F32 || precision == LSVD_PREC_BF16;
     const int lp = precision == LSVD_PREC_F32 ? kPolF32 : kPolBF16;
     lsvd_polar_info_t inf{};
-    Buf bX, bX2, bM, bXf, bXf2, bMf, bS;
+    Buf bX, bX2, bX3, bM, bXf, bXf2, bMf, bS;
 #define PCHK(expr)                                          \
     do {                                                    \
         hipError_t e_ = (expr);                             \
@@ -43,7 +60,7 @@ int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, do
     PCHK(bX.alloc(sizeof(double) * mn));
     PCHK(bX2.alloc(sizeof(double) * mn));
     PCHK(bM.alloc(sizeof(double) * nn));
-    PCHK(bS.alloc(sizeof(double) * (4 + kSumsqBlocks)));
+    PCHK(bS.alloc(sizeof(double) * (4 + kSumsqBlocks + 8)));
     if (lowp) {
         PCHK(bXf.alloc(sizeof(float) * mn));
         PCHK(bXf2.alloc(sizeof(float) * mn));
@@ -113,26 +130,75 @@ int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, do
     const int budget = 120;
     NsSchedule sched;  // plain steps unless LSVD_PREC_F64_SCALED
     sched.init(precision == LSVD_PREC_F64_SCALED);
-    for (int it = 0; it < budget; ++it) {
-        PCHK(polar_zero(S + 1, 3, st));
-        PCHK(ns_gram(kPolF64, m, n, X, bM.p, sched.enabled ? S + 2 : nullptr, st, sched.alpha()));
-        PCHK(ns_update(kPolF64, m, n, X, bM.p, Xn, S + 1, bad, st, sched.c()));
-        PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
-        PCHK(hipStreamSynchronize(st));
-        ++it64;
-        delta = std::sqrt(hs[1]);
-        int b = 0;
-        std::memcpy(&b, &hs[3], sizeof(int));
-        if (b || delta > 1e6 * rootn) {  // (:169-171); the Gram-Schmidt fallback is not run at this size
-            bad64 = true;
-            break;
+    if (!sched.enabled) {
+        // Plain steps, one step ahead of the stop test: step k + 1 is queued
+        // before the host reads step k's ||dX||, so the GPU does not idle
+        // through that read.  Iterate X_k lives in buffer k % 3, so the
+        // queued step (X_{k+1} -> X_{k+2}) leaves X_{k+1}, the result when
+        // step k meets the stop test, in place; step k uses slot k % 2 of
+        // the device and host stop-test slots.  The iterates and the step
+        // count are those of the one-step-at-a-time loop.
+        PCHK(bX3.alloc(sizeof(double) * mn));
+        StepSlots ss;
+        PCHK(ss.create());
+        double* bufs[3] = {X, Xn, (double*)bX3.p};
+        double* T = S + 4 + kSumsqBlocks;  // device slots, 4 doubles each
+        auto launch = [&](int k) -> hipError_t {
+            double* t = T + 4 * (k & 1);
+            hipError_t e = polar_zero(t, 4, st);
+            if (e == hipSuccess) e = ns_gram(kPolF64, m, n, bufs[k % 3], bM.p, nullptr, st, sched.alpha());
+            if (e == hipSuccess)
+                e = ns_update(kPolF64, m, n, bufs[k % 3], bM.p, bufs[(k + 1) % 3], t, (int*)(t + 2), st, sched.c());
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(ss.host + 4 * (k & 1), t, sizeof(double) * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipEventRecord(ss.ev[k & 1], st);
+            return e;
+        };
+        PCHK(launch(0));
+        int k = 0;
+        X = bufs[budget % 3];  // the iterate when the budget runs out
+        for (; k < budget; ++k) {
+            if (k + 1 < budget) PCHK(launch(k + 1));
+            PCHK(hipEventSynchronize(ss.ev[k & 1]));
+            const double* hsk = ss.host + 4 * (k & 1);
+            ++it64;
+            delta = std::sqrt(hsk[0]);
+            int b = 0;
+            std::memcpy(&b, &hsk[2], sizeof(int));
+            if (b || delta > 1e6 * rootn) {  // (:169-171)
+                bad64 = true;
+                X = bufs[k % 3];
+                break;
+            }
+            if (delta < 1e-13 * rootn) {
+                converged = true;
+                X = bufs[(k + 1) % 3];
+                break;
+            }
         }
-        std::swap(X, Xn);
-        const bool plain = sched.plain();
-        sched.after(hs[2], delta);
-        if (plain && delta < 1e-13 * rootn) {
-            converged = true;
-            break;
+        PCHK(hipStreamSynchronize(st));  // the queued step's copy into ss.host has landed
+    } else {
+        for (int it = 0; it < budget; ++it) {
+            PCHK(polar_zero(S + 1, 3, st));
+            PCHK(ns_gram(kPolF64, m, n, X, bM.p, S + 2, st, sched.alpha()));
+            PCHK(ns_update(kPolF64, m, n, X, bM.p, Xn, S + 1, bad, st, sched.c()));
+            PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
+            PCHK(hipStreamSynchronize(st));
+            ++it64;
+            delta = std::sqrt(hs[1]);
+            int b = 0;
+            std::memcpy(&b, &hs[3], sizeof(int));
+            if (b || delta > 1e6 * rootn) {  // (:169-171); the Gram-Schmidt fallback is not run at this size
+                bad64 = true;
+                break;
+            }
+            std::swap(X, Xn);
+            const bool plain = sched.plain();
+            sched.after(hs[2], delta);
+            if (plain && delta < 1e-13 * rootn) {
+                converged = true;
+                break;
+            }
         }
     }
     PCHK(polar_zero(S + 2, 1, st));

@@ -113,11 +113,16 @@ __device__ __forceinline__ void tri_tile(int b, int& ti, int& tj) {
 }
 
 // ---- f64 / f32: C (M x N) = op(A) B, A stored K x M (TRANS_A) or M x K --------
-// BM = BN = 64, BK = 16.  SYM: square symmetric output, upper tiles only.
-template <class T, bool TRANS_A, bool SYM, int EPI>
+// BM = BN = TILE (64, or 32 for products too small to fill the chip with
+// 64-wide tiles), BK = 1024 / TILE, so every thread loads four entries of
+// each operand per K tile; wave w owns the (TILE/2)^2 quadrant (w / 2, w % 2).
+// SYM: square symmetric output, upper tiles only.
+template <class T, bool TRANS_A, bool SYM, int EPI, int TILE>
 __global__ __launch_bounds__(256) void gemm_mfma4_kernel(int M, int N, int K, const T* __restrict__ A,
                                                           const T* __restrict__ B, EpiArgs e) {
-    constexpr int BM = 64, BN = 64, BK = 16, LDA = BM + 1, LDB = BN + 1;
+    constexpr int BM = TILE, BN = TILE, BK = 1024 / TILE, LDA = BM + 1, LDB = BN + 1;
+    constexpr int HM = TILE / 2, NS = TILE / 32;  // quadrant edge, 16 x 16 blocks per quadrant edge
+    static_assert(TILE == 32 || TILE == 64, "tile edge");
     __shared__ T As[BK * LDA];
     __shared__ T Bs[BK * LDB];
     typedef typename Acc4<T>::type acc_t;
@@ -131,11 +136,11 @@ __global__ __launch_bounds__(256) void gemm_mfma4_kernel(int M, int N, int K, co
     const int i0 = ti * BM, j0 = tj * BN;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     const int wm = w >> 1, wn = w & 1;
-    acc_t acc[2][2];
+    acc_t acc[NS][NS];
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NS; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
+        for (int b = 0; b < NS; ++b) acc[a][b] = acc_t{0, 0, 0, 0};
     // The next K tile is read from global memory into registers while the
     // current one is multiplied out of LDS.
     T ra[4], rb[4];
@@ -174,28 +179,28 @@ __global__ __launch_bounds__(256) void gemm_mfma4_kernel(int M, int N, int K, co
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             const int kr = kk + (lane >> 4), c16 = lane & 15;
-            T av[2], bv[2];
+            T av[NS], bv[NS];
 #pragma unroll
-            for (int a = 0; a < 2; ++a) av[a] = As[kr * LDA + wm * 32 + a * 16 + c16];
+            for (int a = 0; a < NS; ++a) av[a] = As[kr * LDA + wm * HM + a * 16 + c16];
 #pragma unroll
-            for (int b = 0; b < 2; ++b) bv[b] = Bs[kr * LDB + wn * 32 + b * 16 + c16];
+            for (int b = 0; b < NS; ++b) bv[b] = Bs[kr * LDB + wn * HM + b * 16 + c16];
 #pragma unroll
-            for (int a = 0; a < 2; ++a)
+            for (int a = 0; a < NS; ++a)
 #pragma unroll
-                for (int b = 0; b < 2; ++b) acc[a][b] = mfma4(av[a], bv[b], acc[a][b]);
+                for (int b = 0; b < NS; ++b) acc[a][b] = mfma4(av[a], bv[b], acc[a][b]);
         }
         __syncthreads();
     }
     double part = 0.0;
     int bad = 0;
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+    for (int a = 0; a < NS; ++a)
 #pragma unroll
-        for (int b = 0; b < 2; ++b)
+        for (int b = 0; b < NS; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wm * 32 + a * 16 + acc_row<T>(lane, r);
-                const int j = j0 + wn * 32 + b * 16 + (lane & 15);
+                const int i = i0 + wm * HM + a * 16 + acc_row<T>(lane, r);
+                const int j = j0 + wn * HM + b * 16 + (lane & 15);
                 if (i < M && j < N && (!SYM || ti < tj || i <= j))
                     epi_apply<T, EPI>(e, N, i, j, acc[a][b][r], part, bad, SYM && i != j);
             }
@@ -347,16 +352,29 @@ __global__ void zero_kernel(double* p, int n) {
 }
 
 // ---- launch wrappers ---------------------------------------------------------------
+template <int TILE>
+static dim3 grid4(int M, int N, bool sym) {
+    if (sym) {
+        const int tn = (N + TILE - 1) / TILE;
+        return dim3(tn * (tn + 1) / 2);
+    }
+    return dim3((N + TILE - 1) / TILE, (M + TILE - 1) / TILE);
+}
+
+// 32-wide tiles when 64-wide ones would leave CUs of the 256 without a
+// workgroup (N <= ~1024 squares): four times the workgroups at half the
+// reuse per LDS load.
+constexpr int kGemmFillWgs = 256;
+
 template <class T, bool TA, bool SYM, int EPI>
 static hipError_t launch4(int M, int N, int K, const T* A, const T* B, const EpiArgs& e, hipStream_t st) {
-    dim3 grid;
-    if (SYM) {
-        const int tn = (N + 63) / 64;
-        grid = dim3(tn * (tn + 1) / 2);
+    const dim3 g64 = grid4<64>(M, N, SYM);
+    if ((int)(g64.x * g64.y) < kGemmFillWgs) {
+        hipLaunchKernelGGL((gemm_mfma4_kernel<T, TA, SYM, EPI, 32>), grid4<32>(M, N, SYM), dim3(256), 0, st, M, N, K,
+                           A, B, e);
     } else {
-        grid = dim3((N + 63) / 64, (M + 63) / 64);
+        hipLaunchKernelGGL((gemm_mfma4_kernel<T, TA, SYM, EPI, 64>), g64, dim3(256), 0, st, M, N, K, A, B, e);
     }
-    hipLaunchKernelGGL((gemm_mfma4_kernel<T, TA, SYM, EPI>), grid, dim3(256), 0, st, M, N, K, A, B, e);
     return hipGetLastError();
 }
 

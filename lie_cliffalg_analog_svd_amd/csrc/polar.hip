@@ -361,10 +361,12 @@ static dim3 grid4(int M, int N, bool sym) {
     return dim3((N + TILE - 1) / TILE, (M + TILE - 1) / TILE);
 }
 
-// 32-wide tiles when 64-wide ones would leave CUs of the 256 without a
-// workgroup (N <= ~1024 squares): four times the workgroups at half the
-// reuse per LDS load.
-constexpr int kGemmFillWgs = 256;
+// 32-wide tiles when 64-wide ones would give fewer than four workgroups per
+// CU (256 CUs; squares up to ~2048): four times the workgroups at half the
+// reuse per LDS load.  Measured on the plain polar iteration: 1024^2 4.9 ->
+// 4.2 ms, 1536^2 13.4 -> 10.0 ms against a threshold of one per CU; two per
+// CU and eight per CU were both slower.
+constexpr int kGemmFillWgs = 1024;
 
 template <class T, bool TA, bool SYM, int EPI>
 static hipError_t launch4(int M, int N, int K, const T* A, const T* B, const EpiArgs& e, hipStream_t st) {

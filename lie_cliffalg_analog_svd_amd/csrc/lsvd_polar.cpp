@@ -130,14 +130,18 @@ int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, do
     const int budget = 120;
     NsSchedule sched;  // plain steps unless LSVD_PREC_F64_SCALED
     sched.init(precision == LSVD_PREC_F64_SCALED);
-    if (!sched.enabled) {
-        // Plain steps, one step ahead of the stop test: step k + 1 is queued
-        // before the host reads step k's ||dX||, so the GPU does not idle
-        // through that read.  Iterate X_k lives in buffer k % 3, so the
-        // queued step (X_{k+1} -> X_{k+2}) leaves X_{k+1}, the result when
-        // step k meets the stop test, in place; step k uses slot k % 2 of
-        // the device and host stop-test slots.  The iterates and the step
-        // count are those of the one-step-at-a-time loop.
+    // Plain steps on a matrix up to about 1600^2, not after a low-precision
+    // phase: one step ahead of the stop test.  Step k + 1 is queued before
+    // the host reads step k's ||dX||, so the GPU does not idle through that
+    // read (~35 us a step); the step queued after the last one is wasted,
+    // which costs less than the reads save only while a step is short and
+    // the iteration long.  Iterate X_k lives in buffer k % 3, so the queued
+    // step (X_{k+1} -> X_{k+2}) leaves X_{k+1}, the result when step k meets
+    // the stop test, in place; step k uses slot k % 2 of the device and host
+    // stop-test slots.  The iterates and the step count are those of the
+    // one-step-at-a-time loop.
+    const bool ahead = !sched.enabled && !lowp && (double)m * n * n <= 4e9;
+    if (ahead) {
         PCHK(bX3.alloc(sizeof(double) * mn));
         StepSlots ss;
         PCHK(ss.create());
@@ -180,7 +184,7 @@ int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, do
     } else {
         for (int it = 0; it < budget; ++it) {
             PCHK(polar_zero(S + 1, 3, st));
-            PCHK(ns_gram(kPolF64, m, n, X, bM.p, S + 2, st, sched.alpha()));
+            PCHK(ns_gram(kPolF64, m, n, X, bM.p, sched.enabled ? S + 2 : nullptr, st, sched.alpha()));
             PCHK(ns_update(kPolF64, m, n, X, bM.p, Xn, S + 1, bad, st, sched.c()));
             PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
             PCHK(hipStreamSynchronize(st));

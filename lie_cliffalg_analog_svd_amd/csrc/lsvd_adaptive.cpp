@@ -17,7 +17,7 @@ struct DevMem {
     void* p = nullptr;
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
     ~DevMem() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 

@@ -414,8 +414,8 @@ static int plan_run_device(lsvd_plan_t p, const Pools& pools, bool use_graph) {
                       p->graph_pools.s == pools.s && p->graph_pools.vt == pools.vt &&
                       p->graph_pools.info == pools.info;
     if (!same) {
-        if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
-        if (p->graph) hipGraphDestroy(p->graph);
+        if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+        if (p->graph) (void)hipGraphDestroy(p->graph);
         p->graph_exec = nullptr;
         p->graph = nullptr;
         e = plan_ensure_streams(p);
@@ -565,12 +565,12 @@ int lsvd_create(int device, lsvd_handle_t* out) {
 
 int lsvd_destroy(lsvd_handle_t h) {
     if (!h) return LSVD_E_ARG;
-    hipSetDevice(h->device);
+    (void)hipSetDevice(h->device);
     for (auto& kv : h->plan_cache) lsvd_batch_plan_destroy(kv.second);
     h->plan_cache.clear();
     lsvd_release_extras(h);
-    if (h->own_stream) hipStreamDestroy(h->own_stream);
-    for (hipStream_t s2 : h->lanes) hipStreamDestroy(s2);
+    if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
+    for (hipStream_t s2 : h->lanes) (void)hipStreamDestroy(s2);
     delete h;
     return LSVD_OK;
 }
@@ -622,13 +622,13 @@ int lsvd_batch_plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool,
 
 int lsvd_batch_plan_destroy(lsvd_plan_t p) {
     if (!p) return LSVD_E_ARG;
-    hipSetDevice(p->h->device);
-    if (p->graph_exec) hipGraphExecDestroy(p->graph_exec);
-    if (p->graph) hipGraphDestroy(p->graph);
-    for (hipEvent_t ev : p->ev) hipEventDestroy(ev);
+    (void)hipSetDevice(p->h->device);
+    if (p->graph_exec) (void)hipGraphExecDestroy(p->graph_exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    for (hipEvent_t ev : p->ev) (void)hipEventDestroy(ev);
     void* ptrs[] = {p->d_items, p->d_counters, p->d_consts, p->d_gws, p->d_a, p->d_u, p->d_s, p->d_vt, p->d_info};
     for (void* q : ptrs)
-        if (q) hipFree(q);
+        if (q) (void)hipFree(q);
     delete p;
     return LSVD_OK;
 }
@@ -683,9 +683,9 @@ int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int3
     hipEvent_t e0 = nullptr, e1 = nullptr;
     auto cleanup = [&]() {
         for (double* q : {da, du, ds, dvt, gws})
-            if (q) hipFree(q);
-        if (e0) hipEventDestroy(e0);
-        if (e1) hipEventDestroy(e1);
+            if (q) (void)hipFree(q);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
         lsvd_batch_plan_destroy(p);
     };
 #define TCHECK(expr)                                  \
@@ -717,7 +717,7 @@ int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int3
             Bucket bk = base;
             const size_t need = configure_bucket(h, bk, vs[vi].id);
             if (gws) {
-                hipFree(gws);
+                (void)hipFree(gws);
                 gws = nullptr;
             }
             if (need) TCHECK(hipMalloc(&gws, sizeof(double) * need));
@@ -804,7 +804,7 @@ int lsvd_batch_plan_time_buckets(lsvd_plan_t p, const double* a_pool, double* u_
         ++slot;
     }
     for (auto& x : ev)
-        if (x) hipEventDestroy(x);
+        if (x) (void)hipEventDestroy(x);
     return rc;
 }
 

@@ -18,7 +18,7 @@ struct DevBuf {
     size_t bytes = 0;
     hipError_t ensure(size_t want) {
         if (want <= bytes && p) return hipSuccess;
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
         p = nullptr;
         bytes = 0;
         hipError_t e = hipMalloc(&p, want ? want : 1);
@@ -26,7 +26,7 @@ struct DevBuf {
         return e;
     }
     ~DevBuf() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 
@@ -44,12 +44,12 @@ int joint_run_sweeps(lsvd_handle_t h, const JointArgs& a, int max_sweeps, JointF
     e = hipStreamEndCapture(st, &graph);
     if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
     if (le != hipSuccess) {
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         return hip_fail(h, le, "joint sweep launch");
     }
     e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
-        hipGraphDestroy(graph);
+        (void)hipGraphDestroy(graph);
         return hip_fail(h, e, "hipGraphInstantiate");
     }
     int rc = LSVD_OK;
@@ -65,8 +65,8 @@ int joint_run_sweeps(lsvd_handle_t h, const JointArgs& a, int max_sweeps, JointF
         for (int f = 0; f < a.F; ++f) all = all && fam[f].done;
         if (all) break;
     }
-    hipGraphExecDestroy(exec);
-    hipGraphDestroy(graph);
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
     return rc;
 }
 
@@ -262,12 +262,12 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
         hipError_t le = jsvd_launch_sweep(a, st);
         JCHECK(hipStreamEndCapture(st, &graph));
         if (le != hipSuccess) {
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             return hip_fail(h, le, "joint svd sweep launch");
         }
         e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (e != hipSuccess) {
-            hipGraphDestroy(graph);
+            (void)hipGraphDestroy(graph);
             return hip_fail(h, e, "hipGraphInstantiate");
         }
         int rc2 = LSVD_OK;
@@ -284,8 +284,8 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
             for (auto& s : fam) all = all && s.done;
             if (all) break;
         }
-        hipGraphExecDestroy(exec);
-        hipGraphDestroy(graph);
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
         if (rc2 != LSVD_OK) return rc2;
     }
     JCHECK(jsvd_launch_final(a, d_sig, d_vt, st));

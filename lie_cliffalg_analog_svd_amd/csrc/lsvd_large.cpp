@@ -26,7 +26,7 @@ struct Mem {
     void* p = nullptr;
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
     ~Mem() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 

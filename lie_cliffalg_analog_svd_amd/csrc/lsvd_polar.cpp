@@ -17,7 +17,7 @@ namespace {
 struct Buf {
     void* p = nullptr;
     ~Buf() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
 };
@@ -34,8 +34,8 @@ struct StepSlots {
     }
     ~StepSlots() {
         for (hipEvent_t e : ev)
-            if (e) hipEventDestroy(e);
-        if (host) hipHostFree(host);
+            if (e) (void)hipEventDestroy(e);
+        if (host) (void)hipHostFree(host);
     }
 };
 

@@ -18,7 +18,7 @@ struct DMem {
     void* p = nullptr;
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
     ~DMem() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
 };
 

@@ -12,7 +12,7 @@ namespace {
 struct Buf {
     void* p = nullptr;
     ~Buf() {
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     }
     hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
 };

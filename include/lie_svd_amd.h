@@ -303,6 +303,36 @@ typedef struct {
 int lsvd_hosvd3_f64(lsvd_handle_t h, int count, int n1, int n2, int n3, const double* x, double* core, double* u1,
                     double* u2, double* u3, lsvd_tensor_trace_t* trace);
 
+/* ---- Phase-BSS (batched consumer of LieSvdSmall and LieSvdJoint) ------- */
+
+/* PhaseBssTrace (src/lie_svd_bss.rs:36-44). */
+typedef struct {
+    int32_t channels, samples;
+    double covariance_floor;
+    double joint_initial_offdiag, joint_final_offdiag;
+    int32_t joint_rotations;
+    double mean_channel_coherence;
+} lsvd_bss_trace_t;
+
+/* Replaces LieSvdBss::separate (src/lie_svd_bss.rs:58-101) for `count`
+ * observation blocks of one shape channels x samples (row-major, packed
+ * back to back; 1 <= channels <= 64, LSVD_E_LIMIT beyond; samples >= 2,
+ * else LSVD_E_SHAPE as the reference's assert): centered covariance, its
+ * LieSvdSmall::solve (all blocks in one batched call), whitening
+ * W[i][j] = u[j][i] / sqrt(max(sigma_i, eps max(max sigma, 1))), the
+ * lagged covariances of the whitened channels for the lags 0 < lag <
+ * samples (the covariance itself when none is usable), Phase-JADE of each
+ * block's family (params NULL: for_n(channels)), then unmixing = V^T W
+ * (channels^2), separated = V^T whitened (channels x samples) and the
+ * channel coherence (channels) per block.  whitening / whitened (optional,
+ * NULL to skip) receive W and the whitened channels.  lags: nlags host
+ * ints (PhaseBssParams::for_channels uses 1, 2, 3, 5).  trace (host memory
+ * in both pointer modes, may be NULL) receives count entries. */
+int lsvd_bss_separate_f64(lsvd_handle_t h, int count, int channels, int samples, const double* x, int nlags,
+                          const int* lags, double eps, const lsvd_joint_params_t* params, double* unmixing,
+                          double* separated, double* coherence, double* whitening, double* whitened,
+                          lsvd_bss_trace_t* trace);
+
 /* Replaces rotate_tensor3 (src/lie_svd_tensor.rs:121-145), as used by
  * reconstruct_hosvd3 (:64-66): out[a][b][c] = sum q1[a][i] q2[b][j] q3[c][k]
  * x[i][j][k] for `count` tensors; q_m is r_m x n_m per tensor (count r_m n_m

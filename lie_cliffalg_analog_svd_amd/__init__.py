@@ -4,7 +4,7 @@ Hand-written HIP kernels behind a C ABI (``include/lie_svd_amd.h``,
 ``liblsvd_amd.so``), with a Python mirror of the reference's Rust API:
 ``LieSvdSmall``, ``LieSvdMicro``, ``LieSvd``, ``eigh_jacobi_full``, the
 batch plans, ``LieSvdJoint`` and the rotor-stream export
-(``HardwareSchedule``), tensor HO-SVD (``LieSvdTensor``).  ``profiles`` and ``metrics`` restate the reference's input
+(``HardwareSchedule``), tensor HO-SVD (``LieSvdTensor``), Phase-BSS (``LieSvdBss``).  ``profiles`` and ``metrics`` restate the reference's input
 generators and residual metrics (src/lib.rs:133-388).
 """
 from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_SCALED, LSVD_PTR_DEVICE,
@@ -21,6 +21,8 @@ from .adaptive import (AdaptiveRoute, AdaptiveTrace, AdaptiveTriage, LieSvdAdapt
                        analyze_batch, analyze_matrix, choose_route)
 from .tensor import (LieSvdTensor, TensorPhaseSvd3, TensorPhaseTrace, hosvd3_batched, reconstruct_hosvd3,
                      rotate_tensor3_batched, tensor_relative_error)
+from .bss import (LieSvdBss, PhaseBssParams, PhaseBssResult, PhaseBssTrace, channel_phase_coherence,
+                  estimate_sir_db, separate_batched)
 from .schedule import (ComplexMziPhase, ComplexPhaseEventKind, HardwarePhaseEvent, HardwareSchedule, HardwareTarget,
                        MziPhase, PhaseRotorKind, givens_batch, schedules_from_orthogonal)
 

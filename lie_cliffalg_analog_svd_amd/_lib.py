@@ -79,6 +79,21 @@ class LsvdTensorTrace(C.Structure):
     ]
 
 
+class LsvdBssTrace(C.Structure):
+    _fields_ = [
+        ("channels", C.c_int32),
+        ("samples", C.c_int32),
+        ("covariance_floor", C.c_double),
+        ("joint_initial_offdiag", C.c_double),
+        ("joint_final_offdiag", C.c_double),
+        ("joint_rotations", C.c_int32),
+        ("mean_channel_coherence", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class LsvdPolarInfo(C.Structure):
     _fields_ = [
         ("polar_ok", C.c_int32),
@@ -196,6 +211,9 @@ SIGNATURES = {
                                   C.POINTER(LsvdTensorTrace)]),
     "lsvd_tensor_rotate3_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, _vp,
                                           _vp, _vp, _vp, _vp]),
+    "lsvd_bss_separate_f64": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_double,
+                                        C.POINTER(LsvdJointParams), _vp, _vp, _vp, _vp, _vp,
+                                        C.POINTER(LsvdBssTrace)]),
     "lsvd_givens_schedule_f64": (C.c_int, [_vp, C.c_int, _i32p, _i64p, _vp, C.c_int, _vp, _vp, _vp, _vp, _vp]),
     "lsvd_givens_events": (C.c_int, [C.c_int, _vp, _vp, C.POINTER(LsvdPhaseEvent)]),
     "lsvd_schedule_json": (C.c_size_t, [C.c_int, C.c_int, C.c_int, C.POINTER(LsvdPhaseEvent), C.c_int, _vp,

@@ -156,6 +156,10 @@ def lib():
         L.lso_schedule_json.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(PhaseEvent), C.c_int, dp,
                                         C.c_char_p, C.c_size_t]
         L.lso_schedule_json.restype = C.c_size_t
+        L.lso_bss_whiten.argtypes = [dp, C.c_int, C.c_int, ip, C.c_int, C.c_double, dp, dp, dp, dp, dp]
+        L.lso_bss_whiten.restype = C.c_int
+        L.lso_bss_unmix.argtypes = [dp, dp, dp, C.c_int, C.c_int, dp, dp, dp]
+        L.lso_bss_unmix.restype = None
         _lib = L
     return _lib
 
@@ -492,3 +496,42 @@ def rotate_tensor3(x, q1, q2, q3):
     lib().lso_rotate_tensor3(_dp(x), n1, n2, n3, _dp(q1), q1.shape[0], _dp(q2), q2.shape[0], _dp(q3), q3.shape[0],
                              _dp(out))
     return out
+
+
+BSS_LAGS = (1, 2, 3, 5)  # PhaseBssParams::for_channels (src/lie_svd_bss.rs:20-26)
+BSS_EPS = 1e-10
+
+
+def bss_whiten(x, lags=BSS_LAGS, eps: float = BSS_EPS):
+    """The first half of LieSvdBss::separate (src/lie_svd_bss.rs:58-75,
+    162-197) for one channels x samples observation block.  Returns
+    (whitening W, whitened, lagged family (K, ch, ch), sigma, floor)."""
+    x = _c(x)
+    ch, samples = x.shape
+    lg = np.ascontiguousarray(np.asarray(lags, dtype=np.int32))
+    w = np.zeros((ch, ch)); z = np.zeros((ch, samples)); fam = np.zeros((max(len(lg), 1), ch, ch))
+    sg = np.zeros(ch); fl = np.zeros(1)
+    k = lib().lso_bss_whiten(_dp(x), ch, samples, lg.ctypes.data_as(C.POINTER(C.c_int)), len(lg), eps, _dp(w),
+                             _dp(z), _dp(fam), _dp(sg), _dp(fl))
+    return w, z, fam[:k].copy(), sg, float(fl[0])
+
+
+def bss_unmix(basis, whitening, whitened):
+    """The second half (src/lie_svd_bss.rs:81-86, 104-126): (unmixing,
+    separated, channel coherence)."""
+    v, w, z = _c(basis), _c(whitening), _c(whitened)
+    ch, samples = z.shape
+    un = np.zeros((ch, ch)); sep = np.zeros((ch, samples)); coh = np.zeros(ch)
+    lib().lso_bss_unmix(_dp(v), _dp(w), _dp(z), ch, samples, _dp(un), _dp(sep), _dp(coh))
+    return un, sep, coh
+
+
+def bss_separate(x, lags=BSS_LAGS, eps: float = BSS_EPS):
+    """LieSvdBss::separate (src/lie_svd_bss.rs:58-101): whiten, Phase-JADE on
+    the lagged family with for_n(channels), unmix.  Returns a dict."""
+    w, z, fam, sg, fl = bss_whiten(x, lags, eps)
+    v, _d, tr = joint_diag_sym(fam, joint_params_for_n(z.shape[0]))
+    un, sep, coh = bss_unmix(v, w, z)
+    return {"unmixing": un, "separated": sep, "channel_coherence": coh, "whitening": w, "whitened": z,
+            "lagged": fam, "basis": v, "sigma": sg, "covariance_floor": fl, "joint": tr,
+            "mean_channel_coherence": float(coh.sum() / max(len(coh), 1))}

@@ -1571,3 +1571,119 @@ void lso_hosvd3(const double* x, int n1, int n2, int n3, double* core, double* u
     trace[1] = tensor_offdiag_norm(core, n1, n2, n3);
     trace[2] = tensor_superdiag_ratio(core, n1, n2, n3);
 }
+
+/* ---- LieSvdBss::separate (src/lie_svd_bss.rs:58-101) ------------------- */
+
+/* The first half, up to the joint diagonalizer: center_channels (:162-171),
+ * covariance = X_c X_c^T / samples (:63), LieSvdSmall::solve (:65), the
+ * floor eps max(max sigma, 1) (:66), whitening W[i][j] = u[j][i] /
+ * sqrt(max(sigma_i, floor)) (:67-73), whitened = W X_c (:74) and the lagged
+ * covariances (:75, :173-197): for each lag in 0 < lag < samples,
+ * c[i][j] = sum_t x[i][t + lag] x[j][t] / (samples - lag), each term divided
+ * as it is added, then (c + c^T) / 2; with no usable lag, x x^T / samples.
+ * lagged must hold max(nlags, 1) x ch x ch; returns the family size. */
+int lso_bss_whiten(const double* x, int ch, int samples, const int* lags, int nlags, double eps, double* whitening,
+                   double* whitened, double* lagged, double* sigma, double* floor_out) {
+    if (ch <= 0 || samples <= 1) return 0; /* the reference asserts (:61) */
+    double* xc = (double*)malloc(sizeof(double) * ch * samples);
+    for (int i = 0; i < ch; ++i) {
+        double mean = 0.0;
+        for (int t = 0; t < samples; ++t) mean += x[(size_t)i * samples + t];
+        mean /= samples;
+        for (int t = 0; t < samples; ++t) xc[(size_t)i * samples + t] = x[(size_t)i * samples + t] - mean;
+    }
+    double* cov = (double*)malloc(sizeof(double) * ch * ch);
+    for (int i = 0; i < ch; ++i)
+        for (int j = 0; j < ch; ++j) {
+            double s = 0.0;
+            for (int t = 0; t < samples; ++t) s += xc[(size_t)i * samples + t] * xc[(size_t)j * samples + t];
+            cov[(size_t)i * ch + j] = s / samples;
+        }
+    double* u = (double*)malloc(sizeof(double) * ch * ch);
+    double* vt = (double*)malloc(sizeof(double) * ch * ch);
+    lso_info info;
+    lso_small_solve(cov, ch, u, sigma, vt, &info);
+    double smax = 0.0;
+    for (int i = 0; i < ch; ++i) smax = fmax(smax, sigma[i]);
+    const double floor_v = eps * fmax(smax, 1.0);
+    for (int i = 0; i < ch; ++i) {
+        const double inv = 1.0 / sqrt(fmax(sigma[i], floor_v));
+        for (int j = 0; j < ch; ++j) whitening[(size_t)i * ch + j] = inv * u[(size_t)j * ch + i];
+    }
+    for (int i = 0; i < ch; ++i)
+        for (int t = 0; t < samples; ++t) {
+            double s = 0.0;
+            for (int k = 0; k < ch; ++k) s += whitening[(size_t)i * ch + k] * xc[(size_t)k * samples + t];
+            whitened[(size_t)i * samples + t] = s;
+        }
+    int kk = 0;
+    double* c = (double*)malloc(sizeof(double) * ch * ch);
+    for (int l = 0; l < nlags; ++l) {
+        const int lag = lags[l];
+        if (lag <= 0 || lag >= samples) continue;
+        const double denom = (double)(samples - lag);
+        for (int e = 0; e < ch * ch; ++e) c[e] = 0.0;
+        for (int t = 0; t < samples - lag; ++t)
+            for (int i = 0; i < ch; ++i)
+                for (int j = 0; j < ch; ++j)
+                    c[(size_t)i * ch + j] +=
+                        whitened[(size_t)i * samples + t + lag] * whitened[(size_t)j * samples + t] / denom;
+        double* out = lagged + (size_t)kk * ch * ch;
+        for (int i = 0; i < ch; ++i)
+            for (int j = 0; j < ch; ++j) out[(size_t)i * ch + j] = 0.5 * (c[(size_t)i * ch + j] + c[(size_t)j * ch + i]);
+        ++kk;
+    }
+    if (kk == 0) {
+        for (int i = 0; i < ch; ++i)
+            for (int j = 0; j < ch; ++j) {
+                double s = 0.0;
+                for (int t = 0; t < samples; ++t)
+                    s += whitened[(size_t)i * samples + t] * whitened[(size_t)j * samples + t];
+                lagged[(size_t)i * ch + j] = s / samples;
+            }
+        kk = 1;
+    }
+    *floor_out = floor_v;
+    free(xc);
+    free(cov);
+    free(u);
+    free(vt);
+    free(c);
+    return kk;
+}
+
+/* channel_phase_coherence (:104-126) of one channel of `samples` values. */
+static double bss_coherence(const double* x, int samples) {
+    if (samples <= 1) return 0.0;
+    double norm = 0.0, delay_dot = 0.0, grad = 0.0;
+    for (int t = 0; t < samples; ++t) {
+        const double a = x[t], b = x[(t + 1) % samples];
+        norm += a * a;
+        delay_dot += a * b;
+        const double d = b - a;
+        grad += d * d;
+    }
+    const double phase_lock = fabs(delay_dot) / fmax(norm, 1e-300);
+    const double smooth = 1.0 / (1.0 + sqrt(grad / fmax(norm, 1e-300)));
+    const double v = 0.5 * phase_lock + 0.5 * smooth;
+    return v < 0.0 ? 0.0 : (v > 1.0 ? 1.0 : v);
+}
+
+/* The second half (:81-86): separated = V^T whitened, unmixing = V^T W,
+ * and the channel coherence of the separated channels. */
+void lso_bss_unmix(const double* basis, const double* whitening, const double* whitened, int ch, int samples,
+                   double* unmixing, double* separated, double* coherence) {
+    for (int i = 0; i < ch; ++i) {
+        for (int t = 0; t < samples; ++t) {
+            double s = 0.0;
+            for (int k = 0; k < ch; ++k) s += basis[(size_t)k * ch + i] * whitened[(size_t)k * samples + t];
+            separated[(size_t)i * samples + t] = s;
+        }
+        for (int j = 0; j < ch; ++j) {
+            double s = 0.0;
+            for (int k = 0; k < ch; ++k) s += basis[(size_t)k * ch + i] * whitening[(size_t)k * ch + j];
+            unmixing[(size_t)i * ch + j] = s;
+        }
+        coherence[i] = bss_coherence(separated + (size_t)i * samples, samples);
+    }
+}

@@ -23,10 +23,11 @@ same workload.  Further legs, each with its CPU oracle beside it:
   joint    Phase-JADE of 32 families x 64 symmetric 256 x 256 matrices,
            GB/s of the layer traffic, off-diagonal residual, recovery of
            the planted basis; the reference's published smoke families;
-  joint_svd, givens, polish, triage, hosvd
+  joint_svd, givens, polish, triage, hosvd, bss
            the two-sided joint SVD, the rotor-stream export of the batch's
-           V factors, the batched digital polish, the adaptive triage, and
-           the tensor HO-SVD of 4096 tensors of 16^3;
+           V factors, the batched digital polish, the adaptive triage, the
+           tensor HO-SVD of 4096 tensors of 16^3, and Phase-BSS of 1024
+           blocks of 8 channels x 4096 samples;
   vjp      gradient of L = sum w_i s_i + <G, U V^T> for 512 96 x 64
            matrices with gaps ~1e-3 plus 32 with one 1e-8 gap, against the
            analytic FP64 gradient and central differences.
@@ -667,6 +668,68 @@ def hosvd_leg(h, dev, args):
     return out
 
 
+def bss_leg(h, dev, args):
+    """LieSvdBss::separate (src/lie_svd_bss.rs:58-101) of a batch of 1024
+    observation blocks of 8 channels x 4096 samples on device pointers:
+    1024 covariances solved by one batched LieSvdSmall call, 1024 lagged
+    families of 4 by one batched Phase-JADE call.  Each block mixes 8
+    sources (sinusoids of normalised frequencies 0.02 + 0.045 i, so their
+    lag-1..5 autocorrelations differ, plus N(0, 0.1^2) noise) by
+    I + 0.3 N(0, 1)."""
+    import torch
+
+    import oracle
+    from lie_cliffalg_analog_svd_amd import PhaseBssParams, _lib, estimate_sir_db
+    count, ch, S = args.bss_count, 8, 4096
+    rng = np.random.default_rng(0x5EED0007)
+    t = np.arange(S)
+    src = np.stack([np.sin(2 * np.pi * (0.02 + 0.045 * i) * t + i) for i in range(ch)])
+    srcs = src[None] + 0.1 * rng.standard_normal((count, ch, S))
+    mix = np.eye(ch)[None] + 0.3 * rng.standard_normal((count, ch, ch))
+    xs = np.einsum("bij,bjt->bit", mix, srcs)
+    xd = torch.from_numpy(xs).to(dev)
+    un = torch.empty((count, ch, ch), dtype=torch.float64, device=dev)
+    sep = torch.empty((count, ch, S), dtype=torch.float64, device=dev)
+    coh = torch.empty((count, ch), dtype=torch.float64, device=dev)
+    lags = np.array([1, 2, 3, 5], dtype=np.int32)
+    cp = PhaseBssParams.for_channels(ch).joint._c()
+    tr = (_lib.LsvdBssTrace * count)()
+    ptr = lambda a: C.c_void_p(a.data_ptr())  # noqa: E731
+
+    def run():
+        h.check(_lib.lib().lsvd_bss_separate_f64(h.raw, count, ch, S, ptr(xd), len(lags),
+                                                 lags.ctypes.data_as(C.c_void_p), 1e-10, C.byref(cp), ptr(un),
+                                                 ptr(sep), ptr(coh), None, None, tr))
+    run()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    sec = float(np.median(ts))
+    sp = sep.cpu().numpy()
+    sample = list(range(0, count, max(1, count // 64)))
+    t1 = time.perf_counter()
+    worst, gain = 0.0, []
+    for b in sample:
+        o = oracle.bss_separate(xs[b])["separated"]
+        sgn = np.sign(np.sum(sp[b] * o, axis=1, keepdims=True))
+        worst = max(worst, float(np.abs(sp[b] * sgn - o).max() / np.abs(o).max()))
+        gain.append(estimate_sir_db(srcs[b], sp[b]) - estimate_sir_db(srcs[b], xs[b]))
+    cpu = (time.perf_counter() - t1) / len(sample)
+    out = {"blocks": count, "channels": ch, "samples": S, "lags": lags.tolist(), "seconds": round(sec, 5),
+           "blocks_per_s": round(count / sec, 1),
+           "rotations_mean": float(np.mean([tr[b].joint_rotations for b in range(count)])),
+           "separated_diff_vs_oracle_sample": worst, "sir_gain_db_sample_mean": round(float(np.mean(gain)), 2),
+           "cpu_baseline": {"value": round(1.0 / cpu, 2), "unit": "blocks/s", "cores": 1, "kind": "port",
+                            "sample": f"{len(sample)} of the blocks, C restatement of LieSvdBss::separate, "
+                                      f"{cpu * len(sample):.2f}s"}}
+    log(f"bss: {out}")
+    return out
+
+
 def polar_leg(h, dev, args):
     """Large NS polar: 4096 x 4096 and 8192 x 2048, A = U diag(s) V^T with Haar
     factors and s log-uniform in [1e-6, 1].  Plain FP64 vs BF16 MFMA
@@ -769,6 +832,8 @@ def main(argv=None) -> int:
     ap.add_argument("--no-vjp", action="store_true", help="skip the vjp leg")
     ap.add_argument("--no-hosvd", action="store_true", help="skip the tensor HO-SVD leg")
     ap.add_argument("--hosvd-count", type=int, default=4096)
+    ap.add_argument("--no-bss", action="store_true", help="skip the Phase-BSS leg")
+    ap.add_argument("--bss-count", type=int, default=1024)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy a MIB-MiB device buffer 3 times (HBM counter calibration) and exit")
     args = ap.parse_args(argv)
@@ -975,6 +1040,8 @@ def main(argv=None) -> int:
         log(f"triage: {legs['triage']}")
     if not args.no_hosvd:
         legs["hosvd"] = hosvd_leg(h, dev, args)
+    if not args.no_bss:
+        legs["bss"] = bss_leg(h, dev, args)
     if not args.no_vjp:
         legs["vjp"] = vjp_leg(dev, args)
         log(f"vjp: {legs['vjp']}")

@@ -7,8 +7,10 @@
 //   unmixing = V^T W and the channel coherence           (:82-84, :104-126)
 // The covariance and the lagged covariances are one kernel: cross-products
 // of a channel block with its lagged copy, staged through LDS in chunks of
-// 32 samples and split over the samples so that a few blocks still fill the
-// chip; the split sums are added in a fixed order by the reduce kernel.
+// 32 samples at 64 channels (proportionally more at fewer channels) and split
+// over the samples into about 8192 workgroups in all, so that the chunks of a
+// block are staged side by side rather than one after another; the split sums
+// are added in a fixed order by the reduce kernel.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -20,8 +22,14 @@
 namespace lsvd {
 namespace {
 
-constexpr int kChunk = 32;  // samples staged per step (two 64 x 33 LDS tiles: 33.8 KB)
+// LDS of one staged operand: 64 channels x 33 (32 samples + pad) doubles;
+// fewer channels stage proportionally more samples per step.
+constexpr int kStage = kBssMaxChannels * 33;
 constexpr int kAccPerThread = kBssMaxChannels * kBssMaxChannels / 256;
+
+// Samples staged per step for `ch` channels: a multiple of 32 with
+// ch (chunk + 1) <= kStage.
+inline int bss_chunk(int ch) { return std::max(32, (kStage / ch - 1) / 32 * 32); }
 
 __global__ __launch_bounds__(256) void bss_mean_kernel(const double* x, int ch, int S, double* mean) {
     __shared__ double red[4];
@@ -36,9 +44,10 @@ __global__ __launch_bounds__(256) void bss_mean_kernel(const double* x, int ch, 
 // Block (p, q) = blockIdx.x (p = / nq, q = % nq), split blockIdx.y of
 // `span` samples.  Thread e owns entries e, e + 256, ... of the ch x ch sums.
 __global__ __launch_bounds__(256) void bss_cross_kernel(const double* x, const double* mean, int ch, int S,
-                                                        const int* lags, int nq, int span, double* part) {
-    constexpr int L = kChunk + 1;
-    __shared__ double sa[kBssMaxChannels * L], sb[kBssMaxChannels * L];
+                                                        const int* lags, int nq, int span, int chunk,
+                                                        double* part) {
+    const int L = chunk + 1;
+    __shared__ double sa[kStage], sb[kStage];
     const int p = blockIdx.x / nq, q = blockIdx.x % nq, split = blockIdx.y;
     const int lag = lags[q];
     const int tn = S - lag;
@@ -48,10 +57,10 @@ __global__ __launch_bounds__(256) void bss_cross_kernel(const double* x, const d
     double acc[kAccPerThread];
 #pragma unroll
     for (int r = 0; r < kAccPerThread; ++r) acc[r] = 0.0;
-    for (int t0 = t_begin; t0 < t_end; t0 += kChunk) {
-        const int cw = min(kChunk, t_end - t0);
-        for (int e = threadIdx.x; e < ch * kChunk; e += 256) {
-            const int r = e / kChunk, c = e % kChunk;
+    for (int t0 = t_begin; t0 < t_end; t0 += chunk) {
+        const int cw = min(chunk, t_end - t0);
+        for (int e = threadIdx.x; e < ch * chunk; e += 256) {
+            const int r = e / chunk, c = e % chunk;
             double va = 0.0, vb = 0.0;
             if (c < cw) {
                 const double m = mean ? mean[(size_t)p * ch + r] : 0.0;
@@ -114,7 +123,9 @@ __global__ void bss_whitening_kernel(const double* u, const double* sigma, int c
     if (threadIdx.x == 0 && floor_out) floor_out[p] = fl;
 }
 
-// One sample t per thread: its ch inputs in registers, op(M) in LDS.
+// One sample t per thread: its ch inputs in registers, op(M) in LDS;
+// MAXC >= ch bounds the unrolled register loops.
+template <int MAXC>
 __global__ __launch_bounds__(256) void bss_apply_kernel(const double* M, int trans, const double* in,
                                                         const double* mean, int ch, int S, double* out) {
     __shared__ double sm[kBssMaxChannels * kBssMaxChannels];
@@ -128,15 +139,15 @@ __global__ __launch_bounds__(256) void bss_apply_kernel(const double* M, int tra
     const int t = blockIdx.y * 256 + threadIdx.x;
     if (t >= S) return;
     const double* ip = in + (size_t)p * ch * S;
-    double v[kBssMaxChannels];
+    double v[MAXC];
 #pragma unroll
-    for (int k = 0; k < kBssMaxChannels; ++k)
+    for (int k = 0; k < MAXC; ++k)
         v[k] = k < ch ? ip[(size_t)k * S + t] - (mean ? mean[(size_t)p * ch + k] : 0.0) : 0.0;
     double* op = out + (size_t)p * ch * S;
     for (int i = 0; i < ch; ++i) {
         double s = 0.0;
 #pragma unroll
-        for (int k = 0; k < kBssMaxChannels; ++k)
+        for (int k = 0; k < MAXC; ++k)
             if (k < ch) s += sm[i * ch + k] * v[k];
         op[(size_t)i * S + t] = s;
     }
@@ -188,12 +199,19 @@ hipError_t bss_means(const double* x, int count, int ch, int S, double* mean, hi
     return hipGetLastError();
 }
 
+int bss_splits(int count, int ch, int S, int nq) {
+    const int chunks = (S + bss_chunk(ch) - 1) / bss_chunk(ch);
+    const int want = (kBssFillBlocks + count * nq - 1) / (count * nq);
+    return std::max(1, std::min(chunks, want));
+}
+
 hipError_t bss_cross(const double* x, const double* mean, int count, int ch, int S, const int* lags, int nq,
                      int nsplit, double* part, hipStream_t st) {
+    const int chunk = bss_chunk(ch);
     const int per = (S + nsplit - 1) / nsplit;
-    const int span = (per + kChunk - 1) / kChunk * kChunk;
+    const int span = (per + chunk - 1) / chunk * chunk;
     hipLaunchKernelGGL(bss_cross_kernel, dim3(count * nq, nsplit), dim3(256), 0, st, x, mean, ch, S, lags, nq, span,
-                       part);
+                       chunk, part);
     return hipGetLastError();
 }
 
@@ -214,8 +232,12 @@ hipError_t bss_whitening(const double* u, const double* sigma, int count, int ch
 
 hipError_t bss_apply(const double* M, bool trans, const double* in, const double* mean, int count, int ch, int S,
                      double* out, hipStream_t st) {
-    hipLaunchKernelGGL(bss_apply_kernel, dim3(count, (S + 255) / 256), dim3(256), 0, st, M, trans ? 1 : 0, in, mean,
-                       ch, S, out);
+    const dim3 grid(count, (S + 255) / 256);
+    const int tr = trans ? 1 : 0;
+    if (ch <= 8) hipLaunchKernelGGL(bss_apply_kernel<8>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
+    else if (ch <= 16) hipLaunchKernelGGL(bss_apply_kernel<16>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
+    else if (ch <= 32) hipLaunchKernelGGL(bss_apply_kernel<32>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
+    else hipLaunchKernelGGL(bss_apply_kernel<64>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
     return hipGetLastError();
 }
 

@@ -33,8 +33,6 @@ struct DeviceMode {
     ~DeviceMode() { h->ptr_mode = saved; }
 };
 
-constexpr int kBssFillBlocks = 512;  // cross-product workgroups aimed for
-
 }  // namespace
 
 #define BCHK(expr)                                          \
@@ -75,11 +73,7 @@ int lsvd_bss_separate_f64(lsvd_handle_t h, int count, int channels, int samples,
         denom.push_back((double)S);
     }
     const int K = (int)lag.size();
-    auto splits = [&](int blocks) {
-        const int most = (S + 63) / 64;
-        return std::max(1, std::min(most, (kBssFillBlocks + blocks - 1) / blocks));
-    };
-    const int ns_cov = splits(count), ns_lag = splits(count * K);
+    const int ns_cov = bss_splits(count, ch, S, 1), ns_lag = bss_splits(count, ch, S, K);
     DMem bx, bmean, bpart, bcov, bu, bs, bvt, bw, bz, bfloor, bfam, bv, bdiag, bun, bsep, bcoh, blag, bden;
     const double* dx = x;
     if (host) {

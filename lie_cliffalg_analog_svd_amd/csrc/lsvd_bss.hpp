@@ -8,6 +8,10 @@
 namespace lsvd {
 
 constexpr int kBssMaxChannels = 64;  // channels of one observation block
+constexpr int kBssFillBlocks = 8192;  // cross-product workgroups aimed for
+
+// Sample splits of the cross-products of `count` blocks at `nq` lags.
+int bss_splits(int count, int ch, int S, int nq);
 
 // mean[p ch + i] = mean over t of x[p][i][t] (count blocks of ch x S).
 hipError_t bss_means(const double* x, int count, int ch, int S, double* mean, hipStream_t st);

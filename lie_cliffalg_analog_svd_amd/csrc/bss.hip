@@ -123,12 +123,17 @@ __global__ void bss_whitening_kernel(const double* u, const double* sigma, int c
     if (threadIdx.x == 0 && floor_out) floor_out[p] = fl;
 }
 
-// One sample t per thread: its ch inputs in registers, op(M) in LDS;
-// MAXC >= ch bounds the unrolled register loops.
+// TPT samples per thread (4 at up to 8 channels, 2 at up to 16, else 1),
+// their ch inputs each in registers, all loaded before the products;
+// op(M) in LDS.  MAXC >= ch bounds the unrolled loops and the LDS tile.
+template <int MAXC>
+constexpr int bss_apply_tpt() { return MAXC <= 8 ? 4 : (MAXC <= 16 ? 2 : 1); }
+
 template <int MAXC>
 __global__ __launch_bounds__(256) void bss_apply_kernel(const double* M, int trans, const double* in,
                                                         const double* mean, int ch, int S, double* out) {
-    __shared__ double sm[kBssMaxChannels * kBssMaxChannels];
+    constexpr int TPT = bss_apply_tpt<MAXC>();
+    __shared__ double sm[MAXC * MAXC];
     const int p = blockIdx.x;
     const double* mp = M + (size_t)p * ch * ch;
     for (int e = threadIdx.x; e < ch * ch; e += 256) {
@@ -136,20 +141,27 @@ __global__ __launch_bounds__(256) void bss_apply_kernel(const double* M, int tra
         sm[e] = trans ? mp[(size_t)k * ch + i] : mp[e];
     }
     __syncthreads();
-    const int t = blockIdx.y * 256 + threadIdx.x;
-    if (t >= S) return;
     const double* ip = in + (size_t)p * ch * S;
-    double v[MAXC];
-#pragma unroll
-    for (int k = 0; k < MAXC; ++k)
-        v[k] = k < ch ? ip[(size_t)k * S + t] - (mean ? mean[(size_t)p * ch + k] : 0.0) : 0.0;
     double* op = out + (size_t)p * ch * S;
-    for (int i = 0; i < ch; ++i) {
-        double s = 0.0;
+    const int tb = blockIdx.y * 256 * TPT + threadIdx.x;
+    double v[TPT][MAXC];
+#pragma unroll
+    for (int u = 0; u < TPT; ++u) {
+        const int t = tb + 256 * u;
 #pragma unroll
         for (int k = 0; k < MAXC; ++k)
-            if (k < ch) s += sm[i * ch + k] * v[k];
-        op[(size_t)i * S + t] = s;
+            v[u][k] = (k < ch && t < S) ? ip[(size_t)k * S + t] - (mean ? mean[(size_t)p * ch + k] : 0.0) : 0.0;
+    }
+    for (int i = 0; i < ch; ++i) {
+#pragma unroll
+        for (int u = 0; u < TPT; ++u) {
+            const int t = tb + 256 * u;
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < MAXC; ++k)
+                if (k < ch) s += sm[i * ch + k] * v[u][k];
+            if (t < S) op[(size_t)i * S + t] = s;
+        }
     }
 }
 
@@ -232,12 +244,20 @@ hipError_t bss_whitening(const double* u, const double* sigma, int count, int ch
 
 hipError_t bss_apply(const double* M, bool trans, const double* in, const double* mean, int count, int ch, int S,
                      double* out, hipStream_t st) {
-    const dim3 grid(count, (S + 255) / 256);
     const int tr = trans ? 1 : 0;
-    if (ch <= 8) hipLaunchKernelGGL(bss_apply_kernel<8>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
-    else if (ch <= 16) hipLaunchKernelGGL(bss_apply_kernel<16>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
-    else if (ch <= 32) hipLaunchKernelGGL(bss_apply_kernel<32>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
-    else hipLaunchKernelGGL(bss_apply_kernel<64>, grid, dim3(256), 0, st, M, tr, in, mean, ch, S, out);
+    auto grid = [&](int tpt) { return dim3(count, (S + 256 * tpt - 1) / (256 * tpt)); };
+    if (ch <= 8)
+        hipLaunchKernelGGL(bss_apply_kernel<8>, grid(bss_apply_tpt<8>()), dim3(256), 0, st, M, tr, in, mean, ch, S,
+                           out);
+    else if (ch <= 16)
+        hipLaunchKernelGGL(bss_apply_kernel<16>, grid(bss_apply_tpt<16>()), dim3(256), 0, st, M, tr, in, mean, ch,
+                           S, out);
+    else if (ch <= 32)
+        hipLaunchKernelGGL(bss_apply_kernel<32>, grid(bss_apply_tpt<32>()), dim3(256), 0, st, M, tr, in, mean, ch,
+                           S, out);
+    else
+        hipLaunchKernelGGL(bss_apply_kernel<64>, grid(bss_apply_tpt<64>()), dim3(256), 0, st, M, tr, in, mean, ch,
+                           S, out);
     return hipGetLastError();
 }
 

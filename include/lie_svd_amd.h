@@ -99,12 +99,16 @@ typedef struct {
 
 /* Polar diagnostics. */
 typedef struct {
-    int32_t polar_ok;
+    int32_t polar_ok;      /* orth_err <= 1e-8 sqrt(n) (the reference's trust
+                              test) and skew_err <= 1e-8 sqrt(n) */
     int32_t lowp_iters;    /* iterations in the chosen low precision */
     int32_t f64_iters;     /* FP64 iterations (refinement, or all for F64) */
     int32_t reserved;
     double orth_err;       /* ||Q^T Q - I||_F of the returned factor */
     double last_delta;     /* ||X_{k+1} - X_k||_F at the last iteration */
+    double skew_err;       /* ||Q^T A - A^T Q||_F / ||A||_F: zero for A's polar
+                              factor (Q^T A is then symmetric), so it sees the
+                              error orthogonality cannot */
 } lsvd_polar_info_t;
 
 /* ---- handle ---------------------------------------------------------- */
@@ -171,10 +175,14 @@ int lsvd_micro_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, dou
  *                    (BF16: BF16 MFMA, FP32 accumulate) to the reference's
  *                    FP64 stopping rule.
  * Low-precision products perturb the singular directions the FP64
- * refinement cannot restore; info->orth_err is ||Q^T Q - I||_F and the
- * caller states the error against the FP64 factor.  The Gram-Schmidt
- * fallback of the reference is not run at this size: on divergence
- * polar_ok is 0 and q holds the last iterate. */
+ * refinement cannot restore: the result is orthogonal to FP64 precision
+ * but is the polar factor of a matrix within about u_low ||A|| of A, so in
+ * directions with sigma_i / sigma_max below the low precision's unit
+ * roundoff u_low it is not A's.  info->orth_err is ||Q^T Q - I||_F and
+ * info->skew_err the A-dependent residual ||Q^T A - A^T Q||_F / ||A||_F;
+ * polar_ok needs both below 1e-8 sqrt(n), so such a factor reports 0.  The
+ * Gram-Schmidt fallback of the reference is not run at this size: on
+ * divergence polar_ok is 0 and q holds the last iterate. */
 int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision, double* q,
                    lsvd_polar_info_t* info);
 
@@ -237,6 +245,26 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
 int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols, const double* mats,
                        const lsvd_joint_params_t* params, double* u, double* sigmas, double* vt,
                        lsvd_joint_trace_t* trace);
+
+/* Host memory only, no GPU work (no reference counterpart: the reference has
+ * no graphs).  Phase-JADE and the joint SVD capture a sweep as hipGraphs of
+ * at most lsvd_max_graph_nodes() kernel nodes each: `layers` layers of
+ * `nodes_per_layer` nodes plus one stop-test node are split into graphs of
+ * at most max_nodes nodes; first[c] / count[c] receive the layer range of
+ * graph c (at most max_chunks entries written; both may be NULL).  Returns
+ * the number of graphs. */
+int lsvd_sweep_graph_chunks(int layers, int nodes_per_layer, int max_nodes, int32_t* first, int32_t* count,
+                            int max_chunks);
+int lsvd_max_graph_nodes(void);
+
+/* Layer-pair map of the Phase-JADE and joint-SVD kernels: the pairs the
+ * device's circle-method schedule (layer_pairs, src/lie_svd_joint.rs:
+ * 516-539) yields for n indices, computed by a kernel with the same device
+ * function the sweeps use.  out (host memory, layers x slots x 2 int32 with
+ * layers = round_robin_layer_count(n), slots = ceil(n / 2)) receives (i, j),
+ * i < j, per slot in the reference's order, or (-1, -1) for a slot paired
+ * with the phantom index of odd n.  n <= 4096.  (Diagnostic.) */
+int lsvd_joint_layer_pairs(lsvd_handle_t h, int n, int32_t* out);
 
 /* ---- Adaptive triage and router (src/lie_svd_adaptive.rs) ------------- */
 

@@ -102,6 +102,7 @@ class LsvdPolarInfo(C.Structure):
         ("reserved", C.c_int32),
         ("orth_err", C.c_double),
         ("last_delta", C.c_double),
+        ("skew_err", C.c_double),
     ]
 
     def as_dict(self):
@@ -221,6 +222,9 @@ SIGNATURES = {
     "lsvd_batch_plan_describe": (C.c_int, [_vp, _vp, C.c_int, C.POINTER(C.c_int), _vp]),
     "lsvd_batch_plan_time_buckets": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp, C.c_int]),
     "lsvd_batch_plan_phase_clocks": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "lsvd_sweep_graph_chunks": (C.c_int, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, C.c_int]),
+    "lsvd_max_graph_nodes": (C.c_int, []),
+    "lsvd_joint_layer_pairs": (C.c_int, [_vp, C.c_int, _i32p]),
 }
 
 

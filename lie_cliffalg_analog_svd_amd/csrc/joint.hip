@@ -202,8 +202,17 @@ __global__ __launch_bounds__(64) void joint_decide_kernel(JointArgs a, int layer
         }
         scale *= 0.5;
     }
+    // rejected (:383-385): the reference rotated rows and columns (i, j) and
+    // rotated them back.  In finite arithmetic that restores them up to
+    // rounding, and the accept test only fails on non-finite energies, so
+    // what the round trip leaves is its NaN pattern: a NaN angle turns
+    // rows and columns i, j of every matrix into NaN (flag 2 with NaN c, s);
+    // a finite angle mixes NaNs between the two rows and the two columns,
+    // which the NaN-propagating identity (c, s) = (1, 0) does (0 * NaN =
+    // NaN, 0 * x = 0).  The basis is left alone (:377-379).
     if (lane == 0) {
-        *rot = JointRot{1.0, 0.0, 0};
+        const double nan = __builtin_nan("");
+        *rot = isfinite(theta) ? JointRot{1.0, 0.0, 2} : JointRot{nan, nan, 2};
         atomicAdd(&a.fam[f].rejected, 1);
     }
 }
@@ -246,16 +255,18 @@ __global__ __launch_bounds__(256) void joint_apply_kernel(JointArgs a, int layer
     for (int q = 0; q < SPB; ++q) {
         const int i = rid[2 * q], j = rid[2 * q + 1];
         double c = 1.0, s = 0.0;
+        bool rotate = false;  // flag 0 (skipped): rows copied unchanged
         if (!is_basis && s0 + q < a.slots && j >= 0) {
             const JointRot r = rt[s0 + q];
             c = r.c;
             s = r.s;
+            rotate = r.flag != 0;
         }
         for (int col = threadIdx.x; col < n; col += blockDim.x) {
             const double xi = i >= 0 ? w[(size_t)i * n + col] : 0.0;
             const double xj = j >= 0 ? w[(size_t)j * n + col] : 0.0;
-            rows[(2 * q) * n + col] = c * xi - s * xj;
-            rows[(2 * q + 1) * n + col] = s * xi + c * xj;
+            rows[(2 * q) * n + col] = rotate ? c * xi - s * xj : xi;
+            rows[(2 * q + 1) * n + col] = rotate ? s * xi + c * xj : xj;
         }
     }
     __syncthreads();
@@ -288,11 +299,15 @@ __global__ __launch_bounds__(256) void joint_apply_kernel(JointArgs a, int layer
                 }
             }
             if (other < n) {
-                partner = other;
-                is_first = col < other;
                 const JointRot r = rt[slot];
-                c = r.c;
-                s = r.s;
+                // matrices take accepted and rejected rotors (flag 1, 2), the
+                // basis accepted ones only
+                if (r.flag == 1 || (r.flag == 2 && !is_basis)) {
+                    partner = other;
+                    is_first = col < other;
+                    c = r.c;
+                    s = r.s;
+                }
             }
         }
 #pragma unroll
@@ -356,6 +371,24 @@ __global__ void joint_diag_kernel(JointArgs a, double* diag) {
     }
 }
 
+// ---- the layer-pair map itself (diagnostic) -----------------------------------
+__global__ void joint_pairs_kernel(int n, int layers, int slots, int32_t* out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= layers * slots) return;
+    int i, j;
+    joint_pair(n, t / slots, t % slots, i, j);
+    out[2 * t] = i;
+    out[2 * t + 1] = j;
+}
+
+hipError_t joint_launch_pairs(int n, int32_t* out, hipStream_t st) {
+    const int layers = n % 2 == 0 ? n - 1 : n, slots = (n + 1) / 2;
+    const int total = layers * slots;
+    if (total <= 0) return hipSuccess;
+    hipLaunchKernelGGL(joint_pairs_kernel, dim3((total + 255) / 256), dim3(256), 0, st, n, layers, slots, out);
+    return hipGetLastError();
+}
+
 // ---- launches --------------------------------------------------------------------
 constexpr int kSPB = 2;
 
@@ -367,13 +400,14 @@ hipError_t joint_launch_init(const JointArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t joint_launch_sweep(const JointArgs& a, hipStream_t st) {
-    const int layers = a.n % 2 == 0 ? a.n - 1 : a.n;
+int joint_layers(const JointArgs& a) { return a.n % 2 == 0 ? a.n - 1 : a.n; }
+
+hipError_t joint_launch_layers(const JointArgs& a, hipStream_t st, int first, int count, bool sweep_end) {
     const int chunks = (a.slots + kSPB - 1) / kSPB;
     const int basis_chunks = ((a.n + 1) / 2 + kSPB - 1) / kSPB;
     const int cz = chunks > basis_chunks ? chunks : basis_chunks;
     const size_t lds = sizeof(double) * 2 * kSPB * a.n;
-    for (int layer = 0; layer < layers; ++layer) {
+    for (int layer = first; layer < first + count; ++layer) {
         hipLaunchKernelGGL(joint_decide_kernel, dim3(a.slots, a.F), dim3(64), 0, st, a, layer);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -381,6 +415,7 @@ hipError_t joint_launch_sweep(const JointArgs& a, hipStream_t st) {
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
+    if (!sweep_end) return hipSuccess;
     hipLaunchKernelGGL(joint_sweep_end_kernel, dim3(a.F), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -386,9 +386,10 @@ hipError_t jsvd_launch_init(const JsvdArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
-hipError_t jsvd_launch_sweep(const JsvdArgs& a, hipStream_t st) {
-    const int layers = (a.kk & 1) ? a.kk : a.kk - 1;
-    for (int layer = 0; layer < layers; ++layer) {
+int jsvd_layers(const JsvdArgs& a) { return (a.kk & 1) ? a.kk : a.kk - 1; }
+
+hipError_t jsvd_launch_layers(const JsvdArgs& a, hipStream_t st, int first, int count, bool sweep_end) {
+    for (int layer = first; layer < first + count; ++layer) {
         hipLaunchKernelGGL(jsvd_decide_kernel, dim3(a.slots, a.F), dim3(64), 0, st, a, layer);
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
@@ -397,6 +398,7 @@ hipError_t jsvd_launch_sweep(const JsvdArgs& a, hipStream_t st) {
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
+    if (!sweep_end) return hipSuccess;
     hipLaunchKernelGGL(jsvd_sweep_end_kernel, dim3(a.F), dim3(256), 0, st, a);
     return hipGetLastError();
 }

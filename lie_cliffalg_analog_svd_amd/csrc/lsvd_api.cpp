@@ -374,7 +374,8 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const
     }
     if (p->n_micro > 0) {
         const int l = least();
-        MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, fb_count};
+        MicroLaunch ml{p->d_items, p->n_micro, pools, p->d_items + p->fallback_first, fb_count,
+                       p->h->micro_max_sweeps};
         if (tev && (e = hipEventRecord(tev[0], p->side[l])) != hipSuccess) return e;
         e = launch_micro(ml, p->side[l]);
         if (e != hipSuccess) return e;
@@ -558,6 +559,11 @@ int lsvd_create(int device, lsvd_handle_t* out) {
         char* end = nullptr;
         const long v = std::strtol(env, &end, 10);
         if (end != env && v >= 2) h->large_min_n = (int)std::min<long>(v, kLargeMaxN + 1);
+    }
+    if (const char* env = std::getenv("LSVD_MICRO_MAX_SWEEPS")) {
+        char* end = nullptr;
+        const long v = std::strtol(env, &end, 10);
+        if (end != env && v >= 1) h->micro_max_sweeps = (int)std::min<long>(v, 24);
     }
     *out = h;
     return LSVD_OK;

@@ -54,6 +54,7 @@ struct lsvd_handle_s {
     std::map<uint64_t, int> tune;                 // autotune winners: bucket key -> variant
     std::map<uint64_t, lsvd_plan_s*> plan_cache;  // one-item plans of the single-matrix API
     int large_min_n = 129;                        // square single solves from this n run chip-wide
+    int micro_max_sweeps = 24;                    // LieSvdMicro's sweep limit for n = 3, 4 (test hook)
     lsvd::Extras* extras = nullptr;
 };
 

@@ -34,45 +34,123 @@ struct DevBuf {
 
 namespace lsvd {
 
-int joint_run_sweeps(lsvd_handle_t h, const JointArgs& a, int max_sweeps, JointFamily* fam) {
-    hipStream_t st = h->stream;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) return hip_fail(h, e, "hipStreamBeginCapture");
-    hipError_t le = joint_launch_sweep(a, st);
-    e = hipStreamEndCapture(st, &graph);
-    if (e != hipSuccess) return hip_fail(h, e, "hipStreamEndCapture");
-    if (le != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return hip_fail(h, le, "joint sweep launch");
+int sweep_graph_chunks(int layers, int nodes_per_layer, int max_nodes, int32_t* first, int32_t* count,
+                       int max_chunks) {
+    if (layers <= 0) return 0;
+    // layers per graph, leaving room for the stop-test node of the last one
+    const int per = std::max(1, (max_nodes - 1) / std::max(nodes_per_layer, 1));
+    int n = 0;
+    for (int f = 0; f < layers; f += per, ++n) {
+        if (n < max_chunks && first && count) {
+            first[n] = f;
+            count[n] = std::min(per, layers - f);
+        }
     }
-    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        (void)hipGraphDestroy(graph);
-        return hip_fail(h, e, "hipGraphInstantiate");
+    return n;
+}
+
+// Captures one sweep as graphs of at most kMaxGraphNodes kernel nodes
+// (launch(st, first, count, sweep_end) enqueues a layer range) and replays
+// them until every family has stopped.  A single graph of a whole sweep
+// reached 2047 nodes at n = 1024; under rocprofv3's kernel tracing such a
+// launch ended in a host-side segmentation fault (graphs of up to 511 nodes
+// replay under the tracer).
+template <class Launch>
+static int run_sweep_graphs(lsvd_handle_t h, int layers, const Launch& launch, int max_sweeps, const JointFamily* d_fam,
+                            JointFamily* fam, int F, const char* what) {
+    hipStream_t st = h->stream;
+    const int nch = sweep_graph_chunks(layers, 2, kMaxGraphNodes, nullptr, nullptr, 0);
+    std::vector<int32_t> first(nch), count(nch);
+    sweep_graph_chunks(layers, 2, kMaxGraphNodes, first.data(), count.data(), nch);
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
+    auto cleanup = [&]() {
+        for (hipGraphExec_t x : execs) (void)hipGraphExecDestroy(x);
+        for (hipGraph_t g : graphs) (void)hipGraphDestroy(g);
+    };
+    for (int c = 0; c < nch; ++c) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t x = nullptr;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+            cleanup();
+            return hip_fail(h, e, "hipStreamBeginCapture");
+        }
+        hipError_t le = launch(st, first[c], count[c], c == nch - 1);
+        e = hipStreamEndCapture(st, &g);
+        if (e != hipSuccess || le != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            cleanup();
+            return hip_fail(h, e != hipSuccess ? e : le, what);
+        }
+        graphs.push_back(g);
+        e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            cleanup();
+            return hip_fail(h, e, "hipGraphInstantiate");
+        }
+        execs.push_back(x);
     }
     int rc = LSVD_OK;
     for (int sweep = 0; sweep < max_sweeps; ++sweep) {
-        e = hipGraphLaunch(exec, st);
-        if (e == hipSuccess) e = hipMemcpyAsync(fam, a.fam, sizeof(JointFamily) * a.F, hipMemcpyDeviceToHost, st);
+        hipError_t e = hipSuccess;
+        for (int c = 0; c < nch && e == hipSuccess; ++c) e = hipGraphLaunch(execs[c], st);
+        if (e == hipSuccess) e = hipMemcpyAsync(fam, d_fam, sizeof(JointFamily) * F, hipMemcpyDeviceToHost, st);
         if (e == hipSuccess) e = hipStreamSynchronize(st);
         if (e != hipSuccess) {
-            rc = hip_fail(h, e, "joint sweep");
+            rc = hip_fail(h, e, what);
             break;
         }
         bool all = true;
-        for (int f = 0; f < a.F; ++f) all = all && fam[f].done;
+        for (int f = 0; f < F; ++f) all = all && fam[f].done;
         if (all) break;
     }
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
+    cleanup();
     return rc;
+}
+
+int joint_run_sweeps(lsvd_handle_t h, const JointArgs& a, int max_sweeps, JointFamily* fam) {
+    auto launch = [&](hipStream_t st, int first, int count, bool end) {
+        return joint_launch_layers(a, st, first, count, end);
+    };
+    return run_sweep_graphs(h, joint_layers(a), launch, max_sweeps, a.fam, fam, a.F, "joint sweep");
+}
+
+int jsvd_run_sweeps(lsvd_handle_t h, const JsvdArgs& a, int max_sweeps, JointFamily* fam) {
+    auto launch = [&](hipStream_t st, int first, int count, bool end) {
+        return jsvd_launch_layers(a, st, first, count, end);
+    };
+    return run_sweep_graphs(h, jsvd_layers(a), launch, max_sweeps, a.fam, fam, a.F, "joint svd sweep");
 }
 
 }  // namespace lsvd
 
 extern "C" {
+
+int lsvd_sweep_graph_chunks(int layers, int nodes_per_layer, int max_nodes, int32_t* first, int32_t* count,
+                            int max_chunks) {
+    return sweep_graph_chunks(layers, nodes_per_layer, max_nodes, first, count, max_chunks);
+}
+
+int lsvd_max_graph_nodes(void) { return kMaxGraphNodes; }
+
+int lsvd_joint_layer_pairs(lsvd_handle_t h, int n, int32_t* out) {
+    if (!h || !out) return LSVD_E_ARG;
+    if (n < 0) return set_error(h, LSVD_E_SHAPE, "negative size");
+    if (n > 4096) return set_error(h, LSVD_E_LIMIT, "n > 4096");
+    const int layers = n % 2 == 0 ? n - 1 : n, slots = (n + 1) / 2;
+    const size_t cnt = (size_t)std::max(layers, 0) * slots * 2;
+    if (cnt == 0) return LSVD_OK;
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    DevBuf buf;
+    if ((e = buf.ensure(sizeof(int32_t) * cnt)) != hipSuccess) return hip_fail(h, e, "hipMalloc");
+    if ((e = joint_launch_pairs(n, (int32_t*)buf.p, h->stream)) != hipSuccess) return hip_fail(h, e, "pairs kernel");
+    if ((e = hipMemcpyAsync(out, buf.p, sizeof(int32_t) * cnt, hipMemcpyDeviceToHost, h->stream)) != hipSuccess)
+        return hip_fail(h, e, "hipMemcpyAsync");
+    if ((e = hipStreamSynchronize(h->stream)) != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
+    return LSVD_OK;
+}
 
 // JointDiagonalizationParams::for_n (src/lie_svd_joint.rs:26-35).
 int lsvd_joint_params_for_n(int n, lsvd_joint_params_t* out) {
@@ -254,38 +332,7 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
     JCHECK(jsvd_launch_init(a, st));
     std::vector<JointFamily> fam(families);
     if (kk > 1 && prm.max_sweeps > 0) {
-        // one sweep (2 launches per layer + the stop test) as a graph,
-        // relaunched until every family has stopped
-        hipGraph_t graph = nullptr;
-        hipGraphExec_t exec = nullptr;
-        JCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
-        hipError_t le = jsvd_launch_sweep(a, st);
-        JCHECK(hipStreamEndCapture(st, &graph));
-        if (le != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            return hip_fail(h, le, "joint svd sweep launch");
-        }
-        e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-        if (e != hipSuccess) {
-            (void)hipGraphDestroy(graph);
-            return hip_fail(h, e, "hipGraphInstantiate");
-        }
-        int rc2 = LSVD_OK;
-        for (int sweep = 0; sweep < prm.max_sweeps; ++sweep) {
-            e = hipGraphLaunch(exec, st);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(fam.data(), a.fam, sizeof(JointFamily) * families, hipMemcpyDeviceToHost, st);
-            if (e == hipSuccess) e = hipStreamSynchronize(st);
-            if (e != hipSuccess) {
-                rc2 = hip_fail(h, e, "joint svd sweep");
-                break;
-            }
-            bool all = true;
-            for (auto& s : fam) all = all && s.done;
-            if (all) break;
-        }
-        (void)hipGraphExecDestroy(exec);
-        (void)hipGraphDestroy(graph);
+        int rc2 = jsvd_run_sweeps(h, a, prm.max_sweeps, fam.data());
         if (rc2 != LSVD_OK) return rc2;
     }
     JCHECK(jsvd_launch_final(a, d_sig, d_vt, st));

@@ -45,7 +45,10 @@ inline int jsvd_matrix_row_tasks(int slots, int rows, int kk) { return slots + (
 inline int jsvd_basis_row_tasks(int n) { return (n + 1) / 2; }
 
 hipError_t jsvd_launch_init(const JsvdArgs& a, hipStream_t st);
-hipError_t jsvd_launch_sweep(const JsvdArgs& a, hipStream_t st);
+// Layers [first, first + count) of a sweep (2 launches each), then the
+// stop test when sweep_end.
+int jsvd_layers(const JsvdArgs& a);
+hipError_t jsvd_launch_layers(const JsvdArgs& a, hipStream_t st, int first, int count, bool sweep_end);
 hipError_t jsvd_launch_final(const JsvdArgs& a, double* sig, double* vt, hipStream_t st);
 
 }  // namespace lsvd

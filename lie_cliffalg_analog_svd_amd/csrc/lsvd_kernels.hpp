@@ -56,6 +56,7 @@ struct MicroLaunch {
     Pools pools;
     Item* fallback;        // device, items escalated to LieSvdSmall
     int* fallback_count;   // device, zeroed before the launch
+    int sweep_cap = 24;    // sweep limit of n = 3, 4 (the reference's 24)
 };
 
 hipError_t launch_micro(const MicroLaunch& L, hipStream_t st);

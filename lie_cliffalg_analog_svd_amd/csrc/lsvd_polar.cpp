@@ -207,10 +207,20 @@ int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, do
     }
     PCHK(polar_zero(S + 2, 1, st));
     PCHK(ns_orth(m, n, X, S + 2, st));
+    // A-dependent check: Q is A's polar factor iff Q^T A is symmetric
+    // (positive semidefinite), so ||Q^T A - A^T Q||_F / ||A||_F measures
+    // how far Q is from it in the directions orthogonality cannot see
+    // (one FP64 n x n x m product into M)
+    PCHK(gemm_f64(true, n, n, m, X, dA, (double*)bM.p, 1.0, st));
+    PCHK(polar_skew_sumsq((const double*)bM.p, n, S + 1, S + 4, st));
     PCHK(hipMemcpyAsync(hs, S, sizeof(hs), hipMemcpyDeviceToHost, st));
     PCHK(hipStreamSynchronize(st));
     inf.orth_err = std::sqrt(std::max(hs[2], 0.0));
-    inf.polar_ok = (!bad64 && inf.orth_err <= 1e-8 * rootn) ? 1 : 0;
+    inf.skew_err = std::sqrt(std::max(hs[1], 0.0)) / std::max(std::sqrt(hs[0]), 1e-300);
+    // the reference's trust test (:174, :178), plus the skew test at the
+    // same threshold: a factor that is orthogonal but rotated away from A's
+    // polar factor in its small-sigma directions is not trusted
+    inf.polar_ok = (!bad64 && inf.orth_err <= 1e-8 * rootn && inf.skew_err <= 1e-8 * rootn) ? 1 : 0;
     (void)converged;
     inf.lowp_iters = it_low;
     inf.f64_iters = it64;

@@ -33,6 +33,8 @@ hipError_t gemm_lowp(int prec, bool trans_a, int M, int N, int K, const float* A
 // *out = ||x||^2 (overwritten), deterministic; partials: kSumsqBlocks doubles.
 constexpr int kSumsqBlocks = 256;
 hipError_t polar_sumsq(const double* x, size_t n, double* out, double* partials, hipStream_t st);
+// *out = ||B - B^T||_F^2 of an n x n matrix (overwritten), deterministic.
+hipError_t polar_skew_sumsq(const double* b, int n, double* out, double* partials, hipStream_t st);
 hipError_t polar_scale(const double* x, size_t n, const double* sumsq, void* y, bool f32, hipStream_t st);
 hipError_t polar_widen(const float* x, size_t n, double* y, hipStream_t st);
 hipError_t polar_zero(double* p, int n, hipStream_t st);

@@ -6,7 +6,8 @@
 // to 24 sweeps.  A matrix whose residual off-diagonal energy stays above
 // 1e-10 * max(||A||_F, 1) escalates to LieSvdSmall (:90-92): it is appended
 // to a fallback queue that a small_svd launch drains later in the same
-// stream (or graph).
+// stream (or graph).  The sweep limit of n = 3, 4 can be lowered from the
+// host (the handle's micro_max_sweeps) to force that escalation in tests.
 #include <hip/hip_runtime.h>
 #include <math.h>
 
@@ -67,9 +68,9 @@ __device__ __forceinline__ void rotor_pair(double (&w)[16], double (&ub)[16], do
 
 // Returns true when converged (false: escalate to LieSvdSmall).
 template <int N>
-__device__ bool micro_solve(const double* a, double* u, double* s, double* vt, int& rotations) {
+__device__ bool micro_solve(const double* a, double* u, double* s, double* vt, int& rotations, int sweep_cap) {
     constexpr int kPairs = N == 2 ? 1 : (N == 3 ? 3 : 6);
-    constexpr int kSweeps = N == 2 ? 1 : 24;
+    const int kSweeps = N == 2 ? 1 : sweep_cap;
     constexpr int pi_[6] = {0, 2, 0, 1, 0, 1};  // 4x4 layers {(0,1),(2,3)},{(0,2),(1,3)},{(0,3),(1,2)}
     constexpr int pj_[6] = {1, 3, 2, 3, 3, 2};
     constexpr int qi_[3] = {0, 0, 1};            // 3x3 cycle (0,1),(0,2),(1,2)
@@ -139,7 +140,7 @@ __device__ bool micro_solve(const double* a, double* u, double* s, double* vt, i
 }
 
 __global__ __launch_bounds__(256) void micro_svd_kernel(const Item* __restrict__ items, int count, Pools P,
-                                                         Item* fallback, int* fallback_count) {
+                                                         Item* fallback, int* fallback_count, int sweep_cap) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= count) return;
     const Item itm = items[b];
@@ -159,13 +160,13 @@ __global__ __launch_bounds__(256) void micro_svd_kernel(const Item* __restrict__
             break;
         }
         case 2:
-            ok = micro_solve<2>(a, u, s, vt, rot);
+            ok = micro_solve<2>(a, u, s, vt, rot, sweep_cap);
             break;
         case 3:
-            ok = micro_solve<3>(a, u, s, vt, rot);
+            ok = micro_solve<3>(a, u, s, vt, rot, sweep_cap);
             break;
         case 4:
-            ok = micro_solve<4>(a, u, s, vt, rot);
+            ok = micro_solve<4>(a, u, s, vt, rot, sweep_cap);
             break;
         default:
             break;
@@ -188,8 +189,9 @@ __global__ __launch_bounds__(256) void micro_svd_kernel(const Item* __restrict__
 hipError_t launch_micro(const MicroLaunch& L, hipStream_t st) {
     if (L.count <= 0) return hipSuccess;
     const int nt = 256;
+    const int cap = L.sweep_cap >= 1 && L.sweep_cap <= 24 ? L.sweep_cap : 24;
     hipLaunchKernelGGL(micro_svd_kernel, dim3((L.count + nt - 1) / nt), dim3(nt), 0, st, L.items, L.count, L.pools,
-                       L.fallback, L.fallback_count);
+                       L.fallback, L.fallback_count, cap);
     return hipGetLastError();
 }
 

@@ -334,6 +334,34 @@ __global__ void sumsq_final_kernel(const double* partials, double* out) {
     if (threadIdx.x == 0) *out = s;
 }
 
+// ||B - B^T||_F^2 of an n x n matrix: every ordered pair (i, j) once, so
+// each skew entry is counted twice as in the Frobenius norm; a 32 x 32
+// tile and its mirror are staged in LDS so both reads coalesce.
+__global__ __launch_bounds__(256) void skew_sumsq_kernel(const double* b, int n, double* partials) {
+    __shared__ double t[32][33];
+    __shared__ double red[4];
+    const int tiles = (n + 31) / 32;
+    double s = 0.0;
+    for (int tIdx = blockIdx.x; tIdx < tiles * tiles; tIdx += gridDim.x) {
+        const int I = tIdx / tiles, J = tIdx % tiles;
+        __syncthreads();
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {  // mirror tile (J, I), stored transposed
+            const int r = e / 32, c = e % 32, gi = J * 32 + r, gj = I * 32 + c;
+            t[c][r] = (gi < n && gj < n) ? b[(size_t)gi * n + gj] : 0.0;
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < 32 * 32; e += 256) {
+            const int r = e / 32, c = e % 32, gi = I * 32 + r, gj = J * 32 + c;
+            if (gi < n && gj < n) {
+                const double d = b[(size_t)gi * n + gj] - t[r][c];
+                s += d * d;
+            }
+        }
+    }
+    s = block_sum<256>(s, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = s;
+}
+
 // y = x / max(sqrt(*sumsq), 1e-300), into f64 or f32 storage
 template <class T>
 __global__ void scale_kernel(const double* x, size_t n, const double* sumsq, T* y) {
@@ -438,6 +466,14 @@ hipError_t gemm_lowp(int prec, bool trans_a, int M, int N, int K, const float* A
 
 hipError_t polar_sumsq(const double* x, size_t n, double* out, double* partials, hipStream_t st) {
     hipLaunchKernelGGL(sumsq_kernel, dim3(kSumsqBlocks), dim3(256), 0, st, x, n, partials);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(kSumsqBlocks), 0, st, partials, out);
+    return hipGetLastError();
+}
+
+hipError_t polar_skew_sumsq(const double* b, int n, double* out, double* partials, hipStream_t st) {
+    hipLaunchKernelGGL(skew_sumsq_kernel, dim3(kSumsqBlocks), dim3(256), 0, st, b, n, partials);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(sumsq_final_kernel, dim3(1), dim3(kSumsqBlocks), 0, st, partials, out);

@@ -124,6 +124,8 @@ def lib():
         L.lso_qr_reduce.argtypes = [dp, C.c_int, C.c_int, dp, dp]
         L.lso_micro_solve.argtypes = [dp, C.c_int, dp, dp, dp, C.POINTER(Info)]
         L.lso_micro_solve.restype = C.c_int
+        L.lso_micro_solve_capped.argtypes = [dp, C.c_int, C.c_int, dp, dp, dp, C.POINTER(Info)]
+        L.lso_micro_solve_capped.restype = C.c_int
         L.lso_joint_params_for_n.argtypes = [C.c_int]
         L.lso_joint_params_for_n.restype = JointParams
         L.lso_joint_diag_sym.argtypes = [dp, C.c_int, C.c_int, JointParams, dp, dp, C.POINTER(JointTrace)]
@@ -233,14 +235,16 @@ def qr_reduce(a):
     return q, r
 
 
-def micro_solve(a):
-    """LieSvdMicro::solve (src/lie_svd_micro.rs:21-52) -> (U, s, Vt, info)."""
+def micro_solve(a, sweep_cap: int = 0):
+    """LieSvdMicro::solve (src/lie_svd_micro.rs:21-52) -> (U, s, Vt, info).
+    sweep_cap > 0 lowers the 24-sweep limit of n = 3, 4 (a test hook that
+    forces the escalation to LieSvdSmall, :90-92)."""
     a = _c(a)
     n = a.shape[0]
     assert a.shape == (n, n) and n <= 4
     u = np.zeros((n, n)); s = np.zeros(n); vt = np.zeros((n, n))
     info = Info()
-    rc = lib().lso_micro_solve(_dp(a), n, _dp(u), _dp(s), _dp(vt), C.byref(info))
+    rc = lib().lso_micro_solve_capped(_dp(a), n, int(sweep_cap), _dp(u), _dp(s), _dp(vt), C.byref(info))
     assert rc == 0
     return u, s, vt, info.as_dict()
 

@@ -412,8 +412,12 @@ static double offdiag_norm(const double* w, int n) { /* src/lie_svd_micro.rs:101
 }
 
 /* LieSvdMicro::solve + solve_fixed_schedule + extract_sorted_svd,
- * src/lie_svd_micro.rs:21-95,141-214.  n <= 4.  Returns 0, or -1 if n > 4. */
-int lso_micro_solve(const double* mat, int n, double* u, double* s, double* vt, lso_info* info) {
+ * src/lie_svd_micro.rs:21-95,141-214.  n <= 4.  Returns 0, or -1 if n > 4.
+ * sweep_cap (test hook, no reference counterpart) lowers the 24-sweep limit
+ * of n = 3, 4 so that the escalation to LieSvdSmall (:90-92) can be forced;
+ * 0 keeps the reference's limit. */
+int lso_micro_solve_capped(const double* mat, int n, int sweep_cap, double* u, double* s, double* vt,
+                           lso_info* info) {
     lso_info local;
     memset(&local, 0, sizeof local);
     if (n < 0 || n > 4) return -1;
@@ -435,6 +439,7 @@ int lso_micro_solve(const double* mat, int n, double* u, double* s, double* vt, 
     const int(*pairs)[2] = n == 2 ? p2 : (n == 3 ? p3 : p4);
     int npairs = n == 2 ? 1 : (n == 3 ? 3 : 6);
     int max_sweeps = n == 2 ? 1 : 24;
+    if (n > 2 && sweep_cap > 0 && sweep_cap < max_sweeps) max_sweeps = sweep_cap;
     double w[16], ub[16], vb[16];
     memcpy(w, mat, sizeof(double) * n * n);
     memset(ub, 0, sizeof ub);
@@ -502,6 +507,10 @@ int lso_micro_solve(const double* mat, int n, double* u, double* s, double* vt, 
     }
     if (info) *info = local;
     return 0;
+}
+
+int lso_micro_solve(const double* mat, int n, double* u, double* s, double* vt, lso_info* info) {
+    return lso_micro_solve_capped(mat, n, 0, u, s, vt, info);
 }
 
 /* ------------------------------------------------------------------ */

@@ -128,3 +128,36 @@ def test_solvers_raise_without_a_gpu():
         LieSvdSmall.solve(np.eye(3))
     with pytest.raises(_lib.LsvdError):
         LieSvdTensor.hosvd3(np.ones((2, 2, 2)))
+
+
+def _chunks(layers, npl, max_nodes):
+    lib = _lib.lib()
+    nch = lib.lsvd_sweep_graph_chunks(layers, npl, max_nodes, None, None, 0)
+    first = (ctypes.c_int32 * max(nch, 1))()
+    count = (ctypes.c_int32 * max(nch, 1))()
+    assert lib.lsvd_sweep_graph_chunks(layers, npl, max_nodes, first, count, nch) == nch
+    return [(first[c], count[c]) for c in range(nch)]
+
+
+@pytest.mark.parametrize("n", [2, 3, 16, 129, 255, 256, 257, 511, 512, 1023, 1024, 2047, 2048, 4096])
+def test_sweep_graph_chunks_bound_and_cover(n):
+    """Phase-JADE / joint-SVD sweeps are captured as graphs of at most
+    lsvd_max_graph_nodes() kernel nodes (two per layer, one stop test at the
+    end): the chunks cover every layer once, in order, and none exceeds the
+    bound (a 2047-node sweep graph, n = 1024, ended in a host-side fault
+    under the kernel tracer)."""
+    layers = n - 1 if n % 2 == 0 else n
+    cap = _lib.lib().lsvd_max_graph_nodes()
+    assert 2 < cap <= 512
+    ch = _chunks(layers, 2, cap)
+    nxt = 0
+    for c, (f, k) in enumerate(ch):
+        assert f == nxt and k >= 1
+        nodes = 2 * k + (1 if c == len(ch) - 1 else 0)
+        assert nodes <= cap
+        nxt = f + k
+    assert nxt == layers
+    if n <= 256:
+        assert len(ch) == 1  # n <= 256 keeps the whole sweep in one graph
+    assert _chunks(0, 2, cap) == []
+    assert _chunks(5, 3, 7) == [(0, 2), (2, 2), (4, 1)]

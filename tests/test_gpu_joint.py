@@ -172,6 +172,7 @@ def test_joint_svd_batched_families_and_errors():
 JADE_SMOKE = {16: (2.537e0, 9.848e-14, 6), 64: (5.835e0, 1.398e-12, 8), 128: (8.353e0, 2.695e-12, 10),
               256: (1.198e1, 8.459e-12, 11)}
 JOINT_SVD_SMOKE = {16: (1.267e1, 9.231e-14, None), 64: (4.688e1, 4.582e-12, 12), 128: (1.076e2, 6.987e-12, 16)}
+JADE_PUBLISHED_ROTATIONS = {16: 681, 64: 14917, 128: 63520}
 
 
 @pytest.mark.parametrize("n", sorted(JADE_SMOKE))
@@ -189,7 +190,15 @@ def test_jade_reference_smoke(n):
     print("  gpu", tr, flush=True)
     assert "%.3e" % tr.initial_offdiag == "%.3e" % p_init
     assert tr.final_offdiag < 4.0 * p_final
-    assert tr.sweeps <= p_sweeps + 2 and tr.rejected_rotations == 0
+    # the published sweep counts exactly, and the published rotation counts
+    # (README.md:1345-1347) at N = 16, 64, 128; at N = 256 the published
+    # line, the oracle and the GPU differ by a few of the 270k rotations
+    # taken at the rounding floor
+    assert tr.sweeps == p_sweeps and tr.rejected_rotations == 0
+    if n in JADE_PUBLISHED_ROTATIONS:
+        assert tr.rotations == JADE_PUBLISHED_ROTATIONS[n]
+    else:
+        assert abs(tr.rotations - 270435) <= 10
     assert np.linalg.norm(v.T @ v - np.eye(n)) < 1e-12 * n
     assert workloads.recovery_error(q, v) < 1e-10
     i = np.arange(n, dtype=np.float64)
@@ -209,13 +218,123 @@ def test_joint_svd_reference_smoke(n):
     p_init, p_final, p_sweeps = JOINT_SVD_SMOKE[n]
     mats, u0, v0 = workloads.joint_svd_stress_family(n)
     u, sg, vt, tr = LieSvdJoint.joint_svd(list(mats))
-    print("  gpu", tr, flush=True)
+    uo, so, vto, tro = oracle.joint_svd(mats, oracle.joint_params_for_n(n))
+    print("  gpu", tr, "oracle", tro, flush=True)
     assert "%.3e" % tr.initial_offdiag == "%.3e" % p_init
     assert tr.final_offdiag < 4.0 * p_final
-    if p_sweeps is not None:
-        assert tr.sweeps <= p_sweeps + 2
+    # the published sweep counts exactly (N = 64: 12 sweeps and the published
+    # 23384 rotations; N = 128: 16 sweeps, rotations within 2 % of the
+    # published 118627 -- the last sweep works at the rounding floor of the
+    # family's bits, where the oracle stops one sweep earlier); N = 16 has no
+    # published count and equals the oracle's
+    if p_sweeps is None:
+        assert tr.sweeps == tro["sweeps"] and tr.rotations == tro["rotations"]
+    else:
+        assert tr.sweeps == p_sweeps
+    if n == 64:
+        assert tr.rotations == 23384
+    if n == 128:
+        assert abs(tr.rotations / 118627 - 1.0) < 0.02
     assert workloads.recovery_error(u0, u) < 1e-10 and workloads.recovery_error(v0, vt.T) < 1e-10
     i = np.arange(n, dtype=np.float64)
     for k in range(len(mats)):
         want = np.sort(1.0 + 0.15 * k + 0.05 * i)
         assert np.max(np.abs(np.sort(sg[k]) - want) / want) < 1e-12
+
+
+@pytest.mark.parametrize("n,k,seed", [(8, 4, 170), (12, 3, 171), (16, 8, 172)])
+def test_joint_svd_line_search_rejects_vs_oracle(n, k, seed):
+    """accept_joint_rotor's halving line search and roll-back (src/lie_svd_
+    joint.rs:264-291, 359-386) on families with no common SVD: the averaged
+    2 x 2 angles are not the minimisers of the joint energy, so the oracle
+    rejects hundreds of rotations.  The GPU takes the same decisions: the
+    sweep, rotation and rejection counts equal the oracle's, and so do the
+    final residual and the factors."""
+    from lie_cliffalg_analog_svd_amd import JointDiagonalizationParams as P
+    print("joint svd random", n, k, seed, flush=True)
+    mats = rng.normal_matrix(seed, k * n, n).reshape(k, n, n)
+    u, sg, vt, tr = LieSvdJoint.joint_svd(list(mats))
+    uo, so, vto, tro = oracle.joint_svd(mats, oracle.joint_params_for_n(n))
+    print("  gpu", tr, "oracle", tro, flush=True)
+    assert tro["rejected_rotations"] > 50
+    assert tr.sweeps == tro["sweeps"] and tr.rotations == tro["rotations"]
+    assert tr.rejected_rotations == tro["rejected_rotations"]
+    assert abs(tr.final_offdiag - tro["final_offdiag"]) <= 1e-10 * tro["initial_offdiag"]
+    assert np.max(np.abs(u - uo)) < 1e-9 and np.max(np.abs(vt - vto)) < 1e-9
+    for f in range(k):
+        assert np.max(np.abs(sg[f] - so[f])) < 1e-9 * max(1.0, float(np.max(so[f])))
+
+
+@pytest.mark.parametrize("n", [2, 3])
+def test_jade_line_search_rejects_nonfinite_vs_oracle(n):
+    """The symmetric line search (src/lie_svd_joint.rs:359-386) rejects only
+    non-finite trials: for finite input the chosen angle is the exact
+    minimiser of the pair energy (it is C + R cos(4 theta - psi) and the
+    pi/4 alternative sits at its minimum), and the oracle rejects nothing
+    even on random families run to the sweep cap.  A NaN entry makes every
+    trial non-finite; the reference then rotates rows and columns (i, j) and
+    rolls them back, which leaves NaN in them.  With n = 2, 3 each layer has
+    one pair, so the GPU's layer-parallel decisions see what the reference's
+    sequential ones see: the counts, the basis and the NaN pattern of the
+    diagonals equal the oracle's."""
+    print("jade nonfinite", n, flush=True)
+    if n == 2:
+        mats = np.array([[[1.0, np.nan], [np.nan, 2.0]], [[0.5, 0.25], [0.25, -1.0]]])
+    else:
+        mats = np.array([[[1.0, 2.0, 3.0], [2.0, np.nan, 1.0], [3.0, 1.0, 0.5]],
+                         [[2.0, 0.5, 0.0], [0.5, 1.0, 0.25], [0.0, 0.25, 3.0]]])
+    v, d, tr = LieSvdJoint.diagonalize_symmetric(list(mats))
+    vo, do, tro = oracle.joint_diag_sym(mats, oracle.joint_params_for_n(n))
+    print("  gpu", tr, "oracle", tro, flush=True)
+    assert tro["rejected_rotations"] > 0
+    assert tr.sweeps == tro["sweeps"] and tr.rotations == tro["rotations"]
+    assert tr.rejected_rotations == tro["rejected_rotations"]
+    assert np.array_equal(v, vo)
+    for f in range(len(mats)):
+        assert np.array_equal(np.isnan(d[f]), np.isnan(do[f]))
+        fin = ~np.isnan(do[f])
+        assert np.allclose(d[f][fin], do[f][fin], rtol=0, atol=1e-14)
+
+
+def test_layer_pair_map_bit_exact():
+    """The device's round-robin map (slot, layer) -> (i, j) that the Phase-JADE
+    and joint-SVD kernels use equals the reference's layer_pairs
+    (src/lie_svd_joint.rs:516-539, restated in the oracle) for even and odd
+    n, pair for pair and in order."""
+    import ctypes
+    from lie_cliffalg_analog_svd_amd import _lib
+    from lie_cliffalg_analog_svd_amd.handle import default_handle
+    h = default_handle()
+    for n in list(range(1, 41)) + [127, 128, 255, 256, 511, 1024, 2047, 2048]:
+        layers = oracle.layer_count(n)
+        slots = (n + 1) // 2
+        out = np.full(max(layers * slots * 2, 1), -7, dtype=np.int32)
+        h.check(_lib.lib().lsvd_joint_layer_pairs(h.raw, n, out.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+        got = out[:layers * slots * 2].reshape(max(layers, 0), slots, 2)
+        for layer in range(layers):
+            dev = [(int(i), int(j)) for i, j in got[layer] if i >= 0]
+            assert all(j == -1 for i, j in got[layer] if i < 0)
+            assert dev == [tuple(p) for p in oracle.layer_pairs(n, layer)], (n, layer)
+
+
+def test_jade_n512_planted_basis():
+    """Phase-JADE above n = 256 (the reference has no size limit): one family
+    of K = 4 symmetric 512 x 512 matrices Q diag(d_k) Q^T + 1e-6 E_k.  A sweep
+    here is 1021 kernel launches, captured as graphs of at most
+    lsvd_max_graph_nodes() nodes.  The planted Q is recovered up to
+    permutation and sign, V stays orthogonal, the off-diagonal residual
+    drops to the noise level and the diagonals match the planted d_k."""
+    from scipy.optimize import linear_sum_assignment
+    n, K, eps = 512, 4, 1e-6
+    fams, qs, ds = workloads.jade_batch(1, K, n, eps, seed=512)
+    print("jade n=512", fams.shape, flush=True)
+    v, d, trs = diagonalize_families(fams)
+    tr = trs[0]
+    print("  gpu", tr, flush=True)
+    assert np.linalg.norm(v[0].T @ v[0] - np.eye(n)) < 1e-10
+    c = np.abs(qs[0].T @ v[0])
+    r, cidx = linear_sum_assignment(-c)
+    assert np.max(np.abs(1.0 - c[r, cidx])) < 1e-6
+    assert tr.final_offdiag < 1e-4 * tr.initial_offdiag
+    for k in range(K):
+        assert np.max(np.abs(np.sort(d[0, k]) - np.sort(ds[0][k]))) < 1e-4

@@ -104,18 +104,90 @@ def test_polar_f64_scaled_schedule(m, n, lo):
 
 
 @pytest.mark.parametrize("prec", [LSVD_PREC_F32, LSVD_PREC_BF16])
-def test_polar_mixed_precision(prec):
-    """Mixed precision: orthogonal to FP64 precision; error against the
-    exact polar factor bounded by the low-precision unit roundoff times
-    the condition number (kappa = 1e2 here)."""
+def test_polar_mixed_precision_well_conditioned(prec):
+    """Mixed precision at kappa = 1e2: orthogonal to FP64 precision after the
+    FP64 refinement; the error against the exact polar factor is that of the
+    low precision (u_low kappa), and the A-dependent residual
+    skew_err = ||Q^T A - A^T Q||_F / ||A||_F sees it, so the factor is only
+    trusted where that residual is below 1e-8 sqrt(n)."""
     m, n = 384, 256
     a, s, u, v = workloads.polar_matrix(m, n, seed=12, lo=1e-2)
     print("polar mixed", prec, flush=True)
     q, ok = newton_schulz_polar(a, prec)
     info = polar.last_info
-    assert ok
+    print("  info", info, flush=True)
     assert info["orth_err"] < 1e-12 * np.sqrt(n)
     assert info["lowp_iters"] > 0 and info["f64_iters"] <= 8
     err = np.linalg.norm(q - u @ v.T, 2)
-    unit = 2.0 ** -8 if prec == LSVD_PREC_BF16 else 2.0 ** -24
-    assert err < 50 * unit * 1e2, err
+    skew_exact = np.linalg.norm(q.T @ a - a.T @ q) / np.linalg.norm(a)
+    assert abs(info["skew_err"] - skew_exact) <= 1e-12 + 1e-6 * skew_exact
+    if prec == LSVD_PREC_F32:
+        assert err < 50 * 2.0 ** -24 * 1e2, err
+    else:
+        assert 1e-4 < err < 0.5, err
+        assert not ok and info["skew_err"] > 1e-8 * np.sqrt(n)
+    assert ok == (info["skew_err"] <= 1e-8 * np.sqrt(n))
+
+
+@pytest.mark.parametrize("m,n", [(512, 512), (512, 256)])
+def test_polar_kappa_1e6_vs_oracle(m, n):
+    """Config-3 spectra (s log-uniform in [1e-6, 1]) at a size the oracle
+    runs in seconds.  FP64 (plain and scaled steps) equals the oracle's
+    newton_schulz_polar to the FP64 limit of this conditioning and is
+    trusted; the FP32 and BF16 iterations end orthogonal to FP64 precision
+    but rotated away from A's polar factor in its small-sigma directions
+    (u_low kappa > 1): the skew residual flags them (polar_ok = 0) and their
+    error against the oracle is stated, not hidden."""
+    from lie_cliffalg_analog_svd_amd import LSVD_PREC_F64_SCALED
+    a, s, u, v = workloads.polar_matrix(m, n, seed=0x5EED0003, lo=1e-6)
+    print("polar kappa 1e6", m, n, flush=True)
+    qo, oko, it_o = oracle.newton_schulz_polar(a)
+    assert oko
+    rootn = np.sqrt(n)
+    for prec in (LSVD_PREC_F64, LSVD_PREC_F64_SCALED):
+        q, ok = newton_schulz_polar(a, prec)
+        info = polar.last_info
+        print("  f64", prec, info, flush=True)
+        assert ok and info["skew_err"] < 1e-12
+        assert np.linalg.norm(q - qo) / rootn < 1e-9
+    for prec in (LSVD_PREC_F32, LSVD_PREC_BF16):
+        q, ok = newton_schulz_polar(a, prec)
+        info = polar.last_info
+        err = np.linalg.norm(q - qo) / rootn
+        print("  lowp", prec, info, "err vs oracle", err, flush=True)
+        assert info["orth_err"] < 1e-12 * rootn
+        assert not ok and info["skew_err"] > 1e-8 * rootn
+        assert err > 1e-6
+
+
+@pytest.mark.parametrize("m,n", [(4096, 4096), (8192, 2048)])
+def test_polar_config3_full_size(m, n):
+    """Config 3 at full size: FP64 scaled steps agree with the plain FP64
+    iteration to 1e-9 (Frobenius / sqrt(n)) and both are trusted; the
+    low-precision routes are orthogonal to FP64 precision and report
+    polar_ok = 0 through the skew residual, with their error against the
+    FP64 factor stated (measured, see the bench's polar leg)."""
+    import torch
+    from lie_cliffalg_analog_svd_amd import LSVD_PREC_F64_SCALED
+    a, s, u, v = workloads.polar_matrix(m, n, seed=0x5EED0003, lo=1e-6)
+    print("polar config 3", m, n, flush=True)
+    h = Handle(0)
+    h.set_pointer_mode(LSVD_PTR_DEVICE)
+    ad = torch.from_numpy(a).cuda()
+    out = {}
+    for prec in (LSVD_PREC_F64, LSVD_PREC_F64_SCALED, LSVD_PREC_F32, LSVD_PREC_BF16):
+        qd = torch.empty_like(ad)
+        info = polar.polar_device(h, ad, m, n, qd, prec)
+        h.synchronize()
+        out[prec] = (qd, info)
+        print("  ", prec, info, flush=True)
+    h.close()
+    rootn = np.sqrt(n)
+    q64 = out[LSVD_PREC_F64][0]
+    for prec, (qd, info) in out.items():
+        assert info["orth_err"] < 1e-12 * rootn
+        e = float(torch.linalg.norm(qd - q64)) / rootn
+        if prec in (LSVD_PREC_F64, LSVD_PREC_F64_SCALED):
+            assert info["polar_ok"] == 1 and info["skew_err"] < 1e-12 and e < 1e-9, (prec, e)
+        else:
+            assert info["polar_ok"] == 0 and info["skew_err"] > 1e-8 * rootn and e > 1e-6, (prec, e)

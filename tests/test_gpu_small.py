@@ -193,6 +193,40 @@ def test_micro_degenerate_and_router():
     assert u3.shape == (9, 9)
 
 
+def test_micro_escalation_to_small(monkeypatch):
+    """LieSvdMicro's fallback to LieSvdSmall::solve (src/lie_svd_micro.rs:
+    90-92).  No finite input found reaches it within the reference's 24
+    sweeps (the cyclic two-sided rotations converge quadratically), so the
+    sweep limit of n = 3, 4 is lowered from the host (LSVD_MICRO_MAX_SWEEPS,
+    read when a handle is created; the oracle's sweep_cap is the same hook):
+    after one sweep most 3 x 3 and 4 x 4 matrices are not converged and go
+    through the batch plan's fallback queue, drained by a LieSvdSmall launch.
+    The escalated set, the info path (2) and sigma must match the oracle's,
+    and 2 x 2 (one rotor, never escalates) stays on the micro path."""
+    from lie_cliffalg_analog_svd_amd import Handle, svd_batched
+    monkeypatch.setenv("LSVD_MICRO_MAX_SWEEPS", "1")
+    h = Handle(0)
+    mats = [uniform_matrix(4400 + b, 2 + b % 3, 2 + b % 3, -2.0, 2.0) for b in range(48)]
+    out, info = svd_batched(mats, handle=h, with_info=True)
+    escalated = 0
+    for b, a in enumerate(mats):
+        uo, so, vto, io = oracle.micro_solve(a, sweep_cap=1)
+        u, s, vt = out[b]
+        what = f"micro capped b={b} n={a.shape[0]}"
+        assert info[b]["micro_fallback"] == io["micro_fallback"], what
+        assert info[b]["path"] == (2 if io["micro_fallback"] else 1), what
+        assert_factors(a, u, s, vt, 1e-12, what)
+        assert_sigma_close(s, so, a.shape[0], what=what)
+        escalated += io["micro_fallback"]
+        if io["micro_fallback"]:
+            assert info[b]["ns_iters"] > 0, what  # the LieSvdSmall path ran
+    h.close()
+    assert escalated >= 16  # the fallback queue really carried work
+    # the same matrices with the reference's limit: nothing escalates
+    out2, info2 = svd_batched(mats, with_info=True)
+    assert all(i["path"] == 1 for i in info2)
+
+
 def test_micro_sign_convention():
     u, s, vt = LieSvdMicro.solve(np.diag([1.0, -5.0, 2.0]))
     assert np.array_equal(s, [5.0, 2.0, 1.0])

@@ -366,6 +366,70 @@ def test_round_robin_schedule(n):
     assert sorted(seen) == [(a, b) for a in range(n) for b in range(a + 1, n)]
 
 
+# ---- metrics: closed forms (src/lib.rs:149-194) -------------------------------
+
+def test_metrics_closed_forms_exact_factors():
+    """Exact factors of a diagonal and of a permuted diagonal matrix give
+    residuals of exactly zero (every product is exact in binary64)."""
+    d = np.array([5.0, 3.0, 2.0, 0.5])
+    o = oracle.metrics(np.diag(d), np.eye(4), d, np.eye(4))
+    assert o["rel_recon"] == 0.0 and o["orth_u"] == 0.0 and o["orth_v"] == 0.0
+    p = np.eye(4)[[2, 0, 3, 1]]
+    o = oracle.metrics(p @ np.diag(d), p, d, np.eye(4))
+    assert o["rel_recon"] == 0.0 and o["orth_u"] == 0.0 and o["orth_v"] == 0.0
+    # A = P diag(d) Q^T with Vt = Q^T a permutation too
+    q = np.eye(4)[[1, 3, 0, 2]]
+    o = oracle.metrics(p @ np.diag(d) @ q.T, p, d, q.T)
+    assert o["rel_recon"] == 0.0 and o["orth_u"] == 0.0 and o["orth_v"] == 0.0
+
+
+def test_metrics_closed_forms_perturbed_factors():
+    """U = I + e E_01: U^T U - I = e (E + E^T) + e^2 E^T E has Frobenius norm
+    sqrt(2 e^2 + e^4); Vt = I + f E_12 the same with f; a perturbed A gives
+    rel_recon = delta / ||A||_F.  With e, f, delta powers of two every
+    intermediate is exact, so the results are the correctly rounded closed
+    forms.  A dropped e^2 term, a missing square root or a wrong norm would
+    each change the value by far more than one ulp."""
+    e, f, delta = 2.0 ** -10, 2.0 ** -7, 2.0 ** -12
+    n = 3
+    u = np.eye(n); u[0, 1] = e
+    vt = np.eye(n); vt[1, 2] = f
+    s = np.array([4.0, 2.0, 1.0])
+    a = (u * s[None, :]) @ vt
+    a2 = a.copy(); a2[2, 0] += delta
+    o = oracle.metrics(a2, u, s, vt)
+    assert o["orth_u"] == math.sqrt(2 * e * e + e ** 4)
+    assert o["orth_v"] == math.sqrt(2 * f * f + f ** 4)
+    assert abs(o["rel_recon"] - delta / math.sqrt(float(np.sum(a2 * a2)))) <= 2 * EPS * o["rel_recon"]
+    # exact factors of the same A: zero reconstruction error
+    assert oracle.metrics(a, u, s, vt)["rel_recon"] == 0.0
+
+
+def test_metrics_closed_forms_sigma_errors():
+    """sigma_max_rel / sigma_tail_rel: both sides sorted descending, the tail
+    from floor(3n/4), the 1e-300 floor under a zero reference value."""
+    n = 4
+    a = np.diag([4.0, 3.0, 2.0, 1.0])
+    eye = np.eye(n)
+    # error only in the largest value: max = 2^-10, tail (i >= 3) = 0
+    o = oracle.metrics(a, eye, np.array([4.0 * (1 + 2.0 ** -10), 3.0, 2.0, 1.0]), eye, np.array([4.0, 3.0, 2.0, 1.0]))
+    assert o["sigma_max_rel"] == 2.0 ** -10 and o["sigma_tail_rel"] == 0.0
+    # error only in the smallest: it is in the tail
+    o = oracle.metrics(a, eye, np.array([4.0, 3.0, 2.0, 1.0 + 2.0 ** -20]), eye, np.array([4.0, 3.0, 2.0, 1.0]))
+    assert o["sigma_max_rel"] == 2.0 ** -20 and o["sigma_tail_rel"] == 2.0 ** -20
+    # unsorted inputs are sorted before comparing: no error
+    o = oracle.metrics(a, eye, np.array([1.0, 2.0, 3.0, 4.0]), eye, np.array([4.0, 2.0, 3.0, 1.0]))
+    assert o["sigma_max_rel"] == 0.0 and o["sigma_tail_rel"] == 0.0
+    # a zero reference value divides by the 1e-300 floor
+    o = oracle.metrics(np.diag([1.0, 0.0]), np.eye(2), np.array([1.0, 1e-310]), np.eye(2), np.array([1.0, 0.0]))
+    assert o["sigma_max_rel"] == 1e-310 / 1e-300 and o["sigma_tail_rel"] == 1e-310 / 1e-300
+    # n = 8: the tail starts at index 6, so an error at index 5 is not in it
+    d = np.arange(8.0, 0.0, -1.0)
+    got = d.copy(); got[5] *= 1 + 2.0 ** -8
+    o = oracle.metrics(np.diag(d), np.eye(8), got, np.eye(8), d)
+    assert o["sigma_max_rel"] == 2.0 ** -8 and o["sigma_tail_rel"] == 0.0
+
+
 # ---- metrics: two independent implementations agree --------------------------
 
 def test_metrics_match_package_restatement():

@@ -63,10 +63,10 @@ struct Sh2 {
     double tau[NP];
     double scr[NP];
     double2 cs[2][Cfg2<NP>::TPR * Cfg2<NP>::CSP];  // per layer parity: (c, s oriented to the slot order),
-                                                   // slot k at csi(k)
-    double tv[2][NP / 2];   // oriented t of the slot's diagonal update
-    int rf[2][NP / 2];      // rotation applied (|a_pq| >= 1e-300)
+                                                   // slot k at csi(k) (the V rows' reads)
+    double2 csp[2][NP / 2];                        // the same, slot k at k (the P blocks' reads)
     int rank[NP];
+    alignas(16) double sink[128];  // write target of angle-wave lanes that hold no block
     int item;
 };
 
@@ -77,6 +77,65 @@ __device__ __forceinline__ int csi(int k) {
 
 __device__ __forceinline__ d4v mfma64(double a, double b, d4v c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// 1/sqrt(x) and 1/x for x within a few binades of 1 (the angle inputs are
+// scaled by a power of two first): the hardware estimates (v_rsq_f64,
+// v_rcp_f64, relative error ~2^-26) refined by two Newton steps each, a
+// shorter dependent chain than the IEEE sqrt and division sequences (whose
+// range fix-ups these inputs never need).
+__device__ __forceinline__ double rsq_nr(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+__device__ __forceinline__ double rcp_nr(double x) {
+    double y = __builtin_amdgcn_rcp(x);
+    y = fma(fma(-x, y, 1.0), y, y);
+    y = fma(fma(-x, y, 1.0), y, y);
+    return y;
+}
+
+// The rotation of one pair (src/lie_svd_small.rs:217-243) in the form of
+// jacobi_angle (lsvd_device.hpp): t = sgn |2 a_pq| / (|d| + r), c = (|d| +
+// r) / h, s = sgn |2 a_pq| / h with r = sqrt(d^2 + 4 a_pq^2), h = sqrt((|d| +
+// r)^2 + 4 a_pq^2), d and 2 a_pq scaled by one power of two so that the
+// larger is in [0.5, 1).
+__device__ __forceinline__ int jacobi_angle_nr(double x00, double x11, double apq, bool fwd, double& c, double& sv,
+                                               double& tv) {
+    const bool rot = fabs(apq) >= 1e-300;
+    const double app = fwd ? x00 : x11, aqq = fwd ? x11 : x00;
+    const double d = aqq - app, a2 = 2.0 * apq;
+    const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(d), fabs(a2)));
+    const double ds = ldexp(d, -ex), as = fabs(ldexp(a2, -ex));
+    const double x = fma(ds, ds, as * as);
+    const double r = x * rsq_nr(x);
+    const double den = fabs(ds) + r;
+    const double sgn = (d == 0.0 || ((d > 0.0) == (a2 > 0.0))) ? 1.0 : -1.0;  // sgn(theta), theta = +-0 -> +
+    const double ih = rsq_nr(fma(den, den, as * as));
+    const double tt = sgn * as * rcp_nr(den);
+    c = rot ? den * ih : 1.0;
+    const double sn = rot ? sgn * as * ih : 0.0;
+    sv = fwd ? sn : -sn;
+    tv = rot ? (fwd ? tt : -tt) : 0.0;
+    return rot ? 1 : 0;
+}
+
+// Lane i takes lane i + 1's value (DPP wave_shl:1) / lane i - 1's (wave_shr:1);
+// a lane without a source keeps its own.  The whole wave must execute them.
+__device__ __forceinline__ double from_next_lane(double x) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false));
+}
+
+__device__ __forceinline__ double from_prev_lane(double x) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x138, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, 0x138, 0xf, 0xf, false));
 }
 
 // One 16 x 16 tile: acc += A (16 x K) B (K x 16); fa(i, k), fb(k, j);
@@ -358,70 +417,141 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     clk.mark(kPhP);
 
     // ---- Jacobi on P (:212-285) in ring-position space.  P is stored
-    // permuted so that the pairs of every layer sit at positions (2k, 2k+1),
-    // and only its upper triangle is kept.  Each thread owns fixed 2x2
-    // blocks (k_a <= k_b) of that layout, rotates them and writes them to
-    // where the next layer needs them (transposed into the upper triangle
-    // when the move crosses the diagonal): one read, four writes per block
-    // and two barriers per layer.  V's update for a layer is deferred to the
-    // next layer's first phase, where it overlaps the angle computation.
-    // Within a row, even positions are stored first and odd ones after
-    // them (pcol), so the lanes of a wave, which hold consecutive k_b, read
-    // and write consecutive doubles (a stride of two doubles would put two
-    // lanes of each half-wave on one LDS bank).
-    constexpr int HP = NP / 2;
-    auto pcol = [](int c) { return (c & 1) * HP + (c >> 1); };
-    for (int e = tid; e < NP * NP; e += NT) ys[e] = main[(e / NP) * LDM + e % NP];
-    __syncthreads();
-    for (int e = tid; e < NP * NP; e += NT) {
-        const int g = e / NP, hh = e % NP;
-        main[g * LDM + pcol(hh)] = ys[ring_idx0(NP, g) * NP + ring_idx0(NP, hh)];
+    // permuted so that the pairs of every layer sit at positions (2k, 2k+1):
+    // its upper triangle as 2 x 2 blocks (ka <= kb) of the S = NP / 2 slots,
+    // packed diagonal by diagonal (block (ka, ka + d) at base(d) + ka, four
+    // planes for the entries 00, 01, 10, 11: a layer moves every entry by
+    // one block along or across the diagonals, so consecutive lanes of a
+    // diagonal read and write consecutive doubles), in two buffers: layer L reads one and writes
+    // its rotated blocks straight to the positions layer L + 1 needs them at
+    // in the other, so a layer needs one barrier.  Wave 0 (the angle wave)
+    // rotates the blocks on and next to the diagonal (kb - ka <= 2) itself:
+    // those are all the next layer's diagonal blocks come from, so it forms
+    // them in registers and computes layer L + 1's angles while the other
+    // waves rotate the rest of P and V for layer L.  V lives in registers
+    // in the same position space (TPR lanes per row): a layer is a rotation
+    // of register pairs plus a fixed cyclic shift (register moves and one
+    // neighbour-lane exchange each way).
+    constexpr int S = NP / 2, NBLK = S * (S + 1) / 2;
+    constexpr int NR = (S - 3) * (S - 2) / 2;  // blocks with kb - ka >= 3, from base(3) = 3S - 3 on
+    constexpr int NRT = NW == 1 ? NT : NT - 64;  // threads taking those
+    constexpr int MBR = (NR + NRT - 1) / NRT;
+    static_assert(S >= 4, "ring layout");
+    auto base = [](int d) { return d * S - (d * (d - 1)) / 2; };  // first block of diagonal d
+    // storage offset (within one buffer) of ring position (r, c)
+    auto poff = [&](int r, int c) {
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        return (((lo & 1) << 1) | (hi & 1)) * NBLK + base((hi >> 1) - (lo >> 1)) + (lo >> 1);
+    };
+    // block index -> (diagonal d, ka)
+    auto decode = [](int b, int& d, int& ka) {
+        d = 0;
+        while (b >= S - d) {
+            b -= S - d;
+            ++d;
+        }
+        ka = b;
+    };
+    double* pb = main;  // 2 x 4 NBLK doubles: NP (NP + 2) <= the workgroup's LDS area
+    double ref = 0.0;
+    {
+        for (int e = tid; e < NP * NP; e += NT) {
+            const double x = main[(e / NP) * LDM + e % NP];
+            ys[e] = x;
+            ref += x * x;
+        }
+        ref = fmax(sqrt(block_sum<NT>(ref, sh.red)), 1e-300);  // its barriers also order the ys writes
+        for (int b = tid; b < NBLK; b += NT) {
+            int dd, ka;
+            decode(b, dd, ka);
+            const int kb = ka + dd;
+#pragma unroll
+            for (int cmp = 0; cmp < 4; ++cmp) {
+                const int g = 2 * ka + (cmp >> 1), hh = 2 * kb + (cmp & 1);
+                pb[cmp * NBLK + b] = ys[ring_idx0(NP, g) * NP + ring_idx0(NP, hh)];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
-    const int vrow = tid / TPR, vch = tid % TPR;
+    // V rows leave the angle wave alone when the other waves can hold them
+    constexpr int VBASE = (NW > 1 && NP * TPR <= NT - 64) ? 64 : 0;
+    const int vrow = tid >= VBASE ? (tid - VBASE) / TPR : NP, vch = (tid - VBASE) % TPR;
     double v[CH];
 #pragma unroll
     for (int j = 0; j < CH; ++j) {
         const int idx = ring_idx0(NP, vch * CH + j);
         v[j] = (vrow < N && idx == vrow) ? 1.0 : 0.0;
     }
-    double ref = 0.0;
-    for (int e = tid; e < NP * NP; e += NT) {
-        const double x = main[(e / NP) * LDM + e % NP];
-        ref += x * x;
-    }
-    ref = fmax(sqrt(block_sum<NT>(ref, sh.red)), 1e-300);
-    constexpr int NBLK = SLOTS * (SLOTS + 1) / 2;
-    constexpr int MB = (NBLK + NT - 1) / NT;
-    // per block: packed (k_a, k_b, csi(k_a), csi(k_b)) and the four
-    // upper-triangle write offsets
-    int bk[MB], bw01[MB], bw23[MB];
+    // regular blocks (kb - ka >= 3) of this thread: packed (ka, kb, csi(ka),
+    // csi(kb)), source block index and the four write offsets
+    const int rt = NW == 1 ? tid : tid - 64;
+    int rk[MBR], rsrc[MBR], rw01[MBR], rw23[MBR];
 #pragma unroll
-    for (int i = 0; i < MB; ++i) {
-        const int bI = tid + i * NT;
-        int ka = 0, rem = bI;
-        while (ka < SLOTS && rem >= SLOTS - ka) {
-            rem -= SLOTS - ka;
-            ++ka;
+    for (int i = 0; i < MBR; ++i) {
+        const int j = rt + i * NRT;
+        rk[i] = -1;
+        rsrc[i] = rw01[i] = rw23[i] = 0;
+        if (rt >= 0 && j < NR) {
+            int dd, ka;
+            decode(base(3) + j, dd, ka);
+            const int kb = ka + dd;
+            const int da0 = ring_dst(NP, 2 * ka), da1 = ring_dst(NP, 2 * ka + 1);
+            const int db0 = ring_dst(NP, 2 * kb), db1 = ring_dst(NP, 2 * kb + 1);
+            rk[i] = ka | (kb << 8);
+            rsrc[i] = base(3) + j;
+            rw01[i] = poff(da0, db0) | (poff(da0, db1) << 16);
+            rw23[i] = poff(da1, db0) | (poff(da1, db1) << 16);
         }
-        const int kb = ka + rem;
-        const int da0 = ring_dst(NP, 2 * ka), da1 = ring_dst(NP, 2 * ka + 1);
-        const int db0 = ring_dst(NP, 2 * kb), db1 = ring_dst(NP, 2 * kb + 1);
-        auto up = [&](int r, int c) { return r <= c ? r * LDM + pcol(c) : c * LDM + pcol(r); };
-        bk[i] = bI < NBLK ? (ka | (kb << 8) | (csi<NP>(ka) << 16) | (csi<NP>(kb) << 24)) : -1;
-        bw01[i] = up(da0, db0) | (up(da0, db1) << 16);
-        bw23[i] = up(da1, db0) | (up(da1, db1) << 16);
     }
+    // angle wave, lane k < S: diagonal block (k, k), (k, k + 1) for
+    // k <= S - 2 and (k - 1, k + 1) for 1 <= k <= S - 2
+    const bool awave = w == 0;
+    const int k = tid;
+    const bool aslot = awave && k < S;
+    const bool has1 = aslot && k <= S - 2, has2 = aslot && k >= 1 && k <= S - 2;
+    int b0 = 0, b1 = 0, b2 = 0, w0d = 0, w1d = 0, w1a = 0, w1b = 0, w2a = 0, w2b = 0;
+    if (aslot) {
+        const int p0 = ring_dst(NP, 2 * k), p1 = ring_dst(NP, 2 * k + 1);
+        b0 = k;
+        w0d = poff(p0, p0) | (poff(p0, p1) << 16);
+        w1d = poff(p1, p1);
+        if (has1) {
+            const int q0 = ring_dst(NP, 2 * k + 2), q1 = ring_dst(NP, 2 * k + 3);
+            b1 = S + k;
+            w1a = poff(p0, q0) | (poff(p0, q1) << 16);
+            w1b = poff(p1, q0) | (poff(p1, q1) << 16);
+        }
+        if (has2) {
+            const int r0 = ring_dst(NP, 2 * k - 2), r1 = ring_dst(NP, 2 * k - 1);
+            const int q0 = ring_dst(NP, 2 * k + 2), q1 = ring_dst(NP, 2 * k + 3);
+            b2 = 2 * S - 1 + k - 1;
+            w2a = poff(r0, q0) | (poff(r0, q1) << 16);
+            w2b = poff(r1, q0) | (poff(r1, q1) << 16);
+        }
+    }
+    // slot k's rotation for the current layer and its neighbours' (c, s)
+    double ac = 1.0, as = 0.0, at = 0.0, acn = 1.0, asn = 0.0, acp = 1.0, asp = 0.0;
+    int arf = 0;
+    auto angle = [&](int L, double x00, double x11, double apq) {
+        int a, b;
+        ring_pair(NP, L, k, a, b);
+        arf = jacobi_angle_nr(x00, x11, apq, a < b, ac, as, at);
+    };
+    auto neighbours = [&]() {  // the whole wave executes the lane moves
+        acn = from_next_lane(ac);
+        asn = from_next_lane(as);
+        acp = from_prev_lane(ac);
+        asp = from_prev_lane(as);
+    };
     const int max_sweeps = 60 + 20 * N;
-    int sweeps = 0, rots = 0;
+    int sweeps = 0, rots = 0, G = 0;
     const int lane = tid & 63;
     const int nbr_next = (lane + 1) & 63, nbr_prev = (lane + 63) & 63;
-    const int ka_thread = tid - (NT - 64);  // the last wave computes the angles
-    // rotate V's own slots with layer `buf`'s angles, then shift the ring
-    auto v_update = [&](int buf) {
+    // rotate V's own slots with the layer's angles, then shift the ring
+    auto v_update = [&](const double2* csl) {
 #pragma unroll
         for (int j = 0; j < CH / 2; ++j) {
-            const double2 r = sh.cs[buf][vch * Cfg2<NP>::CSP + j];
+            const double2 r = csl[vch * Cfg2<NP>::CSP + j];
             const double x = v[2 * j], y = v[2 * j + 1];
             v[2 * j] = r.x * x - r.y * y;
             v[2 * j + 1] = r.y * x + r.x * y;
@@ -438,83 +568,117 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         for (int j = CH - 1; j >= 3; j -= 2) v[j] = v[j - 2];
         v[1] = (vch == 0) ? e2 : plo;
     };
+    if (awave) {  // the first layer's angles
+        if (aslot) {
+            angle(0, pb[b0], pb[3 * NBLK + b0], pb[NBLK + b0]);
+            sh.cs[0][csi<NP>(k)] = make_double2(ac, as);
+            sh.csp[0][k] = make_double2(ac, as);
+        }
+        neighbours();
+    }
+    __syncthreads();
     for (int sw = 0; sw < max_sweeps && N > 1; ++sw) {
-        double off = 0.0;
-        for (int e = tid; e < NP * NP; e += NT) {
-            const int g = e / NP, hh = e % NP;
-            const double x = main[g * LDM + pcol(hh)];
-            if (g < hh) off += x * x;
+        const double* cur = pb + (G & 1) * 4 * NBLK;
+        double off = 0.0;  // upper off-diagonal entries: a diagonal block holds one (plane 01)
+        for (int b = tid; b < NBLK; b += NT) {
+            const double x01 = cur[NBLK + b];
+            off += x01 * x01;
+            if (b >= S) {  // not on the diagonal
+                const double x00 = cur[b], x10 = cur[2 * NBLK + b], x11 = cur[3 * NBLK + b];
+                off += x00 * x00 + x10 * x10 + x11 * x11;
+            }
         }
         off = block_sum<NT>(off, sh.red);
         if (!(sqrt(off) >= kJacobiTol * ref)) break;
         ++sweeps;
 #pragma unroll 1
-        for (int L = 0; L < NP - 1; ++L) {
-            const int buf = L & 1;
-            // phase A: rotation of every slot of layer L (:217-243)
-            if (ka_thread >= 0 && ka_thread < SLOTS) {
-                const int k = ka_thread;
-                int a, b;
-                ring_pair(NP, L, k, a, b);
-                double c, sv, tv;
-                const int rot = jacobi_angle(main[2 * k * LDM + k], main[(2 * k + 1) * LDM + HP + k],
-                                             main[2 * k * LDM + HP + k], a < b, c, sv, tv);
-                rots += rot;
-                sh.cs[buf][csi<NP>(k)] = make_double2(c, sv);
-                sh.tv[buf][k] = tv;
-                sh.rf[buf][k] = rot;
+        for (int L = 0; L < NP - 1; ++L, ++G) {
+            const double* cp = pb + (G & 1) * 4 * NBLK;
+            double* np_ = pb + ((G + 1) & 1) * 4 * NBLK;
+            const double2* csl = sh.cs[G & 1];
+            const double2* cpl = sh.csp[G & 1];
+            if (awave) {
+                // all reads first (lanes without a block read their diagonal
+                // block's slot, which is in range), then the three rotations
+                // as independent chains, then the writes of the blocks held
+                const int s1 = has1 ? b1 : b0, s2 = has2 ? b2 : b0;
+                const double x00 = cp[b0], x01 = cp[NBLK + b0], x11 = cp[3 * NBLK + b0];
+                const double c00 = cp[s1], c01 = cp[NBLK + s1], c10 = cp[2 * NBLK + s1], c11 = cp[3 * NBLK + s1];
+                const double e00 = cp[s2], e01 = cp[NBLK + s2], e10 = cp[2 * NBLK + s2], e11 = cp[3 * NBLK + s2];
+                rots += aslot ? arf : 0;
+                // diagonal block (:244-249): the exact 2 x 2 update
+                const double n00 = arf ? x00 - at * x01 : x00;
+                const double n11 = arf ? x11 + at * x01 : x11;
+                const double n01 = arf ? 0.0 : x01;
+                // (k, k + 1): rows with slot k, columns with slot k + 1
+                const double p00 = ac * c00 - as * c10, p01 = ac * c01 - as * c11;
+                const double p10 = as * c00 + ac * c10, p11 = as * c01 + ac * c11;
+                const double m00 = acn * p00 - asn * p01, m01 = asn * p00 + acn * p01;
+                const double m10 = acn * p10 - asn * p11, d1n11 = asn * p10 + acn * p11;
+                // (k - 1, k + 1): rows with slot k - 1, columns with slot k + 1
+                const double q00 = acp * e00 - asp * e10, q01 = acp * e01 - asp * e11;
+                const double q10 = asp * e00 + acp * e10, q11 = asp * e01 + acp * e11;
+                const double r00 = acn * q00 - asn * q01, r01 = asn * q00 + acn * q01;
+                const double r10 = acn * q10 - asn * q11, r11 = asn * q10 + acn * q11;
+                // one basic block with the angle below, so the scheduler can
+                // fill the angle's dependent chain with these stores
+                double* sink = &sh.sink[2 * lane];
+                *(aslot ? np_ + (w0d & 0xffff) : sink) = n00;
+                *(aslot ? np_ + (w0d >> 16) : sink) = n01;
+                *(aslot ? np_ + w1d : sink) = n11;
+                *(has1 ? np_ + (w1a & 0xffff) : sink) = m00;
+                *(has1 ? np_ + (w1a >> 16) : sink) = m01;
+                *(has1 ? np_ + (w1b & 0xffff) : sink) = m10;
+                *(has1 ? np_ + (w1b >> 16) : sink) = d1n11;
+                *(has2 ? np_ + (w2a & 0xffff) : sink) = r00;
+                *(has2 ? np_ + (w2a >> 16) : sink) = r01;
+                *(has2 ? np_ + (w2b & 0xffff) : sink) = r10;
+                *(has2 ? np_ + (w2b >> 16) : sink) = r11;
+                // next (0, 1) <- (0, 2); next (2k, 2k + 1) <- (2k - 1, 2k + 2)
+                double apqn = k == 0 ? m00 : r10;
+                double e00n = 0.0, e11n = 0.0;
+                // the next layer's diagonal block of slot k (positions move
+                // as ring_dst: 2k <- 2k + 2, 2k + 1 <- 2k - 1, with the ring's
+                // fixed point 0, 1 <- 2 and 2S - 2 <- 2S - 1)
+                const double n00_next = from_next_lane(n00);
+                const double n11_prev = from_prev_lane(n11);
+                const double d1_prev = from_prev_lane(d1n11);
+                e00n = k == 0 ? n00 : (k == S - 1 ? n11 : n00_next);
+                e11n = k == 0 ? n00_next : n11_prev;
+                if (k == S - 1) apqn = d1_prev;  // next (2S-2, 2S-1) <- (2S-3, 2S-1)
+                angle(L + 1 < NP - 1 ? L + 1 : 0, e00n, e11n, apqn);  // lanes >= S: unused
+                *(aslot ? &sh.cs[(G + 1) & 1][csi<NP>(aslot ? k : 0)] : reinterpret_cast<double2*>(sink)) =
+                    make_double2(ac, as);
+                *(aslot ? &sh.csp[(G + 1) & 1][aslot ? k : 0] : reinterpret_cast<double2*>(sink)) =
+                    make_double2(ac, as);
+                neighbours();
             }
-            double blk[MB][4];
+            // the other blocks: rotate (:244-285), write to the next layer's positions
 #pragma unroll
-            for (int i = 0; i < MB; ++i) {
-                if (bk[i] >= 0) {
-                    const int ka = bk[i] & 255, kb = (bk[i] >> 8) & 255;
-                    const double* r0 = main + 2 * ka * LDM + kb;
-                    blk[i][0] = r0[0];
-                    blk[i][1] = r0[HP];
-                    blk[i][2] = ka == kb ? r0[HP] : r0[LDM];
-                    blk[i][3] = r0[LDM + HP];
-                }
-            }
-            if (L > 0) v_update(buf ^ 1);
-            __syncthreads();
-            // P blocks: rotate (:244-285) and move to the next layer's positions
-#pragma unroll
-            for (int i = 0; i < MB; ++i) {
-                if (bk[i] < 0) continue;
-                const int ka = bk[i] & 255, kb = (bk[i] >> 8) & 255;
-                const int ca = (bk[i] >> 16) & 255, cb = bk[i] >> 24;
-                const int w0 = bw01[i] & 0xffff, w1 = bw01[i] >> 16, w2 = bw23[i] & 0xffff, w3 = bw23[i] >> 16;
-                const double b00 = blk[i][0], b01 = blk[i][1], b10 = blk[i][2], b11 = blk[i][3];
-                if (ka == kb) {
-                    double n00 = b00, n11 = b11, n01 = b01;
-                    if (sh.rf[buf][ka]) {  // (:244-249)
-                        const double tt = sh.tv[buf][ka];
-                        n00 = b00 - tt * b01;
-                        n11 = b11 + tt * b01;
-                        n01 = 0.0;
-                    }
-                    main[w0] = n00;
-                    main[w1] = n01;
-                    main[w3] = n11;
-                    continue;
-                }
-                const double2 ra = sh.cs[buf][ca], rb = sh.cs[buf][cb];
+            for (int i = 0; i < MBR; ++i) {
+                if (rk[i] < 0) continue;
+                const int ca = rk[i] & 255, cb = rk[i] >> 8;
+                const int sb = rsrc[i];
+                const double b00 = cp[sb], b01 = cp[NBLK + sb], b10 = cp[2 * NBLK + sb], b11 = cp[3 * NBLK + sb];
+                const double2 ra = cpl[ca], rb = cpl[cb];
                 const double l00 = ra.x * b00 - ra.y * b10, l01 = ra.x * b01 - ra.y * b11;
                 const double l10 = ra.y * b00 + ra.x * b10, l11 = ra.y * b01 + ra.x * b11;
-                main[w0] = rb.x * l00 - rb.y * l01;
-                main[w1] = rb.y * l00 + rb.x * l01;
-                main[w2] = rb.x * l10 - rb.y * l11;
-                main[w3] = rb.y * l10 + rb.x * l11;
+                np_[rw01[i] & 0xffff] = rb.x * l00 - rb.y * l01;
+                np_[rw01[i] >> 16] = rb.y * l00 + rb.x * l01;
+                np_[rw23[i] & 0xffff] = rb.x * l10 - rb.y * l11;
+                np_[rw23[i] >> 16] = rb.y * l10 + rb.x * l11;
             }
+            if (vrow < NP) v_update(csl);
             __syncthreads();
         }
-        v_update((NP - 2) & 1);  // the sweep's last layer
     }
     clk.mark(kPhJacobi);
-    for (int i = tid; i < N; i += NT) {
-        const int q = ring_pos0(NP, i);
-        sh.diag[i] = main[q * LDM + pcol(q)];
+    {
+        const double* cur = pb + (G & 1) * 4 * NBLK;
+        for (int i = tid; i < N; i += NT) {
+            const int q = ring_pos0(NP, i);
+            sh.diag[i] = cur[((q & 1) ? 3 : 0) * NBLK + (q >> 1)];
+        }
     }
     __syncthreads();
     for (int i = tid; i < N; i += NT) {  // rank sort, eigenvalues descending

@@ -71,6 +71,8 @@ def algorithmic_flops(m: int, n: int, k_ns: int, sweeps: int) -> float:
 
 
 def kernel_name(variant: int) -> str:
+    if variant >= 300:
+        return f"small_svd2_multi_kernel (padded sizes 64..{variant - 300})"
     if variant >= 100:
         return f"small_svd2_kernel<{variant - 100}>"
     return "micro_svd_kernel" if variant < 0 else f"small_svd_kernel variant {variant}"

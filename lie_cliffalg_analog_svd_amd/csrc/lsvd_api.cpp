@@ -89,6 +89,17 @@ static double item_cost(int M, int N) {
 // Sets the launch shape of a bucket for `variant`; returns the global arena
 // doubles it needs (0 for LDS variants).
 static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
+    if (variant >= kV2Multi) {  // one launch over sizes 64..128: LDS and slots for the largest item
+        const int np = variant - kV2Multi;
+        bk.variant = variant;
+        const size_t lds = v2_lds_bytes(np, bk.maxM);
+        const int per_cu = small2_multi_blocks_per_cu(lds);
+        const int cus = h->lane0_cus > 0 && h->lane0_cus < h->num_cus ? h->lane0_cus : h->num_cus;
+        bk.grid = std::max(1, std::min(bk.count, per_cu * cus));
+        bk.lds_bytes = lds;
+        bk.gws_stride = v2_slot_doubles(np, bk.maxM);
+        return (size_t)bk.gws_stride * bk.grid;
+    }
     if (variant >= kV2Base) {  // on-chip kernel: LDS sized by the bucket's largest M
         const int np = variant - kV2Base;
         bk.variant = variant;
@@ -176,7 +187,10 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         const int M = std::max(mb, nb), N = std::min(mb, nb);
         int key;
         if (N <= 128 && M <= 128) {
-            key = 1000 + (N + 15) / 16 * 16;  // on-chip kernel, padded size class
+            const int np = (N + 15) / 16 * 16;
+            // on-chip kernels: one launch per padded size below 64, one
+            // launch over all the larger sizes (a queue sorted longest first)
+            key = (np >= kV2MultiMinNP && h->multi_launch) ? 2000 : 1000 + np;
         } else {
             const int cls = n_class(N);
             if (cls < 0) {
@@ -198,9 +212,13 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
     std::vector<Item> all;
     all.insert(all.end(), micro.begin(), micro.end());
     p->n_micro = (int)micro.size();
+    std::vector<int> consts;  // per bucket: its count (and the multi launch's class ends)
     for (auto& kv : by_class) {
         auto& v = kv.second;
         std::stable_sort(v.begin(), v.end(), [](const Tmp& x, const Tmp& y) { return x.cost > y.cost; });
+        if (kv.first == 2000)  // the multi-size launch drains its size classes largest first
+            std::stable_sort(v.begin(), v.end(),
+                             [](const Tmp& x, const Tmp& y) { return (x.N + 15) / 16 > (y.N + 15) / 16; });
         Bucket bk{};
         bk.first = (int)all.size();
         bk.count = (int)v.size();
@@ -217,8 +235,19 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         bk.maxN = maxN;
         bk.maxM = maxM;
         bk.arena_bytes = arena;
+        bk.consts_off = (int)consts.size();
+        consts.push_back(bk.count);
+        if (kv.first == 2000) {  // ends of the classes 128, 112, 96, 80, 64 (relative to bk.first)
+            for (int np = 128; np >= kV2MultiMinNP; np -= 16) {
+                int end = 0;
+                while (end < (int)v.size() && (v[end].N + 15) / 16 * 16 >= np) ++end;
+                consts.push_back(end);
+            }
+        }
         int vid;
-        if (kv.first >= 1000) {
+        if (kv.first == 2000) {
+            vid = kV2Multi + (maxN + 15) / 16 * 16;
+        } else if (kv.first >= 1000) {
             vid = kV2Base + (kv.first - 1000);
         } else {
             vid = default_variant(maxN, arena);
@@ -239,8 +268,7 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
     // counters: [bucket counters][fallback counter][fallback count]
     const int nb = (int)p->buckets.size();
     p->n_counters = nb + 2;
-    std::vector<int> consts(nb + 1, 0);
-    for (int i = 0; i < nb; ++i) consts[i] = p->buckets[i].count;
+    consts.push_back(0);
 
     // global arena slots for non-LDS variants, and launch grids
     size_t gws_doubles = 0;
@@ -270,8 +298,8 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
     if (!all.empty())
         PCHECK(hipMemcpy(p->d_items, all.data(), sizeof(Item) * all.size(), hipMemcpyHostToDevice));
     PCHECK(hipMalloc(&p->d_counters, sizeof(int) * p->n_counters));
-    PCHECK(hipMalloc(&p->d_consts, sizeof(int) * (nb + 1)));
-    PCHECK(hipMemcpy(p->d_consts, consts.data(), sizeof(int) * (nb + 1), hipMemcpyHostToDevice));
+    PCHECK(hipMalloc(&p->d_consts, sizeof(int) * consts.size()));
+    PCHECK(hipMemcpy(p->d_consts, consts.data(), sizeof(int) * consts.size(), hipMemcpyHostToDevice));
     if (gws_doubles) PCHECK(hipMalloc(&p->d_gws, sizeof(double) * gws_doubles));
     if (h->ptr_mode == LSVD_PTR_HOST) {
         PCHECK(hipMalloc(&p->d_a, sizeof(double) * std::max<int64_t>(1, p->a_len)));
@@ -307,6 +335,14 @@ hipError_t plan_ensure_streams(lsvd_plan_t p) {
         uint32_t mask[32];
         for (uint32_t& w : mask) w = 0xffffffffu;
         const uint32_t words = (uint32_t)std::min((h->num_cus + 31) / 32, 32);
+        if (h->lane0_cus > 0 && h->lane0_cus < h->num_cus) {  // lane 0: CUs [0, lane0_cus), the others the rest
+            const bool first = h->lanes.empty();
+            for (int c = 0; c < 32 * 32; ++c) {
+                const bool on = c < h->num_cus && (first ? c < h->lane0_cus : c >= h->lane0_cus);
+                if (on) mask[c / 32] |= 1u << (c % 32);
+                else mask[c / 32] &= ~(1u << (c % 32));
+            }
+        }
         hipStream_t s2;
         hipError_t e = hipExtStreamCreateWithCUMask(&s2, words, mask);
         if (e != hipSuccess) e = hipStreamCreateWithFlags(&s2, hipStreamNonBlocking);
@@ -361,7 +397,7 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const
         L.grid = bk.grid;
         L.lds_bytes = bk.lds_bytes;
         L.items = p->d_items + bk.first;
-        L.count_ptr = p->d_consts + i;
+        L.count_ptr = p->d_consts + bk.consts_off;
         L.counter = p->d_counters + i;
         L.pools = pools;
         L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
@@ -560,6 +596,8 @@ int lsvd_create(int device, lsvd_handle_t* out) {
         const long v = std::strtol(env, &end, 10);
         if (end != env && v >= 2) h->large_min_n = (int)std::min<long>(v, kLargeMaxN + 1);
     }
+    if (const char* env = std::getenv("LSVD_BATCH_MULTI")) h->multi_launch = std::strtol(env, nullptr, 10) != 0;
+    if (const char* env = std::getenv("LSVD_LANE0_CUS")) h->lane0_cus = (int)std::strtol(env, nullptr, 10);
     if (const char* env = std::getenv("LSVD_MICRO_MAX_SWEEPS")) {
         char* end = nullptr;
         const long v = std::strtol(env, &end, 10);
@@ -735,7 +773,7 @@ int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int3
                 L.grid = bk.grid;
                 L.lds_bytes = bk.lds_bytes;
                 L.items = p->d_items + bk.first;
-                L.count_ptr = p->d_consts + bi;
+                L.count_ptr = p->d_consts + bk.consts_off;
                 L.counter = p->d_counters + bi;
                 L.pools = pools;
                 L.gws = gws;
@@ -766,7 +804,8 @@ int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, in
     if (p->n_micro > 0) d.push_back({0, p->n_micro, 4, -1, (p->n_micro + 255) / 256, 256, 0});
     for (auto& bk : p->buckets) {
         const SmallVariant* v = bk.variant >= kV2Base ? nullptr : find_variant(bk.variant);
-        const int nt = bk.variant >= kV2Base ? v2_threads(bk.variant - kV2Base) : (v ? v->nt : 0);
+        const int nt = bk.variant >= kV2Multi ? 1024
+                       : bk.variant >= kV2Base ? v2_threads(bk.variant - kV2Base) : (v ? v->nt : 0);
         d.push_back({bk.first, bk.count, bk.maxN, bk.variant, bk.grid, nt, (int64_t)bk.lds_bytes});
     }
     *nbuckets = (int)d.size();

@@ -24,6 +24,8 @@ struct Bucket {
     int64_t gws_stride;
     double cost_sum;     // item_cost summed over the bucket
     double cost_max;
+    int consts_off = 0;  // its entries in the plan's constants: the count, and
+                         // for the multi-size launch the end of each size class
 };
 
 inline uint64_t tune_key(int maxN, size_t arena_bytes, int precision) {
@@ -55,6 +57,9 @@ struct lsvd_handle_s {
     std::map<uint64_t, lsvd_plan_s*> plan_cache;  // one-item plans of the single-matrix API
     int large_min_n = 129;                        // square single solves from this n run chip-wide
     int micro_max_sweeps = 24;                    // LieSvdMicro's sweep limit for n = 3, 4 (test hook)
+    bool multi_launch = true;                     // sizes 64..128 of a batch in one launch
+    int lane0_cus = 0;                            // CUs of lane 0 (the longest launch); the other
+                                                  // lanes take the rest; 0: every lane on every CU
     lsvd::Extras* extras = nullptr;
 };
 

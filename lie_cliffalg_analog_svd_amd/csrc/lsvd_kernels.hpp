@@ -47,6 +47,13 @@ int64_t v2_slot_doubles(int np, int max_m);
 hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st);
 int small2_blocks_per_cu(int np, size_t lds_bytes);
 constexpr int kV2Base = 100;
+// One persistent launch over the items of every padded size 64..128 (1024
+// threads per matrix, LDS and slots sized for np_max): variant id =
+// kV2Multi + np_max.
+constexpr int kV2Multi = 300;
+constexpr int kV2MultiMinNP = 64;
+hipError_t launch_small2_multi(const SmallLaunch& L, hipStream_t st);
+int small2_multi_blocks_per_cu(size_t lds_bytes);
 
 // ---- LieSvdMicro (micro_svd.hip) ----------------------------------------
 

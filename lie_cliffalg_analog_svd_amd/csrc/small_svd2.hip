@@ -35,11 +35,16 @@ typedef double d4v __attribute__((ext_vector_type(4)));
 constexpr int kV2YModeMaxNP = 96;
 constexpr size_t kV2LdsBudget = 150 * 1024;
 
+// Threads per matrix of the per-size kernels: measured per-matrix latency is
+// lowest with 4 (N <= 32), 8 (N <= 64) and 16 (N > 64) waves per matrix.
 template <int NP>
+constexpr int v2_default_nt() {
+    return NP <= 16 ? 64 : (NP <= 32 ? 256 : (NP <= 64 ? 512 : 1024));
+}
+
+template <int NP, int NTT = v2_default_nt<NP>()>
 struct Cfg2 {
-    // measured per-matrix latency is lowest with 4 (N <= 32), 8 (N <= 64)
-    // and 16 (N > 64) waves per matrix
-    static constexpr int NT = NP <= 16 ? 64 : (NP <= 32 ? 256 : (NP <= 64 ? 512 : 1024));
+    static constexpr int NT = NTT;
     static constexpr int NW = NT / 64;
     static constexpr int TPR = NP <= 16 ? 4 : 8;  // lanes per row of V (divides 64)
     static constexpr int CH = NP / TPR;           // ring positions per lane
@@ -56,17 +61,18 @@ struct Cfg2 {
     static constexpr int PANEL = (NP * LDP > 16 * LDM) ? NP * LDP : 16 * LDM;
 };
 
-template <int NP>
+template <int NP, int NT = v2_default_nt<NP>()>
 struct Sh2 {
-    double red[2 * Cfg2<NP>::NW];
+    double red[2 * Cfg2<NP, NT>::NW];
     double diag[NP];
     double tau[NP];
     double scr[NP];
-    double2 cs[2][Cfg2<NP>::TPR * Cfg2<NP>::CSP];  // per layer parity: (c, s oriented to the slot order),
-                                                   // slot k at csi(k) (the V rows' reads)
-    double2 csp[2][NP / 2];                        // the same, slot k at k (the P blocks' reads)
+    // per layer parity: (c, s oriented to the slot order), slot k at csi(k)
+    // (the V rows' reads); 64 more entries take the writes of angle-wave
+    // lanes that hold no slot
+    double2 cs[2][Cfg2<NP, NT>::TPR * Cfg2<NP, NT>::CSP + 64];
+    double2 csp[2][NP / 2 + 64];                   // the same, slot k at k (the P blocks' reads)
     int rank[NP];
-    alignas(16) double sink[128];  // write target of angle-wave lanes that hold no block
     int item;
 };
 
@@ -204,10 +210,10 @@ struct PhaseClock2 {
     }
 };
 
-template <int NP>
+template <int NP, int NTT = v2_default_nt<NP>()>
 __device__ void solve_item2(const Item& itm, const Pools& P, double* main, double* panel, int panel_rows,
-                            double* xs, double* vs, double* ys, Sh2<NP>& sh, int ns_scaled) {
-    using C = Cfg2<NP>;
+                            double* xs, double* vs, double* ys, Sh2<NP, NTT>& sh, int ns_scaled) {
+    using C = Cfg2<NP, NTT>;
     constexpr int NT = C::NT, NW = C::NW, LDM = C::LDM, NTILE = C::NTILE;
     constexpr int TPR = C::TPR, CH = C::CH, SLOTS = C::SLOTS, LDP = C::LDP;
     const int tid = tix(), w = tid >> 6;
@@ -452,7 +458,10 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         }
         ka = b;
     };
-    double* pb = main;  // 2 x 4 NBLK doubles: NP (NP + 2) <= the workgroup's LDS area
+    // 2 x 4 NBLK doubles, then 128 doubles that take the writes of
+    // angle-wave lanes holding no block: NP (NP + 2) + 128 <= the LDS area
+    double* pb = main;
+    constexpr int SINK = 8 * NBLK;
     double ref = 0.0;
     {
         for (int e = tid; e < NP * NP; e += NT) {
@@ -551,7 +560,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     auto v_update = [&](const double2* csl) {
 #pragma unroll
         for (int j = 0; j < CH / 2; ++j) {
-            const double2 r = csl[vch * Cfg2<NP>::CSP + j];
+            const double2 r = csl[vch * C::CSP + j];
             const double x = v[2 * j], y = v[2 * j + 1];
             v[2 * j] = r.x * x - r.y * y;
             v[2 * j + 1] = r.y * x + r.x * y;
@@ -621,19 +630,20 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                 const double r00 = acn * q00 - asn * q01, r01 = asn * q00 + acn * q01;
                 const double r10 = acn * q10 - asn * q11, r11 = asn * q10 + acn * q11;
                 // one basic block with the angle below, so the scheduler can
-                // fill the angle's dependent chain with these stores
-                double* sink = &sh.sink[2 * lane];
-                *(aslot ? np_ + (w0d & 0xffff) : sink) = n00;
-                *(aslot ? np_ + (w0d >> 16) : sink) = n01;
-                *(aslot ? np_ + w1d : sink) = n11;
-                *(has1 ? np_ + (w1a & 0xffff) : sink) = m00;
-                *(has1 ? np_ + (w1a >> 16) : sink) = m01;
-                *(has1 ? np_ + (w1b & 0xffff) : sink) = m10;
-                *(has1 ? np_ + (w1b >> 16) : sink) = d1n11;
-                *(has2 ? np_ + (w2a & 0xffff) : sink) = r00;
-                *(has2 ? np_ + (w2a >> 16) : sink) = r01;
-                *(has2 ? np_ + (w2b & 0xffff) : sink) = r10;
-                *(has2 ? np_ + (w2b >> 16) : sink) = r11;
+                // fill the angle's dependent chain with these stores (offsets
+                // from pb; lanes without the block write their sink slot)
+                const int nb = ((G + 1) & 1) * 4 * NBLK, sk = SINK + 2 * lane;
+                pb[aslot ? nb + (w0d & 0xffff) : sk] = n00;
+                pb[aslot ? nb + (w0d >> 16) : sk] = n01;
+                pb[aslot ? nb + w1d : sk] = n11;
+                pb[has1 ? nb + (w1a & 0xffff) : sk] = m00;
+                pb[has1 ? nb + (w1a >> 16) : sk] = m01;
+                pb[has1 ? nb + (w1b & 0xffff) : sk] = m10;
+                pb[has1 ? nb + (w1b >> 16) : sk] = d1n11;
+                pb[has2 ? nb + (w2a & 0xffff) : sk] = r00;
+                pb[has2 ? nb + (w2a >> 16) : sk] = r01;
+                pb[has2 ? nb + (w2b & 0xffff) : sk] = r10;
+                pb[has2 ? nb + (w2b >> 16) : sk] = r11;
                 // next (0, 1) <- (0, 2); next (2k, 2k + 1) <- (2k - 1, 2k + 2)
                 double apqn = k == 0 ? m00 : r10;
                 double e00n = 0.0, e11n = 0.0;
@@ -647,10 +657,8 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                 e11n = k == 0 ? n00_next : n11_prev;
                 if (k == S - 1) apqn = d1_prev;  // next (2S-2, 2S-1) <- (2S-3, 2S-1)
                 angle(L + 1 < NP - 1 ? L + 1 : 0, e00n, e11n, apqn);  // lanes >= S: unused
-                *(aslot ? &sh.cs[(G + 1) & 1][csi<NP>(aslot ? k : 0)] : reinterpret_cast<double2*>(sink)) =
-                    make_double2(ac, as);
-                *(aslot ? &sh.csp[(G + 1) & 1][aslot ? k : 0] : reinterpret_cast<double2*>(sink)) =
-                    make_double2(ac, as);
+                sh.cs[(G + 1) & 1][aslot ? csi<NP>(k) : C::TPR * C::CSP + lane] = make_double2(ac, as);
+                sh.csp[(G + 1) & 1][aslot ? k : S + lane] = make_double2(ac, as);
                 neighbours();
             }
             // the other blocks: rotate (:244-285), write to the next layer's positions
@@ -835,6 +843,89 @@ __global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __
         const int panel_rows = (lds_doubles - rows * Cfg2<NP>::LDM) / Cfg2<NP>::LDM;
         solve_item2<NP>(itm, pools, main, panel, panel_rows, xs, vs, ys, sh, ns_scaled);
     }
+}
+
+// One persistent launch over the items of every padded size from 64 to 128
+// (the queue sorted longest first across sizes): 16 waves per workgroup,
+// LDS and global slot sized for the largest item.  Each workgroup takes the
+// next item and runs the solver instance of its padded size.
+constexpr int kMultiNT = 1024;
+union Sh2Multi {
+    Sh2<64, kMultiNT> s64;
+    Sh2<80, kMultiNT> s80;
+    Sh2<96, kMultiNT> s96;
+    Sh2<112, kMultiNT> s112;
+    Sh2<128, kMultiNT> s128;
+};
+
+// The queue holds the items ordered by padded size, largest first (and by
+// cost within a size); count_ptr[1 + j] is the end of size class j
+// (128, 112, 96, 80, 64).  Workgroups drain the classes in that order, each
+// class with its own solver instance, so the largest matrices start first.
+// An index fetched past the end of the current class is carried into the
+// next class's loop, so every index is solved exactly once.
+template <int NP>
+__device__ __forceinline__ void multi_drain(const Item* items, int end, int* counter, const Pools& pools,
+                                            double* dyn_lds, int lds_doubles, double* xs, double* vs, double* ys,
+                                            Sh2<NP, kMultiNT>& sh, int* next_item, int ns_scaled) {
+    for (;;) {
+        if (*next_item < 0) {  // uniform: next_item is in LDS
+            __syncthreads();
+            if (threadIdx.x == 0) *next_item = atomicAdd(counter, 1);
+            __syncthreads();
+        }
+        const int it = *next_item;
+        if (it >= end) return;  // keep it for a later class
+        __syncthreads();
+        if (threadIdx.x == 0) *next_item = -1;
+        const Item itm = items[it];
+        const int M = itm.m > itm.n ? itm.m : itm.n;
+        const int MP = (M + 15) & ~15;
+        const int rows = MP > NP ? MP : NP;
+        double* main = dyn_lds;
+        double* panel = dyn_lds + (size_t)rows * Cfg2<NP, kMultiNT>::LDM;
+        const int panel_rows = (lds_doubles - rows * Cfg2<NP, kMultiNT>::LDM) / Cfg2<NP, kMultiNT>::LDM;
+        solve_item2<NP, kMultiNT>(itm, pools, main, panel, panel_rows, xs, vs, ys, sh, ns_scaled);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kMultiNT) void small_svd2_multi_kernel(const Item* __restrict__ items,
+                                                                    const int* count_ptr, int* counter, Pools pools,
+                                                                    double* gws, int64_t gws_stride, int lds_doubles,
+                                                                    int ns_scaled) {
+    extern __shared__ double dyn_lds[];
+    __shared__ Sh2Multi sh;
+    __shared__ int next_item;
+    const int64_t part = gws_stride / 3;
+    double* xs = gws + (size_t)blockIdx.x * gws_stride;
+    double* vs = xs + part;
+    double* ys = vs + part;
+    if (threadIdx.x == 0) next_item = -1;
+    __syncthreads();
+    multi_drain<128>(items, count_ptr[1], counter, pools, dyn_lds, lds_doubles, xs, vs, ys, sh.s128, &next_item, ns_scaled);
+    multi_drain<112>(items, count_ptr[2], counter, pools, dyn_lds, lds_doubles, xs, vs, ys, sh.s112, &next_item, ns_scaled);
+    multi_drain<96>(items, count_ptr[3], counter, pools, dyn_lds, lds_doubles, xs, vs, ys, sh.s96, &next_item, ns_scaled);
+    multi_drain<80>(items, count_ptr[4], counter, pools, dyn_lds, lds_doubles, xs, vs, ys, sh.s80, &next_item, ns_scaled);
+    multi_drain<64>(items, count_ptr[5], counter, pools, dyn_lds, lds_doubles, xs, vs, ys, sh.s64, &next_item, ns_scaled);
+}
+
+hipError_t launch_small2_multi(const SmallLaunch& L, hipStream_t st) {
+    if (L.lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&small_svd2_multi_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(small_svd2_multi_kernel, dim3(L.grid), dim3(kMultiNT), L.lds_bytes, st, L.items, L.count_ptr,
+                       L.counter, L.pools, L.gws, L.gws_stride, (int)(L.lds_bytes / sizeof(double)), L.ns_scaled);
+    return hipGetLastError();
+}
+
+int small2_multi_blocks_per_cu(size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, small_svd2_multi_kernel, kMultiNT, lds) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
 }
 
 template <int NP>

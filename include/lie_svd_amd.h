@@ -151,6 +151,19 @@ int lsvd_small_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, dou
 int lsvd_small_solve_rectangular_f64(lsvd_handle_t h, const double* a, int m, int n, double* u, double* s,
                                      double* vt, lsvd_info_t* info);
 
+/* No counterpart in the reference: an allocation-free variant of
+ * lsvd_small_solve_rectangular_f64 for callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and results (bit-identical: the same
+ * on-chip kernel), but every pointer is a device pointer (the handle must
+ * be in LSVD_PTR_DEVICE mode, else LSVD_E_ARG), max(m, n) <= 128 (else
+ * LSVD_E_LIMIT), and the call only enqueues work on the handle's stream.
+ * The work queue, counter and staging slot live in a fixed 393,728-byte
+ * device array that each GPU holds until the process ends; calls that
+ * overlap on one GPU (other streams or host threads) race on it. */
+int lsvd_small_solve_rectangular_f64_noalloc(lsvd_handle_t h, const double* a, int m, int n, double* u, double* s,
+                                             double* vt, lsvd_info_t* info);
+
 /* Replaces eigh_jacobi_full (src/lie_svd_small.rs:212-285): symmetric p
  * (n x n, n <= 1024) -> v (n x n, columns are eigenvectors), eig (n,
  * descending).  Above 128 (LSVD_LARGE_MIN_N) the Jacobi sweeps run as the

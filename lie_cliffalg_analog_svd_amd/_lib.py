@@ -182,6 +182,7 @@ SIGNATURES = {
     "lsvd_version": (C.c_char_p, []),
     "lsvd_small_solve_f64": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
     "lsvd_small_solve_rectangular_f64": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
+    "lsvd_small_solve_rectangular_f64_noalloc": (C.c_int, [_vp, _vp, C.c_int, C.c_int, _vp, _vp, _vp, _vp]),
     "lsvd_eigh_jacobi_f64": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp]),
     "lsvd_micro_solve_f64": (C.c_int, [_vp, _vp, C.c_int, _vp, _vp, _vp, _vp]),
     "lsvd_batch_plan_create": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _i64p, _i64p, _i64p, C.c_int,

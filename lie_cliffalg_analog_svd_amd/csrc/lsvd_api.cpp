@@ -590,6 +590,11 @@ int lsvd_create(int device, lsvd_handle_t* out) {
         return LSVD_E_HIP;
     }
     h->stream = h->own_stream;
+    if (noalloc_init(device) != hipSuccess) {
+        (void)hipStreamDestroy(h->own_stream);
+        delete h;
+        return LSVD_E_HIP;
+    }
     h->large_min_n = kLargeMinNDefault;
     if (const char* env = std::getenv("LSVD_LARGE_MIN_N")) {
         char* end = nullptr;

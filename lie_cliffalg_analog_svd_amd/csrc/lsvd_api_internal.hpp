@@ -41,6 +41,9 @@ int svd_device_small(struct lsvd_handle_s* h, int count, const int32_t* n, const
                      const int64_t* s_off, int precision, const double* a, double* u, double* s, double* vt,
                      lsvd_info_t* info);
 int hip_fail(struct lsvd_handle_s* h, hipError_t e, const char* what);
+// Looks up this device's address of the no-alloc entry point's scratch
+// array (once per device; `device` must be current).
+hipError_t noalloc_init(int device);
 
 }  // namespace lsvd
 

@@ -145,14 +145,41 @@ __device__ __forceinline__ double from_prev_lane(double x) {
 }
 
 // One 16 x 16 tile: acc += A (16 x K) B (K x 16); fa(i, k), fb(k, j);
-// K is a multiple of 4.
+// K is a multiple of 16.
 // Operand lanes: A[i][k] at lane (i + 16 k'), B[k][j] at lane (j + 16 k').
+// The k each lane group k' reads is ordered for the LDS banks (64 banks of
+// 4 bytes; a 64-bit read is served 32 lanes -- k' = 0, 1 or k' = 2, 3 -- per
+// pass).  Every operand here is a row (address row * ld + k) or a column
+// (address k * ld + col) of a matrix with an odd leading dimension ld, so
+// lanes i = 0..15 of one k' cover 32 banks, and the two groups of a pass
+// fill the other 32 when their k differ by 16: 2 * 16 * ld = 32 mod 64 for
+// the column reads, and 2 * 16 = 32 for the row reads.  With consecutive k
+// (k0 + k') the two groups overlapped in 30 of their 32 banks.  So a
+// 64-wide block of k is taken as k = q + 16 k' (q < 16), a 32-wide one as
+// q + 16 (k' & 1) + 8 (k' >> 1) (q < 8), and a last 16 as k0 + k'.
 template <class FA, class FB>
 __device__ __forceinline__ d4v tile_mm(int K, const FA& fa, const FB& fb, d4v acc) {
     const int lane = tix() & 63;
     const int i = lane & 15, kk = lane >> 4;
+    const int K64 = K & ~63;
+    const int o64 = 16 * kk;
+#pragma unroll 1
+    for (int kb = 0; kb < K64; kb += 64) {
 #pragma unroll 4
-    for (int k0 = 0; k0 < K; k0 += 4) acc = mfma64(fa(i, k0 + kk), fb(k0 + kk, i), acc);
+        for (int q = 0; q < 16; ++q) {
+            const int k = kb + q + o64;
+            acc = mfma64(fa(i, k), fb(k, i), acc);
+        }
+    }
+    int k0 = K64;
+    if (K - k0 >= 32) {
+        const int o32 = k0 + 16 * (kk & 1) + 8 * (kk >> 1);
+#pragma unroll 4
+        for (int q = 0; q < 8; ++q) acc = mfma64(fa(i, o32 + q), fb(o32 + q, i), acc);
+        k0 += 32;
+    }
+#pragma unroll 4
+    for (; k0 < K; k0 += 4) acc = mfma64(fa(i, k0 + kk), fb(k0 + kk, i), acc);
     return acc;
 }
 

@@ -279,6 +279,15 @@ int lsvd_max_graph_nodes(void);
  * with the phantom index of odd n.  n <= 4096.  (Diagnostic.) */
 int lsvd_joint_layer_pairs(lsvd_handle_t h, int n, int32_t* out);
 
+/* Diagnostic, no reference counterpart: the device memory the handle keeps
+ * for the temporaries of its entry points (the chip-wide route, the large
+ * polar, Phase-JADE, the joint SVD, Phase-BSS, HO-SVD, polish, rotor export,
+ * triage, vjp).  Those take their buffers from this workspace, which grows
+ * to the largest call and is then reused, so repeated calls of one size
+ * allocate nothing; it is freed by lsvd_destroy.  Either pointer may be
+ * NULL. */
+int lsvd_workspace_info(lsvd_handle_t h, size_t* bytes, int* chunks);
+
 /* ---- Adaptive triage and router (src/lie_svd_adaptive.rs) ------------- */
 
 /* AdaptiveRoute (src/lie_svd_adaptive.rs:8-15). */

@@ -226,6 +226,7 @@ SIGNATURES = {
     "lsvd_sweep_graph_chunks": (C.c_int, [C.c_int, C.c_int, C.c_int, _i32p, _i32p, C.c_int]),
     "lsvd_max_graph_nodes": (C.c_int, []),
     "lsvd_joint_layer_pairs": (C.c_int, [_vp, C.c_int, _i32p]),
+    "lsvd_workspace_info": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
 }
 
 

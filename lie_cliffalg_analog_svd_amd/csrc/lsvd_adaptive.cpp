@@ -13,13 +13,8 @@ using namespace lsvd;
 
 namespace {
 
-struct DevMem {
-    void* p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-    ~DevMem() {
-        if (p) (void)hipFree(p);
-    }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using DevMem = lsvd::WsMem;
 
 lsvd_adaptive_params_t defaults() {
     lsvd_adaptive_params_t p{};
@@ -72,6 +67,7 @@ int lsvd_analyze_batched_f64(lsvd_handle_t h, int count, const int32_t* n, const
         if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
     } while (0)
     const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    lsvd::WsScope ws_scope(h);
     DevMem bitems, bpool, bout;
     ACHECK(bitems.alloc(sizeof(TriageItem) * (size_t)count));
     ACHECK(hipMemcpyAsync(bitems.p, items.data(), sizeof(TriageItem) * (size_t)count, hipMemcpyHostToDevice, st));

@@ -504,26 +504,60 @@ static int plan_execute(lsvd_plan_t p, const double* a_pool, double* u_pool, dou
 }
 
 namespace lsvd {
-int svd_device_small(lsvd_handle_t h, int count, const int32_t* n, const int64_t* mat_off, const int64_t* s_off,
-                     int precision, const double* a, double* u, double* s, double* vt, lsvd_info_t* info) {
+// Device-pointer plans of the library's own batched calls (the BSS
+// covariances, HOSVD mode Grams, polish cores, vjp forwards), cached per
+// handle by their exact item list so that a repeated call builds and
+// allocates nothing.
+static lsvd_plan_t cached_device_plan(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n,
+                                      const int64_t* ao, const int64_t* uo, const int64_t* so, const int64_t* vo,
+                                      int precision, int* rc) {
+    std::vector<int64_t> key;
+    key.reserve(2 + 6 * (size_t)count);
+    key.push_back(count);
+    key.push_back(precision);
+    for (int b = 0; b < count; ++b) {
+        key.push_back(m[b]);
+        key.push_back(n[b]);
+        key.push_back(ao[b]);
+        key.push_back(uo[b]);
+        key.push_back(so[b]);
+        key.push_back(vo[b]);
+    }
+    for (auto& e : h->device_plans)
+        if (e.first == key) {
+            *rc = LSVD_OK;
+            return e.second;
+        }
     std::vector<int32_t> modes(count, kModeSvd);
     const int saved = h->ptr_mode;
     h->ptr_mode = LSVD_PTR_DEVICE;
     lsvd_plan_t p = nullptr;
-    int rc = plan_build(h, count, n, n, mat_off, mat_off, s_off, mat_off, modes.data(), precision, &p);
+    *rc = plan_build(h, count, m, n, ao, uo, so, vo, modes.data(), precision, &p);
     h->ptr_mode = saved;
-    if (rc != LSVD_OK) return rc;
+    if (*rc != LSVD_OK) return nullptr;
+    if (h->device_plans.size() >= 8) {
+        lsvd_batch_plan_destroy(h->device_plans.front().second);
+        h->device_plans.erase(h->device_plans.begin());
+    }
+    h->device_plans.emplace_back(std::move(key), p);
+    return p;
+}
+
+int svd_device_small(lsvd_handle_t h, int count, const int32_t* n, const int64_t* mat_off, const int64_t* s_off,
+                     int precision, const double* a, double* u, double* s, double* vt, lsvd_info_t* info) {
+    int rc = LSVD_OK;
+    lsvd_plan_t p = cached_device_plan(h, count, n, n, mat_off, mat_off, s_off, mat_off, precision, &rc);
+    if (!p) return rc;
     rc = plan_execute(p, a, u, s, vt, info, plan_lanes(p) <= 1);
     if (rc == LSVD_OK) {
         hipError_t e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) rc = hip_fail(h, e, "hipStreamSynchronize");
     }
-    lsvd_batch_plan_destroy(p);
     return rc;
 }
 
 int svd_device_uniform(lsvd_handle_t h, int count, int m, int n, const double* a, double* u, double* s, double* vt) {
-    std::vector<int32_t> mm(count, m), nn(count, n), modes(count, kModeSvd);
+    std::vector<int32_t> mm(count, m), nn(count, n);
     std::vector<int64_t> ao(count), uo(count), so(count), vo(count);
     const int k = std::min(m, n);
     for (int b = 0; b < count; ++b) {
@@ -532,19 +566,15 @@ int svd_device_uniform(lsvd_handle_t h, int count, int m, int n, const double* a
         so[b] = (int64_t)b * k;
         vo[b] = (int64_t)b * k * n;
     }
-    const int saved = h->ptr_mode;
-    h->ptr_mode = LSVD_PTR_DEVICE;
-    lsvd_plan_t p = nullptr;
-    int rc = plan_build(h, count, mm.data(), nn.data(), ao.data(), uo.data(), so.data(), vo.data(), modes.data(),
-                        LSVD_PREC_F64, &p);
-    h->ptr_mode = saved;
-    if (rc != LSVD_OK) return rc;
+    int rc = LSVD_OK;
+    lsvd_plan_t p = cached_device_plan(h, count, mm.data(), nn.data(), ao.data(), uo.data(), so.data(), vo.data(),
+                                       LSVD_PREC_F64, &rc);
+    if (!p) return rc;
     rc = plan_execute(p, a, u, s, vt, nullptr, false);
     if (rc == LSVD_OK) {
         hipError_t e = hipStreamSynchronize(h->stream);
         if (e != hipSuccess) rc = hip_fail(h, e, "hipStreamSynchronize");
     }
-    lsvd_batch_plan_destroy(p);
     return rc;
 }
 }  // namespace lsvd
@@ -617,7 +647,11 @@ int lsvd_destroy(lsvd_handle_t h) {
     (void)hipSetDevice(h->device);
     for (auto& kv : h->plan_cache) lsvd_batch_plan_destroy(kv.second);
     h->plan_cache.clear();
+    for (auto& e : h->device_plans) lsvd_batch_plan_destroy(e.second);
+    h->device_plans.clear();
     lsvd_release_extras(h);
+    delete h->ws;
+    h->ws = nullptr;
     if (h->own_stream) (void)hipStreamDestroy(h->own_stream);
     for (hipStream_t s2 : h->lanes) (void)hipStreamDestroy(s2);
     delete h;
@@ -626,7 +660,13 @@ int lsvd_destroy(lsvd_handle_t h) {
 
 int lsvd_set_stream(lsvd_handle_t h, void* stream) {
     if (!h) return LSVD_E_ARG;
-    h->stream = stream ? (hipStream_t)stream : h->own_stream;
+    // the workspace's bytes are reused in stream order: drain the old stream
+    const hipStream_t next = stream ? (hipStream_t)stream : h->own_stream;
+    if (next != h->stream && h->stream) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) return hip_fail(h, e, "hipStreamSynchronize");
+    }
+    h->stream = next;
     return LSVD_OK;
 }
 

@@ -8,6 +8,7 @@
 #include <vector>
 
 #include "lsvd_kernels.hpp"
+#include "lsvd_workspace.hpp"
 
 namespace lsvd {
 
@@ -58,12 +59,14 @@ struct lsvd_handle_s {
     std::string last_error;
     std::map<uint64_t, int> tune;                 // autotune winners: bucket key -> variant
     std::map<uint64_t, lsvd_plan_s*> plan_cache;  // one-item plans of the single-matrix API
+    std::vector<std::pair<std::vector<int64_t>, lsvd_plan_s*>> device_plans;  // the library's own batched calls
     int large_min_n = 129;                        // square single solves from this n run chip-wide
     int micro_max_sweeps = 24;                    // LieSvdMicro's sweep limit for n = 3, 4 (test hook)
     bool multi_launch = true;                     // sizes 64..128 of a batch in one launch
     int lane0_cus = 0;                            // CUs of lane 0 (the longest launch); the other
                                                   // lanes take the rest; 0: every lane on every CU
     lsvd::Extras* extras = nullptr;
+    lsvd::Workspace* ws = nullptr;                // temporaries of the entry points (grown, then reused)
 };
 
 struct lsvd_plan_s {

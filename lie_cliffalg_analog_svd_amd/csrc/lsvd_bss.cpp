@@ -16,13 +16,8 @@ using namespace lsvd;
 
 namespace {
 
-struct DMem {
-    void* p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-    ~DMem() {
-        if (p) (void)hipFree(p);
-    }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using DMem = lsvd::WsMem;
 
 // Runs the joint diagonalizer on device buffers whatever the caller's pointer
 // mode, restoring it afterwards.
@@ -74,6 +69,7 @@ int lsvd_bss_separate_f64(lsvd_handle_t h, int count, int channels, int samples,
     }
     const int K = (int)lag.size();
     const int ns_cov = bss_splits(count, ch, S, 1), ns_lag = bss_splits(count, ch, S, K);
+    lsvd::WsScope ws_scope(h);
     DMem bx, bmean, bpart, bcov, bu, bs, bvt, bw, bz, bfloor, bfam, bv, bdiag, bun, bsep, bcoh, blag, bden;
     const double* dx = x;
     if (host) {

@@ -13,20 +13,14 @@ using namespace lsvd;
 
 namespace {
 
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
 struct DevBuf {
     void* p = nullptr;
-    size_t bytes = 0;
     hipError_t ensure(size_t want) {
-        if (want <= bytes && p) return hipSuccess;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-        hipError_t e = hipMalloc(&p, want ? want : 1);
-        if (e == hipSuccess) bytes = want;
+        lsvd::WsMem m;
+        const hipError_t e = m.alloc(want);
+        p = m.p;
         return e;
-    }
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
     }
 };
 
@@ -143,6 +137,7 @@ int lsvd_joint_layer_pairs(lsvd_handle_t h, int n, int32_t* out) {
     if (cnt == 0) return LSVD_OK;
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    lsvd::WsScope ws_scope(h);
     DevBuf buf;
     if ((e = buf.ensure(sizeof(int32_t) * cnt)) != hipSuccess) return hip_fail(h, e, "hipMalloc");
     if ((e = joint_launch_pairs(n, (int32_t*)buf.p, h->stream)) != hipSuccess) return hip_fail(h, e, "pairs kernel");
@@ -182,6 +177,7 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
     const size_t mats_n = (size_t)families * k * nn;
     const int slots = (n + (n & 1)) / 2;
 
+    lsvd::WsScope ws_scope(h);
     DevBuf bmats, bwork, bbasis, boff, bpart, brot, bfam, bdiag;
 #define JCHECK(expr)                                        \
     do {                                                    \
@@ -276,6 +272,7 @@ int lsvd_joint_svd_f64(lsvd_handle_t h, int families, int k, int rows, int cols,
     const size_t rc = (size_t)rows * cols, uu = (size_t)rows * rows, vv = (size_t)cols * cols;
     const size_t mats_n = (size_t)families * k * rc;
     const int slots = (kk + (kk & 1)) / 2;
+    lsvd::WsScope ws_scope(h);
     DevBuf bmats, bwork, bu, bv, bsig, bvt, brow, bcol, bpart, brot, bfam;
 #define JCHECK(expr)                                        \
     do {                                                    \

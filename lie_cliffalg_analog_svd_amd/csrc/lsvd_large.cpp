@@ -22,13 +22,8 @@ namespace lsvd {
 
 namespace {
 
-struct Mem {
-    void* p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-    ~Mem() {
-        if (p) (void)hipFree(p);
-    }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using Mem = lsvd::WsMem;
 
 }  // namespace
 
@@ -58,6 +53,7 @@ int run_block_jacobi(lsvd_handle_t h, const double* dP, int n, double* dDiag, do
     const int ld = nbp * bb;
     const size_t ll = (size_t)ld * ld;
     const size_t wsz = (size_t)nbp * bb * bb;  // one layer's block rotations
+    lsvd::WsScope ws_scope(h);
     Mem bApad, bVpad, bWrot, bPart, bRot;
     LCHK(bApad.alloc(sizeof(double) * ll));
     LCHK(bVpad.alloc(sizeof(double) * ll));
@@ -126,6 +122,7 @@ int svd_large_tall(lsvd_handle_t h, const double* dA, int m, int n, double* dU, 
     *ok = false;
     hipStream_t st = h->stream;
     const size_t mn = (size_t)m * n, nn = (size_t)n * n;
+    lsvd::WsScope ws_scope(h);
     Mem bQ, bP, bVs, bVc, bDiag, bPerm, bScal;
     LCHK(bQ.alloc(sizeof(double) * mn));
     lsvd_polar_info_t pinf{};
@@ -186,6 +183,7 @@ int svd_large(lsvd_handle_t h, const double* a, int m, int n, double* u, double*
     const bool wide = n > m;  // solved transposed (:348-352): SVD(A^T) = (V, sigma, U^T)
     const int tm = wide ? n : m, tn = k;  // the tall problem is tm x tn
     const size_t mn = (size_t)m * n, kk = (size_t)k * k;
+    lsvd::WsScope ws_scope(h);
     Mem bA, bAt, bU, bS, bVt, bTu, bTvt;
     const double* dA = a;
     double *dU = u, *dS = s, *dVt = vt;
@@ -235,6 +233,7 @@ int eigh_large(lsvd_handle_t h, const double* p, int n, double* v, double* eig, 
     hipStream_t st = h->stream;
     const bool host = h->ptr_mode == LSVD_PTR_HOST;
     const size_t nn = (size_t)n * n;
+    lsvd::WsScope ws_scope(h);
     Mem bP, bVc, bDiag, bPerm, bFlag, bV, bE;
     const double* dP = p;
     double *dV = v, *dE = eig;

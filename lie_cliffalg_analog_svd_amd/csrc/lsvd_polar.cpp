@@ -14,13 +14,8 @@ using namespace lsvd;
 
 namespace {
 
-struct Buf {
-    void* p = nullptr;
-    ~Buf() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using Buf = lsvd::WsMem;
 
 // Pinned host slots and one event per slot for the stop-test reads of the
 // plain FP64 steps.
@@ -50,6 +45,7 @@ int polar_run(lsvd_handle_t h, const double* dA, int m, int n, int precision, do
     const bool lowp = precision == LSVD_PREC_F32 || precision == LSVD_PREC_BF16;
     const int lp = precision == LSVD_PREC_F32 ? kPolF32 : kPolBF16;
     lsvd_polar_info_t inf{};
+    lsvd::WsScope ws_scope(h);
     Buf bX, bX2, bX3, bM, bXf, bXf2, bMf, bS;
 #define PCHK(expr)                                          \
     do {                                                    \
@@ -267,6 +263,7 @@ int lsvd_polar_f64(lsvd_handle_t h, const double* a, int m, int n, int precision
         if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
     } while (0)
     if (h->ptr_mode != LSVD_PTR_HOST) return polar_run(h, a, m, n, precision, q, info);
+    lsvd::WsScope ws_scope(h);
     Buf bA, bQ;
     PCHK(bA.alloc(sizeof(double) * mn));
     PCHK(bQ.alloc(sizeof(double) * mn));

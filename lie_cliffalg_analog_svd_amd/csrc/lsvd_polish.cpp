@@ -15,13 +15,8 @@ using namespace lsvd;
 
 namespace {
 
-struct DevMem {
-    void* p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-    ~DevMem() {
-        if (p) (void)hipFree(p);
-    }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using DevMem = lsvd::WsMem;
 
 }  // namespace
 
@@ -59,6 +54,7 @@ int lsvd_polish_batched_f64(lsvd_handle_t h, int count, const int32_t* n, const 
     } while (0)
     const bool host = h->ptr_mode == LSVD_PTR_HOST;
     const size_t mb = sizeof(double) * (size_t)mat_end, sb = sizeof(double) * (size_t)s_end;
+    lsvd::WsScope ws_scope(h);
     DevMem bitems, bt, bcore, bur, bvrt, ba, bu0, bvt0, bu, bs, bvt, binfo;
     PCHECK(bitems.alloc(sizeof(BgemmItem) * (size_t)count));
     PCHECK(hipMemcpyAsync(bitems.p, items.data(), sizeof(BgemmItem) * (size_t)count, hipMemcpyHostToDevice, st));

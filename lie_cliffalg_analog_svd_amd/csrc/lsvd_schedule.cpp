@@ -17,13 +17,8 @@ using namespace lsvd;
 
 namespace {
 
-struct DevMem {
-    void* p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-    ~DevMem() {
-        if (p) (void)hipFree(p);
-    }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using DevMem = lsvd::WsMem;
 
 // Size classes of one launch each: bands of 16 up to the LDS limit (the
 // LDS bytes, hence the occupancy, follow the band's largest d), then the
@@ -168,6 +163,7 @@ int lsvd_givens_schedule_f64(lsvd_handle_t h, int count, const int32_t* d, const
         if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
     } while (0)
     const bool host = h->ptr_mode == LSVD_PTR_HOST;
+    lsvd::WsScope ws_scope(h);
     DevMem bitems, bpool, bn, bij, bcs, bth, bsg, bscratch;
     GCHECK(bitems.alloc(sizeof(GivensItem) * (size_t)count));
     GCHECK(hipMemcpyAsync(bitems.p, sorted.data(), sizeof(GivensItem) * (size_t)count, hipMemcpyHostToDevice, st));

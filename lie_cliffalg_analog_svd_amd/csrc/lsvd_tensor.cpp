@@ -14,13 +14,8 @@ using namespace lsvd;
 
 namespace {
 
-struct DMem {
-    void* p = nullptr;
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-    ~DMem() {
-        if (p) (void)hipFree(p);
-    }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using DMem = lsvd::WsMem;
 
 int check_dims(lsvd_handle_t h, int count, int n1, int n2, int n3) {
     if (count < 0) return set_error(h, LSVD_E_SHAPE, "negative count");
@@ -66,6 +61,7 @@ int lsvd_hosvd3_f64(lsvd_handle_t h, int count, int n1, int n2, int n3, const do
             so += dims[m];
         }
     }
+    lsvd::WsScope ws_scope(h);
     DMem bx, bg, bu, bs, bvt, boff, bt1, bt2, bcore, bu1, bu2, bu3, bnorm;
     const double* dx = x;
     if (host) {
@@ -149,6 +145,7 @@ int lsvd_tensor_rotate3_f64(lsvd_handle_t h, int count, int n1, int n2, int n3, 
     std::vector<int64_t> qoff((size_t)3 * count);
     for (int t = 0; t < count; ++t)
         for (int m = 0; m < 3; ++m) qoff[(size_t)3 * t + m] = (int64_t)(t * qs[m]);
+    lsvd::WsScope ws_scope(h);
     DMem bx, bq[3], boff, bt1, bt2, bout;
     const double* dx = x;
     const double* dq[3] = {q1, q2, q3};

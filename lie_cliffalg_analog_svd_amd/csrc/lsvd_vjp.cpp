@@ -9,13 +9,8 @@
 using namespace lsvd;
 
 namespace {
-struct Buf {
-    void* p = nullptr;
-    ~Buf() {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t bytes) { return hipMalloc(&p, bytes ? bytes : 1); }
-};
+// temporaries come from the handle's workspace (lsvd_workspace.hpp)
+using Buf = lsvd::WsMem;
 }  // namespace
 
 static int vjp_common(lsvd_handle_t h, int count, int m, int n, const double* a, const double* u, const double* s,
@@ -37,6 +32,7 @@ static int vjp_common(lsvd_handle_t h, int count, int m, int n, const double* a,
         hipError_t e_ = (expr);                             \
         if (e_ != hipSuccess) return hip_fail(h, e_, #expr); \
     } while (0)
+    lsvd::WsScope ws_scope(h);
     Buf bA, bU, bS, bVt, bgU, bgS, bgVt, bQ, bG, bW;
     auto stage = [&](Buf& b, const double* src, size_t len, const double** dst) -> hipError_t {
         if (!src) {

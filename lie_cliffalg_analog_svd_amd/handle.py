@@ -78,6 +78,14 @@ class Handle:
         check(self._h, lib().lsvd_set_pointer_mode(self._h, mode))
         self.pointer_mode = mode
 
+    def workspace_info(self) -> tuple[int, int]:
+        """(bytes, chunks) of the device workspace the handle keeps for its
+        entry points' temporaries (lsvd_workspace_info)."""
+        b = C.c_size_t()
+        c = C.c_int()
+        check(self._h, lib().lsvd_workspace_info(self._h, C.byref(b), C.byref(c)))
+        return int(b.value), int(c.value)
+
     def set_stream(self, stream_ptr: int | None):
         check(self._h, lib().lsvd_set_stream(self._h, C.c_void_p(stream_ptr or 0)))
 

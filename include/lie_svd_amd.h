@@ -176,6 +176,34 @@ int lsvd_eigh_jacobi_f64(lsvd_handle_t h, const double* p, int n, double* v, dou
 int lsvd_micro_solve_f64(lsvd_handle_t h, const double* a, int n, double* u, double* s, double* vt,
                          lsvd_info_t* info);
 
+/* ---- metrics::compute (src/lib.rs:133-194) ---------------------------- */
+
+/* SvdMetrics (src/lib.rs:136-142); sigma_max_rel / sigma_tail_rel are NaN
+ * when no reference spectrum is given (the reference's None). */
+typedef struct {
+    double rel_recon;
+    double orth_u;
+    double orth_v;
+    double sigma_max_rel;
+    double sigma_tail_rel;
+} lsvd_metrics_t;
+
+/* Replaces metrics::compute (src/lib.rs:149-194) for a batch of
+ * factorizations in the batch plan's pool layout (matrix b: a at a_off[b],
+ * m x n; u at u_off[b], m x k; s at s_off[b], k; vt at vt_off[b], k x n;
+ * k = min(m, n) <= 1024), computed on the device, one workgroup per matrix:
+ * rel_recon = ||U diag(s) Vt - A||_F / max(||A||_F, 1e-300), orth_u =
+ * ||U^T U - I||_F, orth_v = ||Vt Vt^T - I||_F (k x k identities: the
+ * reference's n x n for its square factors), and with sigma_ref (same
+ * layout as s, or NULL) the largest relative error of the descending-sorted
+ * spectra over all i and over i >= floor(3 m / 4).  out[b] receives matrix
+ * b's metrics.  Offsets are host arrays; pools and out follow the pointer
+ * mode.  Synchronous. */
+int lsvd_metrics_batched_f64(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                             const int64_t* u_off, const int64_t* s_off, const int64_t* vt_off, const double* a,
+                             const double* u, const double* s, const double* vt, const double* sigma_ref,
+                             lsvd_metrics_t* out);
+
 /* ---- Newton-Schulz polar factor --------------------------------------- */
 
 /* Replaces newton_schulz_polar (src/lie_svd_small.rs:158-183), which the

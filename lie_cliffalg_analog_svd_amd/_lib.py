@@ -49,6 +49,16 @@ class LsvdInfo(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class LsvdMetrics(C.Structure):
+    _fields_ = [
+        ("rel_recon", C.c_double),
+        ("orth_u", C.c_double),
+        ("orth_v", C.c_double),
+        ("sigma_max_rel", C.c_double),
+        ("sigma_tail_rel", C.c_double),
+    ]
+
+
 class LsvdJointParams(C.Structure):
     _fields_ = [
         ("max_sweeps", C.c_int32),
@@ -227,6 +237,8 @@ SIGNATURES = {
     "lsvd_max_graph_nodes": (C.c_int, []),
     "lsvd_joint_layer_pairs": (C.c_int, [_vp, C.c_int, _i32p]),
     "lsvd_workspace_info": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
+    "lsvd_metrics_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _i64p, _i64p, _i64p, _vp, _vp, _vp,
+                                            _vp, _vp, _vp]),
 }
 
 

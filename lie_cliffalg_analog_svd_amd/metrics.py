@@ -86,3 +86,35 @@ def subspace_angle_clusters(u_got, u_ref, sigma_ref, rtol: float = 1e-8):
         out.append((i, j, float(np.linalg.norm(r, 2)) if r.size else 0.0))
         i = j
     return out
+
+
+def compute_batched(handle, ms, ns, layout, a_pool, u_pool, s_pool, vt_pool, sigma_ref_pool=None):
+    """metrics::compute (src/lib.rs:149-194) for every matrix of a batch on
+    the device (lsvd_metrics_batched_f64): pools in the batch plan's packed
+    layout (`layout` from pack_offsets), numpy arrays in host pointer mode
+    or device tensors in device mode; `sigma_ref_pool` has the layout of
+    s_pool.  Returns a structured array with fields rel_recon, orth_u,
+    orth_v, sigma_max_rel, sigma_tail_rel (NaN without a reference)."""
+    import ctypes as C
+
+    from . import _lib
+    from .handle import _as_i32p, _as_i64p, _i64, _ptr
+    ms = np.ascontiguousarray(ms, dtype=np.int32)
+    ns = np.ascontiguousarray(ns, dtype=np.int32)
+    count = int(ms.shape[0])
+    offs = [_i64(layout[k]) for k in ("a_off", "u_off", "s_off", "vt_off")]
+    dt = np.dtype([(f, np.float64) for f in ("rel_recon", "orth_u", "orth_v", "sigma_max_rel", "sigma_tail_rel")])
+    dev_mode = handle.pointer_mode == _lib.LSVD_PTR_DEVICE
+    if dev_mode:
+        import torch
+        out_t = torch.empty((max(count, 1), 5), dtype=torch.float64, device=a_pool.device)
+        out_ptr = _ptr(out_t)
+    else:
+        out = np.zeros(max(count, 1), dtype=dt)
+        out_ptr = C.c_void_p(out.ctypes.data)
+    handle.check(_lib.lib().lsvd_metrics_batched_f64(
+        handle.raw, count, _as_i32p(ms), _as_i32p(ns), *[_as_i64p(o) for o in offs], _ptr(a_pool), _ptr(u_pool),
+        _ptr(s_pool), _ptr(vt_pool), _ptr(sigma_ref_pool), out_ptr))
+    if dev_mode:
+        out = np.ascontiguousarray(out_t.cpu().numpy()).view(dt).reshape(-1)
+    return out[:count]

@@ -46,9 +46,17 @@ template <int NP, int NTT = v2_default_nt<NP>()>
 struct Cfg2 {
     static constexpr int NT = NTT;
     static constexpr int NW = NT / 64;
-    static constexpr int TPR = NP <= 16 ? 4 : 8;  // lanes per row of V (divides 64)
-    static constexpr int CH = NP / TPR;           // ring positions per lane
-    static_assert(CH % 2 == 0 && CH >= 4 && NP * TPR <= NT, "V ring layout");
+    // V in registers: a lane holds VR rows x CH ring positions, a row spread
+    // over TPR consecutive lanes.  The lanes of a row group read the
+    // layer's (c, s) of their CH / 2 slots once for all VR rows, so the
+    // (c, s) traffic per layer is NP^2 / VR x 8 bytes: at NP = 128 two rows
+    // per lane (16 lanes a row) halve it for the same 2 CH VR registers
+    // (Jacobi cycles -3.4 %; at NP = 96 the same layout measured +7 %).
+    static constexpr int TPR = NP <= 16 ? 4 : (NP == 128 ? 16 : 8);  // divides 64
+    static constexpr int VR = NP == 128 ? 2 : 1;
+    static constexpr int CH = NP / TPR;  // ring positions per lane
+    static constexpr int VLANES = NP / VR * TPR;
+    static_assert(CH % 2 == 0 && CH >= 4 && NP % VR == 0 && VLANES <= NT, "V ring layout");
     static constexpr int NTILE = NP / 16;
     static constexpr int LDM = NP + 1;
     static constexpr int SLOTS = NP / 2;
@@ -165,11 +173,12 @@ __device__ __forceinline__ d4v tile_mm(int K, const FA& fa, const FB& fb, d4v ac
     const int o64 = 16 * kk;
 #pragma unroll 1
     for (int kb = 0; kb < K64; kb += 64) {
-#pragma unroll 4
+#pragma unroll 8
         for (int q = 0; q < 16; ++q) {
             const int k = kb + q + o64;
             acc = mfma64(fa(i, k), fb(k, i), acc);
         }
+        __builtin_amdgcn_iglp_opt(0);  // interleave the LDS reads with the MFMAs
     }
     int k0 = K64;
     if (K - k0 >= 32) {
@@ -333,38 +342,93 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                 }
             }
             if (!ymode) {
+                if constexpr (NP >= 80 && NW >= 16) {
+                    // X_new tile t = J MT + R (panel J's row tile R) belongs to
+                    // wave t % NW, register slot t / NW, until every panel is
+                    // done: the products read the old X throughout
+                    constexpr int QMAX = (8 * NTILE + NW - 1) / NW;  // MT <= 8
+                    d4v xacc[QMAX];
 #pragma unroll 1
-                for (int J = 0; J < NTILE; ++J) {
-                    // panel = (alpha I - X^T X)[:, J]
-                    for (int I = w; I < NTILE; I += NW) {
-                        const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
-                                              [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+                    for (int J = 0; J < NTILE; ++J) {
+                        // panel = (alpha I - X^T X)[:, J]
+                        for (int I = w; I < NTILE; I += NW) {
+                            const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                                  [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = I * 16 + trow(r), col = tcol();
-                            panel[row * LDP + col] = (row == J * 16 + col ? alpha : 0.0) - g[r];
-                            if (row < N && J * 16 + col < N) {
-                                const double e = g[r] - (row == J * 16 + col ? 1.0 : 0.0);
-                                gi2 += e * e;
+                            for (int r = 0; r < 4; ++r) {
+                                const int row = I * 16 + trow(r), col = tcol();
+                                panel[row * LDP + col] = (row == J * 16 + col ? alpha : 0.0) - g[r];
+                                if (row < N && J * 16 + col < N) {
+                                    const double e = g[r] - (row == J * 16 + col ? 1.0 : 0.0);
+                                    gi2 += e * e;
+                                }
                             }
                         }
-                    }
-                    __syncthreads();
-                    // X_new[:, J] = c X panel -> staging slot; |X_new - X|^2 on the fly
-                    for (int R = w; R < MT; R += NW) {
-                        const d4v x = tile_mm(NP, [&](int i, int k) { return main[(R * 16 + i) * LDM + k]; },
-                                              [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
+                        __syncthreads();
 #pragma unroll
-                        for (int r = 0; r < 4; ++r) {
-                            const int row = R * 16 + trow(r), col = J * 16 + tcol();
-                            const double xn = cstep * x[r];
-                            const double d = xn - main[row * LDM + col];
-                            ys[row * NP + col] = xn;
-                            bad |= !isfinite(xn);
-                            dsq += d * d;
+                        for (int q = 0; q < QMAX; ++q) {
+                            const int t = w + q * NW;
+                            if (t >= J * MT && t < (J + 1) * MT) {
+                                const int R = t - J * MT;
+                                xacc[q] = tile_mm(NP, [&](int i, int k) { return main[(R * 16 + i) * LDM + k]; },
+                                                  [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
+#pragma unroll
+                                for (int r = 0; r < 4; ++r) {
+                                    const int row = R * 16 + trow(r), col = J * 16 + tcol();
+                                    const double xn = cstep * xacc[q][r];
+                                    const double d = xn - main[row * LDM + col];
+                                    xacc[q][r] = xn;
+                                    bad |= !isfinite(xn);
+                                    dsq += d * d;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
+#pragma unroll
+                    for (int q = 0; q < QMAX; ++q) {
+                        const int t = w + q * NW;
+                        if (t < MT * NTILE) {
+                            const int J = t / MT, R = t % MT;
+#pragma unroll
+                            for (int r = 0; r < 4; ++r)
+                                main[(R * 16 + trow(r)) * LDM + J * 16 + tcol()] = xacc[q][r];
                         }
                     }
-                    __syncthreads();
+                } else {
+#pragma unroll 1
+                    for (int J = 0; J < NTILE; ++J) {
+                        // panel = (alpha I - X^T X)[:, J]
+                        for (int I = w; I < NTILE; I += NW) {
+                            const d4v g = tile_mm(MP, [&](int i, int k) { return main[k * LDM + I * 16 + i]; },
+                                                  [&](int k, int j) { return main[k * LDM + J * 16 + j]; }, zero4);
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int row = I * 16 + trow(r), col = tcol();
+                                panel[row * LDP + col] = (row == J * 16 + col ? alpha : 0.0) - g[r];
+                                if (row < N && J * 16 + col < N) {
+                                    const double e = g[r] - (row == J * 16 + col ? 1.0 : 0.0);
+                                    gi2 += e * e;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                        // X_new[:, J] = c X panel -> staging slot; |X_new - X|^2 on the fly
+                        for (int R = w; R < MT; R += NW) {
+                            const d4v x = tile_mm(NP, [&](int i, int k) { return main[(R * 16 + i) * LDM + k]; },
+                                                  [&](int k, int j) { return panel[k * LDP + j]; }, zero4);
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) {
+                                const int row = R * 16 + trow(r), col = J * 16 + tcol();
+                                const double xn = cstep * x[r];
+                                const double d = xn - main[row * LDM + col];
+                                ys[row * NP + col] = xn;
+                                bad |= !isfinite(xn);
+                                dsq += d * d;
+                            }
+                        }
+                        __syncthreads();
+                    }
                 }
             }
             // a non-finite entry makes the step infinite
@@ -377,7 +441,7 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
                 diverged = 1;
                 break;
             }
-            if (!ymode) {
+            if (!ymode && !(NP >= 80 && NW >= 16)) {
                 for (int e = tid; e < MP * NP; e += NT) main[(e / NP) * LDM + e % NP] = ys[e];
                 __syncthreads();
             }
@@ -510,14 +574,18 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
         __syncthreads();
     }
     // V rows leave the angle wave alone when the other waves can hold them
-    constexpr int VBASE = (NW > 1 && NP * TPR <= NT - 64) ? 64 : 0;
-    const int vrow = tid >= VBASE ? (tid - VBASE) / TPR : NP, vch = (tid - VBASE) % TPR;
-    double v[CH];
+    constexpr int VR = C::VR;
+    constexpr int VBASE = (NW > 1 && C::VLANES <= NT - 64) ? 64 : 0;
+    // rows vrow + rr, rr < VR (vrow >= NP: no V rows)
+    const int vrow = tid >= VBASE ? (tid - VBASE) / TPR * VR : NP, vch = (tid - VBASE) % TPR;
+    double v[VR][CH];
 #pragma unroll
-    for (int j = 0; j < CH; ++j) {
-        const int idx = ring_idx0(NP, vch * CH + j);
-        v[j] = (vrow < N && idx == vrow) ? 1.0 : 0.0;
-    }
+    for (int rr = 0; rr < VR; ++rr)
+#pragma unroll
+        for (int j = 0; j < CH; ++j) {
+            const int idx = ring_idx0(NP, vch * CH + j);
+            v[rr][j] = (vrow + rr < N && idx == vrow + rr) ? 1.0 : 0.0;
+        }
     // regular blocks (kb - ka >= 3) of this thread: packed (ka, kb, csi(ka),
     // csi(kb)), source block index and the four write offsets
     const int rt = NW == 1 ? tid : tid - 64;
@@ -582,27 +650,35 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     const int max_sweeps = 60 + 20 * N;
     int sweeps = 0, rots = 0, G = 0;
     const int lane = tid & 63;
-    const int nbr_next = (lane + 1) & 63, nbr_prev = (lane + 63) & 63;
     // rotate V's own slots with the layer's angles, then shift the ring
     auto v_update = [&](const double2* csl) {
 #pragma unroll
         for (int j = 0; j < CH / 2; ++j) {
             const double2 r = csl[vch * C::CSP + j];
-            const double x = v[2 * j], y = v[2 * j + 1];
-            v[2 * j] = r.x * x - r.y * y;
-            v[2 * j + 1] = r.y * x + r.x * y;
+#pragma unroll
+            for (int rr = 0; rr < VR; ++rr) {
+                const double x = v[rr][2 * j], y = v[rr][2 * j + 1];
+                v[rr][2 * j] = r.x * x - r.y * y;
+                v[rr][2 * j + 1] = r.y * x + r.x * y;
+            }
         }
         // in-place ring shift (verified against the circle-method schedule):
         // even slots move down by one pair, odd slots up
-        const double nfe = __shfl(v[0], nbr_next, 64);       // next lane's first even
-        const double plo = __shfl(v[CH - 1], nbr_prev, 64);  // previous lane's last odd
-        const double e2 = v[2], olast = v[CH - 1];
+        // (DPP lane moves: a row's TPR lanes are consecutive lanes of one
+        // wave, and the row ends do not use what crosses them)
 #pragma unroll
-        for (int j = 0; j + 2 < CH; j += 2) v[j] = (vch == 0 && j == 0) ? v[0] : v[j + 2];
-        v[CH - 2] = (vch == TPR - 1) ? olast : nfe;
+        for (int rr = 0; rr < VR; ++rr) {
+            double* vv = v[rr];
+            const double nfe = from_next_lane(vv[0]);       // next lane's first even
+            const double plo = from_prev_lane(vv[CH - 1]);  // previous lane's last odd
+            const double e2 = vv[2], olast = vv[CH - 1];
 #pragma unroll
-        for (int j = CH - 1; j >= 3; j -= 2) v[j] = v[j - 2];
-        v[1] = (vch == 0) ? e2 : plo;
+            for (int j = 0; j + 2 < CH; j += 2) vv[j] = (vch == 0 && j == 0) ? vv[0] : vv[j + 2];
+            vv[CH - 2] = (vch == TPR - 1) ? olast : nfe;
+#pragma unroll
+            for (int j = CH - 1; j >= 3; j -= 2) vv[j] = vv[j - 2];
+            vv[1] = (vch == 0) ? e2 : plo;
+        }
     };
     if (awave) {  // the first layer's angles
         if (aslot) {
@@ -728,13 +804,15 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     // V from registers to LDS (index space), zero padding
     for (int e = tid; e < NP * NP; e += NT) main[(e / NP) * LDM + e % NP] = 0.0;
     __syncthreads();
-    if (vrow < N) {
 #pragma unroll
-        for (int j = 0; j < CH; ++j) {
-            const int idx = ring_idx0(NP, vch * CH + j);
-            if (idx < N) main[vrow * LDM + idx] = v[j];
+    for (int rr = 0; rr < VR; ++rr)
+        if (vrow + rr < N) {
+#pragma unroll
+            for (int j = 0; j < CH; ++j) {
+                const int idx = ring_idx0(NP, vch * CH + j);
+                if (idx < N) main[(vrow + rr) * LDM + idx] = v[rr][j];
+            }
         }
-    }
     __syncthreads();
     rots = (int)block_sum<NT>((double)rots, sh.red);
     info.jacobi_sweeps = sweeps;

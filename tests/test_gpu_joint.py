@@ -338,3 +338,31 @@ def test_jade_n512_planted_basis():
     assert tr.final_offdiag < 1e-4 * tr.initial_offdiag
     for k in range(K):
         assert np.max(np.abs(np.sort(d[0, k]) - np.sort(ds[0][k]))) < 1e-4
+
+
+def test_jade_config4_full_size_planted():
+    """The Phase-JADE workload at full size (64 symmetric 256 x 256 matrices
+    per family, Q diag(d_k) Q^T + 1e-6 sym(G_k)), four families in one call.
+    Per family: V orthogonal, the planted Q recovered up to permutation and
+    sign (1 - |q_i^T v_j| below 1e-9, i.e. angles below ~4e-5, the noise
+    over the gaps of d), the final off-diagonal norm at or below the noise's
+    Frobenius norm (which the planted basis itself leaves), no rejected
+    rotation (finite data; the accept test's chosen angle is the 2 x 2
+    minimiser) and the diagonals equal to the planted d_k to 1e-5."""
+    from scipy.optimize import linear_sum_assignment
+    F, K, n, eps = 4, 64, 256, 1e-6
+    fams, qs, ds = workloads.jade_batch(F, K, n, eps)
+    print("jade config 4", fams.shape, flush=True)
+    v, d, trs = diagonalize_families(fams)
+    for f in range(F):
+        tr = trs[f]
+        print("  family", f, tr, flush=True)
+        assert np.linalg.norm(v[f].T @ v[f] - np.eye(n)) < 1e-10
+        c = np.abs(qs[f].T @ v[f])
+        r, cidx = linear_sum_assignment(-c)
+        assert np.max(np.abs(1.0 - c[r, cidx])) < 1e-9
+        noise = fams[f] - np.einsum("ij,kj,lj->kil", qs[f], ds[f], qs[f], optimize=True)
+        assert tr.final_offdiag <= np.linalg.norm(noise), (tr.final_offdiag, np.linalg.norm(noise))
+        assert tr.rejected_rotations == 0 and tr.rotations > 0
+        perm = cidx[np.argsort(r)]  # v column perm[i] carries q column i
+        assert np.max(np.abs(d[f][:, perm] - ds[f])) < 1e-5

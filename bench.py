@@ -71,6 +71,8 @@ def algorithmic_flops(m: int, n: int, k_ns: int, sweeps: int) -> float:
 
 
 def kernel_name(variant: int) -> str:
+    if variant >= 500:
+        return f"small_svd2_kernel<{variant - 500}, twice the default threads>"
     if variant >= 300:
         return f"small_svd2_multi_kernel (padded sizes 64..{variant - 300})"
     if variant >= 100:
@@ -876,19 +878,23 @@ def main(argv=None) -> int:
     h = Handle(0)
     h.set_pointer_mode(L.LSVD_PTR_DEVICE)
     h.set_stream(stream.cuda_stream)
-    if not args.no_tune:
-        big = ~((ms == ns) & (ns <= 4))
-        t1 = time.time()
-        st = _lib.lib().lsvd_autotune_batch(h.raw, int(big.sum()), ms[big].ctypes.data_as(_lib._i32p),
-                                            ns[big].ctypes.data_as(_lib._i32p), prec)
-        h.check(st)
-        log(f"autotune: {time.time() - t1:.1f}s")
-    plan = BatchPlan(h, ms, ns, prec)
-    lay = plan.layout
+    from lie_cliffalg_analog_svd_amd.handle import pack_offsets
+    lay = pack_offsets(ms, ns)
     a_pool = torch.from_numpy(workloads.pack(mats)).to(dev)
     u_pool = torch.zeros(lay["u_len"], dtype=torch.float64, device=dev)
     s_pool = torch.zeros(lay["s_len"], dtype=torch.float64, device=dev)
     vt_pool = torch.zeros(lay["vt_len"], dtype=torch.float64, device=dev)
+    # the autotuner (before the timed region): shapes on synthetic inputs,
+    # then the plan knobs on this batch's pools
+    tune = None
+    if not args.no_tune:
+        t1 = time.time()
+        tune = {"synthetic": h.autotune(ms, ns, prec)}
+        tune["on_data"] = h.autotune_pools(ms, ns, lay, a_pool, u_pool, s_pool, vt_pool, prec)
+        tune["seconds"] = round(time.time() - t1, 2)
+        log(f"autotune: {tune}")
+    plan = BatchPlan(h, ms, ns, prec)
+    assert plan.layout["u_len"] == lay["u_len"] and plan.layout["vt_len"] == lay["vt_len"]
     info_dev = torch.zeros((count, 8), dtype=torch.int32, device=dev)
     log(f"inputs: {a_pool.numel() * 8 / 1e6:.1f} MB in HBM")
 
@@ -1080,6 +1086,7 @@ def main(argv=None) -> int:
         "fp64_tflops_whole_batch": round(total_tflops, 4),
         "buckets": [{**dd, "ms": round(t, 4)} for dd, t in zip(descs, bucket_ms)],
         "phase_split": phase_split,
+        "autotune": tune,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "legs": legs,

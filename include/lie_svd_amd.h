@@ -546,9 +546,34 @@ int lsvd_svd_batched_f64(lsvd_handle_t h, int count, const int32_t* m, const int
                          double* s_pool, const int64_t* vt_off, double* vt_pool, int precision,
                          lsvd_info_t* info);
 
-/* Runs the autotuner for the shape buckets of a batch and stores the
- * winners in the handle's cache (used by later plans). */
+/* What lsvd_autotune_batch chose for the on-chip buckets (N, M <= 128) of
+ * the last batch it tuned, and the plan times it measured (synthetic
+ * U(-1, 1) inputs of the batch's shapes, median of 3 executions). */
+typedef struct {
+    int32_t multi_min_np;  /* padded sizes >= this share the multi-size launch (129: none) */
+    int32_t max_lanes;     /* concurrent lanes (1..4) */
+    int32_t wide_mask;     /* bit (np / 16 - 1): size-np bucket at twice the default threads */
+    int32_t evaluations;   /* plans timed */
+    double default_ms;     /* the handle's settings before tuning */
+    double tuned_ms;       /* the chosen settings */
+} lsvd_tune_report_t;
+
+/* Runs the autotuner for a batch's shapes and stores its choices in the
+ * handle (used by later plans): for the on-chip buckets (N, M <= 128) a
+ * coordinate search over the split point of the multi-size launch (64, 80,
+ * 96 or none), the lane count (1..4) and the threads per workgroup of each
+ * per-size bucket (default or twice), timing whole plans; for the buckets
+ * above 128 the kernel variant of each bucket, timing that bucket. */
 int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, int precision);
+/* The plan-level part of lsvd_autotune_batch timed on the caller's own
+ * device pools (the handle in LSVD_PTR_DEVICE mode, the layout of
+ * lsvd_batch_plan_create), so that the search sees the data's real cost
+ * mix (Newton-Schulz steps and Jacobi sweeps grow with the conditioning);
+ * the outputs are overwritten by the timing runs. */
+int lsvd_autotune_batch_pools(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                              const int64_t* u_off, const int64_t* s_off, const int64_t* vt_off, int precision,
+                              const double* a_pool, double* u_pool, double* s_pool, double* vt_pool);
+int lsvd_autotune_report(lsvd_handle_t h, lsvd_tune_report_t* out);
 
 /* ---- Plan diagnostics (no reference counterpart: profiling aids) ------ */
 

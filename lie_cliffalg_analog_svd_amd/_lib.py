@@ -59,6 +59,20 @@ class LsvdMetrics(C.Structure):
     ]
 
 
+class LsvdTuneReport(C.Structure):
+    _fields_ = [
+        ("multi_min_np", C.c_int32),
+        ("max_lanes", C.c_int32),
+        ("wide_mask", C.c_int32),
+        ("evaluations", C.c_int32),
+        ("default_ms", C.c_double),
+        ("tuned_ms", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class LsvdJointParams(C.Structure):
     _fields_ = [
         ("max_sweeps", C.c_int32),
@@ -237,6 +251,9 @@ SIGNATURES = {
     "lsvd_max_graph_nodes": (C.c_int, []),
     "lsvd_joint_layer_pairs": (C.c_int, [_vp, C.c_int, _i32p]),
     "lsvd_workspace_info": (C.c_int, [_vp, C.POINTER(C.c_size_t), C.POINTER(C.c_int)]),
+    "lsvd_autotune_report": (C.c_int, [_vp, _vp]),
+    "lsvd_autotune_batch_pools": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _i64p, _i64p, _i64p, C.c_int, _vp,
+                                             _vp, _vp, _vp]),
     "lsvd_metrics_batched_f64": (C.c_int, [_vp, C.c_int, _i32p, _i32p, _i64p, _i64p, _i64p, _i64p, _vp, _vp, _vp,
                                             _vp, _vp, _vp]),
 }

@@ -1,11 +1,14 @@
 // lsvd_api.cpp -- host side of the C ABI declared in include/lie_svd_amd.h.
 //
-// A handle owns a HIP stream and an autotune cache.  Every solve is
-// expressed as a batch plan: a list of Items sorted longest-first and split
-// into buckets by the tall-ified shape (M x N, M >= N).  Each bucket is one
-// persistent launch of a small_svd variant sized for that bucket; square
-// n <= 4 items go to the micro kernel, whose escalations are drained by a
-// last small_svd launch.  A plan executes as one hipGraph launch.
+// A handle owns a HIP stream, its lanes and the autotuner's choices.  Every
+// solve is expressed as a batch plan: a list of Items sorted longest-first
+// and split into buckets by the tall-ified shape (M x N, M >= N).  Each
+// bucket is one persistent launch of a small_svd variant sized for that
+// bucket (the padded sizes 64..128 share one launch); square n <= 4 items go
+// to the micro kernel, whose escalations are drained by a last small_svd
+// launch.  A plan whose buckets fit one lane replays a captured hipGraph; a
+// plan spread over several lanes is enqueued on them directly (see
+// lsvd_batch_plan_execute for the measured reason).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -89,7 +92,7 @@ static double item_cost(int M, int N) {
 // Sets the launch shape of a bucket for `variant`; returns the global arena
 // doubles it needs (0 for LDS variants).
 static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
-    if (variant >= kV2Multi) {  // one launch over sizes 64..128: LDS and slots for the largest item
+    if (variant >= kV2Multi && variant < kV2Wide) {  // one launch over sizes 64..128: LDS and slots for the largest item
         const int np = variant - kV2Multi;
         bk.variant = variant;
         const size_t lds = v2_lds_bytes(np, bk.maxM);
@@ -101,10 +104,11 @@ static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
         return (size_t)bk.gws_stride * bk.grid;
     }
     if (variant >= kV2Base) {  // on-chip kernel: LDS sized by the bucket's largest M
-        const int np = variant - kV2Base;
+        const bool wide = variant >= kV2Wide;
+        const int np = variant - (wide ? kV2Wide : kV2Base);
         bk.variant = variant;
         const size_t lds = v2_lds_bytes(np, bk.maxM);
-        const int per_cu = small2_blocks_per_cu(np, lds);
+        const int per_cu = small2_blocks_per_cu(np, wide, lds);
         bk.grid = std::max(1, std::min(bk.count, per_cu * h->num_cus));
         bk.lds_bytes = lds;
         bk.gws_stride = v2_slot_doubles(np, bk.maxM);
@@ -190,7 +194,7 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
             const int np = (N + 15) / 16 * 16;
             // on-chip kernels: one launch per padded size below 64, one
             // launch over all the larger sizes (a queue sorted longest first)
-            key = (np >= kV2MultiMinNP && h->multi_launch) ? 2000 : 1000 + np;
+            key = onchip_multi(h, np) ? 2000 : 1000 + np;
         } else {
             const int cls = n_class(N);
             if (cls < 0) {
@@ -248,7 +252,8 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         if (kv.first == 2000) {
             vid = kV2Multi + (maxN + 15) / 16 * 16;
         } else if (kv.first >= 1000) {
-            vid = kV2Base + (kv.first - 1000);
+            const int np = kv.first - 1000;
+            vid = onchip_variant(h, np);
         } else {
             vid = default_variant(maxN, arena);
             auto tk = h->tune.find(tune_key(maxN, arena, precision));
@@ -325,7 +330,8 @@ namespace lsvd {
 constexpr int kPlanLanes = 4;
 
 static int plan_lanes(lsvd_plan_t p) {
-    return std::min(kPlanLanes, (int)p->buckets.size() + (p->n_micro > 0 ? 1 : 0));
+    const int cap = std::max(1, std::min(kPlanLanes, p->h->max_lanes));
+    return std::min(cap, (int)p->buckets.size() + (p->n_micro > 0 ? 1 : 0));
 }
 
 hipError_t plan_ensure_streams(lsvd_plan_t p) {
@@ -738,9 +744,118 @@ int lsvd_svd_batched_f64(lsvd_handle_t h, int count, const int32_t* m, const int
     return rc;
 }
 
-// Autotuner: for every bucket of the batch, times each compatible kernel
-// variant on synthetic inputs of the bucket's shapes (best of 3 runs) and
-// records the fastest in the handle's cache, keyed by (maxN, arena, precision).
+// The plan-level knobs of the on-chip buckets: a coordinate search over
+// the multi-size launch's first padded size, the lane count and the threads
+// per workgroup of each per-size bucket, timing the whole plan on `pools`
+// (median of 5 executions after a warm-up; a candidate replaces the best so
+// far when it saves more than 1.5 %).  Leaves the chosen knobs in the handle
+// and the report in h->tune_report.
+static int tune_plan_knobs(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                           const int64_t* u_off, const int64_t* s_off, const int64_t* vt_off, int precision,
+                           const Pools& pools, hipEvent_t e0, hipEvent_t e1) {
+    const int saved_ptr_mode = h->ptr_mode;
+    std::vector<int32_t> pmodes(count);
+    for (int b = 0; b < count; ++b) pmodes[b] = (m[b] == n[b] && n[b] <= 4) ? kModeMicro : kModeSvd;
+    const int saved_mm = h->multi_min_np, saved_lanes = h->max_lanes;
+    const uint32_t saved_wide = h->wide_mask;
+    int evals = 0;
+    float t_ms = 0.f;
+    auto time_plan = [&](float* out) -> int {
+        h->ptr_mode = LSVD_PTR_DEVICE;
+        lsvd_plan_t q = nullptr;
+        int r = plan_build(h, count, m, n, a_off, u_off, s_off, vt_off, pmodes.data(), precision, &q);
+        h->ptr_mode = saved_ptr_mode;
+        if (r != LSVD_OK) return r;
+        float ts[5];
+        const char* what = "warm-up run";
+        hipError_t e = plan_enqueue(q, pools, h->stream);
+        for (int rep = 0; rep < 5 && e == hipSuccess; ++rep) {
+            what = "event record";
+            e = hipEventRecord(e0, h->stream);
+            if (e == hipSuccess) {
+                what = "timed run";
+                e = plan_enqueue(q, pools, h->stream);
+            }
+            if (e == hipSuccess) {
+                what = "event record";
+                e = hipEventRecord(e1, h->stream);
+            }
+            if (e == hipSuccess) {
+                what = "event wait";
+                e = hipEventSynchronize(e1);
+            }
+            if (e == hipSuccess) {
+                what = "event time";
+                e = hipEventElapsedTime(&ts[rep], e0, e1);
+            }
+        }
+        lsvd_batch_plan_destroy(q);
+        if (e != hipSuccess) {
+            const std::string msg = std::string("autotune plan (multi from ") + std::to_string(h->multi_min_np) +
+                                    ", lanes " + std::to_string(h->max_lanes) + ", wide " +
+                                    std::to_string(h->wide_mask) + "): " + what;
+            return hip_fail(h, e, msg.c_str());
+        }
+        std::sort(ts, ts + 5);
+        *out = ts[2];
+        ++evals;
+        return LSVD_OK;
+    };
+    int rc2 = time_plan(&t_ms);
+    if (rc2 != LSVD_OK) {
+        return rc2;
+    }
+    const float t_default = t_ms;
+    float t_best = t_ms;
+    // coordinate search: a candidate replaces the best so far when it
+    // saves more than 1.5 %
+    int best_mm = saved_mm, best_lanes = saved_lanes;
+    uint32_t best_wide = saved_wide;
+    auto try_knobs = [&](int mm, int lanes, uint32_t wide) -> int {
+        if (mm == best_mm && lanes == best_lanes && wide == best_wide) return LSVD_OK;
+        h->multi_min_np = mm;
+        h->max_lanes = lanes;
+        h->wide_mask = wide;
+        float t = 0.f;
+        const int r = time_plan(&t);
+        if (r == LSVD_OK && t < t_best * 0.985f) {
+            t_best = t;
+            best_mm = mm;
+            best_lanes = lanes;
+            best_wide = wide;
+        }
+        h->multi_min_np = best_mm;
+        h->max_lanes = best_lanes;
+        h->wide_mask = best_wide;
+        return r;
+    };
+    const int mms[] = {64, 80, 96, 129};
+    for (int mm : mms) {
+        const int r = try_knobs(mm, best_lanes, best_wide);
+        if (r != LSVD_OK) return r;
+    }
+    for (int lanes = 1; lanes <= kPlanLanes; ++lanes) {
+        const int r = try_knobs(best_mm, lanes, best_wide);
+        if (r != LSVD_OK) return r;
+    }
+    for (int np = 16; np <= 64; np += 16) {
+        if (np >= best_mm && h->multi_launch) continue;  // inside the multi-size launch
+        const int r = try_knobs(best_mm, best_lanes, best_wide ^ (1u << (np / 16 - 1)));
+        if (r != LSVD_OK) return r;
+    }
+    h->tune_report = lsvd_tune_report_t{best_mm, best_lanes, (int32_t)best_wide, evals, (double)t_default,
+                                        (double)t_best};
+        return LSVD_OK;
+}
+
+// Autotuner.  Buckets above 128: every compatible kernel variant is timed on
+// the bucket alone (best of 3) and the fastest recorded in the handle's
+// cache, keyed by (maxN, arena, precision).  Then the plan-level knobs of
+// the on-chip buckets (the multi-size launch's first padded size, the lane
+// count, default or doubled threads per workgroup for each per-size bucket)
+// are chosen by a coordinate search that times the whole plan (median of 5
+// executions; a change is kept when it saves more than 1.5 %), since the
+// buckets share the CUs and the best choice for one depends on the others.
 int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, int precision) {
     if (!h || count < 0 || (count > 0 && (!m || !n))) return LSVD_E_ARG;
     std::vector<int64_t> a_off(count), u_off(count), s_off(count), vt_off(count);
@@ -838,8 +953,42 @@ int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int3
         }
         h->tune[tune_key(base.maxN, base.arena_bytes, precision)] = best;
     }
+    // ---- plan-level knobs, timing whole plans on the same inputs
+    rc = tune_plan_knobs(h, count, m, n, a_off.data(), u_off.data(), s_off.data(), vt_off.data(), precision, pools,
+                         e0, e1);
+    if (rc != LSVD_OK) {
+        cleanup();
+        return rc;
+    }
 #undef TCHECK
     cleanup();
+    return LSVD_OK;
+}
+
+int lsvd_autotune_batch_pools(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, const int64_t* a_off,
+                              const int64_t* u_off, const int64_t* s_off, const int64_t* vt_off, int precision,
+                              const double* a_pool, double* u_pool, double* s_pool, double* vt_pool) {
+    if (!h || count < 0 || (count > 0 && (!m || !n || !a_off || !u_off || !s_off || !vt_off))) return LSVD_E_ARG;
+    if (h->ptr_mode != LSVD_PTR_DEVICE) return set_error(h, LSVD_E_ARG, "lsvd_autotune_batch_pools takes device pools");
+    if (count > 0 && (!a_pool || !u_pool || !s_pool || !vt_pool)) return set_error(h, LSVD_E_ARG, "null pool pointer");
+    hipError_t e = hipSetDevice(h->device);
+    if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(h, e, "hipEventCreate");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(h, e, "hipEventCreate");
+    }
+    const Pools pools{a_pool, u_pool, s_pool, vt_pool, nullptr};
+    const int rc = tune_plan_knobs(h, count, m, n, a_off, u_off, s_off, vt_off, precision, pools, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+}
+
+int lsvd_autotune_report(lsvd_handle_t h, lsvd_tune_report_t* out) {
+    if (!h || !out) return LSVD_E_ARG;
+    *out = h->tune_report;
     return LSVD_OK;
 }
 
@@ -849,8 +998,10 @@ int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, in
     if (p->n_micro > 0) d.push_back({0, p->n_micro, 4, -1, (p->n_micro + 255) / 256, 256, 0});
     for (auto& bk : p->buckets) {
         const SmallVariant* v = bk.variant >= kV2Base ? nullptr : find_variant(bk.variant);
-        const int nt = bk.variant >= kV2Multi ? 1024
-                       : bk.variant >= kV2Base ? v2_threads(bk.variant - kV2Base) : (v ? v->nt : 0);
+        const int nt = bk.variant >= kV2Wide    ? 2 * v2_threads(bk.variant - kV2Wide)
+                       : bk.variant >= kV2Multi ? 1024
+                       : bk.variant >= kV2Base  ? v2_threads(bk.variant - kV2Base)
+                                                : (v ? v->nt : 0);
         d.push_back({bk.first, bk.count, bk.maxN, bk.variant, bk.grid, nt, (int64_t)bk.lds_bytes});
     }
     *nbuckets = (int)d.size();

@@ -63,11 +63,30 @@ struct lsvd_handle_s {
     int large_min_n = 129;                        // square single solves from this n run chip-wide
     int micro_max_sweeps = 24;                    // LieSvdMicro's sweep limit for n = 3, 4 (test hook)
     bool multi_launch = true;                     // sizes 64..128 of a batch in one launch
+    // autotuned plan knobs (lsvd_autotune_batch): padded sizes from
+    // multi_min_np up share the multi-size launch (129: none), at most
+    // max_lanes concurrent lanes, and bit (np / 16 - 1) of wide_mask runs the
+    // size-np bucket with twice the default threads per workgroup
+    int multi_min_np = 64;
+    int max_lanes = 4;
+    uint32_t wide_mask = 0;
+    lsvd_tune_report_t tune_report{};
     int lane0_cus = 0;                            // CUs of lane 0 (the longest launch); the other
                                                   // lanes take the rest; 0: every lane on every CU
     lsvd::Extras* extras = nullptr;
     lsvd::Workspace* ws = nullptr;                // temporaries of the entry points (grown, then reused)
 };
+
+namespace lsvd {
+// Whether a padded size np <= 128 goes to the multi-size launch under the
+// handle's knobs, and the per-size variant id otherwise.
+inline bool onchip_multi(const lsvd_handle_s* h, int np) {
+    return np >= (h->multi_min_np > kV2MultiMinNP ? h->multi_min_np : kV2MultiMinNP) && h->multi_launch;
+}
+inline int onchip_variant(const lsvd_handle_s* h, int np) {
+    return ((h->wide_mask >> (np / 16 - 1)) & 1) && v2_has_wide(np) ? kV2Wide + np : kV2Base + np;
+}
+}  // namespace lsvd
 
 struct lsvd_plan_s {
     lsvd_handle_t h = nullptr;

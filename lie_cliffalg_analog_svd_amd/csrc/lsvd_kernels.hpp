@@ -40,13 +40,16 @@ int small_variant_blocks_per_cu(int id, size_t lds_bytes);
 hipError_t launch_reset_counters(int* counters, int n, hipStream_t st);
 
 // ---- on-chip batched LieSvdSmall for N, M <= 128 (small_svd2.hip) -------
-// np: padded size 16, 32, ..., 128 (N <= np).  Variant id = 100 + np.
+// np: padded size 16, 32, ..., 128 (N <= np).  Variant id = 100 + np with
+// the default threads per workgroup, 500 + np with twice as many (np <= 64).
 int v2_threads(int np);
+bool v2_has_wide(int np);
 size_t v2_lds_bytes(int np, int max_m);
 int64_t v2_slot_doubles(int np, int max_m);
-hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st);
-int small2_blocks_per_cu(int np, size_t lds_bytes);
+hipError_t launch_small2(int np, bool wide, const SmallLaunch& L, hipStream_t st);
+int small2_blocks_per_cu(int np, bool wide, size_t lds_bytes);
 constexpr int kV2Base = 100;
+constexpr int kV2Wide = 500;
 // One persistent launch over the items of every padded size 64..128 (1024
 // threads per matrix, LDS and slots sized for np_max): variant id =
 // kV2Multi + np_max.

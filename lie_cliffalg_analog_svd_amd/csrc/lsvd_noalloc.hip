@@ -96,7 +96,7 @@ extern "C" int lsvd_small_solve_rectangular_f64_noalloc(lsvd_handle_t h, const d
 
     const int M = m > n ? m : n, N = m < n ? m : n;
     const int np = (N + 15) / 16 * 16;
-    const bool multi = np >= kV2MultiMinNP && h->multi_launch;  // as the batch plan buckets it
+    const bool multi = onchip_multi(h, np);  // as the batch plan buckets it
     NoallocSetup su{};
     su.item = Item{m, n, 0, kModeSvd, 0, 0, 0, 0};
     su.ints[0] = 0;  // queue counter
@@ -121,6 +121,6 @@ extern "C" int lsvd_small_solve_rectangular_f64_noalloc(lsvd_handle_t h, const d
     L.ns_scaled = 0;  // LSVD_PREC_F64, as the single-matrix entry points
     if (kNoallocHead + sizeof(double) * (size_t)L.gws_stride > kNoallocScratchBytes)
         return set_error(h, LSVD_E_LIMIT, "no-alloc staging slot exceeds the scratch");
-    e = launch_small_variant(multi ? kV2Multi + np : kV2Base + np, L, h->stream);
+    e = launch_small_variant(multi ? kV2Multi + np : onchip_variant(h, np), L, h->stream);
     return e == hipSuccess ? LSVD_OK : hip_fail(h, e, "kernel launch");
 }

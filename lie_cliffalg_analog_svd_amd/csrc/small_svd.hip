@@ -508,8 +508,9 @@ const SmallVariant* small_variants(int* count) {
 }
 
 hipError_t launch_small_variant(int id, const SmallLaunch& L, hipStream_t st) {
+    if (id >= kV2Wide) return launch_small2(id - kV2Wide, true, L, st);
     if (id >= kV2Multi) return launch_small2_multi(L, st);
-    if (id >= kV2Base) return launch_small2(id - kV2Base, L, st);
+    if (id >= kV2Base) return launch_small2(id - kV2Base, false, L, st);
     switch (id) {
 #define X(vid, nt, maxn, lds) \
     case vid:                 \
@@ -521,8 +522,9 @@ hipError_t launch_small_variant(int id, const SmallLaunch& L, hipStream_t st) {
 }
 
 int small_variant_blocks_per_cu(int id, size_t lds) {
+    if (id >= kV2Wide) return small2_blocks_per_cu(id - kV2Wide, true, lds);
     if (id >= kV2Multi) return small2_multi_blocks_per_cu(lds);
-    if (id >= kV2Base) return small2_blocks_per_cu(id - kV2Base, lds);
+    if (id >= kV2Base) return small2_blocks_per_cu(id - kV2Base, false, lds);
     switch (id) {
 #define X(vid, nt, maxn, lds_) \
     case vid:                  \

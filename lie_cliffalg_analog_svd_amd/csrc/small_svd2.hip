@@ -921,13 +921,12 @@ __device__ void solve_item2(const Item& itm, const Pools& P, double* main, doubl
     __syncthreads();
 }
 
-template <int NP>
-__global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __restrict__ items, const int* count_ptr,
-                                                                  int* counter, Pools pools, double* gws,
-                                                                  int64_t gws_stride, int lds_doubles,
-                                                                  int ns_scaled) {
+template <int NP, int NT = v2_default_nt<NP>()>
+__global__ __launch_bounds__(NT) void small_svd2_kernel(const Item* __restrict__ items, const int* count_ptr,
+                                                        int* counter, Pools pools, double* gws, int64_t gws_stride,
+                                                        int lds_doubles, int ns_scaled) {
     extern __shared__ double dyn_lds[];
-    __shared__ Sh2<NP> sh;
+    __shared__ Sh2<NP, NT> sh;
     const int count = *count_ptr;
     const int64_t part = gws_stride / 3;
     double* xs = gws + (size_t)blockIdx.x * gws_stride;
@@ -944,9 +943,9 @@ __global__ __launch_bounds__(Cfg2<NP>::NT) void small_svd2_kernel(const Item* __
         const int MP = (M + 15) & ~15;
         const int rows = MP > NP ? MP : NP;
         double* main = dyn_lds;
-        double* panel = dyn_lds + (size_t)rows * Cfg2<NP>::LDM;
-        const int panel_rows = (lds_doubles - rows * Cfg2<NP>::LDM) / Cfg2<NP>::LDM;
-        solve_item2<NP>(itm, pools, main, panel, panel_rows, xs, vs, ys, sh, ns_scaled);
+        double* panel = dyn_lds + (size_t)rows * Cfg2<NP, NT>::LDM;
+        const int panel_rows = (lds_doubles - rows * Cfg2<NP, NT>::LDM) / Cfg2<NP, NT>::LDM;
+        solve_item2<NP, NT>(itm, pools, main, panel, panel_rows, xs, vs, ys, sh, ns_scaled);
     }
 }
 
@@ -1033,28 +1032,30 @@ int small2_multi_blocks_per_cu(size_t lds) {
     return nb > 0 ? nb : 1;
 }
 
-template <int NP>
+template <int NP, int NT = v2_default_nt<NP>()>
 static hipError_t launch2(const SmallLaunch& L, hipStream_t st) {
     if (L.lds_bytes > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&small_svd2_kernel<NP>),
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&small_svd2_kernel<NP, NT>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes);
         if (e != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((small_svd2_kernel<NP>), dim3(L.grid), dim3(Cfg2<NP>::NT), L.lds_bytes, st, L.items,
-                       L.count_ptr, L.counter, L.pools, L.gws, L.gws_stride, (int)(L.lds_bytes / sizeof(double)),
-                       L.ns_scaled);
+    hipLaunchKernelGGL((small_svd2_kernel<NP, NT>), dim3(L.grid), dim3(NT), L.lds_bytes, st, L.items, L.count_ptr,
+                       L.counter, L.pools, L.gws, L.gws_stride, (int)(L.lds_bytes / sizeof(double)), L.ns_scaled);
     return hipGetLastError();
 }
 
-template <int NP>
+template <int NP, int NT = v2_default_nt<NP>()>
 static int blocks2(size_t lds) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, small_svd2_kernel<NP>, Cfg2<NP>::NT, lds) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, small_svd2_kernel<NP, NT>, NT, lds) != hipSuccess)
         return 1;
     return nb > 0 ? nb : 1;
 }
 
 #define LSVD_V2_SIZES(X) X(16) X(32) X(48) X(64) X(80) X(96) X(112) X(128)
+// Sizes with a second instance at twice the default threads (the autotuner's
+// choice per bucket).
+#define LSVD_V2_WIDE_SIZES(X) X(16) X(32) X(48) X(64)
 
 int v2_threads(int np) {
     switch (np) {
@@ -1065,6 +1066,17 @@ int v2_threads(int np) {
 #undef X
     }
     return 0;
+}
+
+bool v2_has_wide(int np) {
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return true;
+        LSVD_V2_WIDE_SIZES(X)
+#undef X
+    }
+    return false;
 }
 
 // LDS of one workgroup: the matrix area (max(16 ceil(M/16), NP) rows of
@@ -1090,7 +1102,17 @@ int64_t v2_slot_doubles(int np, int M) {
     return 3 * part;
 }
 
-hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st) {
+hipError_t launch_small2(int np, bool wide, const SmallLaunch& L, hipStream_t st) {
+    if (wide) {
+        switch (np) {
+#define X(np_) \
+    case np_:  \
+        return launch2<np_, 2 * v2_default_nt<np_>()>(L, st);
+            LSVD_V2_WIDE_SIZES(X)
+#undef X
+        }
+        return hipErrorInvalidValue;
+    }
     switch (np) {
 #define X(np_) \
     case np_:  \
@@ -1101,7 +1123,17 @@ hipError_t launch_small2(int np, const SmallLaunch& L, hipStream_t st) {
     return hipErrorInvalidValue;
 }
 
-int small2_blocks_per_cu(int np, size_t lds) {
+int small2_blocks_per_cu(int np, bool wide, size_t lds) {
+    if (wide) {
+        switch (np) {
+#define X(np_) \
+    case np_:  \
+        return blocks2<np_, 2 * v2_default_nt<np_>()>(lds);
+            LSVD_V2_WIDE_SIZES(X)
+#undef X
+        }
+        return 1;
+    }
     switch (np) {
 #define X(np_) \
     case np_:  \

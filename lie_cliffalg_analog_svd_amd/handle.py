@@ -78,6 +78,30 @@ class Handle:
         check(self._h, lib().lsvd_set_pointer_mode(self._h, mode))
         self.pointer_mode = mode
 
+    def autotune(self, ms, ns, precision: int = _lib.LSVD_PREC_F64) -> dict:
+        """lsvd_autotune_batch for a batch's shapes; returns the report
+        (chosen plan knobs and the default / tuned plan times)."""
+        ms = _i32(ms)
+        ns = _i32(ns)
+        check(self._h, lib().lsvd_autotune_batch(self._h, int(ms.shape[0]), _as_i32p(ms), _as_i32p(ns), precision))
+        rep = _lib.LsvdTuneReport()
+        check(self._h, lib().lsvd_autotune_report(self._h, C.byref(rep)))
+        return rep.as_dict()
+
+    def autotune_pools(self, ms, ns, layout, a_pool, u_pool, s_pool, vt_pool,
+                       precision: int = _lib.LSVD_PREC_F64) -> dict:
+        """lsvd_autotune_batch_pools: the plan knobs tuned on the caller's
+        device pools (packed layout from pack_offsets); returns the report."""
+        ms = _i32(ms)
+        ns = _i32(ns)
+        offs = [_i64(layout[k]) for k in ("a_off", "u_off", "s_off", "vt_off")]
+        check(self._h, lib().lsvd_autotune_batch_pools(self._h, int(ms.shape[0]), _as_i32p(ms), _as_i32p(ns),
+                                                         *[_as_i64p(o) for o in offs], precision, _ptr(a_pool),
+                                                         _ptr(u_pool), _ptr(s_pool), _ptr(vt_pool)))
+        rep = _lib.LsvdTuneReport()
+        check(self._h, lib().lsvd_autotune_report(self._h, C.byref(rep)))
+        return rep.as_dict()
+
     def workspace_info(self) -> tuple[int, int]:
         """(bytes, chunks) of the device workspace the handle keeps for its
         entry points' temporaries (lsvd_workspace_info)."""

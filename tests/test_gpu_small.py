@@ -352,3 +352,35 @@ def test_batch_rejects_low_precision():
     for prec in (LSVD_PREC_F32, LSVD_PREC_BF16):
         with pytest.raises(LsvdError):
             svd_batched(mats, precision=prec)
+
+
+def test_autotune_plan_knobs_and_results():
+    """lsvd_autotune_batch on a ragged batch (512 ragged + 256 tiny): the
+    report names valid knobs, times at least the default plan and one
+    candidate per knob, and the tuned plan is no slower than the default by
+    its own measurement; the tuned plan's sigma equal the default plan's
+    (the knobs change scheduling and threads per workgroup only) to the
+    two-tier FP64 bar."""
+    from lie_cliffalg_analog_svd_amd import LSVD_PREC_F64_SCALED, workloads
+    from lie_cliffalg_analog_svd_amd.handle import Handle
+    batch = workloads.ragged_batch(512, 256)
+    mats = batch.mats
+    ms = np.array([a.shape[0] for a in mats], dtype=np.int32)
+    ns = np.array([a.shape[1] for a in mats], dtype=np.int32)
+    h0 = Handle()
+    h1 = Handle()
+    try:
+        r0 = svd_batched(mats, precision=LSVD_PREC_F64_SCALED, handle=h0)
+        rep = h1.autotune(ms, ns, LSVD_PREC_F64_SCALED)
+        _say("autotune report", rep)
+        assert rep["multi_min_np"] in (64, 80, 96, 129)
+        assert 1 <= rep["max_lanes"] <= 4 and 0 <= rep["wide_mask"] < 16
+        assert rep["evaluations"] >= 8
+        assert 0.0 < rep["tuned_ms"] <= rep["default_ms"]
+        r1 = svd_batched(mats, precision=LSVD_PREC_F64_SCALED, handle=h1)
+        for b in range(len(mats)):
+            k = min(mats[b].shape)
+            assert_sigma_close(r1[b][1], r0[b][1], k, what=f"tuned vs default {b}")
+    finally:
+        h0.close()
+        h1.close()

@@ -23,7 +23,6 @@ from .tensor import (LieSvdTensor, TensorPhaseSvd3, TensorPhaseTrace, hosvd3_bat
                      rotate_tensor3_batched, tensor_relative_error)
 from .bss import (LieSvdBss, PhaseBssParams, PhaseBssResult, PhaseBssTrace, channel_phase_coherence,
                   estimate_sir_db, separate_batched)
-from .schedule import (ComplexMziPhase, ComplexPhaseEventKind, HardwarePhaseEvent, HardwareSchedule, HardwareTarget,
-                       MziPhase, PhaseRotorKind, givens_batch, schedules_from_orthogonal)
+from .schedule import HardwarePhaseEvent, HardwareSchedule, HardwareTarget, givens_batch, schedules_from_orthogonal
 
 __version__ = "0.1.0"

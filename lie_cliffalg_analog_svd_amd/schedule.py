@@ -5,9 +5,9 @@
 events of the reference's "phase-schedule-v1" format; the elimination runs
 on the GPU (``lsvd_givens_schedule_f64``) and ``schedules_from_orthogonal``
 does a whole batch in one call.  ``to_json_string`` writes the reference's
-JSON text byte for byte (``lsvd_schedule_json``).  ``from_real_phaseflow`` /
-``from_complex_phaseflow`` wrap event logs of the PhaseFlow routes, which
-are not part of this library, in the same schedule shape.
+JSON text byte for byte (``lsvd_schedule_json``), including the
+source and kind names the format defines for the PhaseFlow routes' events
+(those routes and their event producers are not part of this library).
 """
 from __future__ import annotations
 
@@ -32,54 +32,11 @@ class HardwareTarget(enum.IntEnum):
         return "mzi_mesh" if self is HardwareTarget.MziMesh else "fpga_rotor_mesh"
 
 
-class PhaseRotorKind(enum.IntEnum):
-    """The real PhaseFlow rotor kinds and their schedule names
-    (src/lie_svd_compiler.rs:270-282)."""
-    GoldenPreSpin = 0
-    CausalAntiSpin = 1
-    CrossPhaseYinYang = 2
-    PhaseConjugate = 3
-    Bottleneck = 4
-    PhaseJump = 5
-    Unwrap = 6
-    Directional = 7
-    Surgery = 8
-
-
-class ComplexPhaseEventKind(enum.IntEnum):
-    """src/lie_svd_compiler.rs:284-289 (codes follow the real kinds)."""
-    GoldenPreSpin = 9
-    DiagonalPhase = 10
-
-
 _SOURCES = {"real_phaseflow": 0, "complex_phaseflow": 1, "orthogonal_givens": 2}
 _KINDS = ["golden_prespin", "causal_antispin", "cross_phase_yinyang", "phase_conjugate", "bottleneck",
           "phase_jump", "unwrap", "directional", "surgery", "complex_golden_prespin", "complex_diagonal_phase",
           "givens_elimination"]
 _KIND_CODE = {k: c for c, k in enumerate(_KINDS)}
-
-
-@dataclass
-class MziPhase:
-    """A real PhaseFlow rotor event (the fields the schedule reads)."""
-    pass_: int
-    i: int
-    j: int
-    theta_l: float
-    theta_r: float
-    kind: PhaseRotorKind
-
-
-@dataclass
-class ComplexMziPhase:
-    """A complex PhaseFlow event (the fields the schedule reads)."""
-    layer: int
-    i: int
-    j: int
-    phi_l: float
-    phi_r: float
-    theta: float
-    kind: ComplexPhaseEventKind
 
 
 @dataclass
@@ -107,20 +64,6 @@ class HardwareSchedule:
     channels: int
     events: list = field(default_factory=list)
     diagonal_signs: list = field(default_factory=list)
-
-    @staticmethod
-    def from_real_phaseflow(events, channels: int, target: HardwareTarget) -> "HardwareSchedule":
-        """src/lie_svd_compiler.rs:59-88."""
-        ev = [HardwarePhaseEvent(e.pass_, e.i, e.j, 0.0, 0.0, 0.5 * (e.theta_l + e.theta_r), e.theta_l, e.theta_r,
-                                 None, None, "real_phaseflow", _KINDS[int(e.kind)]) for e in events]
-        return HardwareSchedule(FORMAT_VERSION, HardwareTarget(target), channels, ev, [])
-
-    @staticmethod
-    def from_complex_phaseflow(events, channels: int, target: HardwareTarget) -> "HardwareSchedule":
-        """src/lie_svd_compiler.rs:90-118."""
-        ev = [HardwarePhaseEvent(e.layer, e.i, e.j, e.phi_l, e.phi_r, e.theta, e.theta, e.theta, None, None,
-                                 "complex_phaseflow", _KINDS[int(e.kind)]) for e in events]
-        return HardwareSchedule(FORMAT_VERSION, HardwareTarget(target), channels, ev, [])
 
     @staticmethod
     def from_orthogonal_matrix(v, target: HardwareTarget, handle: Handle | None = None) -> "HardwareSchedule":

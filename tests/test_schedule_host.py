@@ -13,32 +13,38 @@ import pytest
 import oracle
 from lie_cliffalg_analog_svd_amd import _lib
 from lie_cliffalg_analog_svd_amd.handle import _ptr
-from lie_cliffalg_analog_svd_amd.schedule import (ComplexMziPhase, ComplexPhaseEventKind, HardwarePhaseEvent,
-                                                   HardwareSchedule, HardwareTarget, MziPhase, PhaseRotorKind,
-                                                   schedule_json)
+from lie_cliffalg_analog_svd_amd.schedule import HardwarePhaseEvent, HardwareSchedule, HardwareTarget, schedule_json
+
+
+def _real(layer, i, j, theta_l, theta_r, kind):
+    """An event as the reference's from_real_phaseflow shapes it
+    (src/lie_svd_compiler.rs:59-88: theta = (theta_l + theta_r) / 2, no
+    phases, no energies); the producers themselves are out of scope."""
+    return HardwarePhaseEvent(layer, i, j, 0.0, 0.0, 0.5 * (theta_l + theta_r), theta_l, theta_r, None, None,
+                              "real_phaseflow", kind)
+
+
+def _sched(events, channels, target):
+    return HardwareSchedule("phase-schedule-v1", target, channels, events, [])
 
 
 def test_real_schedule_json_contains_layers_and_angles():
-    s = HardwareSchedule.from_real_phaseflow([MziPhase(2, 0, 1, 0.125, -0.25, PhaseRotorKind.Unwrap)], 4,
-                                             HardwareTarget.MziMesh)
+    s = _sched([_real(2, 0, 1, 0.125, -0.25, "unwrap")], 4, HardwareTarget.MziMesh)
     js = s.to_json_string()
     assert s.total_events() == 1 and s.conflict_free_layer_count() == 3
     assert '"target": "mzi_mesh"' in js and '"kind": "unwrap"' in js
 
 
 def test_real_schedule_json_names_kinds():
-    s = HardwareSchedule.from_real_phaseflow([MziPhase(0, 2, 3, 0.1, -0.1, PhaseRotorKind.CausalAntiSpin)], 4,
-                                             HardwareTarget.MziMesh)
+    s = _sched([_real(0, 2, 3, 0.1, -0.1, "causal_antispin")], 4, HardwareTarget.MziMesh)
     assert "causal_antispin" in s.to_json_string()
-    s = HardwareSchedule.from_real_phaseflow([MziPhase(1, 0, 2, 0.03, 0.0, PhaseRotorKind.CrossPhaseYinYang)], 4,
-                                             HardwareTarget.MziMesh)
+    s = _sched([_real(1, 0, 2, 0.03, 0.0, "cross_phase_yinyang")], 4, HardwareTarget.MziMesh)
     assert "cross_phase_yinyang" in s.to_json_string()
-    s = HardwareSchedule.from_real_phaseflow([MziPhase(0, 0, 1, -0.02, 0.03, PhaseRotorKind.PhaseConjugate),
-                                              MziPhase(1, 2, 3, 0.04, -0.05, PhaseRotorKind.Bottleneck)], 4,
-                                             HardwareTarget.MziMesh)
+    s = _sched([_real(0, 0, 1, -0.02, 0.03, "phase_conjugate"), _real(1, 2, 3, 0.04, -0.05, "bottleneck")], 4,
+               HardwareTarget.MziMesh)
     js = s.to_json_string()
     assert "phase_conjugate" in js and "bottleneck" in js
-    empty = HardwareSchedule.from_real_phaseflow([], 3, HardwareTarget.FpgaRotorMesh)
+    empty = _sched([], 3, HardwareTarget.FpgaRotorMesh)
     assert empty.conflict_free_layer_count() == 0 and empty.total_events() == 0
 
 
@@ -75,9 +81,9 @@ def test_native_writer_matches_oracle_writer(seed):
 
 
 def test_complex_schedule_fields():
-    s = HardwareSchedule.from_complex_phaseflow(
-        [ComplexMziPhase(5, 1, 2, 0.25, -0.5, 0.75, ComplexPhaseEventKind.DiagonalPhase)], 3,
-        HardwareTarget.FpgaRotorMesh)
+    # the shape of src/lie_svd_compiler.rs:90-118 (theta_l = theta_r = theta)
+    s = _sched([HardwarePhaseEvent(5, 1, 2, 0.25, -0.5, 0.75, 0.75, 0.75, None, None, "complex_phaseflow",
+                                   "complex_diagonal_phase")], 3, HardwareTarget.FpgaRotorMesh)
     e = s.events[0]
     assert (e.layer, e.theta_l, e.theta_r, e.source, e.kind) == (5, 0.75, 0.75, "complex_phaseflow",
                                                                  "complex_diagonal_phase")
@@ -114,8 +120,7 @@ def test_writer_rejects_unknown_codes():
     with pytest.raises(ValueError):
         schedule_json(bad)
     # truncation: cap bytes written including the NUL, full length returned
-    s = HardwareSchedule.from_real_phaseflow([MziPhase(0, 0, 1, 0.5, 0.5, PhaseRotorKind.Surgery)], 2,
-                                             HardwareTarget.MziMesh)
+    s = _sched([_real(0, 0, 1, 0.5, 0.5, "surgery")], 2, HardwareTarget.MziMesh)
     full = s.to_json_string()
     from lie_cliffalg_analog_svd_amd.schedule import _c_events
     buf = C.create_string_buffer(11)

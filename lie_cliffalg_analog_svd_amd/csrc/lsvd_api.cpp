@@ -638,6 +638,8 @@ int lsvd_create(int device, lsvd_handle_t* out) {
         if (end != env && v >= 2) h->large_min_n = (int)std::min<long>(v, kLargeMaxN + 1);
     }
     if (const char* env = std::getenv("LSVD_BATCH_MULTI")) h->multi_launch = std::strtol(env, nullptr, 10) != 0;
+    if (const char* env = std::getenv("LSVD_JADE_CHUNK_MB"))
+        h->jade_chunk_bytes = (size_t)std::max(0L, std::strtol(env, nullptr, 10)) << 20;
     if (const char* env = std::getenv("LSVD_LANE0_CUS")) h->lane0_cus = (int)std::strtol(env, nullptr, 10);
     if (const char* env = std::getenv("LSVD_MICRO_MAX_SWEEPS")) {
         char* end = nullptr;

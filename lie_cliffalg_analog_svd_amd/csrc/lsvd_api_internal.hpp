@@ -71,6 +71,11 @@ struct lsvd_handle_s {
     int max_lanes = 4;
     uint32_t wide_mask = 0;
     lsvd_tune_report_t tune_report{};
+    // Phase-JADE sweeps run this many bytes of families (working copies and
+    // bases) at a time (0: all families together).  About 3/4 of the 256 MB
+    // Infinity Cache: 32 families of 64 x 256^2 took 1.25 s unchunked, 1.30 /
+    // 1.15 / 1.04 / 1.23 s in chunks of 96 / 128 / 192 / 320 MB.
+    size_t jade_chunk_bytes = size_t(192) << 20;
     int lane0_cus = 0;                            // CUs of lane 0 (the longest launch); the other
                                                   // lanes take the rest; 0: every lane on every CU
     lsvd::Extras* extras = nullptr;

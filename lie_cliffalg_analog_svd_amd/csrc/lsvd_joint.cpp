@@ -227,8 +227,27 @@ int lsvd_joint_diag_sym_f64(lsvd_handle_t h, int families, int k, int n, const d
     JCHECK(joint_launch_init(a, st));
     std::vector<JointFamily> fam(families);
     if (n > 1 && prm.max_sweeps > 0) {
-        int rc = joint_run_sweeps(h, a, prm.max_sweeps, fam.data());
-        if (rc != LSVD_OK) return rc;
+        // The families are independent: their sweeps run a chunk of
+        // families at a time, sized so that a chunk's working copies and
+        // bases stay in the Infinity Cache across the layers of a sweep
+        // (every layer reads and writes all of them once).
+        const size_t per_family = sizeof(double) * ((size_t)k + 1) * nn;
+        const int chunk = h->jade_chunk_bytes == 0
+                              ? families
+                              : (int)std::max<size_t>(1, std::min<size_t>(families, h->jade_chunk_bytes / per_family));
+        for (int f0 = 0; f0 < families; f0 += chunk) {
+            JointArgs c = a;
+            c.F = std::min(chunk, families - f0);
+            c.mats = a.mats + (size_t)f0 * k * nn;
+            c.work = a.work + (size_t)f0 * k * nn;
+            c.basis = a.basis + (size_t)f0 * nn;
+            c.offrow = a.offrow + (size_t)f0 * k * n;
+            c.part = a.part + (size_t)f0 * k * 2;
+            c.rot = a.rot + (size_t)f0 * slots;
+            c.fam = a.fam + f0;
+            int rc = joint_run_sweeps(h, c, prm.max_sweeps, fam.data() + f0);
+            if (rc != LSVD_OK) return rc;
+        }
     }
     JCHECK(joint_launch_diag(a, d_diag, st));
     JCHECK(hipMemcpyAsync(fam.data(), a.fam, sizeof(JointFamily) * families, hipMemcpyDeviceToHost, st));

@@ -800,9 +800,19 @@ def polar_leg(h, dev, args):
                                  "sample": f"2 NS iterations of a {xs.shape[0]}x1024 matrix in the C restatement "
                                            f"on {threads} threads, scaled by 4mn^2 to {m}x{n} and by the "
                                            f"{f64_iters} FP64 iterations"}
-        entry["speedup_f64_scaled_vs_f64"] = round(entry["f64"]["seconds"] / entry["f64_scaled"]["seconds"], 3)
-        entry["speedup_bf16_vs_f64"] = round(entry["f64"]["seconds"] / entry["bf16"]["seconds"], 3)
-        entry["speedup_f32_vs_f64"] = round(entry["f64"]["seconds"] / entry["f32"]["seconds"], 3)
+        # speedups only at a stated error: every variant's time ratio with its
+        # error against the FP64 factor and its trust flag; the chosen route is
+        # the fastest trusted variant within 1e-9 of the FP64 factor
+        entry["speedup_at_err_vs_f64"] = {
+            v: {"speedup": round(entry["f64"]["seconds"] / entry[v]["seconds"], 3),
+                "err_vs_f64_fro_over_sqrt_n": entry[v]["err_vs_f64_fro_over_sqrt_n"],
+                "polar_ok": entry[v]["polar_ok"]} for v in ("f64_scaled", "f32", "bf16")}
+        ok = [v for v in ("f64", "f64_scaled", "f32", "bf16")
+              if entry[v]["polar_ok"] and entry[v]["err_vs_f64_fro_over_sqrt_n"] <= 1e-9]
+        best = min(ok, key=lambda v: entry[v]["seconds"])
+        entry["chosen"] = {"variant": best, "speedup_vs_f64": round(entry["f64"]["seconds"] / entry[best]["seconds"], 3),
+                           "err_vs_f64_fro_over_sqrt_n": entry[best]["err_vs_f64_fro_over_sqrt_n"],
+                           "rule": "fastest variant with polar_ok = 1 within 1e-9 of the FP64 factor"}
         res[f"{m}x{n}"] = entry
         del ad
     return res

@@ -71,6 +71,8 @@ def algorithmic_flops(m: int, n: int, k_ns: int, sweeps: int) -> float:
 
 
 def kernel_name(variant: int) -> str:
+    if variant >= 700:
+        return f"os_svd_kernel<{variant - 700}> (QR + one-sided Jacobi)"
     if variant >= 500:
         return f"small_svd2_kernel<{variant - 500}, twice the default threads>"
     if variant >= 300:
@@ -823,8 +825,8 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="scaled", choices=["scaled", "f64"],
-                    help="batch NS schedule: scaled FP64 steps (chosen) or the reference iteration")
+    ap.add_argument("--precision", default="scaled", choices=["scaled", "f64", "onesided"],
+                    help="batch route: FP64 NS with scaled steps, the reference iteration, or QR + one-sided Jacobi")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--tiny", type=int, default=8192)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
@@ -873,7 +875,8 @@ def main(argv=None) -> int:
     from lie_cliffalg_analog_svd_amd import _lib, metrics, workloads
     from lie_cliffalg_analog_svd_amd.handle import BatchPlan, Handle, info_array
 
-    prec = {"f64": L.LSVD_PREC_F64, "scaled": L.LSVD_PREC_F64_SCALED}[args.precision]
+    prec = {"f64": L.LSVD_PREC_F64, "scaled": L.LSVD_PREC_F64_SCALED,
+            "onesided": L.LSVD_PREC_F64_ONESIDED}[args.precision]
     t0 = time.time()
     batch = workloads.ragged_batch(args.count, args.tiny)
     mats = batch.mats

@@ -57,6 +57,14 @@ enum lsvd_pointer_mode { LSVD_PTR_HOST = 0, LSVD_PTR_DEVICE = 1 };
  *                        applied to 1.5 X (they grow 2.25x instead of
  *                        1.5x per step); plain steps finish.  No reference
  *                        counterpart.
+ *   LSVD_PREC_F64_ONESIDED batch plans only: SVD items with max(m, n) <= 128
+ *                        and 16 ceil(min(m, n) / 16) <= the handle's
+ *                        os_max_np (default 64, environment LSVD_OS_MAX_NP at
+ *                        handle creation, chosen by lsvd_autotune_batch) are
+ *                        solved by Householder QR and one-sided (Hestenes)
+ *                        Jacobi on R^T in FP64 (no Newton-Schulz; info.path =
+ *                        4, ns_iters = 0); the rest of the plan runs as
+ *                        LSVD_PREC_F64_SCALED.
  *   LSVD_PREC_F32 / BF16 low-precision MFMA iterations (FP32 accumulate,
  *                        scaled schedule), then FP64 refinement steps; large
  *                        polar only (lsvd_polar_f64), see there. */
@@ -64,7 +72,8 @@ enum lsvd_precision {
     LSVD_PREC_F64 = 0,
     LSVD_PREC_F32 = 1,
     LSVD_PREC_BF16 = 2,
-    LSVD_PREC_F64_SCALED = 3
+    LSVD_PREC_F64_SCALED = 3,
+    LSVD_PREC_F64_ONESIDED = 4
 };
 
 /* Per-matrix diagnostics (fills the reference's solver traces). */
@@ -76,7 +85,8 @@ typedef struct {
     int32_t repaired;       /* left basis repaired from the right one */
     int32_t micro_fallback; /* LieSvdMicro escalated to LieSvdSmall */
     int32_t path;           /* 0 small, 1 micro, 2 micro->small fallback,
-                               3 chip-wide route of a large single matrix */
+                               3 chip-wide route of a large single matrix,
+                               4 QR + one-sided Jacobi (LSVD_PREC_F64_ONESIDED) */
     int32_t refine_iters;   /* FP64 refinement iterations (mixed precision) */
 } lsvd_info_t;
 
@@ -556,13 +566,16 @@ typedef struct {
     int32_t evaluations;   /* plans timed */
     double default_ms;     /* the handle's settings before tuning */
     double tuned_ms;       /* the chosen settings */
+    int32_t os_max_np;     /* LSVD_PREC_F64_ONESIDED: padded sizes <= this take QR + one-sided Jacobi */
+    int32_t reserved;
 } lsvd_tune_report_t;
 
 /* Runs the autotuner for a batch's shapes and stores its choices in the
  * handle (used by later plans): for the on-chip buckets (N, M <= 128) a
- * coordinate search over the split point of the multi-size launch (64, 80,
- * 96 or none), the lane count (1..4) and the threads per workgroup of each
- * per-size bucket (default or twice), timing whole plans; for the buckets
+ * coordinate search over (LSVD_PREC_F64_ONESIDED) the largest padded size
+ * that takes QR + one-sided Jacobi, the split point of the multi-size launch
+ * (64, 80, 96 or none), the lane count (1..4) and the threads per workgroup
+ * of each per-size bucket (default or twice), timing whole plans; for the buckets
  * above 128 the kernel variant of each bucket, timing that bucket. */
 int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, int precision);
 /* The plan-level part of lsvd_autotune_batch timed on the caller's own

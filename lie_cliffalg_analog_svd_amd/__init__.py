@@ -7,7 +7,8 @@ batch plans, ``LieSvdJoint`` and the rotor-stream export
 (``HardwareSchedule``), tensor HO-SVD (``LieSvdTensor``), Phase-BSS (``LieSvdBss``).  ``profiles`` and ``metrics`` restate the reference's input
 generators and residual metrics (src/lib.rs:133-388).
 """
-from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_SCALED, LSVD_PTR_DEVICE,
+from ._lib import (LSVD_PREC_BF16, LSVD_PREC_F32, LSVD_PREC_F64, LSVD_PREC_F64_ONESIDED, LSVD_PREC_F64_SCALED,
+                   LSVD_PTR_DEVICE,
                    LSVD_PTR_HOST, LsvdError, ShapeError)
 from .handle import BatchPlan, Handle, default_handle, pack_offsets
 from .solvers import (MICRO_MAX_N, SMALL_MAX_N, LieSvd, LieSvdMicro, LieSvdSmall, digital_polish,

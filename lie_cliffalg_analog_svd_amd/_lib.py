@@ -28,6 +28,7 @@ LSVD_PREC_F64 = 0
 LSVD_PREC_F32 = 1
 LSVD_PREC_BF16 = 2
 LSVD_PREC_F64_SCALED = 3
+LSVD_PREC_F64_ONESIDED = 4
 
 LSVD_TARGET_MZI_MESH = 0
 LSVD_TARGET_FPGA_ROTOR_MESH = 1
@@ -67,6 +68,8 @@ class LsvdTuneReport(C.Structure):
         ("evaluations", C.c_int32),
         ("default_ms", C.c_double),
         ("tuned_ms", C.c_double),
+        ("os_max_np", C.c_int32),
+        ("reserved", C.c_int32),
     ]
 
     def as_dict(self):

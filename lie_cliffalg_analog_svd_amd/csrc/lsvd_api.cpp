@@ -92,6 +92,16 @@ static double item_cost(int M, int N) {
 // Sets the launch shape of a bucket for `variant`; returns the global arena
 // doubles it needs (0 for LDS variants).
 static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
+    if (variant >= kOsBase) {  // QR + one-sided Jacobi: LDS and reflector slot sized by the largest M
+        const int np = variant - kOsBase;
+        bk.variant = variant;
+        const size_t lds = os_lds_bytes(np, bk.maxM);
+        const int per_cu = onesided_blocks_per_cu(np, lds);
+        bk.grid = std::max(1, std::min(bk.count, per_cu * h->num_cus));
+        bk.lds_bytes = lds;
+        bk.gws_stride = os_slot_doubles(np, bk.maxM);
+        return (size_t)bk.gws_stride * bk.grid;
+    }
     if (variant >= kV2Multi && variant < kV2Wide) {  // one launch over sizes 64..128: LDS and slots for the largest item
         const int np = variant - kV2Multi;
         bk.variant = variant;
@@ -133,9 +143,10 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
                       int precision, lsvd_plan_t* out) {
     if (!h || !out) return LSVD_E_ARG;
     if (count < 0) return set_error(h, LSVD_E_ARG, "negative count");
-    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F64_SCALED)
+    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F64_SCALED && precision != LSVD_PREC_F64_ONESIDED)
         return set_error(h, LSVD_E_ARG,
-                         "batch plans compute in FP64: precision must be LSVD_PREC_F64 or LSVD_PREC_F64_SCALED");
+                         "batch plans compute in FP64: precision must be LSVD_PREC_F64, LSVD_PREC_F64_SCALED or "
+                         "LSVD_PREC_F64_ONESIDED");
     hipError_t e = hipSetDevice(h->device);
     if (e != hipSuccess) return hip_fail(h, e, "hipSetDevice");
 
@@ -190,7 +201,10 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         }
         const int M = std::max(mb, nb), N = std::min(mb, nb);
         int key;
-        if (N <= 128 && M <= 128) {
+        if (N <= 128 && M <= 128 && precision == LSVD_PREC_F64_ONESIDED && mode != kModeEigh &&
+            (N + 15) / 16 * 16 <= h->os_max_np) {
+            key = 3000 + (N + 15) / 16 * 16;  // QR + one-sided Jacobi, one launch per padded size
+        } else if (N <= 128 && M <= 128) {
             const int np = (N + 15) / 16 * 16;
             // on-chip kernels: one launch per padded size below 64, one
             // launch over all the larger sizes (a queue sorted longest first)
@@ -249,7 +263,9 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
             }
         }
         int vid;
-        if (kv.first == 2000) {
+        if (kv.first >= 3000) {
+            vid = kOsBase + (kv.first - 3000);
+        } else if (kv.first == 2000) {
             vid = kV2Multi + (maxN + 15) / 16 * 16;
         } else if (kv.first >= 1000) {
             const int np = kv.first - 1000;
@@ -408,7 +424,7 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const
         L.pools = pools;
         L.gws = p->d_gws ? p->d_gws + bk.gws_offset : nullptr;
         L.gws_stride = bk.gws_stride;
-        L.ns_scaled = p->precision == LSVD_PREC_F64_SCALED;
+        L.ns_scaled = p->precision != LSVD_PREC_F64;
         if (tev && (e = hipEventRecord(tev[2 * (i + 1)], p->side[l])) != hipSuccess) return e;
         e = launch_small_variant(bk.variant, L, p->side[l]);
         if (e != hipSuccess) return e;
@@ -637,6 +653,11 @@ int lsvd_create(int device, lsvd_handle_t* out) {
         const long v = std::strtol(env, &end, 10);
         if (end != env && v >= 2) h->large_min_n = (int)std::min<long>(v, kLargeMaxN + 1);
     }
+    if (const char* env = std::getenv("LSVD_OS_MAX_NP")) {
+        char* end = nullptr;
+        const long v = std::strtol(env, &end, 10);
+        if (end != env && v >= 0) h->os_max_np = (int)std::min<long>(v, 128);
+    }
     if (const char* env = std::getenv("LSVD_BATCH_MULTI")) h->multi_launch = std::strtol(env, nullptr, 10) != 0;
     if (const char* env = std::getenv("LSVD_JADE_CHUNK_MB"))
         h->jade_chunk_bytes = (size_t)std::max(0L, std::strtol(env, nullptr, 10)) << 20;
@@ -758,7 +779,7 @@ static int tune_plan_knobs(lsvd_handle_t h, int count, const int32_t* m, const i
     const int saved_ptr_mode = h->ptr_mode;
     std::vector<int32_t> pmodes(count);
     for (int b = 0; b < count; ++b) pmodes[b] = (m[b] == n[b] && n[b] <= 4) ? kModeMicro : kModeSvd;
-    const int saved_mm = h->multi_min_np, saved_lanes = h->max_lanes;
+    const int saved_mm = h->multi_min_np, saved_lanes = h->max_lanes, saved_os = h->os_max_np;
     const uint32_t saved_wide = h->wide_mask;
     int evals = 0;
     float t_ms = 0.f;
@@ -811,13 +832,14 @@ static int tune_plan_knobs(lsvd_handle_t h, int count, const int32_t* m, const i
     float t_best = t_ms;
     // coordinate search: a candidate replaces the best so far when it
     // saves more than 1.5 %
-    int best_mm = saved_mm, best_lanes = saved_lanes;
+    int best_mm = saved_mm, best_lanes = saved_lanes, best_os = saved_os;
     uint32_t best_wide = saved_wide;
-    auto try_knobs = [&](int mm, int lanes, uint32_t wide) -> int {
-        if (mm == best_mm && lanes == best_lanes && wide == best_wide) return LSVD_OK;
+    auto try_knobs = [&](int mm, int lanes, uint32_t wide, int osn) -> int {
+        if (mm == best_mm && lanes == best_lanes && wide == best_wide && osn == best_os) return LSVD_OK;
         h->multi_min_np = mm;
         h->max_lanes = lanes;
         h->wide_mask = wide;
+        h->os_max_np = osn;
         float t = 0.f;
         const int r = time_plan(&t);
         if (r == LSVD_OK && t < t_best * 0.985f) {
@@ -825,28 +847,36 @@ static int tune_plan_knobs(lsvd_handle_t h, int count, const int32_t* m, const i
             best_mm = mm;
             best_lanes = lanes;
             best_wide = wide;
+            best_os = osn;
         }
         h->multi_min_np = best_mm;
         h->max_lanes = best_lanes;
         h->wide_mask = best_wide;
+        h->os_max_np = best_os;
         return r;
     };
+    if (precision == LSVD_PREC_F64_ONESIDED) {  // where QR + one-sided Jacobi hands over to Newton-Schulz
+        for (int osn = 16; osn <= 128; osn += 16) {
+            const int r = try_knobs(best_mm, best_lanes, best_wide, osn);
+            if (r != LSVD_OK) return r;
+        }
+    }
     const int mms[] = {64, 80, 96, 129};
     for (int mm : mms) {
-        const int r = try_knobs(mm, best_lanes, best_wide);
+        const int r = try_knobs(mm, best_lanes, best_wide, best_os);
         if (r != LSVD_OK) return r;
     }
     for (int lanes = 1; lanes <= kPlanLanes; ++lanes) {
-        const int r = try_knobs(best_mm, lanes, best_wide);
+        const int r = try_knobs(best_mm, lanes, best_wide, best_os);
         if (r != LSVD_OK) return r;
     }
     for (int np = 16; np <= 64; np += 16) {
         if (np >= best_mm && h->multi_launch) continue;  // inside the multi-size launch
-        const int r = try_knobs(best_mm, best_lanes, best_wide ^ (1u << (np / 16 - 1)));
+        const int r = try_knobs(best_mm, best_lanes, best_wide ^ (1u << (np / 16 - 1)), best_os);
         if (r != LSVD_OK) return r;
     }
     h->tune_report = lsvd_tune_report_t{best_mm, best_lanes, (int32_t)best_wide, evals, (double)t_default,
-                                        (double)t_best};
+                                        (double)t_best, best_os, 0};
         return LSVD_OK;
 }
 

@@ -68,6 +68,10 @@ struct lsvd_handle_s {
     // max_lanes concurrent lanes, and bit (np / 16 - 1) of wide_mask runs the
     // size-np bucket with twice the default threads per workgroup
     int multi_min_np = 64;
+    // LSVD_PREC_F64_ONESIDED plans: SVD items of padded size <= os_max_np
+    // take QR + one-sided Jacobi, the larger ones the scaled Newton-Schulz
+    // kernels (autotuned)
+    int os_max_np = 64;
     int max_lanes = 4;
     uint32_t wide_mask = 0;
     lsvd_tune_report_t tune_report{};

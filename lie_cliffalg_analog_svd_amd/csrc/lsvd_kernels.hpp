@@ -58,6 +58,15 @@ constexpr int kV2MultiMinNP = 64;
 hipError_t launch_small2_multi(const SmallLaunch& L, hipStream_t st);
 int small2_multi_blocks_per_cu(size_t lds_bytes);
 
+// ---- QR + one-sided Jacobi for N, M <= 128 (onesided.hip) --------------
+// np: padded size 16, 32, ..., 128.  Variant id = kOsBase + np.
+constexpr int kOsBase = 700;
+int os_threads(int np);
+size_t os_lds_bytes(int np, int max_m);
+int64_t os_slot_doubles(int np, int max_m);
+hipError_t launch_onesided(int np, const SmallLaunch& L, hipStream_t st);
+int onesided_blocks_per_cu(int np, size_t lds_bytes);
+
 // ---- LieSvdMicro (micro_svd.hip) ----------------------------------------
 
 struct MicroLaunch {

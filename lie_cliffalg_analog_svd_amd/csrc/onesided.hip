@@ -1,0 +1,825 @@
+// onesided.hip -- batched thin SVD of M x N matrices (N <= 128, M <= 128
+// after transposing wide inputs) by Householder QR and one-sided (Hestenes)
+// Jacobi on R^T, with the whole Jacobi state in registers.
+//
+// The same factorisation as LieSvdSmall::solve_rectangular
+// (src/lie_svd_small.rs:340-360): A = Q R, then the SVD of the small factor,
+// U = Q U_r.  The small factor's SVD is computed by one-sided Jacobi on
+// X = R^T instead of Newton-Schulz + eigh_jacobi_full (:296-323): with
+// X W = B (columns pairwise orthogonal to working precision), sigma_j =
+// ||b_j||, V_A = B diag(sigma)^-1 and U_A = Q W.  R^T is the classical
+// preconditioned form of one-sided Jacobi (Drmac and Veselic, "New fast and
+// accurate Jacobi SVD algorithm", SIAM J. Matrix Anal. Appl. 29, 2008): its
+// columns are graded, so the sweeps converge in about as many sweeps as the
+// reference's eigh_jacobi_full while no polar iteration runs at all.
+//
+// Rotation rule: the pair (p, q) with Gram entries a = ||x_p||^2, b =
+// ||x_q||^2, g = x_p . x_q is rotated when g^2 > tol^2 a b, tol = sqrt(N)
+// 2^-52 (the relative test of LAPACK's dgesvj), by the Jacobi rotation of
+// the 2 x 2 Gram matrix (the reference's Rutishauser form, shared with the
+// two-sided kernels).  The iteration ends when a whole sweep's worth of
+// consecutive layers applies no rotation, or at the reference's sweep cap
+// 60 + 20 N.
+//
+// Parallel ordering: odd-even ("caterpillar") layers over NP positions --
+// even layers pair (2k, 2k+1), odd layers (2k+1, 2k+2) -- and the two
+// columns of every pair always trade positions (rotated or not), so the
+// movement of the columns is independent of the data: any NP consecutive
+// layers meet every pair exactly once, and the index held by a position
+// after G layers has a closed form (index_at).  In registers a trade is
+// free (the rotation writes its two outputs to swapped registers); only the
+// pair that straddles two lanes in an odd layer needs one neighbour-lane
+// move of each column.
+//
+// Register layout (per workgroup of NW waves, one matrix): positions are
+// split over TPR consecutive lanes (CH = NP / TPR per lane), rows over the
+// lane groups ("rowgroups"), XR rows of X and the same XR rows of W per
+// lane.  Per layer a lane forms the partial Gram entries of its P = CH / 2
+// pairs over its rows; the rowgroups of a wave are reduced with the gfx950
+// permlane swaps (reduce-scatter over lane bits 5 and 4, so each lane ends
+// up owning the sums of PP / 4 pairs, PP = P rounded up to a power of two)
+// and DPP (bit 3), the waves through LDS; each lane computes the angles of
+// the pairs it owns, and the swaps gather the (c, s) back.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include <type_traits>
+
+#include "lsvd_device.hpp"
+#include "lsvd_kernels.hpp"
+#include "lsvd_small_common.hpp"
+
+namespace lsvd {
+
+// ---- configuration -------------------------------------------------------
+
+template <int NP>
+struct OsDefault {
+    // positions per lane CH = NP / TPR even, X rows per lane XR = 2
+    static constexpr int TPR = NP == 128 ? 16 : 8;
+    static constexpr int NW = NP == 128 ? 8 : (NP / 16 > 1 ? NP / 16 : 1);
+};
+
+constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
+constexpr int pow2up(int x) { return x <= 1 ? 1 : 2 * pow2up((x + 1) / 2); }
+
+template <int NP, int TPR, int NW>
+struct OsCfg {
+    static constexpr int NT = 64 * NW;
+    static constexpr int CH = NP / TPR;        // positions per lane
+    static constexpr int P = CH / 2;           // pairs per lane and layer
+    static constexpr int PP = pow2up(P);       // padded to a power of two for the reduction
+    static constexpr int RGW = 64 / TPR;       // rowgroups per wave
+    static constexpr int LB = ilog2c(RGW);     // lane bits of the rowgroup (the top LB of 6)
+    static constexpr int RG = NW * RGW;
+    static constexpr int XR = NP / RG;         // X rows (and W rows) per lane
+    static constexpr int SC = ilog2c(PP) < 2 ? ilog2c(PP) : 2;  // scatter steps (bits 5, 4)
+    static constexpr int NA = PP >> SC;        // pairs a lane owns after the scatter
+    static constexpr int S = NP / 2;
+    static constexpr int LDA = NP + 1;
+    static_assert(NP % TPR == 0 && CH % 2 == 0 && CH >= 2, "positions per lane");
+    static_assert(TPR == 8 || TPR == 16, "rowgroup bits 5 and 4 must be rowgroup bits");
+    static_assert(RG * XR == NP && XR >= 1, "rows per lane");
+};
+
+// Column norms (of the input scaled to ||A||_F = 1) at or below this are
+// treated as zero: the Gram products of two such columns underflow, so the
+// rotation test cannot orthogonalise them.
+constexpr double kOsTiny = 1e-140;
+
+struct OsShared {
+    double red[2 * 16];
+    int rank[128];
+    double sig[128];     // normalised singular values by index
+    double tw[128];      // tau * w per column (Householder steps)
+    double vbuf[128];    // a reflector staged for the application of Q
+    double tau[128];
+    int item;
+    int flag;
+};
+
+// ---- lane exchanges ------------------------------------------------------
+
+// a' = [a lanes 0-31 | b lanes 0-31], b' = [a lanes 32-63 | b lanes 32-63]
+__device__ __forceinline__ void xch32(double& a, double& b) {
+    const int al = __double2loint(a), ah = __double2hiint(a);
+    const int bl = __double2loint(b), bh = __double2hiint(b);
+    const auto l = __builtin_amdgcn_permlane32_swap(al, bl, false, false);
+    const auto h = __builtin_amdgcn_permlane32_swap(ah, bh, false, false);
+    a = __hiloint2double(h[0], l[0]);
+    b = __hiloint2double(h[1], l[1]);
+}
+
+// rows of 16 lanes: a' = [a r0, b r0, a r2, b r2], b' = [a r1, b r1, a r3, b r3]
+__device__ __forceinline__ void xch16(double& a, double& b) {
+    const int al = __double2loint(a), ah = __double2hiint(a);
+    const int bl = __double2loint(b), bh = __double2hiint(b);
+    const auto l = __builtin_amdgcn_permlane16_swap(al, bl, false, false);
+    const auto h = __builtin_amdgcn_permlane16_swap(ah, bh, false, false);
+    a = __hiloint2double(h[0], l[0]);
+    b = __hiloint2double(h[1], l[1]);
+}
+
+// lane l ^ 8's value (DPP row_ror:8 within rows of 16)
+__device__ __forceinline__ double from_xor8(double x) {
+    const int lo = __double2loint(x), hi = __double2hiint(x);
+    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x128, 0xf, 0xf, false),
+                            __builtin_amdgcn_update_dpp(lo, lo, 0x128, 0xf, 0xf, false));
+}
+
+// lane l + 1's value (wave_shl:1); lane 63 gets `old`
+__device__ __forceinline__ double os_from_next(double x, double old) {
+    return __hiloint2double(
+        __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(x), 0x130, 0xf, 0xf, false),
+        __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(x), 0x130, 0xf, 0xf, false));
+}
+
+// lane l - 1's value (wave_shr:1); lane 0 gets `old`
+__device__ __forceinline__ double os_from_prev(double x, double old) {
+    return __hiloint2double(
+        __builtin_amdgcn_update_dpp(__double2hiint(old), __double2hiint(x), 0x138, 0xf, 0xf, false),
+        __builtin_amdgcn_update_dpp(__double2loint(old), __double2loint(x), 0x138, 0xf, 0xf, false));
+}
+
+// Sum over the lanes that differ in bit 5 / bit 4 / bit 3 (every lane gets
+// the same bits: a + b is evaluated as lower-lane value + upper-lane value,
+// or commutatively).
+__device__ __forceinline__ double bfly32(double v) {
+    double a = v, b = v;
+    xch32(a, b);
+    return a + b;
+}
+__device__ __forceinline__ double bfly16(double v) {
+    double a = v, b = v;
+    xch16(a, b);
+    return a + b;
+}
+__device__ __forceinline__ double bfly8(double v) { return v + from_xor8(v); }
+
+// Index held by position g after G layers of the odd-even ordering over NP
+// positions (position p holds index p before the first layer).  Unfolded,
+// every column walks a cycle of 2 NP virtual positions one step per layer
+// (virtual v is position v going up, position 2 NP - 1 - v going down;
+// even positions start going up); the column at g is the one whose virtual
+// position has the parity of G.
+__device__ __forceinline__ int os_index_at(int NP, int G, int g) {
+    const int two = 2 * NP;
+    const int v = ((g & 1) == (G & 1)) ? g : two - 1 - g;
+    const int v0 = ((v - G) % two + two) % two;
+    return v0 < NP ? v0 : two - 1 - v0;
+}
+
+// 1/sqrt(x) for x within a few binades of 1: the hardware estimate
+// (v_rsq_f64, relative error ~2^-26) refined by two Newton steps.
+__device__ __forceinline__ double os_rsq(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x;
+    y = y * fma(-h * y, y, 1.5);
+    y = y * fma(-h * y, y, 1.5);
+    return y;
+}
+
+// The Jacobi rotation of the 2 x 2 Gram matrix [[a, g], [g, b]]
+// (src/lie_svd_small.rs:226-243 in the form of jacobi_angle): columns
+// (x_p, x_q) -> (c x_p - s x_q, s x_p + c x_q) makes them orthogonal.  Its
+// inputs are scaled by a power of two so that max(|b - a|, |2 g|) is in
+// [0.5, 1); r and 1 / h from reciprocal square roots.
+__device__ __forceinline__ void os_angle(double a, double b, double g, double& c, double& s) {
+    const double d = b - a, a2 = 2.0 * g;
+    const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(d), fabs(a2)));
+    const double ds = ldexp(d, -ex), as = fabs(ldexp(a2, -ex));
+    const double x = fma(ds, ds, as * as);  // in [0.25, 2)
+    const double r = x * os_rsq(x);
+    const double den = fabs(ds) + r;
+    const double sgn = (d == 0.0 || ((d > 0.0) == (a2 > 0.0))) ? 1.0 : -1.0;
+    const double ih = os_rsq(fma(den, den, as * as));  // in [0.25, 6)
+    c = den * ih;
+    s = sgn * as * ih;
+}
+
+// ---- Householder QR and the application of Q (in LDS) --------------------
+
+// QR of the M x N matrix in `a` (leading dimension lda), in place: R on and
+// above the diagonal, reflector tails (LAPACK dlarfg form, v_j = 1 implied)
+// below, tau in sh.tau.  Thread t works on column t % NPC, rows
+// t / NPC + K i (K = NT / NPC row chunks).  Two barriers per column: the
+// column sums of the reflector with the trailing columns, then the update,
+// which also forms the next column's squared norm (per chunk) and scales
+// the previous reflector's tail.  `part` holds 2 NT doubles.
+template <int NT, int NPC>
+__device__ void os_householder(int M, int N, double* a, int lda, OsShared& sh, double* part) {
+    const int tid = tix();
+    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;
+    static_assert(NT % NPC == 0, "whole row chunks");
+    const int col = tid % NPC, chunk = tid / NPC;
+    double* npart = part + (K + 1) * NPC;  // [2][K]: squared norms of the next column, per chunk
+    if (col == 0) {
+        double s = 0.0;
+        for (int i = 1 + chunk; i < M; i += K) s = fma(a[(size_t)i * lda], a[(size_t)i * lda], s);
+        npart[chunk] = s;
+    }
+    __syncthreads();
+    double prev_scal = 1.0;
+    for (int j = 0; j < N; ++j) {
+        double xnorm2 = 0.0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) xnorm2 += npart[(j & 1) * K + k];
+        const double alpha = a[(size_t)j * lda + j];
+        double tau = 0.0, beta = alpha, scal = 0.0;
+        if (xnorm2 > 0.0) {
+            beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
+            tau = (beta - alpha) / beta;
+            scal = 1.0 / (alpha - beta);
+        }
+        // column sums sum_{i>j} a[i][j] a[i][col] (and row j of col, kept
+        // unmodified for the update in part[K][col])
+        if (col > j && col < N) {
+            // four rows per round, loads first (the round trips overlap)
+            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int i = j + 1 + chunk;
+            for (; i + 3 * K < M; i += 4 * K) {
+                const double* r0 = a + (size_t)i * lda;
+                const double p0 = r0[j], q0 = r0[col], p1 = r0[K * lda + j], q1 = r0[K * lda + col];
+                const double p2 = r0[2 * K * lda + j], q2 = r0[2 * K * lda + col];
+                const double p3 = r0[3 * K * lda + j], q3 = r0[3 * K * lda + col];
+                s0 = fma(p0, q0, s0);
+                s1 = fma(p1, q1, s1);
+                s2 = fma(p2, q2, s2);
+                s3 = fma(p3, q3, s3);
+            }
+            for (; i < M; i += K) s0 = fma(a[(size_t)i * lda + j], a[(size_t)i * lda + col], s0);
+            part[chunk * NPC + col] = (s0 + s1) + (s2 + s3);
+            if (chunk == 0) part[K * NPC + col] = a[(size_t)j * lda + col];
+        }
+        if (tid == 0) sh.tau[j] = tau;
+        __syncthreads();
+        if (col > j && col < N) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) sum += part[k * NPC + col];
+            const double twc = tau * (part[K * NPC + col] + scal * sum);
+            double nn = 0.0;
+            int i = j + chunk;
+            if (chunk == 0) {  // row j: v_j = 1
+                a[(size_t)j * lda + col] -= twc;
+                i += K;
+            }
+            for (; i + 3 * K < M; i += 4 * K) {
+                double* r0 = a + (size_t)i * lda;
+                const double v0 = r0[j], v1 = r0[K * lda + j], v2 = r0[2 * K * lda + j], v3 = r0[3 * K * lda + j];
+                const double x0 = r0[col], x1 = r0[K * lda + col], x2 = r0[2 * K * lda + col], x3 = r0[3 * K * lda + col];
+                const double n0 = fma(-scal * v0, twc, x0), n1 = fma(-scal * v1, twc, x1);
+                const double n2 = fma(-scal * v2, twc, x2), n3 = fma(-scal * v3, twc, x3);
+                r0[col] = n0;
+                r0[K * lda + col] = n1;
+                r0[2 * K * lda + col] = n2;
+                r0[3 * K * lda + col] = n3;
+                nn = fma(i > j + 1 ? n0 : 0.0, n0, nn);
+                nn = fma(i + K > j + 1 ? n1 : 0.0, n1, nn);
+                nn = fma(n2, n2, nn);
+                nn = fma(n3, n3, nn);
+            }
+            for (; i < M; i += K) {
+                const double nv = fma(-scal * a[(size_t)i * lda + j], twc, a[(size_t)i * lda + col]);
+                a[(size_t)i * lda + col] = nv;
+                if (i > j + 1) nn = fma(nv, nv, nn);
+            }
+            if (col == j + 1) npart[((j + 1) & 1) * K + chunk] = nn;
+        } else if (col == j) {
+            if (chunk == 0) a[(size_t)j * lda + j] = beta;
+            if (j > 0)  // the previous reflector's tail (no longer read)
+                for (int i = j + chunk; i < M; i += K) a[(size_t)i * lda + j - 1] *= prev_scal;
+        }
+        __syncthreads();
+        prev_scal = scal;
+    }
+    if (N > 0 && col == 0)  // the last reflector's tail
+        for (int i = N + chunk; i < M; i += K) a[(size_t)i * lda + N - 1] *= prev_scal;
+    __syncthreads();
+}
+
+// Y (M x ncol in `y`, leading dimension ldy) <- H_0 H_1 ... H_{N-1} Y, the
+// reflectors' tails in global memory (refl: M x ldr, row-major) and tau in
+// sh.tau; three barriers per reflector.
+template <int NT, int NPC>
+__device__ void os_apply_q(int M, int N, int ncol, const double* refl, int ldr, double* y, int ldy, OsShared& sh,
+                           double* part) {
+    const int tid = tix();
+    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;
+    const int col = tid % NPC, chunk = tid / NPC;
+    for (int j = N - 1; j >= 0; --j) {
+        const double tau = sh.tau[j];
+        if (tau == 0.0) continue;  // uniform
+        // the reflector, staged once per column (rows > j)
+        for (int i = j + 1 + tid; i < M; i += NT) sh.vbuf[i] = refl[(size_t)i * ldr + j];
+        __syncthreads();
+        if (col < ncol) {
+            double s0 = chunk == 0 ? y[(size_t)j * ldy + col] : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+            int i = j + (chunk == 0 ? K : chunk);
+            for (; i + 3 * K < M; i += 4 * K) {
+                const double* r0 = y + (size_t)i * ldy;
+                const double x0 = r0[col], x1 = r0[K * ldy + col], x2 = r0[2 * K * ldy + col], x3 = r0[3 * K * ldy + col];
+                s0 = fma(sh.vbuf[i], x0, s0);
+                s1 = fma(sh.vbuf[i + K], x1, s1);
+                s2 = fma(sh.vbuf[i + 2 * K], x2, s2);
+                s3 = fma(sh.vbuf[i + 3 * K], x3, s3);
+            }
+            for (; i < M; i += K) s0 = fma(sh.vbuf[i], y[(size_t)i * ldy + col], s0);
+            part[chunk * NPC + col] = (s0 + s1) + (s2 + s3);
+        }
+        __syncthreads();
+        if (col < ncol) {
+            double sum = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) sum += part[k * NPC + col];
+            const double twc = tau * sum;
+            int i = j + chunk;
+            if (chunk == 0) {
+                y[(size_t)j * ldy + col] -= twc;
+                i += K;
+            }
+            for (; i + 3 * K < M; i += 4 * K) {
+                double* r0 = y + (size_t)i * ldy;
+                const double x0 = r0[col], x1 = r0[K * ldy + col], x2 = r0[2 * K * ldy + col], x3 = r0[3 * K * ldy + col];
+                r0[col] = fma(-sh.vbuf[i], twc, x0);
+                r0[K * ldy + col] = fma(-sh.vbuf[i + K], twc, x1);
+                r0[2 * K * ldy + col] = fma(-sh.vbuf[i + 2 * K], twc, x2);
+                r0[3 * K * ldy + col] = fma(-sh.vbuf[i + 3 * K], twc, x3);
+            }
+            for (; i < M; i += K) y[(size_t)i * ldy + col] = fma(-sh.vbuf[i], twc, y[(size_t)i * ldy + col]);
+        }
+        __syncthreads();
+    }
+}
+
+// ---- the per-matrix solver -----------------------------------------------
+
+template <int NP, int TPR, int NW>
+__device__ void os_solve(const Item& itm, const Pools& P, double* am, double* part, double* refl, OsShared& sh) {
+    using C = OsCfg<NP, TPR, NW>;
+    constexpr int NT = C::NT, CH = C::CH, PC = C::P, PP = C::PP, XR = C::XR, LDA = C::LDA;
+    constexpr int SC = C::SC, NA = C::NA, LB = C::LB, RGW = C::RGW;
+    const int tid = tix(), w = tid >> 6, lane = tid & 63;
+    const int rgw = lane / TPR, vch = lane % TPR, rg = w * RGW + rgw;
+    const int m0 = itm.m, n0 = itm.n;
+    const bool wide = m0 < n0;
+    const int M = wide ? n0 : m0, N = wide ? m0 : n0;
+    const int MP = (M + 15) & ~15;
+    const double* A = P.a + itm.a_off;
+    const View ain{A, n0, wide ? 1 : 0};
+    lsvd_info_t info = {};
+    struct Clk {
+        long long* out;
+        long long t;
+        __device__ void mark(int ph) {
+            if (out && threadIdx.x == 0) {
+                const long long now = clock64();
+                out[ph] += now - t;
+                t = now;
+            }
+        }
+    } clk{P.timing ? P.timing + (size_t)itm.idx * kPhases : nullptr, P.timing && threadIdx.x == 0 ? clock64() : 0};
+
+    // ---- load A / ||A||_F (zero padded to MP x NP) and QR
+    for (int e = tid; e < MP * NP; e += NT) {
+        const int i = e / NP, j = e % NP;
+        am[i * LDA + j] = (i < M && j < N) ? ain(i, j) : 0.0;
+    }
+    double sc = 0.0;
+    __syncthreads();
+    for (int e = tid; e < MP * NP; e += NT) {
+        const double x = am[(e / NP) * LDA + e % NP];
+        sc = fma(x, x, sc);
+    }
+    sc = fmax(sqrt(block_sum<NT>(sc, sh.red)), 1e-300);
+    const double isc = 1.0 / sc;
+    for (int e = tid; e < MP * NP; e += NT) am[(e / NP) * LDA + e % NP] *= isc;
+    __syncthreads();
+    os_householder<NT, NP>(M, N, am, LDA, sh, part);
+    clk.mark(kPhQR);
+    // reflector tails to the global slot (read back when U = Q W is formed)
+    for (int e = tid; e < MP * NP; e += NT) {
+        const int i = e / NP, j = e % NP;
+        if (i > j && i < M && j < N) refl[(size_t)i * NP + j] = am[i * LDA + j];
+    }
+
+    // ---- registers: X = R^T (row r, position g holds R[g][r]), W = I
+    double x[XR][CH], wv[XR][CH];
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int r = rg * XR + k;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int g = vch * CH + c;
+            x[k][c] = (g <= r && r < N) ? am[g * LDA + r] : 0.0;
+            wv[k][c] = g == r ? 1.0 : 0.0;
+        }
+    }
+    __syncthreads();  // am is reused below
+
+    const double tol = sqrt((double)N) * 0x1p-52;
+    const double tol2 = tol * tol;
+    const int maxG = (60 + 20 * N) * NP;
+    int G = 0, quiet = 0, rots = 0;
+    const bool lastv = vch == TPR - 1, firstv = vch == 0;
+    // lane bits that the reduce-scatter did not split (replicated sums)
+    const int rep_mask = ((1 << LB) - 1) & ~((SC >= 1 ? (RGW >> 1) : 0) | (SC >= 2 ? (RGW >> 2) : 0));
+    const bool rep_lane = (rgw & rep_mask) == 0 && w == 0;
+
+    // one layer; ODD: pairs (2j+1, 2j+2) in the lane and (CH-1, next lane's 0)
+    auto layer = [&](auto odd_tag) {
+        constexpr bool ODD = decltype(odd_tag)::value;
+        double xn[XR], xp[XR], wn[XR], wp[XR];
+        if constexpr (ODD) {
+#pragma unroll
+            for (int k = 0; k < XR; ++k) {
+                xn[k] = os_from_next(x[k][0], x[k][0]);
+                xp[k] = os_from_prev(x[k][CH - 1], x[k][CH - 1]);
+                wn[k] = os_from_next(wv[k][0], wv[k][0]);
+                wp[k] = os_from_prev(wv[k][CH - 1], wv[k][CH - 1]);
+            }
+        }
+        auto lo = [](int j) { return ODD ? 2 * j + 1 : 2 * j; };
+        // partial Gram entries of the lane's pairs over its rows
+        double v[PP][3];
+#pragma unroll
+        for (int j = 0; j < PP; ++j) v[j][0] = v[j][1] = v[j][2] = 0.0;
+#pragma unroll
+        for (int j = 0; j < PC; ++j)
+#pragma unroll
+            for (int k = 0; k < XR; ++k) {
+                const double p = x[k][lo(j)];
+                const double q = (ODD && j == PC - 1) ? xn[k] : x[k][lo(j) + 1];
+                v[j][0] = fma(p, p, v[j][0]);
+                v[j][1] = fma(q, q, v[j][1]);
+                v[j][2] = fma(p, q, v[j][2]);
+            }
+        // reduce-scatter over lane bit 5, then bit 4: pair index owned
+        int kid = 0;
+        if constexpr (SC >= 1) {
+#pragma unroll
+            for (int j = 0; j < PP / 2; ++j)
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    double a = v[j][t], b = v[j + PP / 2][t];
+                    xch32(a, b);
+                    v[j][t] = a + b;
+                }
+            kid += (lane >> 5) ? PP / 2 : 0;
+        }
+        if constexpr (SC >= 2) {
+#pragma unroll
+            for (int j = 0; j < PP / 4; ++j)
+#pragma unroll
+                for (int t = 0; t < 3; ++t) {
+                    double a = v[j][t], b = v[j + PP / 4][t];
+                    xch16(a, b);
+                    v[j][t] = a + b;
+                }
+            kid += ((lane >> 4) & 1) ? PP / 4 : 0;
+        }
+        // butterflies over the remaining rowgroup bits
+#pragma unroll
+        for (int j = 0; j < NA; ++j)
+#pragma unroll
+            for (int t = 0; t < 3; ++t) {
+                double s = v[j][t];
+                if constexpr (SC < 1) s = bfly32(s);
+                if constexpr (SC < 2 && LB >= 2) s = bfly16(s);
+                if constexpr (LB >= 3) s = bfly8(s);
+                v[j][t] = s;
+            }
+        // across the waves (LDS, one barrier; the buffer alternates by layer)
+        if constexpr (NW > 1) {
+            // planes [wave][entry][pair]: a wave's lanes touch consecutive pairs
+            double* pb = am + (size_t)(ODD ? 1 : 0) * NW * 3 * C::S;
+#pragma unroll
+            for (int j = 0; j < NA; ++j) {
+                const int pr = vch * PC + kid + j;
+                if (kid + j < PC) {
+                    double* d = pb + (size_t)w * 3 * C::S + pr;
+                    d[0] = v[j][0];
+                    d[C::S] = v[j][1];
+                    d[2 * C::S] = v[j][2];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < NA; ++j) {
+                const int pr = vch * PC + (kid + j < PC ? kid + j : 0);
+                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int ww = 0; ww < NW; ++ww) {
+                    const double* d = pb + (size_t)ww * 3 * C::S + pr;
+                    s0 += d[0];
+                    s1 += d[C::S];
+                    s2 += d[2 * C::S];
+                }
+                v[j][0] = s0;
+                v[j][1] = s1;
+                v[j][2] = s2;
+            }
+        }
+        // angles of the owned pairs
+        double cs[PP][2];
+        int nrot = 0;
+        bool any = false;
+#pragma unroll
+        for (int j = 0; j < NA; ++j) {
+            const int jk = kid + j;
+            const bool boundary_dead = ODD && lastv && jk == PC - 1;  // position NP - 1 has no partner
+            const bool valid = jk < PC && !boundary_dead;
+            const double a = v[j][0], b = v[j][1], g = v[j][2];
+            const bool rot = valid && g * g > tol2 * (a * b);
+            double c = 1.0, s = 0.0;
+            if (rot) os_angle(a, b, g, c, s);
+            if (boundary_dead) {
+                c = 0.0;
+                s = 1.0;
+            }
+            cs[j][0] = c;
+            cs[j][1] = s;
+            nrot += (rot && rep_lane) ? 1 : 0;
+            any = any || rot;
+        }
+        // gather the (c, s) of all the lane's pairs (reverse of the scatter)
+        if constexpr (SC >= 2) {
+#pragma unroll
+            for (int j = PP / 4 - 1; j >= 0; --j)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double a = cs[j][t], b = cs[j][t];
+                    xch16(a, b);
+                    cs[j][t] = a;
+                    cs[j + PP / 4][t] = b;
+                }
+        }
+        if constexpr (SC >= 1) {
+#pragma unroll
+            for (int j = PP / 2 - 1; j >= 0; --j)
+#pragma unroll
+                for (int t = 0; t < 2; ++t) {
+                    double a = cs[j][t], b = cs[j][t];
+                    xch32(a, b);
+                    cs[j][t] = a;
+                    cs[j + PP / 2][t] = b;
+                }
+        }
+        // rotate and trade: the pair's low position takes s p + c q, the high c p - s q
+        double cpv = 1.0, spv = 0.0;
+        if constexpr (ODD) {  // the previous lane's straddling pair
+            cpv = os_from_prev(cs[PC - 1][0], 0.0);
+            spv = os_from_prev(cs[PC - 1][1], 1.0);
+            if (firstv) spv = -spv;  // row start: (0, 1) of the dead pair -> keep q
+        }
+        auto rot_rows = [&](double (*m)[CH], const double* mn, const double* mp) {
+#pragma unroll
+            for (int k = 0; k < XR; ++k) {
+                double nx[CH];
+#pragma unroll
+                for (int j = 0; j < PC; ++j) {
+                    const int l = lo(j);
+                    const double c = cs[j][0], s = cs[j][1];
+                    const double p = m[k][l];
+                    if (ODD && j == PC - 1) {
+                        nx[l] = fma(s, p, c * mn[k]);
+                    } else {
+                        const double q = m[k][l + 1];
+                        nx[l] = fma(s, p, c * q);
+                        nx[l + 1] = fma(c, p, -s * q);
+                    }
+                }
+                if constexpr (ODD) nx[0] = fma(cpv, mp[k], -spv * m[k][0]);
+#pragma unroll
+                for (int c = 0; c < CH; ++c) m[k][c] = nx[c];
+            }
+        };
+        rot_rows(x, xn, xp);
+        rot_rows(wv, wn, wp);
+        any = __any(any);
+        rots += nrot;
+        return any;
+    };
+
+    if (N > 1) {
+        while (quiet < NP && G < maxG) {
+            const bool a0 = layer(std::integral_constant<bool, false>{});
+            ++G;
+            quiet = a0 ? 0 : quiet + 1;
+            if (quiet >= NP) break;
+            const bool a1 = layer(std::integral_constant<bool, true>{});
+            ++G;
+            quiet = a1 ? 0 : quiet + 1;
+        }
+    }
+    __syncthreads();  // every wave's last partial-sum reads are done before am is reused
+    clk.mark(kPhJacobi);
+    info.ns_iters = 0;
+    info.jacobi_sweeps = (G - quiet + NP - 1) / NP;
+    info.polar_ok = 1;
+
+    // ---- column norms -> sigma (normalised), rank
+    {
+        double cn[CH];
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            double s = 0.0;
+#pragma unroll
+            for (int k = 0; k < XR; ++k) s = fma(x[k][c], x[k][c], s);
+            s = bfly32(s);
+            if constexpr (LB >= 2) s = bfly16(s);
+            if constexpr (LB >= 3) s = bfly8(s);
+            cn[c] = s;
+        }
+        if (rgw == 0)
+#pragma unroll
+            for (int c = 0; c < CH; ++c) am[w * NP + vch * CH + c] = cn[c];
+        __syncthreads();
+        for (int g = tid; g < NP; g += NT) {
+            double s = 0.0;
+            for (int ww = 0; ww < NW; ++ww) s += am[ww * NP + g];
+            sh.sig[os_index_at(NP, G, g)] = sqrt(s);
+        }
+        __syncthreads();
+    }
+    for (int i = tid; i < N; i += NT) {  // rank: descending, NaN last, ties by index
+        const double di = isnan(sh.sig[i]) ? -INFINITY : sh.sig[i];
+        int r = 0;
+        for (int j = 0; j < N; ++j) {
+            const double dj = isnan(sh.sig[j]) ? -INFINITY : sh.sig[j];
+            r += (dj > di) || (dj == di && j < i);
+        }
+        sh.rank[i] = r;
+    }
+    if (tid == 0) sh.flag = 0;
+    __syncthreads();
+
+    // ---- V = B diag(sigma)^-1 in sorted column order (am: N x LDA)
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int r = rg * XR + k;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int idx = os_index_at(NP, G, vch * CH + c);
+            if (r < N && idx < N) {
+                // columns whose norm is below kOsTiny (the input is scaled to
+                // ||A||_F = 1) are zero to the rotation test (their Gram
+                // products underflow), so their direction is not orthogonal to
+                // working precision: they are completed instead
+                const double sg = sh.sig[idx];
+                const bool tiny = !(sg > kOsTiny) && !isnan(sg);
+                am[r * LDA + sh.rank[idx]] = tiny ? 0.0 : x[k][c] / sg;
+                if (tiny) sh.flag = 1;
+            }
+        }
+    }
+    __syncthreads();
+    if (sh.flag) {  // zero (or underflowing) columns of B: complete V (the reference's Gram-Schmidt rule)
+        gram_schmidt_g<NT>(N, N, View{am, LDA, 0}, am, LDA, sh.red);
+        info.repaired = 1;
+        __syncthreads();
+    }
+    {
+        double* sdst = P.s + itm.s_off;
+        for (int i = tid; i < N; i += NT) sdst[sh.rank[i]] = sh.sig[i] * sc;
+        double* vdst = wide ? P.u + itm.u_off : P.vt + itm.vt_off;
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int r = e / NP, j = e % NP;  // V[r][j]
+            if (r < N && j < N) {
+                if (!wide) vdst[(size_t)j * N + r] = am[r * LDA + j];
+                else vdst[(size_t)r * N + j] = am[r * LDA + j];
+            }
+        }
+    }
+    __syncthreads();
+    clk.mark(kPhU);
+
+    // ---- U = Q [W; 0] in sorted column order
+    for (int e = tid; e < MP * NP; e += NT) am[(e / NP) * LDA + e % NP] = 0.0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int r = rg * XR + k;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int idx = os_index_at(NP, G, vch * CH + c);
+            if (r < N && idx < N) am[r * LDA + sh.rank[idx]] = wv[k][c];
+        }
+    }
+    __syncthreads();
+    os_apply_q<NT, NP>(M, N, N, refl, NP, am, LDA, sh, part);
+    {
+        double* u = wide ? P.vt + itm.vt_off : P.u + itm.u_off;  // wide: (N x M) = U_t^T
+        for (int e = tid; e < MP * NP; e += NT) {
+            const int i = e / NP, j = e % NP;
+            if (i < M && j < N) {
+                if (!wide) u[(size_t)i * N + j] = am[i * LDA + j];
+                else u[(size_t)j * M + i] = am[i * LDA + j];
+            }
+        }
+    }
+    clk.mark(kPhOut);
+    rots = (int)block_sum<NT>((double)rots, sh.red);
+    info.rotations = rots;
+    if (tid == 0 && P.info) {
+        info.path = itm.mode == kModeMicroFallback ? 2 : 4;
+        info.micro_fallback = itm.mode == kModeMicroFallback ? 1 : 0;
+        P.info[itm.idx] = info;
+    }
+    __syncthreads();
+}
+
+// ---- kernels ---------------------------------------------------------------
+
+template <int NP, int TPR = OsDefault<NP>::TPR, int NW = OsDefault<NP>::NW>
+__global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict__ items, const int* count_ptr,
+                                                         int* counter, Pools pools, double* gws, int64_t gws_stride) {
+    using C = OsCfg<NP, TPR, NW>;
+    extern __shared__ double dyn_lds[];
+    __shared__ OsShared sh;
+    const int count = *count_ptr;
+    double* refl = gws + (size_t)blockIdx.x * gws_stride;
+    // LDS: the column partial sums of the Householder steps (2 NT doubles),
+    // then the matrix area (MP x LDA; the sweeps' partial sums use it too)
+    double* part = dyn_lds;
+    double* am = dyn_lds + 2 * C::NT;
+    for (;;) {
+        if (threadIdx.x == 0) sh.item = atomicAdd(counter, 1);
+        __syncthreads();
+        const int it = sh.item;
+        __syncthreads();
+        if (it >= count) break;
+        const Item itm = items[it];
+        os_solve<NP, TPR, NW>(itm, pools, am, part, refl, sh);
+    }
+}
+
+#define LSVD_OS_SIZES(X) X(16) X(32) X(48) X(64) X(80) X(96) X(112) X(128)
+
+int os_threads(int np) {
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return OsCfg<np_, OsDefault<np_>::TPR, OsDefault<np_>::NW>::NT;
+        LSVD_OS_SIZES(X)
+#undef X
+    }
+    return 0;
+}
+
+size_t os_lds_bytes(int np, int M) {
+    const int mp = (M + 15) & ~15;
+    const int rows = mp > np ? mp : np;
+    return sizeof(double) * (2 * (size_t)os_threads(np) + (size_t)rows * (np + 1));
+}
+
+int64_t os_slot_doubles(int np, int M) {
+    const int mp = (M + 15) & ~15;
+    const int rows = mp > np ? mp : np;
+    return ((int64_t)rows * np + 31) / 32 * 32;
+}
+
+template <int NP>
+static hipError_t os_launch(const SmallLaunch& L, hipStream_t st) {
+    constexpr int NT = OsCfg<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW>::NT;
+    if (L.lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&os_svd_kernel<NP>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((os_svd_kernel<NP>), dim3(L.grid), dim3(NT), L.lds_bytes, st, L.items, L.count_ptr, L.counter,
+                       L.pools, L.gws, L.gws_stride);
+    return hipGetLastError();
+}
+
+template <int NP>
+static int os_blocks(size_t lds) {
+    constexpr int NT = OsCfg<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW>::NT;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, os_svd_kernel<NP>, NT, lds) != hipSuccess) return 1;
+    return nb > 0 ? nb : 1;
+}
+
+hipError_t launch_onesided(int np, const SmallLaunch& L, hipStream_t st) {
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return os_launch<np_>(L, st);
+        LSVD_OS_SIZES(X)
+#undef X
+    }
+    return hipErrorInvalidValue;
+}
+
+int onesided_blocks_per_cu(int np, size_t lds) {
+    switch (np) {
+#define X(np_) \
+    case np_:  \
+        return os_blocks<np_>(lds);
+        LSVD_OS_SIZES(X)
+#undef X
+    }
+    return 1;
+}
+
+}  // namespace lsvd

@@ -32,11 +32,14 @@ same workload.  Further legs, each with its CPU oracle beside it:
            matrices with gaps ~1e-3 plus 32 with one 1e-8 gap, against the
            analytic FP64 gradient and central differences.
 
-The batch value is the chosen schedule (FP64 Newton-Schulz with scaled
-steps while small singular values grow, LSVD_PREC_F64_SCALED); the plain
-FP64 reference iteration is timed beside it on the same workload.
+The batch value is the chosen route (LSVD_PREC_F64_ONESIDED: matrices of
+padded size up to the autotuned split take Householder QR + one-sided
+Jacobi on R^T, the larger ones FP64 Newton-Schulz with scaled steps and the
+ring-order Jacobi); the plain FP64 reference iteration (LSVD_PREC_F64) and
+the scaled Newton-Schulz route alone (LSVD_PREC_F64_SCALED) are timed
+beside it on the same workload.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision scaled|f64]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision onesided|scaled|f64]
 """
 from __future__ import annotations
 
@@ -71,6 +74,8 @@ def algorithmic_flops(m: int, n: int, k_ns: int, sweeps: int) -> float:
 
 
 def kernel_name(variant: int) -> str:
+    if variant >= 900:
+        return f"os_svd_multi_kernel (QR + one-sided Jacobi, padded sizes 80..{variant - 900})"
     if variant >= 700:
         return f"os_svd_kernel<{variant - 700}> (QR + one-sided Jacobi)"
     if variant >= 500:
@@ -825,7 +830,7 @@ def main(argv=None) -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--precision", default="scaled", choices=["scaled", "f64", "onesided"],
+    ap.add_argument("--precision", default="onesided", choices=["onesided", "scaled", "f64"],
                     help="batch route: FP64 NS with scaled steps, the reference iteration, or QR + one-sided Jacobi")
     ap.add_argument("--count", type=int, default=4096)
     ap.add_argument("--tiny", type=int, default=8192)
@@ -933,10 +938,10 @@ def main(argv=None) -> int:
     value = count / (ms_per_step / 1e3)
     log(f"timed: {args.steps} steps, {ms_per_step:.3f} ms/step (wall {wall / args.steps * 1e3:.3f} ms/step)")
 
-    # ---- the plain FP64 reference iteration on the same workload, same count
-    plain = None
-    if prec != L.LSVD_PREC_F64:
-        plan0 = BatchPlan(h, ms, ns, L.LSVD_PREC_F64)
+    # ---- the plain FP64 reference iteration (and the scaled Newton-Schulz
+    # route) on the same workload, same step count
+    def time_other(p2):
+        plan0 = BatchPlan(h, ms, ns, p2)
         u0 = torch.zeros_like(u_pool); s0 = torch.zeros_like(s_pool); vt0 = torch.zeros_like(vt_pool)
         for _ in range(args.warmup + 1):
             plan0.execute(a_pool, u0, s0, vt0)
@@ -948,10 +953,17 @@ def main(argv=None) -> int:
         torch.cuda.synchronize()
         ms0 = ev0.elapsed_time(ev1) / args.steps
         d_s = float((s0 - s_pool).abs().max() / s_pool.abs().max().clamp_min(1e-300))
-        plain = {"ms_per_step": round(ms0, 4), "value": round(count / (ms0 / 1e3), 1),
-                 "chosen_speedup": round(ms0 / ms_per_step, 3), "max_sigma_diff_vs_chosen_over_max": d_s}
-        log(f"plain FP64 iteration: {ms0:.3f} ms/step, chosen speedup {ms0 / ms_per_step:.3f}")
         plan0.close()
+        return {"ms_per_step": round(ms0, 4), "value": round(count / (ms0 / 1e3), 1),
+                "chosen_speedup": round(ms0 / ms_per_step, 3), "max_sigma_diff_vs_chosen_over_max": d_s}
+
+    plain = scaled = None
+    if prec != L.LSVD_PREC_F64:
+        plain = time_other(L.LSVD_PREC_F64)
+        log(f"plain FP64 iteration: {plain['ms_per_step']:.3f} ms/step, chosen speedup {plain['chosen_speedup']:.3f}")
+    if prec == L.LSVD_PREC_F64_ONESIDED:
+        scaled = time_other(L.LSVD_PREC_F64_SCALED)
+        log(f"scaled Newton-Schulz route: {scaled['ms_per_step']:.3f} ms/step")
 
     # ---- per-launch times of each bucket (HIP events on the plan's stream)
     # each launch timed while the other lanes run beside it, averaged over
@@ -998,6 +1010,7 @@ def main(argv=None) -> int:
     # ---- CPU baseline: the oracle on all host cores, same workload
     cpu = None
     oracle_info = None
+    gvo = None
     if not args.no_cpu:
         import oracle
         threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -1014,6 +1027,7 @@ def main(argv=None) -> int:
             e_rel, e_abs = metrics.sigma_errors(outs[b][1], ref_out[b][1])
             gvo_rel = max(gvo_rel, e_rel)
             gvo_abs = max(gvo_abs, e_abs)
+        gvo = {"rel": gvo_rel, "abs": gvo_abs}
         log(f"cpu oracle: {cpu['value']:.0f} matrices/s on {threads} threads; "
             f"gpu vs oracle sigma rel {gvo_rel:.2e} abs {gvo_abs:.2e}")
 
@@ -1092,8 +1106,14 @@ def main(argv=None) -> int:
                                "10% clustered + 8192 square 2..4 (LieSvdMicro), one plan execution per step",
                    "matrices": count, "schedule": args.precision, "parallelism": "single GPU"},
         "plain_fp64": plain,
+        "scaled_ns_fp64": scaled,
         "sigma_err": {"rel_max_vs_imposed": e_rel_max, "abs_max_vs_imposed": e_abs_max,
+                      "rel_max_vs_oracle": gvo["rel"] if gvo else None,
+                      "abs_max_vs_oracle": gvo["abs"] if gvo else None,
                       "rule": "rel over sigma>=1e-3*sigma_max, abs normalised by sigma_max"},
+        "routes": {name: int((info_h[:, 6] == code).sum())
+                   for code, name in [(0, "newton_schulz_jacobi"), (1, "micro"), (2, "micro_fallback"),
+                                      (4, "qr_onesided_jacobi")]},
         "residuals_all": {"rel_recon_max": worst_recon, "orth_max": worst_orth, "matrices": count,
                           "device_metrics_ms": round(met_ms, 2)},
         "fp64_tflops_whole_batch": round(total_tflops, 4),

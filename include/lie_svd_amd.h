@@ -59,7 +59,7 @@ enum lsvd_pointer_mode { LSVD_PTR_HOST = 0, LSVD_PTR_DEVICE = 1 };
  *                        counterpart.
  *   LSVD_PREC_F64_ONESIDED batch plans only: SVD items with max(m, n) <= 128
  *                        and 16 ceil(min(m, n) / 16) <= the handle's
- *                        os_max_np (default 64, environment LSVD_OS_MAX_NP at
+ *                        os_max_np (default 128, environment LSVD_OS_MAX_NP at
  *                        handle creation, chosen by lsvd_autotune_batch) are
  *                        solved by Householder QR and one-sided (Hestenes)
  *                        Jacobi on R^T in FP64 (no Newton-Schulz; info.path =

@@ -92,6 +92,16 @@ static double item_cost(int M, int N) {
 // Sets the launch shape of a bucket for `variant`; returns the global arena
 // doubles it needs (0 for LDS variants).
 static size_t configure_bucket(lsvd_handle_t h, Bucket& bk, int variant) {
+    if (variant >= kOsMulti) {  // one-sided sizes 80..128 in one launch: LDS and slot for the largest
+        const int np = variant - kOsMulti;
+        bk.variant = variant;
+        const size_t lds = os_lds_bytes(np, bk.maxM);
+        const int per_cu = onesided_multi_blocks_per_cu(lds);
+        bk.grid = std::max(1, std::min(bk.count, per_cu * h->num_cus));
+        bk.lds_bytes = lds;
+        bk.gws_stride = os_slot_doubles(np, bk.maxM);
+        return (size_t)bk.gws_stride * bk.grid;
+    }
     if (variant >= kOsBase) {  // QR + one-sided Jacobi: LDS and reflector slot sized by the largest M
         const int np = variant - kOsBase;
         bk.variant = variant;
@@ -203,7 +213,10 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         int key;
         if (N <= 128 && M <= 128 && precision == LSVD_PREC_F64_ONESIDED && mode != kModeEigh &&
             (N + 15) / 16 * 16 <= h->os_max_np) {
-            key = 3000 + (N + 15) / 16 * 16;  // QR + one-sided Jacobi, one launch per padded size
+            const int np = (N + 15) / 16 * 16;
+            // QR + one-sided Jacobi: one launch per padded size below 80, one
+            // launch over the larger sizes (they share the launch shape)
+            key = np >= kOsMultiMinNP ? 3500 : 3000 + np;
         } else if (N <= 128 && M <= 128) {
             const int np = (N + 15) / 16 * 16;
             // on-chip kernels: one launch per padded size below 64, one
@@ -234,7 +247,7 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
     for (auto& kv : by_class) {
         auto& v = kv.second;
         std::stable_sort(v.begin(), v.end(), [](const Tmp& x, const Tmp& y) { return x.cost > y.cost; });
-        if (kv.first == 2000)  // the multi-size launch drains its size classes largest first
+        if (kv.first == 2000 || kv.first == 3500)  // the multi-size launches drain their size classes largest first
             std::stable_sort(v.begin(), v.end(),
                              [](const Tmp& x, const Tmp& y) { return (x.N + 15) / 16 > (y.N + 15) / 16; });
         Bucket bk{};
@@ -255,15 +268,17 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         bk.arena_bytes = arena;
         bk.consts_off = (int)consts.size();
         consts.push_back(bk.count);
-        if (kv.first == 2000) {  // ends of the classes 128, 112, 96, 80, 64 (relative to bk.first)
-            for (int np = 128; np >= kV2MultiMinNP; np -= 16) {
+        if (kv.first == 2000 || kv.first == 3500) {  // class ends 128, 112, 96, 80[, 64] (relative to bk.first)
+            for (int np = 128; np >= (kv.first == 2000 ? kV2MultiMinNP : kOsMultiMinNP); np -= 16) {
                 int end = 0;
                 while (end < (int)v.size() && (v[end].N + 15) / 16 * 16 >= np) ++end;
                 consts.push_back(end);
             }
         }
         int vid;
-        if (kv.first >= 3000) {
+        if (kv.first == 3500) {
+            vid = kOsMulti + (maxN + 15) / 16 * 16;
+        } else if (kv.first >= 3000) {
             vid = kOsBase + (kv.first - 3000);
         } else if (kv.first == 2000) {
             vid = kV2Multi + (maxN + 15) / 16 * 16;
@@ -1030,7 +1045,9 @@ int lsvd_batch_plan_describe(lsvd_plan_t p, lsvd_bucket_desc_t* out, int max, in
     if (p->n_micro > 0) d.push_back({0, p->n_micro, 4, -1, (p->n_micro + 255) / 256, 256, 0});
     for (auto& bk : p->buckets) {
         const SmallVariant* v = bk.variant >= kV2Base ? nullptr : find_variant(bk.variant);
-        const int nt = bk.variant >= kV2Wide    ? 2 * v2_threads(bk.variant - kV2Wide)
+        const int nt = bk.variant >= kOsMulti   ? 512
+                       : bk.variant >= kOsBase  ? os_threads(bk.variant - kOsBase)
+                       : bk.variant >= kV2Wide  ? 2 * v2_threads(bk.variant - kV2Wide)
                        : bk.variant >= kV2Multi ? 1024
                        : bk.variant >= kV2Base  ? v2_threads(bk.variant - kV2Base)
                                                 : (v ? v->nt : 0);

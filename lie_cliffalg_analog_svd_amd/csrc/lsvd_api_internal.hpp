@@ -70,8 +70,9 @@ struct lsvd_handle_s {
     int multi_min_np = 64;
     // LSVD_PREC_F64_ONESIDED plans: SVD items of padded size <= os_max_np
     // take QR + one-sided Jacobi, the larger ones the scaled Newton-Schulz
-    // kernels (autotuned)
-    int os_max_np = 64;
+    // kernels (autotuned; on the ragged batch every size is faster one-sided:
+    // whole plan 7.59 ms at 128 against 7.93 at 64 and 8.35-9.0 in between)
+    int os_max_np = 128;
     int max_lanes = 4;
     uint32_t wide_mask = 0;
     lsvd_tune_report_t tune_report{};

@@ -66,6 +66,13 @@ size_t os_lds_bytes(int np, int max_m);
 int64_t os_slot_doubles(int np, int max_m);
 hipError_t launch_onesided(int np, const SmallLaunch& L, hipStream_t st);
 int onesided_blocks_per_cu(int np, size_t lds_bytes);
+// One persistent launch over the one-sided items of padded sizes 80..128
+// (512 threads, LDS and slot sized for the largest): variant id =
+// kOsMulti + np_max; count_ptr[1..4] end the classes 128, 112, 96, 80.
+constexpr int kOsMulti = 900;
+constexpr int kOsMultiMinNP = 80;
+hipError_t launch_onesided_multi(const SmallLaunch& L, hipStream_t st);
+int onesided_multi_blocks_per_cu(size_t lds_bytes);
 
 // ---- LieSvdMicro (micro_svd.hip) ----------------------------------------
 

@@ -53,33 +53,77 @@ namespace lsvd {
 
 // ---- configuration -------------------------------------------------------
 
-template <int NP>
-struct OsDefault {
-    // positions per lane CH = NP / TPR even, X rows per lane XR = 2
-    static constexpr int TPR = NP == 128 ? 16 : 8;
-    static constexpr int NW = NP == 128 ? 8 : (NP / 16 > 1 ? NP / 16 : 1);
+// Launch shape per padded size NP: TPR lanes per row, NW waves, CH
+// positions per lane.  The layout holds NL = TPR * CH >= NP positions; with
+// NL > NP the positions from NP on are zero columns that never meet the
+// real ones (the odd-layer pair across position NP - 1 is dead).
+#define LSVD_OS_CFG_16 8, 1, 2
+#ifndef LSVD_OS_CFG_32
+#define LSVD_OS_CFG_32 8, 2, 4
+#endif
+#ifndef LSVD_OS_CFG_48
+#define LSVD_OS_CFG_48 8, 3, 6
+#endif
+#ifndef LSVD_OS_CFG_64
+#define LSVD_OS_CFG_64 32, 4, 2
+#endif
+#ifndef LSVD_OS_CFG_80
+#define LSVD_OS_CFG_80 64, 8, 2
+#endif
+#ifndef LSVD_OS_CFG_96
+#define LSVD_OS_CFG_96 64, 8, 2
+#endif
+#ifndef LSVD_OS_CFG_112
+#define LSVD_OS_CFG_112 64, 8, 2
+#endif
+#ifndef LSVD_OS_CFG_128
+#define LSVD_OS_CFG_128 64, 8, 2
+#endif
+template <int TPR_, int NW_, int CH_>
+struct OsShape {
+    static constexpr int TPR = TPR_, NW = NW_, CH = CH_;
 };
+template <int NP>
+struct OsDefault;
+template <>
+struct OsDefault<16> : OsShape<LSVD_OS_CFG_16> {};
+template <>
+struct OsDefault<32> : OsShape<LSVD_OS_CFG_32> {};
+template <>
+struct OsDefault<48> : OsShape<LSVD_OS_CFG_48> {};
+template <>
+struct OsDefault<64> : OsShape<LSVD_OS_CFG_64> {};
+template <>
+struct OsDefault<80> : OsShape<LSVD_OS_CFG_80> {};
+template <>
+struct OsDefault<96> : OsShape<LSVD_OS_CFG_96> {};
+template <>
+struct OsDefault<112> : OsShape<LSVD_OS_CFG_112> {};
+template <>
+struct OsDefault<128> : OsShape<LSVD_OS_CFG_128> {};
 
 constexpr int ilog2c(int x) { return x <= 1 ? 0 : 1 + ilog2c(x / 2); }
 constexpr int pow2up(int x) { return x <= 1 ? 1 : 2 * pow2up((x + 1) / 2); }
 
-template <int NP, int TPR, int NW>
+template <int NP, int TPR, int NW, int CHH>
 struct OsCfg {
     static constexpr int NT = 64 * NW;
-    static constexpr int CH = NP / TPR;        // positions per lane
+    static constexpr int CH = CHH;             // positions per lane
+    static constexpr int NL = TPR * CH;        // positions in the layout (>= NP)
     static constexpr int P = CH / 2;           // pairs per lane and layer
     static constexpr int PP = pow2up(P);       // padded to a power of two for the reduction
     static constexpr int RGW = 64 / TPR;       // rowgroups per wave
     static constexpr int LB = ilog2c(RGW);     // lane bits of the rowgroup (the top LB of 6)
     static constexpr int RG = NW * RGW;
-    static constexpr int XR = NP / RG;         // X rows (and W rows) per lane
-    static constexpr int SC = ilog2c(PP) < 2 ? ilog2c(PP) : 2;  // scatter steps (bits 5, 4)
+    static constexpr int XR = (NP + RG - 1) / RG;  // X rows (and W rows) per lane
+    static constexpr int SCMAX = LB < 2 ? LB : 2;  // lane bits 5 and 4, where they are rowgroup bits
+    static constexpr int SC = ilog2c(PP) < SCMAX ? ilog2c(PP) : SCMAX;  // scatter steps (bits 5, 4)
     static constexpr int NA = PP >> SC;        // pairs a lane owns after the scatter
-    static constexpr int S = NP / 2;
+    static constexpr int S = NL / 2;           // pairs in the layout
     static constexpr int LDA = NP + 1;
-    static_assert(NP % TPR == 0 && CH % 2 == 0 && CH >= 2, "positions per lane");
-    static_assert(TPR == 8 || TPR == 16, "rowgroup bits 5 and 4 must be rowgroup bits");
-    static_assert(RG * XR == NP && XR >= 1, "rows per lane");
+    static_assert(CH % 2 == 0 && CH >= 2 && NL >= NP && NP % CH == 0, "positions per lane");
+    static_assert(TPR == 8 || TPR == 16 || TPR == 32 || TPR == 64, "rowgroups of whole lane bit groups");
+    static_assert(XR >= 1 && 3 * NW * NL <= NP * LDA, "rows per lane; the sweeps' partial sums fit the matrix area");
 };
 
 // Column norms (of the input scaled to ||A||_F = 1) at or below this are
@@ -123,8 +167,20 @@ __device__ __forceinline__ void xch16(double& a, double& b) {
 // lane l ^ 8's value (DPP row_ror:8 within rows of 16)
 __device__ __forceinline__ double from_xor8(double x) {
     const int lo = __double2loint(x), hi = __double2hiint(x);
-    return __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, 0x128, 0xf, 0xf, false),
-                            __builtin_amdgcn_update_dpp(lo, lo, 0x128, 0xf, 0xf, false));
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(hi, 0x128, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(lo, 0x128, 0xf, 0xf, true));
+}
+
+// lane l + 1's value (wave_shl:1), lane l - 1's (wave_shr:1); a lane
+// without a source (63, resp. 0) reads 0.  No `old` operand, so the
+// compiler needs no copy of the destination first.
+__device__ __forceinline__ double os_next0(double x) {
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), 0x130, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(__double2loint(x), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ double os_prev0(double x) {
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), 0x138, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(__double2loint(x), 0x138, 0xf, 0xf, true));
 }
 
 // lane l + 1's value (wave_shl:1); lane 63 gets `old`
@@ -209,11 +265,11 @@ __device__ __forceinline__ void os_angle(double a, double b, double g, double& c
 template <int NT, int NPC>
 __device__ void os_householder(int M, int N, double* a, int lda, OsShared& sh, double* part) {
     const int tid = tix();
-    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;
-    static_assert(NT % NPC == 0, "whole row chunks");
+    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;  // threads from K NPC on idle
     const int col = tid % NPC, chunk = tid / NPC;
+    const bool act = chunk < K;
     double* npart = part + (K + 1) * NPC;  // [2][K]: squared norms of the next column, per chunk
-    if (col == 0) {
+    if (col == 0 && act) {
         double s = 0.0;
         for (int i = 1 + chunk; i < M; i += K) s = fma(a[(size_t)i * lda], a[(size_t)i * lda], s);
         npart[chunk] = s;
@@ -233,7 +289,7 @@ __device__ void os_householder(int M, int N, double* a, int lda, OsShared& sh, d
         }
         // column sums sum_{i>j} a[i][j] a[i][col] (and row j of col, kept
         // unmodified for the update in part[K][col])
-        if (col > j && col < N) {
+        if (col > j && col < N && act) {
             // four rows per round, loads first (the round trips overlap)
             double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
             int i = j + 1 + chunk;
@@ -253,7 +309,7 @@ __device__ void os_householder(int M, int N, double* a, int lda, OsShared& sh, d
         }
         if (tid == 0) sh.tau[j] = tau;
         __syncthreads();
-        if (col > j && col < N) {
+        if (col > j && col < N && act) {
             double sum = 0.0;
 #pragma unroll
             for (int k = 0; k < K; ++k) sum += part[k * NPC + col];
@@ -285,7 +341,7 @@ __device__ void os_householder(int M, int N, double* a, int lda, OsShared& sh, d
                 if (i > j + 1) nn = fma(nv, nv, nn);
             }
             if (col == j + 1) npart[((j + 1) & 1) * K + chunk] = nn;
-        } else if (col == j) {
+        } else if (col == j && act) {
             if (chunk == 0) a[(size_t)j * lda + j] = beta;
             if (j > 0)  // the previous reflector's tail (no longer read)
                 for (int i = j + chunk; i < M; i += K) a[(size_t)i * lda + j - 1] *= prev_scal;
@@ -293,7 +349,7 @@ __device__ void os_householder(int M, int N, double* a, int lda, OsShared& sh, d
         __syncthreads();
         prev_scal = scal;
     }
-    if (N > 0 && col == 0)  // the last reflector's tail
+    if (N > 0 && col == 0 && act)  // the last reflector's tail
         for (int i = N + chunk; i < M; i += K) a[(size_t)i * lda + N - 1] *= prev_scal;
     __syncthreads();
 }
@@ -305,15 +361,16 @@ template <int NT, int NPC>
 __device__ void os_apply_q(int M, int N, int ncol, const double* refl, int ldr, double* y, int ldy, OsShared& sh,
                            double* part) {
     const int tid = tix();
-    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;
+    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;  // threads from K NPC on idle
     const int col = tid % NPC, chunk = tid / NPC;
+    const bool act = chunk < K;
     for (int j = N - 1; j >= 0; --j) {
         const double tau = sh.tau[j];
         if (tau == 0.0) continue;  // uniform
         // the reflector, staged once per column (rows > j)
         for (int i = j + 1 + tid; i < M; i += NT) sh.vbuf[i] = refl[(size_t)i * ldr + j];
         __syncthreads();
-        if (col < ncol) {
+        if (col < ncol && act) {
             double s0 = chunk == 0 ? y[(size_t)j * ldy + col] : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
             int i = j + (chunk == 0 ? K : chunk);
             for (; i + 3 * K < M; i += 4 * K) {
@@ -328,7 +385,7 @@ __device__ void os_apply_q(int M, int N, int ncol, const double* refl, int ldr, 
             part[chunk * NPC + col] = (s0 + s1) + (s2 + s3);
         }
         __syncthreads();
-        if (col < ncol) {
+        if (col < ncol && act) {
             double sum = 0.0;
 #pragma unroll
             for (int k = 0; k < K; ++k) sum += part[k * NPC + col];
@@ -354,9 +411,10 @@ __device__ void os_apply_q(int M, int N, int ncol, const double* refl, int ldr, 
 
 // ---- the per-matrix solver -----------------------------------------------
 
-template <int NP, int TPR, int NW>
+template <int NP, int TPR, int NW, int CHH>
 __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* part, double* refl, OsShared& sh) {
-    using C = OsCfg<NP, TPR, NW>;
+    using C = OsCfg<NP, TPR, NW, CHH>;
+    constexpr int NL = C::NL;
     constexpr int NT = C::NT, CH = C::CH, PC = C::P, PP = C::PP, XR = C::XR, LDA = C::LDA;
     constexpr int SC = C::SC, NA = C::NA, LB = C::LB, RGW = C::RGW;
     const int tid = tix(), w = tid >> 6, lane = tid & 63;
@@ -421,39 +479,34 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
     const double tol2 = tol * tol;
     const int maxG = (60 + 20 * N) * NP;
     int G = 0, quiet = 0, rots = 0;
-    const bool lastv = vch == TPR - 1, firstv = vch == 0;
+    // the lane whose last position is NP - 1, and a row's last lane: their
+    // straddling pairs are dead; the lanes after them start a segment
+    const bool lastv = vch == NP / CH - 1 || vch == TPR - 1, firstv = vch == 0 || vch == NP / CH;
     // lane bits that the reduce-scatter did not split (replicated sums)
-    const int rep_mask = ((1 << LB) - 1) & ~((SC >= 1 ? (RGW >> 1) : 0) | (SC >= 2 ? (RGW >> 2) : 0));
+    const int rep_mask = (RGW - 1) & ~((SC >= 1 ? (RGW >> 1) : 0) | (SC >= 2 ? (RGW >> 2) : 0));
     const bool rep_lane = (rgw & rep_mask) == 0 && w == 0;
 
     // one layer; ODD: pairs (2j+1, 2j+2) in the lane and (CH-1, next lane's 0)
     auto layer = [&](auto odd_tag) {
         constexpr bool ODD = decltype(odd_tag)::value;
-        double xn[XR], xp[XR], wn[XR], wp[XR];
-        if constexpr (ODD) {
-#pragma unroll
-            for (int k = 0; k < XR; ++k) {
-                xn[k] = os_from_next(x[k][0], x[k][0]);
-                xp[k] = os_from_prev(x[k][CH - 1], x[k][CH - 1]);
-                wn[k] = os_from_next(wv[k][0], wv[k][0]);
-                wp[k] = os_from_prev(wv[k][CH - 1], wv[k][CH - 1]);
-            }
-        }
         auto lo = [](int j) { return ODD ? 2 * j + 1 : 2 * j; };
         // partial Gram entries of the lane's pairs over its rows
         double v[PP][3];
 #pragma unroll
         for (int j = 0; j < PP; ++j) v[j][0] = v[j][1] = v[j][2] = 0.0;
 #pragma unroll
-        for (int j = 0; j < PC; ++j)
+        for (int k = 0; k < XR; ++k) {
+            // the straddling pair's partner: the next lane's first position
+            const double xn = ODD ? os_next0(x[k][0]) : 0.0;
 #pragma unroll
-            for (int k = 0; k < XR; ++k) {
+            for (int j = 0; j < PC; ++j) {
                 const double p = x[k][lo(j)];
-                const double q = (ODD && j == PC - 1) ? xn[k] : x[k][lo(j) + 1];
+                const double q = (ODD && j == PC - 1) ? xn : x[k][lo(j) + 1];
                 v[j][0] = fma(p, p, v[j][0]);
                 v[j][1] = fma(q, q, v[j][1]);
                 v[j][2] = fma(p, q, v[j][2]);
             }
+        }
         // reduce-scatter over lane bit 5, then bit 4: pair index owned
         int kid = 0;
         if constexpr (SC >= 1) {
@@ -484,7 +537,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
                 double s = v[j][t];
-                if constexpr (SC < 1) s = bfly32(s);
+                if constexpr (SC < 1 && LB >= 1) s = bfly32(s);
                 if constexpr (SC < 2 && LB >= 2) s = bfly16(s);
                 if constexpr (LB >= 3) s = bfly8(s);
                 v[j][t] = s;
@@ -527,7 +580,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
 #pragma unroll
         for (int j = 0; j < NA; ++j) {
             const int jk = kid + j;
-            const bool boundary_dead = ODD && lastv && jk == PC - 1;  // position NP - 1 has no partner
+            const bool boundary_dead = ODD && lastv && jk == PC - 1;  // no partner across the segment end
             const bool valid = jk < PC && !boundary_dead;
             const double a = v[j][0], b = v[j][1], g = v[j][2];
             const bool rot = valid && g * g > tol2 * (a * b);
@@ -572,9 +625,14 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
             spv = os_from_prev(cs[PC - 1][1], 1.0);
             if (firstv) spv = -spv;  // row start: (0, 1) of the dead pair -> keep q
         }
-        auto rot_rows = [&](double (*m)[CH], const double* mn, const double* mp) {
+        auto rot_rows = [&](double (*m)[CH]) {
 #pragma unroll
             for (int k = 0; k < XR; ++k) {
+                double mn = 0.0, mp = 0.0;
+                if constexpr (ODD) {  // the neighbours' boundary columns, before this row changes
+                    mn = os_next0(m[k][0]);
+                    mp = os_prev0(m[k][CH - 1]);
+                }
                 double nx[CH];
 #pragma unroll
                 for (int j = 0; j < PC; ++j) {
@@ -582,20 +640,20 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
                     const double c = cs[j][0], s = cs[j][1];
                     const double p = m[k][l];
                     if (ODD && j == PC - 1) {
-                        nx[l] = fma(s, p, c * mn[k]);
+                        nx[l] = fma(s, p, c * mn);
                     } else {
                         const double q = m[k][l + 1];
                         nx[l] = fma(s, p, c * q);
                         nx[l + 1] = fma(c, p, -s * q);
                     }
                 }
-                if constexpr (ODD) nx[0] = fma(cpv, mp[k], -spv * m[k][0]);
+                if constexpr (ODD) nx[0] = fma(cpv, mp, -spv * m[k][0]);
 #pragma unroll
                 for (int c = 0; c < CH; ++c) m[k][c] = nx[c];
             }
         };
-        rot_rows(x, xn, xp);
-        rot_rows(wv, wn, wp);
+        rot_rows(x);
+        rot_rows(wv);
         any = __any(any);
         rots += nrot;
         return any;
@@ -626,18 +684,18 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
             double s = 0.0;
 #pragma unroll
             for (int k = 0; k < XR; ++k) s = fma(x[k][c], x[k][c], s);
-            s = bfly32(s);
+            if constexpr (LB >= 1) s = bfly32(s);
             if constexpr (LB >= 2) s = bfly16(s);
             if constexpr (LB >= 3) s = bfly8(s);
             cn[c] = s;
         }
         if (rgw == 0)
 #pragma unroll
-            for (int c = 0; c < CH; ++c) am[w * NP + vch * CH + c] = cn[c];
+            for (int c = 0; c < CH; ++c) am[w * NL + vch * CH + c] = cn[c];
         __syncthreads();
         for (int g = tid; g < NP; g += NT) {
             double s = 0.0;
-            for (int ww = 0; ww < NW; ++ww) s += am[ww * NP + g];
+            for (int ww = 0; ww < NW; ++ww) s += am[ww * NL + g];
             sh.sig[os_index_at(NP, G, g)] = sqrt(s);
         }
         __syncthreads();
@@ -660,7 +718,8 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
         const int r = rg * XR + k;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-            const int idx = os_index_at(NP, G, vch * CH + c);
+            const int g = vch * CH + c;
+            const int idx = g < NP ? os_index_at(NP, G, g) : NP;
             if (r < N && idx < N) {
                 // columns whose norm is below kOsTiny (the input is scaled to
                 // ||A||_F = 1) are zero to the rotation test (their Gram
@@ -702,7 +761,8 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
         const int r = rg * XR + k;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
-            const int idx = os_index_at(NP, G, vch * CH + c);
+            const int g = vch * CH + c;
+            const int idx = g < NP ? os_index_at(NP, G, g) : NP;
             if (r < N && idx < N) am[r * LDA + sh.rank[idx]] = wv[k][c];
         }
     }
@@ -731,10 +791,10 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
 
 // ---- kernels ---------------------------------------------------------------
 
-template <int NP, int TPR = OsDefault<NP>::TPR, int NW = OsDefault<NP>::NW>
+template <int NP, int TPR = OsDefault<NP>::TPR, int NW = OsDefault<NP>::NW, int CHH = OsDefault<NP>::CH>
 __global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict__ items, const int* count_ptr,
                                                          int* counter, Pools pools, double* gws, int64_t gws_stride) {
-    using C = OsCfg<NP, TPR, NW>;
+    using C = OsCfg<NP, TPR, NW, CHH>;
     extern __shared__ double dyn_lds[];
     __shared__ OsShared sh;
     const int count = *count_ptr;
@@ -750,8 +810,77 @@ __global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict_
         __syncthreads();
         if (it >= count) break;
         const Item itm = items[it];
-        os_solve<NP, TPR, NW>(itm, pools, am, part, refl, sh);
+        os_solve<NP, TPR, NW, CHH>(itm, pools, am, part, refl, sh);
     }
+}
+
+template <int NP>
+using OsDefCfg = OsCfg<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW, OsDefault<NP>::CH>;
+
+// One persistent launch over the padded sizes 80..128, which share the
+// launch shape (64 lanes per row, 8 waves, 2 positions per lane): the queue
+// holds the items ordered by padded size, largest first; count_ptr[1 + j]
+// is the end of size class j (128, 112, 96, 80).  Each workgroup drains the
+// classes in that order with the solver instance of each size; an index
+// fetched past a class's end is carried into the next class.
+constexpr int kOsMultiNT = 512;
+static_assert(OsDefCfg<80>::NT == kOsMultiNT && OsDefCfg<96>::NT == kOsMultiNT &&
+                  OsDefCfg<112>::NT == kOsMultiNT && OsDefCfg<128>::NT == kOsMultiNT,
+              "one block size for the multi-size launch");
+
+template <int NP>
+__device__ __forceinline__ void os_multi_drain(const Item* items, int end, int* counter, const Pools& pools,
+                                               double* am, double* part, double* refl, OsShared& sh,
+                                               int* next_item) {
+    for (;;) {
+        if (*next_item < 0) {  // uniform: next_item is in LDS
+            __syncthreads();
+            if (threadIdx.x == 0) *next_item = atomicAdd(counter, 1);
+            __syncthreads();
+        }
+        const int it = *next_item;
+        if (it >= end) return;  // keep it for a later class
+        __syncthreads();
+        if (threadIdx.x == 0) *next_item = -1;
+        const Item itm = items[it];
+        os_solve<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW, OsDefault<NP>::CH>(itm, pools, am, part, refl, sh);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kOsMultiNT) void os_svd_multi_kernel(const Item* __restrict__ items,
+                                                                  const int* count_ptr, int* counter, Pools pools,
+                                                                  double* gws, int64_t gws_stride) {
+    extern __shared__ double dyn_lds[];
+    __shared__ OsShared sh;
+    __shared__ int next_item;
+    double* refl = gws + (size_t)blockIdx.x * gws_stride;
+    double* part = dyn_lds;
+    double* am = dyn_lds + 2 * kOsMultiNT;
+    if (threadIdx.x == 0) next_item = -1;
+    __syncthreads();
+    os_multi_drain<128>(items, count_ptr[1], counter, pools, am, part, refl, sh, &next_item);
+    os_multi_drain<112>(items, count_ptr[2], counter, pools, am, part, refl, sh, &next_item);
+    os_multi_drain<96>(items, count_ptr[3], counter, pools, am, part, refl, sh, &next_item);
+    os_multi_drain<80>(items, count_ptr[4], counter, pools, am, part, refl, sh, &next_item);
+}
+
+hipError_t launch_onesided_multi(const SmallLaunch& L, hipStream_t st) {
+    if (L.lds_bytes > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&os_svd_multi_kernel),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(os_svd_multi_kernel, dim3(L.grid), dim3(kOsMultiNT), L.lds_bytes, st, L.items, L.count_ptr,
+                       L.counter, L.pools, L.gws, L.gws_stride);
+    return hipGetLastError();
+}
+
+int onesided_multi_blocks_per_cu(size_t lds) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, os_svd_multi_kernel, kOsMultiNT, lds) != hipSuccess)
+        return 1;
+    return nb > 0 ? nb : 1;
 }
 
 #define LSVD_OS_SIZES(X) X(16) X(32) X(48) X(64) X(80) X(96) X(112) X(128)
@@ -760,7 +889,7 @@ int os_threads(int np) {
     switch (np) {
 #define X(np_) \
     case np_:  \
-        return OsCfg<np_, OsDefault<np_>::TPR, OsDefault<np_>::NW>::NT;
+        return OsDefCfg<np_>::NT;
         LSVD_OS_SIZES(X)
 #undef X
     }
@@ -781,7 +910,7 @@ int64_t os_slot_doubles(int np, int M) {
 
 template <int NP>
 static hipError_t os_launch(const SmallLaunch& L, hipStream_t st) {
-    constexpr int NT = OsCfg<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW>::NT;
+    constexpr int NT = OsDefCfg<NP>::NT;
     if (L.lds_bytes > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&os_svd_kernel<NP>),
                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds_bytes);
@@ -794,7 +923,7 @@ static hipError_t os_launch(const SmallLaunch& L, hipStream_t st) {
 
 template <int NP>
 static int os_blocks(size_t lds) {
-    constexpr int NT = OsCfg<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW>::NT;
+    constexpr int NT = OsDefCfg<NP>::NT;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, os_svd_kernel<NP>, NT, lds) != hipSuccess) return 1;
     return nb > 0 ? nb : 1;

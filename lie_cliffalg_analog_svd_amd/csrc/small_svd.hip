@@ -508,6 +508,7 @@ const SmallVariant* small_variants(int* count) {
 }
 
 hipError_t launch_small_variant(int id, const SmallLaunch& L, hipStream_t st) {
+    if (id >= kOsMulti) return launch_onesided_multi(L, st);
     if (id >= kOsBase) return launch_onesided(id - kOsBase, L, st);
     if (id >= kV2Wide) return launch_small2(id - kV2Wide, true, L, st);
     if (id >= kV2Multi) return launch_small2_multi(L, st);
@@ -523,6 +524,7 @@ hipError_t launch_small_variant(int id, const SmallLaunch& L, hipStream_t st) {
 }
 
 int small_variant_blocks_per_cu(int id, size_t lds) {
+    if (id >= kOsMulti) return onesided_multi_blocks_per_cu(lds);
     if (id >= kOsBase) return onesided_blocks_per_cu(id - kOsBase, lds);
     if (id >= kV2Wide) return small2_blocks_per_cu(id - kV2Wide, true, lds);
     if (id >= kV2Multi) return small2_multi_blocks_per_cu(lds);

@@ -167,12 +167,23 @@ def test_onesided_nonfinite_input_propagates():
     assert_factors(good, u, s, vt, 1e-11, "finite neighbour")
 
 
-def test_onesided_default_split_routes_large_sizes_to_newton_schulz():
-    """Under the default handle (os_max_np = 64) padded sizes up to 64 take
-    the one-sided kernels (path 4) and larger ones the scaled Newton-Schulz
+def test_onesided_split_routes_large_sizes_to_newton_schulz():
+    """With the split at 64 (LSVD_OS_MAX_NP) padded sizes up to 64 take the
+    one-sided kernels (path 4) and larger ones the scaled Newton-Schulz
     kernels (path 0); both meet the bar."""
+    import os
+    from lie_cliffalg_analog_svd_amd import Handle
+    old = os.environ.get("LSVD_OS_MAX_NP")
+    os.environ["LSVD_OS_MAX_NP"] = "64"
+    try:
+        h = Handle(0)
+    finally:
+        if old is None:
+            del os.environ["LSVD_OS_MAX_NP"]
+        else:
+            os.environ["LSVD_OS_MAX_NP"] = old
     mats = [uniform_matrix(950 + i, m, n) for i, (m, n) in enumerate([(70, 60), (100, 100), (40, 33), (128, 90)])]
-    out, info = svd_batched(mats, precision=OS, with_info=True)
+    out, info = svd_batched(mats, precision=OS, handle=h, with_info=True)
     paths = [d["path"] for d in info]
     assert paths == [4, 0, 4, 0], paths
     for a, (u, s, vt) in zip(mats, out):

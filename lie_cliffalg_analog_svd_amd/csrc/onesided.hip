@@ -135,8 +135,8 @@ struct OsShared {
     double red[2 * 16];
     int rank[128];
     double sig[128];     // normalised singular values by index
-    double tw[128];      // tau * w per column (Householder steps)
-    double vbuf[128];    // a reflector staged for the application of Q
+    double tw[128];      // tau scal^2 per column (the QR's update coefficient)
+    double scal[128];    // 1 / (alpha - beta) per column (reflector tails = scal x)
     double tau[128];
     int item;
     int flag;
@@ -239,180 +239,189 @@ __device__ __forceinline__ double os_rsq(double x) {
 // (src/lie_svd_small.rs:226-243 in the form of jacobi_angle): columns
 // (x_p, x_q) -> (c x_p - s x_q, s x_p + c x_q) makes them orthogonal.  Its
 // inputs are scaled by a power of two so that max(|b - a|, |2 g|) is in
-// [0.5, 1); r and 1 / h from reciprocal square roots.
-__device__ __forceinline__ void os_angle(double a, double b, double g, double& c, double& s) {
+// [0.5, 1); r and 1 / h from reciprocal square roots.  With WANT_T it also
+// returns t = s / c (from the hardware reciprocal refined by two Newton
+// steps): the rotated Gram diagonal is then a - t g (the column c x_p - s x_q)
+// and b + t g (s x_p + c x_q).
+template <bool WANT_T = false>
+__device__ __forceinline__ void os_angle(double a, double b, double g, double& c, double& s, double* t = nullptr) {
     const double d = b - a, a2 = 2.0 * g;
     const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(d), fabs(a2)));
     const double ds = ldexp(d, -ex), as = fabs(ldexp(a2, -ex));
     const double x = fma(ds, ds, as * as);  // in [0.25, 2)
     const double r = x * os_rsq(x);
-    const double den = fabs(ds) + r;
+    const double den = fabs(ds) + r;  // in [0.5, 2.5)
     const double sgn = (d == 0.0 || ((d > 0.0) == (a2 > 0.0))) ? 1.0 : -1.0;
     const double ih = os_rsq(fma(den, den, as * as));  // in [0.25, 6)
     c = den * ih;
     s = sgn * as * ih;
+    if constexpr (WANT_T) {
+        double y = __builtin_amdgcn_rcp(den);
+        y = y * fma(-den, y, 2.0);
+        y = y * fma(-den, y, 2.0);
+        *t = sgn * as * y;
+    }
 }
 
-// ---- Householder QR and the application of Q (in LDS) --------------------
+// Incremental column norms: a rotated column whose squared norm falls below
+// this fraction of its old value has lost that many bits to cancellation,
+// so the next two layers form the Gram diagonal from the columns again.
+constexpr double kOsRefresh = 0.0625;
 
-// QR of the M x N matrix in `a` (leading dimension lda), in place: R on and
-// above the diagonal, reflector tails (LAPACK dlarfg form, v_j = 1 implied)
-// below, tau in sh.tau.  Thread t works on column t % NPC, rows
-// t / NPC + K i (K = NT / NPC row chunks).  Two barriers per column: the
-// column sums of the reflector with the trailing columns, then the update,
-// which also forms the next column's squared norm (per chunk) and scales
-// the previous reflector's tail.  `part` holds 2 NT doubles.
-template <int NT, int NPC>
-__device__ void os_householder(int M, int N, double* a, int lda, OsShared& sh, double* part) {
-    const int tid = tix();
-    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;  // threads from K NPC on idle
-    const int col = tid % NPC, chunk = tid / NPC;
-    const bool act = chunk < K;
-    double* npart = part + (K + 1) * NPC;  // [2][K]: squared norms of the next column, per chunk
-    if (col == 0 && act) {
-        double s = 0.0;
-        for (int i = 1 + chunk; i < M; i += K) s = fma(a[(size_t)i * lda], a[(size_t)i * lda], s);
-        npart[chunk] = s;
-    }
-    __syncthreads();
-    double prev_scal = 1.0;
+// ---- Householder QR and the application of Q (registers) ------------------
+
+// Column-resident layout of an up to 128-row matrix over the workgroup:
+// thread t holds column t / 4 (of NT / 4 >= NP column slots), rows
+// t % 4 + 4 i (i < 32), so the four threads of a column are one quad of a
+// wave and its sums are quad reductions (DPP), never LDS round trips.
+constexpr int kOsQK = 4;               // threads per column
+constexpr int kOsQR = 128 / kOsQK;     // row slots per thread
+constexpr int kOsRows = 128;           // rows of the LDS matrix area
+
+template <int CTRL>
+__device__ __forceinline__ double os_dpp(double x) {
+    return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true),
+                            __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true));
+}
+
+// Sum over the four lanes of a quad (quad_perm [1,0,3,2], then [2,3,0,1]);
+// every lane gets the same bits.
+__device__ __forceinline__ double os_quad_sum(double v) {
+    v = v + os_dpp<0xB1>(v);
+    return v + os_dpp<0x4E>(v);
+}
+
+// Householder QR of the M x N matrix held in registers (a[i] = row
+// t % 4 + 4 i of column t / 4), in the LAPACK dlarfg convention: H_j = I -
+// tau_j v_j v_j^T, v_j = 1 at row j.  On return `am` (leading dimension
+// lda) holds R on and above the diagonal and the reflector tails v_j below
+// it, sh.tau the tau_j.
+//
+// Step j needs the reflector of column j in every thread.  The quad of
+// column j writes the column into vb[j % 2] (128 rows), reads alpha = a_jj
+// back, zeroes the rows <= j (the whole wave, after the quad's stores, in
+// one wave's program order), forms ||x||^2 of the rows > j from the buffer,
+// then beta, tau and 1 / (alpha - beta) = scal, and stores alpha - beta at
+// row j.  With that buffer w (w_r = 0 for r < j, alpha - beta at j, x_r
+// below), y_c = sum_r w_r a_rc = (alpha - beta) v_j^T a_c, and column c's
+// update is a_c -= tau scal^2 y_c w: one pass over the column's registers
+// for the sum, one for the update, no masks.  The quad of column j + 1 builds
+// the next buffer right after its own update, so a column costs one barrier.
+template <int NT>
+__device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda, double* vb, OsShared& sh) {
+    const int tid = tix(), c = tid >> 2, ch = tid & 3, lane = tid & 63;
+    // the reflector buffer of column p, built by the wave that holds it
+    auto pivot = [&](int p) {
+        if ((tid >> 6) != (p >> 4)) return;  // wave-uniform
+        double* w = vb + (p & 1) * kOsRows;
+        if (c == p)
+#pragma unroll
+            for (int i = 0; i < kOsQR; ++i) w[ch + kOsQK * i] = a[i];
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        const double alpha = w[p];
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (lane <= p) w[lane] = 0.0;
+        if (lane + 64 <= p) w[lane + 64] = 0.0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (c == p) {
+            double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+            for (int i = 0; i < kOsQR; i += 2) {
+                const double x0 = w[ch + kOsQK * i], x1 = w[ch + kOsQK * (i + 1)];
+                s0 = fma(x0, x0, s0);
+                s1 = fma(x1, x1, s1);
+            }
+            const double xnorm2 = os_quad_sum(s0 + s1);
+            double tau = 0.0, beta = alpha, scal = 0.0;
+            if (xnorm2 > 0.0) {
+                beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
+                tau = (beta - alpha) / beta;
+                scal = 1.0 / (alpha - beta);
+            }
+            if (ch == 0) {
+                w[p] = alpha - beta;
+                sh.tau[p] = tau;
+                sh.tw[p] = tau * scal * scal;
+                sh.scal[p] = scal;
+                am[(size_t)p * lda + p] = beta;
+            }
+        }
+    };
+    if (N > 0) pivot(0);
     for (int j = 0; j < N; ++j) {
-        double xnorm2 = 0.0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) xnorm2 += npart[(j & 1) * K + k];
-        const double alpha = a[(size_t)j * lda + j];
-        double tau = 0.0, beta = alpha, scal = 0.0;
-        if (xnorm2 > 0.0) {
-            beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
-            tau = (beta - alpha) / beta;
-            scal = 1.0 / (alpha - beta);
-        }
-        // column sums sum_{i>j} a[i][j] a[i][col] (and row j of col, kept
-        // unmodified for the update in part[K][col])
-        if (col > j && col < N && act) {
-            // four rows per round, loads first (the round trips overlap)
-            double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            int i = j + 1 + chunk;
-            for (; i + 3 * K < M; i += 4 * K) {
-                const double* r0 = a + (size_t)i * lda;
-                const double p0 = r0[j], q0 = r0[col], p1 = r0[K * lda + j], q1 = r0[K * lda + col];
-                const double p2 = r0[2 * K * lda + j], q2 = r0[2 * K * lda + col];
-                const double p3 = r0[3 * K * lda + j], q3 = r0[3 * K * lda + col];
-                s0 = fma(p0, q0, s0);
-                s1 = fma(p1, q1, s1);
-                s2 = fma(p2, q2, s2);
-                s3 = fma(p3, q3, s3);
-            }
-            for (; i < M; i += K) s0 = fma(a[(size_t)i * lda + j], a[(size_t)i * lda + col], s0);
-            part[chunk * NPC + col] = (s0 + s1) + (s2 + s3);
-            if (chunk == 0) part[K * NPC + col] = a[(size_t)j * lda + col];
-        }
-        if (tid == 0) sh.tau[j] = tau;
         __syncthreads();
-        if (col > j && col < N && act) {
-            double sum = 0.0;
+        const double* w = vb + (j & 1) * kOsRows;
+        // the tail of reflector j into am's lower part (v = scal x)
+        for (int r = j + 1 + tid; r < M; r += NT) am[(size_t)r * lda + j] = w[r] * sh.scal[j];
+        if (c > j && c < N) {  // quad-uniform
+            double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
-            for (int k = 0; k < K; ++k) sum += part[k * NPC + col];
-            const double twc = tau * (part[K * NPC + col] + scal * sum);
-            double nn = 0.0;
-            int i = j + chunk;
-            if (chunk == 0) {  // row j: v_j = 1
-                a[(size_t)j * lda + col] -= twc;
-                i += K;
+            for (int i = 0; i < kOsQR; i += 4) {
+                y0 = fma(w[ch + kOsQK * i], a[i], y0);
+                y1 = fma(w[ch + kOsQK * (i + 1)], a[i + 1], y1);
+                y2 = fma(w[ch + kOsQK * (i + 2)], a[i + 2], y2);
+                y3 = fma(w[ch + kOsQK * (i + 3)], a[i + 3], y3);
             }
-            for (; i + 3 * K < M; i += 4 * K) {
-                double* r0 = a + (size_t)i * lda;
-                const double v0 = r0[j], v1 = r0[K * lda + j], v2 = r0[2 * K * lda + j], v3 = r0[3 * K * lda + j];
-                const double x0 = r0[col], x1 = r0[K * lda + col], x2 = r0[2 * K * lda + col], x3 = r0[3 * K * lda + col];
-                const double n0 = fma(-scal * v0, twc, x0), n1 = fma(-scal * v1, twc, x1);
-                const double n2 = fma(-scal * v2, twc, x2), n3 = fma(-scal * v3, twc, x3);
-                r0[col] = n0;
-                r0[K * lda + col] = n1;
-                r0[2 * K * lda + col] = n2;
-                r0[3 * K * lda + col] = n3;
-                nn = fma(i > j + 1 ? n0 : 0.0, n0, nn);
-                nn = fma(i + K > j + 1 ? n1 : 0.0, n1, nn);
-                nn = fma(n2, n2, nn);
-                nn = fma(n3, n3, nn);
-            }
-            for (; i < M; i += K) {
-                const double nv = fma(-scal * a[(size_t)i * lda + j], twc, a[(size_t)i * lda + col]);
-                a[(size_t)i * lda + col] = nv;
-                if (i > j + 1) nn = fma(nv, nv, nn);
-            }
-            if (col == j + 1) npart[((j + 1) & 1) * K + chunk] = nn;
-        } else if (col == j && act) {
-            if (chunk == 0) a[(size_t)j * lda + j] = beta;
-            if (j > 0)  // the previous reflector's tail (no longer read)
-                for (int i = j + chunk; i < M; i += K) a[(size_t)i * lda + j - 1] *= prev_scal;
+            const double k = sh.tw[j] * os_quad_sum((y0 + y1) + (y2 + y3));
+#pragma unroll
+            for (int i = 0; i < kOsQR; ++i) a[i] = fma(-k, w[ch + kOsQK * i], a[i]);
         }
-        __syncthreads();
-        prev_scal = scal;
+        if (j + 1 < N) pivot(j + 1);
     }
-    if (N > 0 && col == 0 && act)  // the last reflector's tail
-        for (int i = N + chunk; i < M; i += K) a[(size_t)i * lda + N - 1] *= prev_scal;
     __syncthreads();
+    // R above the diagonal (the diagonal is in am already)
+#pragma unroll
+    for (int i = 0; i < kOsQR; ++i) {
+        const int r = ch + kOsQK * i;
+        if (r < c && c < N) am[(size_t)r * lda + c] = a[i];
+    }
 }
 
-// Y (M x ncol in `y`, leading dimension ldy) <- H_0 H_1 ... H_{N-1} Y, the
-// reflectors' tails in global memory (refl: M x ldr, row-major) and tau in
-// sh.tau; three barriers per reflector.
-template <int NT, int NPC>
-__device__ void os_apply_q(int M, int N, int ncol, const double* refl, int ldr, double* y, int ldy, OsShared& sh,
-                           double* part) {
-    const int tid = tix();
-    constexpr int K = NT / NPC > 0 ? NT / NPC : 1;  // threads from K NPC on idle
-    const int col = tid % NPC, chunk = tid / NPC;
-    const bool act = chunk < K;
-    for (int j = N - 1; j >= 0; --j) {
-        const double tau = sh.tau[j];
-        if (tau == 0.0) continue;  // uniform
-        // the reflector, staged once per column (rows > j)
-        for (int i = j + 1 + tid; i < M; i += NT) sh.vbuf[i] = refl[(size_t)i * ldr + j];
-        __syncthreads();
-        if (col < ncol && act) {
-            double s0 = chunk == 0 ? y[(size_t)j * ldy + col] : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-            int i = j + (chunk == 0 ? K : chunk);
-            for (; i + 3 * K < M; i += 4 * K) {
-                const double* r0 = y + (size_t)i * ldy;
-                const double x0 = r0[col], x1 = r0[K * ldy + col], x2 = r0[2 * K * ldy + col], x3 = r0[3 * K * ldy + col];
-                s0 = fma(sh.vbuf[i], x0, s0);
-                s1 = fma(sh.vbuf[i + K], x1, s1);
-                s2 = fma(sh.vbuf[i + 2 * K], x2, s2);
-                s3 = fma(sh.vbuf[i + 3 * K], x3, s3);
-            }
-            for (; i < M; i += K) s0 = fma(sh.vbuf[i], y[(size_t)i * ldy + col], s0);
-            part[chunk * NPC + col] = (s0 + s1) + (s2 + s3);
-        }
-        __syncthreads();
-        if (col < ncol && act) {
-            double sum = 0.0;
+// U = H_0 H_1 ... H_{N-1} [W; 0] for the M x N result: on entry am's first N
+// rows hold W (leading dimension lda); the reflector tails are read from
+// refl (rows > j of column j, leading dimension NP).  The columns of U are
+// independent, so after the reflectors are staged in am (zero above row j,
+// 1 at row j) each column's quad applies all of them without a barrier.
+// The result stays in u (the same layout as the QR's registers).
+template <int NT, int NP>
+__device__ void os_apply_q_regs(int M, int N, double (&u)[kOsQR], double* am, int lda, const double* refl,
+                                const OsShared& sh) {
+    const int tid = tix(), c = tid >> 2, ch = tid & 3;
 #pragma unroll
-            for (int k = 0; k < K; ++k) sum += part[k * NPC + col];
-            const double twc = tau * sum;
-            int i = j + chunk;
-            if (chunk == 0) {
-                y[(size_t)j * ldy + col] -= twc;
-                i += K;
+    for (int i = 0; i < kOsQR; ++i) {
+        const int r = ch + kOsQK * i;
+        u[i] = (r < N && c < N) ? am[(size_t)r * lda + c] : 0.0;
+    }
+    __syncthreads();
+    for (int e = tid; e < kOsRows * NP; e += NT) {
+        const int r = e / NP, j = e % NP;  // refl: leading dimension NP
+        if (j < N) am[(size_t)r * lda + j] = r > j ? (r < M ? refl[(size_t)r * NP + j] : 0.0) : (r == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (c < N) {  // quad-uniform
+        for (int j = N - 1; j >= 0; --j) {
+            const double tau = sh.tau[j];
+            if (tau == 0.0) continue;  // uniform
+            const double* vj = am + (size_t)ch * lda + j;  // row ch + 4 i: vj[4 i lda]
+            double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+#pragma unroll
+            for (int i = 0; i < kOsQR; i += 4) {
+                y0 = fma(vj[(size_t)kOsQK * i * lda], u[i], y0);
+                y1 = fma(vj[(size_t)kOsQK * (i + 1) * lda], u[i + 1], y1);
+                y2 = fma(vj[(size_t)kOsQK * (i + 2) * lda], u[i + 2], y2);
+                y3 = fma(vj[(size_t)kOsQK * (i + 3) * lda], u[i + 3], y3);
             }
-            for (; i + 3 * K < M; i += 4 * K) {
-                double* r0 = y + (size_t)i * ldy;
-                const double x0 = r0[col], x1 = r0[K * ldy + col], x2 = r0[2 * K * ldy + col], x3 = r0[3 * K * ldy + col];
-                r0[col] = fma(-sh.vbuf[i], twc, x0);
-                r0[K * ldy + col] = fma(-sh.vbuf[i + K], twc, x1);
-                r0[2 * K * ldy + col] = fma(-sh.vbuf[i + 2 * K], twc, x2);
-                r0[3 * K * ldy + col] = fma(-sh.vbuf[i + 3 * K], twc, x3);
-            }
-            for (; i < M; i += K) y[(size_t)i * ldy + col] = fma(-sh.vbuf[i], twc, y[(size_t)i * ldy + col]);
+            const double k = tau * os_quad_sum((y0 + y1) + (y2 + y3));
+#pragma unroll
+            for (int i = 0; i < kOsQR; ++i) u[i] = fma(-k, vj[(size_t)kOsQK * i * lda], u[i]);
         }
-        __syncthreads();
     }
 }
 
 // ---- the per-matrix solver -----------------------------------------------
 
 template <int NP, int TPR, int NW, int CHH>
-__device__ void os_solve(const Item& itm, const Pools& P, double* am, double* part, double* refl, OsShared& sh) {
+__device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb, double* refl, OsShared& sh) {
     using C = OsCfg<NP, TPR, NW, CHH>;
     constexpr int NL = C::NL;
     constexpr int NT = C::NT, CH = C::CH, PC = C::P, PP = C::PP, XR = C::XR, LDA = C::LDA;
@@ -422,10 +431,10 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
     const int m0 = itm.m, n0 = itm.n;
     const bool wide = m0 < n0;
     const int M = wide ? n0 : m0, N = wide ? m0 : n0;
-    const int MP = (M + 15) & ~15;
     const double* A = P.a + itm.a_off;
     const View ain{A, n0, wide ? 1 : 0};
     lsvd_info_t info = {};
+    double sc = 0.0;  // ||A||_F
     struct Clk {
         long long* out;
         long long t;
@@ -438,27 +447,30 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
         }
     } clk{P.timing ? P.timing + (size_t)itm.idx * kPhases : nullptr, P.timing && threadIdx.x == 0 ? clock64() : 0};
 
-    // ---- load A / ||A||_F (zero padded to MP x NP) and QR
-    for (int e = tid; e < MP * NP; e += NT) {
-        const int i = e / NP, j = e % NP;
-        am[i * LDA + j] = (i < M && j < N) ? ain(i, j) : 0.0;
+    // ---- A / ||A||_F into the column-resident registers, QR
+    {
+        static_assert(NT / kOsQK >= NP, "a column slot per column");
+        const int qc = tid >> 2, qch = tid & 3;
+        double a[kOsQR];
+        double ss = 0.0;
+#pragma unroll
+        for (int i = 0; i < kOsQR; ++i) {
+            const int r = qch + kOsQK * i;
+            a[i] = (r < M && qc < N) ? ain(r, qc) : 0.0;
+            ss = fma(a[i], a[i], ss);
+        }
+        sc = fmax(sqrt(block_sum<NT>(ss, sh.red)), 1e-300);
+        const double isc = 1.0 / sc;
+#pragma unroll
+        for (int i = 0; i < kOsQR; ++i) a[i] *= isc;
+        os_qr_regs<NT>(M, N, a, am, LDA, vb, sh);
     }
-    double sc = 0.0;
     __syncthreads();
-    for (int e = tid; e < MP * NP; e += NT) {
-        const double x = am[(e / NP) * LDA + e % NP];
-        sc = fma(x, x, sc);
-    }
-    sc = fmax(sqrt(block_sum<NT>(sc, sh.red)), 1e-300);
-    const double isc = 1.0 / sc;
-    for (int e = tid; e < MP * NP; e += NT) am[(e / NP) * LDA + e % NP] *= isc;
-    __syncthreads();
-    os_householder<NT, NP>(M, N, am, LDA, sh, part);
     clk.mark(kPhQR);
     // reflector tails to the global slot (read back when U = Q W is formed)
-    for (int e = tid; e < MP * NP; e += NT) {
+    for (int e = tid; e < M * NP; e += NT) {
         const int i = e / NP, j = e % NP;
-        if (i > j && i < M && j < N) refl[(size_t)i * NP + j] = am[i * LDA + j];
+        if (i > j && j < N) refl[(size_t)i * NP + j] = am[i * LDA + j];
     }
 
     // ---- registers: X = R^T (row r, position g holds R[g][r]), W = I
@@ -485,26 +497,54 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
     // lane bits that the reduce-scatter did not split (replicated sums)
     const int rep_mask = (RGW - 1) & ~((SC >= 1 ? (RGW >> 1) : 0) | (SC >= 2 ? (RGW >> 2) : 0));
     const bool rep_lane = (rgw & rep_mask) == 0 && w == 0;
+    // Two positions per lane (one pair per layer, its angle formed by the
+    // lane itself, replicated over rowgroups and waves): the squared column
+    // norms travel with the columns in registers (nr0, nr1 for the lane's
+    // positions 0 and 1) and are updated from each rotation, so a layer
+    // reduces only g.  The Gram diagonal is formed from the columns again on
+    // the first layer of every sweep (an even layer: every position is in a
+    // pair) and on the two layers after a rotation whose norm update lost
+    // more than kOsRefresh of a column.  The decisions are the same in every
+    // wave (the reduced sums are bitwise equal there), so the layer's branch
+    // is uniform.
+    constexpr bool INC = CH == 2;
+    double nr0 = 0.0, nr1 = 0.0;
+    bool refresh = false;
 
-    // one layer; ODD: pairs (2j+1, 2j+2) in the lane and (CH-1, next lane's 0)
-    auto layer = [&](auto odd_tag) {
+    // one layer; ODD: pairs (2j+1, 2j+2) in the lane and (CH-1, next lane's 0);
+    // FULL: all three Gram entries from the columns (always, unless INC)
+    auto layer = [&](auto odd_tag, const bool full_in) {
         constexpr bool ODD = decltype(odd_tag)::value;
+        const bool full = !INC || full_in;
         auto lo = [](int j) { return ODD ? 2 * j + 1 : 2 * j; };
         // partial Gram entries of the lane's pairs over its rows
         double v[PP][3];
 #pragma unroll
         for (int j = 0; j < PP; ++j) v[j][0] = v[j][1] = v[j][2] = 0.0;
+        if (full) {
 #pragma unroll
-        for (int k = 0; k < XR; ++k) {
-            // the straddling pair's partner: the next lane's first position
-            const double xn = ODD ? os_next0(x[k][0]) : 0.0;
+            for (int k = 0; k < XR; ++k) {
+                // the straddling pair's partner: the next lane's first position
+                const double xn = ODD ? os_next0(x[k][0]) : 0.0;
 #pragma unroll
-            for (int j = 0; j < PC; ++j) {
-                const double p = x[k][lo(j)];
-                const double q = (ODD && j == PC - 1) ? xn : x[k][lo(j) + 1];
-                v[j][0] = fma(p, p, v[j][0]);
-                v[j][1] = fma(q, q, v[j][1]);
-                v[j][2] = fma(p, q, v[j][2]);
+                for (int j = 0; j < PC; ++j) {
+                    const double p = x[k][lo(j)];
+                    const double q = (ODD && j == PC - 1) ? xn : x[k][lo(j) + 1];
+                    v[j][0] = fma(p, p, v[j][0]);
+                    v[j][1] = fma(q, q, v[j][1]);
+                    v[j][2] = fma(p, q, v[j][2]);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < XR; ++k) {
+                const double xn = ODD ? os_next0(x[k][0]) : 0.0;
+#pragma unroll
+                for (int j = 0; j < PC; ++j) {
+                    const double p = x[k][lo(j)];
+                    const double q = (ODD && j == PC - 1) ? xn : x[k][lo(j) + 1];
+                    v[j][2] = fma(p, q, v[j][2]);
+                }
             }
         }
         // reduce-scatter over lane bit 5, then bit 4: pair index owned
@@ -536,6 +576,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
         for (int j = 0; j < NA; ++j)
 #pragma unroll
             for (int t = 0; t < 3; ++t) {
+                if (t < 2 && !full) continue;  // uniform
                 double s = v[j][t];
                 if constexpr (SC < 1 && LB >= 1) s = bfly32(s);
                 if constexpr (SC < 2 && LB >= 2) s = bfly16(s);
@@ -551,8 +592,10 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
                 const int pr = vch * PC + kid + j;
                 if (kid + j < PC) {
                     double* d = pb + (size_t)w * 3 * C::S + pr;
-                    d[0] = v[j][0];
-                    d[C::S] = v[j][1];
+                    if (full) {
+                        d[0] = v[j][0];
+                        d[C::S] = v[j][1];
+                    }
                     d[2 * C::S] = v[j][2];
                 }
             }
@@ -561,22 +604,35 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
             for (int j = 0; j < NA; ++j) {
                 const int pr = vch * PC + (kid + j < PC ? kid + j : 0);
                 double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                if (full) {
 #pragma unroll
-                for (int ww = 0; ww < NW; ++ww) {
-                    const double* d = pb + (size_t)ww * 3 * C::S + pr;
-                    s0 += d[0];
-                    s1 += d[C::S];
-                    s2 += d[2 * C::S];
+                    for (int ww = 0; ww < NW; ++ww) {
+                        const double* d = pb + (size_t)ww * 3 * C::S + pr;
+                        s0 += d[0];
+                        s1 += d[C::S];
+                        s2 += d[2 * C::S];
+                    }
+                } else {
+#pragma unroll
+                    for (int ww = 0; ww < NW; ++ww) s2 += pb[(size_t)ww * 3 * C::S + 2 * C::S + pr];
                 }
                 v[j][0] = s0;
                 v[j][1] = s1;
                 v[j][2] = s2;
             }
         }
+        if constexpr (INC) {  // the Gram diagonal from the norms the columns carry
+            const double nb = ODD ? os_next0(nr0) : nr1;
+            if (!full) {
+                v[0][0] = ODD ? nr1 : nr0;
+                v[0][1] = nb;
+            }
+        }
         // angles of the owned pairs
         double cs[PP][2];
         int nrot = 0;
-        bool any = false;
+        bool any = false, lost = false;
+        double nlo = 0.0, nhi = 0.0;  // INC: the pair's squared norms after the layer (low, high position)
 #pragma unroll
         for (int j = 0; j < NA; ++j) {
             const int jk = kid + j;
@@ -585,7 +641,16 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
             const double a = v[j][0], b = v[j][1], g = v[j][2];
             const bool rot = valid && g * g > tol2 * (a * b);
             double c = 1.0, s = 0.0;
-            if (rot) os_angle(a, b, g, c, s);
+            if constexpr (INC) {
+                double t = 0.0;
+                if (rot) os_angle<true>(a, b, g, c, s, &t);
+                const double tg = rot ? t * g : 0.0;
+                nlo = boundary_dead ? a : b + tg;
+                nhi = a - tg;
+                lost = rot && (nhi < kOsRefresh * a || nlo < kOsRefresh * b);
+            } else {
+                if (rot) os_angle(a, b, g, c, s);
+            }
             if (boundary_dead) {
                 c = 0.0;
                 s = 1.0;
@@ -594,6 +659,17 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
             cs[j][1] = s;
             nrot += (rot && rep_lane) ? 1 : 0;
             any = any || rot;
+        }
+        if constexpr (INC) {  // the norms trade with the columns
+            if constexpr (ODD) {
+                const double ph = os_prev0(nhi);  // the previous lane's pair's high position is this lane's 0
+                nr1 = nlo;
+                if (!firstv) nr0 = ph;
+            } else {
+                nr0 = nlo;
+                nr1 = nhi;
+            }
+            refresh = __any(lost);
         }
         // gather the (c, s) of all the lane's pairs (reverse of the scatter)
         if constexpr (SC >= 2) {
@@ -660,14 +736,19 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
     };
 
     if (N > 1) {
+        int since = 0, full_left = 0;  // layers since the sweep's first; full layers still owed
         while (quiet < NP && G < maxG) {
-            const bool a0 = layer(std::integral_constant<bool, false>{});
+            const bool a0 = layer(std::integral_constant<bool, false>{}, since == 0 || full_left > 0);
+            full_left = refresh ? 2 : (full_left > 0 ? full_left - 1 : 0);
             ++G;
             quiet = a0 ? 0 : quiet + 1;
             if (quiet >= NP) break;
-            const bool a1 = layer(std::integral_constant<bool, true>{});
+            const bool a1 = layer(std::integral_constant<bool, true>{}, full_left > 0);
+            full_left = refresh ? 2 : (full_left > 0 ? full_left - 1 : 0);
             ++G;
             quiet = a1 ? 0 : quiet + 1;
+            since += 2;
+            if (since >= NP) since = 0;
         }
     }
     __syncthreads();  // every wave's last partial-sum reads are done before am is reused
@@ -753,9 +834,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
     __syncthreads();
     clk.mark(kPhU);
 
-    // ---- U = Q [W; 0] in sorted column order
-    for (int e = tid; e < MP * NP; e += NT) am[(e / NP) * LDA + e % NP] = 0.0;
-    __syncthreads();
+    // ---- U = Q [W; 0] in sorted column order (W: am's first N rows)
 #pragma unroll
     for (int k = 0; k < XR; ++k) {
         const int r = rg * XR + k;
@@ -767,16 +846,20 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* pa
         }
     }
     __syncthreads();
-    os_apply_q<NT, NP>(M, N, N, refl, NP, am, LDA, sh, part);
     {
-        double* u = wide ? P.vt + itm.vt_off : P.u + itm.u_off;  // wide: (N x M) = U_t^T
-        for (int e = tid; e < MP * NP; e += NT) {
-            const int i = e / NP, j = e % NP;
-            if (i < M && j < N) {
-                if (!wide) u[(size_t)i * N + j] = am[i * LDA + j];
-                else u[(size_t)j * M + i] = am[i * LDA + j];
+        double u[kOsQR];
+        os_apply_q_regs<NT, NP>(M, N, u, am, LDA, refl, sh);
+        const int qc = tid >> 2, qch = tid & 3;
+        double* ud = wide ? P.vt + itm.vt_off : P.u + itm.u_off;  // wide: (N x M) = U_t^T
+        if (qc < N)
+#pragma unroll
+            for (int i = 0; i < kOsQR; ++i) {
+                const int r = qch + kOsQK * i;
+                if (r < M) {
+                    if (!wide) ud[(size_t)r * N + qc] = u[i];
+                    else ud[(size_t)qc * M + r] = u[i];
+                }
             }
-        }
     }
     clk.mark(kPhOut);
     rots = (int)block_sum<NT>((double)rots, sh.red);
@@ -799,10 +882,10 @@ __global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict_
     __shared__ OsShared sh;
     const int count = *count_ptr;
     double* refl = gws + (size_t)blockIdx.x * gws_stride;
-    // LDS: the column partial sums of the Householder steps (2 NT doubles),
-    // then the matrix area (MP x LDA; the sweeps' partial sums use it too)
-    double* part = dyn_lds;
-    double* am = dyn_lds + 2 * C::NT;
+    // LDS: the QR's two reflector buffers (2 x 128 doubles), then the matrix
+    // area (128 x LDA; the sweeps' partial sums use it too)
+    double* vb = dyn_lds;
+    double* am = dyn_lds + 2 * kOsRows;
     for (;;) {
         if (threadIdx.x == 0) sh.item = atomicAdd(counter, 1);
         __syncthreads();
@@ -810,7 +893,7 @@ __global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict_
         __syncthreads();
         if (it >= count) break;
         const Item itm = items[it];
-        os_solve<NP, TPR, NW, CHH>(itm, pools, am, part, refl, sh);
+        os_solve<NP, TPR, NW, CHH>(itm, pools, am, vb, refl, sh);
     }
 }
 
@@ -830,7 +913,7 @@ static_assert(OsDefCfg<80>::NT == kOsMultiNT && OsDefCfg<96>::NT == kOsMultiNT &
 
 template <int NP>
 __device__ __forceinline__ void os_multi_drain(const Item* items, int end, int* counter, const Pools& pools,
-                                               double* am, double* part, double* refl, OsShared& sh,
+                                               double* am, double* vb, double* refl, OsShared& sh,
                                                int* next_item) {
     for (;;) {
         if (*next_item < 0) {  // uniform: next_item is in LDS
@@ -843,7 +926,7 @@ __device__ __forceinline__ void os_multi_drain(const Item* items, int end, int* 
         __syncthreads();
         if (threadIdx.x == 0) *next_item = -1;
         const Item itm = items[it];
-        os_solve<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW, OsDefault<NP>::CH>(itm, pools, am, part, refl, sh);
+        os_solve<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW, OsDefault<NP>::CH>(itm, pools, am, vb, refl, sh);
         __syncthreads();
     }
 }
@@ -855,14 +938,14 @@ __global__ __launch_bounds__(kOsMultiNT) void os_svd_multi_kernel(const Item* __
     __shared__ OsShared sh;
     __shared__ int next_item;
     double* refl = gws + (size_t)blockIdx.x * gws_stride;
-    double* part = dyn_lds;
-    double* am = dyn_lds + 2 * kOsMultiNT;
+    double* vb = dyn_lds;
+    double* am = dyn_lds + 2 * kOsRows;
     if (threadIdx.x == 0) next_item = -1;
     __syncthreads();
-    os_multi_drain<128>(items, count_ptr[1], counter, pools, am, part, refl, sh, &next_item);
-    os_multi_drain<112>(items, count_ptr[2], counter, pools, am, part, refl, sh, &next_item);
-    os_multi_drain<96>(items, count_ptr[3], counter, pools, am, part, refl, sh, &next_item);
-    os_multi_drain<80>(items, count_ptr[4], counter, pools, am, part, refl, sh, &next_item);
+    os_multi_drain<128>(items, count_ptr[1], counter, pools, am, vb, refl, sh, &next_item);
+    os_multi_drain<112>(items, count_ptr[2], counter, pools, am, vb, refl, sh, &next_item);
+    os_multi_drain<96>(items, count_ptr[3], counter, pools, am, vb, refl, sh, &next_item);
+    os_multi_drain<80>(items, count_ptr[4], counter, pools, am, vb, refl, sh, &next_item);
 }
 
 hipError_t launch_onesided_multi(const SmallLaunch& L, hipStream_t st) {
@@ -897,9 +980,8 @@ int os_threads(int np) {
 }
 
 size_t os_lds_bytes(int np, int M) {
-    const int mp = (M + 15) & ~15;
-    const int rows = mp > np ? mp : np;
-    return sizeof(double) * (2 * (size_t)os_threads(np) + (size_t)rows * (np + 1));
+    (void)M;  // the matrix area always has 128 rows (the register layout's row slots)
+    return sizeof(double) * (2 * (size_t)kOsRows + (size_t)kOsRows * (np + 1));
 }
 
 int64_t os_slot_doubles(int np, int M) {

@@ -521,11 +521,15 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
         double v[PP][3];
 #pragma unroll
         for (int j = 0; j < PP; ++j) v[j][0] = v[j][1] = v[j][2] = 0.0;
+        // the straddling pair's partner: the next lane's first position (kept
+        // for the rotation of X)
+        double xnx[ODD ? XR : 1];
+#pragma unroll
+        for (int k = 0; k < (ODD ? XR : 1); ++k) xnx[k] = ODD ? os_next0(x[k][0]) : 0.0;
         if (full) {
 #pragma unroll
             for (int k = 0; k < XR; ++k) {
-                // the straddling pair's partner: the next lane's first position
-                const double xn = ODD ? os_next0(x[k][0]) : 0.0;
+                const double xn = ODD ? xnx[k] : 0.0;
 #pragma unroll
                 for (int j = 0; j < PC; ++j) {
                     const double p = x[k][lo(j)];
@@ -538,7 +542,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
         } else {
 #pragma unroll
             for (int k = 0; k < XR; ++k) {
-                const double xn = ODD ? os_next0(x[k][0]) : 0.0;
+                const double xn = ODD ? xnx[k] : 0.0;
 #pragma unroll
                 for (int j = 0; j < PC; ++j) {
                     const double p = x[k][lo(j)];
@@ -701,12 +705,13 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
             spv = os_from_prev(cs[PC - 1][1], 1.0);
             if (firstv) spv = -spv;  // row start: (0, 1) of the dead pair -> keep q
         }
-        auto rot_rows = [&](double (*m)[CH]) {
+        auto rot_rows = [&](double (*m)[CH], auto is_x_tag) {
 #pragma unroll
             for (int k = 0; k < XR; ++k) {
                 double mn = 0.0, mp = 0.0;
                 if constexpr (ODD) {  // the neighbours' boundary columns, before this row changes
-                    mn = os_next0(m[k][0]);
+                    if constexpr (decltype(is_x_tag)::value) mn = xnx[k];
+                    else mn = os_next0(m[k][0]);
                     mp = os_prev0(m[k][CH - 1]);
                 }
                 double nx[CH];
@@ -728,8 +733,8 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
                 for (int c = 0; c < CH; ++c) m[k][c] = nx[c];
             }
         };
-        rot_rows(x);
-        rot_rows(wv);
+        rot_rows(x, std::true_type{});
+        rot_rows(wv, std::false_type{});
         any = __any(any);
         rots += nrot;
         return any;

@@ -76,15 +76,17 @@ def algorithmic_flops(m: int, n: int, k_ns: int, sweeps: int) -> float:
 def executed_flops(m: int, n: int, inf) -> float:
     """Flops the GPU route executes for one matrix by its own method and
     counts: QR + one-sided Jacobi (path 4): Householder QR 2 M N^2 - 2/3 N^3,
-    Q applied to [W; 0] 4 M N^2 - 2 N^3, per sweep 9 N^3 (per pair and layer
-    6 N for the Gram entries and 12 N for the rotation of X and W) plus the
-    final sweep without rotations 3 N^3; Newton-Schulz routes (path 0): the
-    algorithmic formula with the kernel's own NS and sweep counts."""
+    Q applied to [W; 0] 4 M N^2 - 2 N^3, per sweep 7 N^3 (per pair and layer
+    2 N for the Gram entry g -- the column norms are carried and updated --
+    and 12 N for the rotation of X and W), for the sweeps that rotate and the
+    final one that confirms convergence (its rotations are the identity but
+    are executed); Newton-Schulz routes (path 0): the algorithmic formula with
+    the kernel's own NS and sweep counts."""
     M, N = max(m, n), min(m, n)
     if N <= 4 and m == n:
         return 0.0
     if int(inf[6]) == 4:
-        return 6.0 * M * N * N - (8.0 / 3.0) * N ** 3 + (9.0 * int(inf[2]) + 3.0) * N ** 3
+        return 6.0 * M * N * N - (8.0 / 3.0) * N ** 3 + 7.0 * (int(inf[2]) + 1.0) * N ** 3
     return algorithmic_flops(m, n, int(inf[1]), int(inf[2]))
 
 

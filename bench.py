@@ -150,9 +150,34 @@ def single_leg(h, dev, args):
     cpu_ms = float(np.median(ct[1:])) * 1e3
     e_rel, e_abs = metrics.sigma_errors(sh, rs)
     mm = metrics.compute(a, uh, sh, vth)
+    # the same solve by QR + one-sided Jacobi (lsvd_set_solve_precision),
+    # then the handle back to the default
+    h.set_solve_precision(_lib.LSVD_PREC_F64_ONESIDED)
+    try:
+        for _ in range(5):
+            call()
+        torch.cuda.synchronize()
+        times_os = []
+        for _ in range(args.single_reps):
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            call()
+            e1.record()
+            torch.cuda.synchronize()
+            times_os.append(e0.elapsed_time(e1))
+    finally:
+        h.set_solve_precision(_lib.LSVD_PREC_F64)
+    ms_os = float(np.median(times_os))
+    uo, so_, vto = u.cpu().numpy(), s.cpu().numpy(), vt.cpu().numpy()
+    eo_rel, eo_abs = metrics.sigma_errors(so_, rs)
+    mo = metrics.compute(a, uo, so_, vto)
     return {"ms_per_matrix": round(ms, 4), "sigma_vs_oracle": {"rel": e_rel, "abs": e_abs},
             "rel_recon": mm.rel_recon, "orth_u": mm.orth_u, "orth_v": mm.orth_v,
             "oracle_counts": {"ns_iters": rinfo["ns_iters"], "jacobi_sweeps": rinfo["jacobi_sweeps"]},
+            "onesided": {"ms_per_matrix": round(ms_os, 4), "sigma_vs_oracle": {"rel": eo_rel, "abs": eo_abs},
+                         "rel_recon": mo.rel_recon, "orth_u": mo.orth_u, "orth_v": mo.orth_v,
+                         "route": "QR + one-sided Jacobi (lsvd_set_solve_precision)"},
             "cpu_baseline": {"value": round(cpu_ms, 4), "unit": "ms/matrix", "cores": 1, "kind": "port",
                              "sample": "the same matrix, C restatement of LieSvdSmall::solve, median of 5"}}
 

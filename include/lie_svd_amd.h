@@ -130,6 +130,14 @@ int lsvd_destroy(lsvd_handle_t h);
 int lsvd_set_stream(lsvd_handle_t h, void* hip_stream);
 void* lsvd_get_stream(lsvd_handle_t h);
 int lsvd_set_pointer_mode(lsvd_handle_t h, int mode);
+/* No counterpart in the reference (its solves have one algorithm): the
+ * precision mode of the single-matrix entry points below (LSVD_PREC_F64,
+ * the reference iteration, by default; LSVD_PREC_F64_SCALED or
+ * LSVD_PREC_F64_ONESIDED, else LSVD_E_ARG).  With LSVD_PREC_F64_ONESIDED a
+ * single solve of max(m, n) <= 128 runs Householder QR + one-sided Jacobi
+ * (as the batch route of that name).  The allocation-free entry point
+ * always runs the reference iteration. */
+int lsvd_set_solve_precision(lsvd_handle_t h, int precision);
 int lsvd_synchronize(lsvd_handle_t h);
 const char* lsvd_status_string(int status);
 /* Text of the last error on this handle ("" if none). */
@@ -165,7 +173,8 @@ int lsvd_small_solve_rectangular_f64(lsvd_handle_t h, const double* a, int m, in
  * lsvd_small_solve_rectangular_f64 for callers that cannot allocate device
  * memory (code under stream capture, tight serving loops, harnesses that
  * forbid hipMalloc).  Same arguments and results (bit-identical: the same
- * on-chip kernel), but every pointer is a device pointer (the handle must
+ * on-chip kernel, at the default solve precision), but every pointer is a
+ * device pointer (the handle must
  * be in LSVD_PTR_DEVICE mode, else LSVD_E_ARG), max(m, n) <= 128 (else
  * LSVD_E_LIMIT), and the call only enqueues work on the handle's stream.
  * The work queue, counter and staging slot live in a fixed 393,728-byte

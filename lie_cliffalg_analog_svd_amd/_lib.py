@@ -203,6 +203,7 @@ SIGNATURES = {
     "lsvd_set_stream": (C.c_int, [_vp, _vp]),
     "lsvd_get_stream": (C.c_void_p, [_vp]),
     "lsvd_set_pointer_mode": (C.c_int, [_vp, C.c_int]),
+    "lsvd_set_solve_precision": (C.c_int, [_vp, C.c_int]),
     "lsvd_synchronize": (C.c_int, [_vp]),
     "lsvd_status_string": (C.c_char_p, [C.c_int]),
     "lsvd_last_error": (C.c_char_p, [_vp]),

@@ -727,6 +727,14 @@ int lsvd_set_pointer_mode(lsvd_handle_t h, int mode) {
     return LSVD_OK;
 }
 
+int lsvd_set_solve_precision(lsvd_handle_t h, int precision) {
+    if (!h) return LSVD_E_ARG;
+    if (precision != LSVD_PREC_F64 && precision != LSVD_PREC_F64_SCALED && precision != LSVD_PREC_F64_ONESIDED)
+        return set_error(h, LSVD_E_ARG, "single solves take LSVD_PREC_F64, _F64_SCALED or _F64_ONESIDED");
+    h->solve_precision = precision;
+    return LSVD_OK;
+}
+
 int lsvd_synchronize(lsvd_handle_t h) {
     if (!h) return LSVD_E_ARG;
     hipError_t e = hipStreamSynchronize(h->stream);
@@ -1132,7 +1140,8 @@ static int single(lsvd_handle_t h, int mode, const double* a, int m, int n, doub
                                   : eigh_large(h, a, n, u, s, info, &done);
         if (rc != LSVD_OK || done) return rc;
     }
-    const uint64_t key = ((uint64_t)mode << 60) | ((uint64_t)(uint32_t)m << 30) | (uint64_t)(uint32_t)n;
+    const uint64_t key = ((uint64_t)mode << 60) | ((uint64_t)(uint32_t)h->solve_precision << 54) |
+                         ((uint64_t)(uint32_t)m << 27) | (uint64_t)(uint32_t)n;
     lsvd_plan_t p = nullptr;
     auto it = h->plan_cache.find(key);
     if (it != h->plan_cache.end()) {
@@ -1140,7 +1149,7 @@ static int single(lsvd_handle_t h, int mode, const double* a, int m, int n, doub
     } else {
         int32_t mm = m, nn = n, md = mode;
         int64_t zero = 0;
-        int rc = plan_build(h, 1, &mm, &nn, &zero, &zero, &zero, &zero, &md, LSVD_PREC_F64, &p);
+        int rc = plan_build(h, 1, &mm, &nn, &zero, &zero, &zero, &zero, &md, h->solve_precision, &p);
         if (rc != LSVD_OK) return rc;
         if (h->plan_cache.size() > 64) {
             for (auto& kv : h->plan_cache) lsvd_batch_plan_destroy(kv.second);

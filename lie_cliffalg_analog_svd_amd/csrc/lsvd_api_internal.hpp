@@ -73,6 +73,9 @@ struct lsvd_handle_s {
     // kernels (autotuned; on the ragged batch every size is faster one-sided:
     // whole plan 7.59 ms at 128 against 7.93 at 64 and 8.35-9.0 in between)
     int os_max_np = 128;
+    // precision of the single-matrix entry points' one-item plans
+    // (lsvd_set_solve_precision): the reference iteration by default
+    int solve_precision = LSVD_PREC_F64;
     int max_lanes = 4;
     uint32_t wide_mask = 0;
     lsvd_tune_report_t tune_report{};

@@ -332,6 +332,7 @@ template <int NT, int NP>
 __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda, double* vb, double* refl,
                            OsShared& sh) {
     constexpr bool COMPACT = NP <= kOsCompactMaxNP;
+    constexpr bool TIGHT = COMPACT || NT >= 1024;  // at most 128 VGPRs
     const int tid = tix(), c = tid >> 2, ch = tid & 3, lane = tid & 63;
     // the reflector buffer of column p, built by the wave that holds it
     auto pivot = [&](int p) {
@@ -390,9 +391,9 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
                 y3 = fma(w[ch + kOsQK * (i + 3)], a[i + 3], y3);
             }
             const double k = sh.tw[j] * os_quad_sum((y0 + y1) + (y2 + y3));
-            // compact sizes (register budget): read w again rather than
-            // keep 32 more values live across the sum
-            if constexpr (COMPACT) __asm__ __volatile__("" ::: "memory");
+            // tight register budget: read w again rather than keep 32 more
+            // values live across the sum
+            if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");
 #pragma unroll
             for (int i = 0; i < kOsQR; ++i) a[i] = fma(-k, w[ch + kOsQK * i], a[i]);
         }
@@ -418,6 +419,7 @@ template <int NT, int NP>
 __device__ void os_apply_q_regs(int M, int N, double (&u)[kOsQR], double* am, int lda, const double* refl,
                                 const OsShared& sh) {
     constexpr bool COMPACT = NP <= kOsCompactMaxNP;
+    constexpr bool TIGHT = COMPACT || NT >= 1024;  // at most 128 VGPRs
     constexpr int CK = COMPACT ? kOsCK : NP;        // reflectors per staged chunk
     constexpr int SL = COMPACT ? kOsCKL : NP + 1;   // its leading dimension (lda when not compact)
     const int tid = tix(), c = tid >> 2, ch = tid & 3;
@@ -449,7 +451,7 @@ __device__ void os_apply_q_regs(int M, int N, double (&u)[kOsQR], double* am, in
                     y3 = fma(vj[kOsQK * (i + 3) * SL], u[i + 3], y3);
                 }
                 const double k = tau * os_quad_sum((y0 + y1) + (y2 + y3));
-                if constexpr (COMPACT) __asm__ __volatile__("" ::: "memory");  // reload v (register budget)
+                if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");  // reload v (register budget)
 #pragma unroll
                 for (int i = 0; i < kOsQR; ++i) u[i] = fma(-k, vj[kOsQK * i * SL], u[i]);
             }
@@ -945,6 +947,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(os_min_
 
 template <int NP>
 using OsDefCfg = OsCfg<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW, OsDefault<NP>::CH>;
+
 
 // One persistent launch over the padded sizes 80..128, which share the
 // launch shape (64 lanes per row, 8 waves, 2 positions per lane): the queue

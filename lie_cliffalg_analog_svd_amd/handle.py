@@ -78,6 +78,12 @@ class Handle:
         check(self._h, lib().lsvd_set_pointer_mode(self._h, mode))
         self.pointer_mode = mode
 
+    def set_solve_precision(self, precision: int):
+        """Precision mode of the single-matrix solves (LSVD_PREC_F64, the
+        reference iteration, by default; LSVD_PREC_F64_ONESIDED runs
+        Householder QR + one-sided Jacobi)."""
+        check(self._h, lib().lsvd_set_solve_precision(self._h, precision))
+
     def autotune(self, ms, ns, precision: int = _lib.LSVD_PREC_F64) -> dict:
         """lsvd_autotune_batch for a batch's shapes; returns the report
         (chosen plan knobs and the default / tuned plan times)."""

@@ -24,26 +24,31 @@ from tests._util import vector_parity
 pytestmark = pytest.mark.gpu
 
 OS = LSVD_PREC_F64_ONESIDED
-_HANDLE = None
+_HANDLES = {}
+
+
+def _handle_env(**env):
+    """A handle created under the given environment (read at creation)."""
+    import os
+    from lie_cliffalg_analog_svd_amd import Handle
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update({k: str(v) for k, v in env.items()})
+    try:
+        return Handle(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 def _handle():
     """A handle that sends every padded size (16..128) to the one-sided
-    kernels (LSVD_OS_MAX_NP is read when a handle is created)."""
-    global _HANDLE
-    if _HANDLE is None:
-        import os
-        from lie_cliffalg_analog_svd_amd import Handle
-        old = os.environ.get("LSVD_OS_MAX_NP")
-        os.environ["LSVD_OS_MAX_NP"] = "128"
-        try:
-            _HANDLE = Handle(0)
-        finally:
-            if old is None:
-                del os.environ["LSVD_OS_MAX_NP"]
-            else:
-                os.environ["LSVD_OS_MAX_NP"] = old
-    return _HANDLE
+    kernels."""
+    if "os" not in _HANDLES:
+        _HANDLES["os"] = _handle_env(LSVD_OS_MAX_NP=128)
+    return _HANDLES["os"]
 
 
 def _say(*a):
@@ -188,3 +193,33 @@ def test_onesided_split_routes_large_sizes_to_newton_schulz():
     assert paths == [4, 0, 4, 0], paths
     for a, (u, s, vt) in zip(mats, out):
         assert_factors(a, u, s, vt, 1e-11 * max(1.0, max(a.shape) / 32), str(a.shape))
+
+
+def test_onesided_single_solve():
+    """lsvd_set_solve_precision(LSVD_PREC_F64_ONESIDED): the single-matrix
+    entry points (LieSvdSmall::solve / solve_rectangular) run QR + one-sided
+    Jacobi (path 4) and meet the oracle bar; the default handle still runs
+    the reference iteration (path 0)."""
+    from lie_cliffalg_analog_svd_amd import LSVD_PREC_F64, LieSvdSmall, workloads as wl
+    h = _handle_env(LSVD_OS_MAX_NP=128)
+    h.set_solve_precision(OS)
+    cases = [wl.single_64x64(), uniform_matrix(990, 128, 128), uniform_matrix(991, 100, 37),
+             uniform_matrix(992, 20, 90), imposed_spectrum(993, 64, 64, np.logspace(0.0, -12.0, 64))]
+    for a in cases:
+        _say("single solve", a.shape)
+        if a.shape[0] == a.shape[1]:
+            u, s, vt = LieSvdSmall.solve(a, handle=h)
+        else:
+            u, s, vt = LieSvdSmall.solve_rectangular(a, handle=h)
+        info = LieSvdSmall.last_info
+        assert info["path"] == 4, info
+        n = max(a.shape)
+        assert_factors(a, u, s, vt, 1e-11 * max(1.0, n / 32), str(a.shape))
+        (uo, so, vto), = oracle.svd_batch([a])
+        assert_sigma_close(s, so, n, what=f"single {a.shape}")
+        if a.shape == (64, 64) and so[-1] > 1e-6 * so[0]:
+            vector_parity(u, uo, so, n, "single U")
+            vector_parity(vt.T, vto.T, so, n, "single V")
+    h.set_solve_precision(LSVD_PREC_F64)
+    u, s, vt = LieSvdSmall.solve(cases[0], handle=h)
+    assert LieSvdSmall.last_info["path"] == 0, LieSvdSmall.last_info

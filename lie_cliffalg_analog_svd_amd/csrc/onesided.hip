@@ -276,33 +276,12 @@ constexpr double kOsRefresh = 0.0625;
 // wave and its sums are quad reductions (DPP), never LDS round trips.
 constexpr int kOsQK = 4;               // threads per column
 constexpr int kOsQR = 128 / kOsQK;     // row slots per thread
-constexpr int kOsRows = 128;           // rows of the LDS matrix area (large sizes)
-
-// Padded sizes up to 64 keep a compact LDS area (np x (np + 1)): the QR
-// writes the reflector tails straight to the global slot and the
-// application of Q stages them kOsCK at a time, so several matrices share a
-// CU; their kernels are compiled for os_min_waves(np) waves per SIMD.
-constexpr int kOsCompactMaxNP = 64;
-constexpr int kOsCK = 4;
-constexpr int kOsCKL = kOsCK + 1;
-constexpr int os_area_doubles(int np) {
-    return np > kOsCompactMaxNP ? kOsRows * (np + 1)
-                                : (np * (np + 1) > kOsRows * kOsCKL ? np * (np + 1) : kOsRows * kOsCKL);
-}
-constexpr int os_min_waves(int np) { return np <= 48 ? 4 : (np <= kOsCompactMaxNP ? 3 : 1); }
+constexpr int kOsRows = 128;           // rows of the LDS matrix area
 
 template <int CTRL>
 __device__ __forceinline__ double os_dpp(double x) {
     return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true),
                             __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true));
-}
-
-// A workgroup barrier that orders LDS only: global stores still in flight
-// (the QR's reflector tails) are not waited for.
-__device__ __forceinline__ void os_lds_barrier() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-    __builtin_amdgcn_s_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Sum over the four lanes of a quad (quad_perm [1,0,3,2], then [2,3,0,1]);
@@ -328,11 +307,8 @@ __device__ __forceinline__ double os_quad_sum(double v) {
 // update is a_c -= tau scal^2 y_c w: one pass over the column's registers
 // for the sum, one for the update, no masks.  The quad of column j + 1 builds
 // the next buffer right after its own update, so a column costs one barrier.
-template <int NT, int NP>
-__device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda, double* vb, double* refl,
-                           OsShared& sh) {
-    constexpr bool COMPACT = NP <= kOsCompactMaxNP;
-    constexpr bool TIGHT = COMPACT || NT >= 1024;  // at most 128 VGPRs
+template <int NT>
+__device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda, double* vb, OsShared& sh) {
     const int tid = tix(), c = tid >> 2, ch = tid & 3, lane = tid & 63;
     // the reflector buffer of column p, built by the wave that holds it
     auto pivot = [&](int p) {
@@ -373,14 +349,10 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
     };
     if (N > 0) pivot(0);
     for (int j = 0; j < N; ++j) {
-        os_lds_barrier();
+        __syncthreads();
         const double* w = vb + (j & 1) * kOsRows;
-        // the tail of reflector j (v = scal x): into am's lower part, or
-        // straight to the global slot for the compact sizes
-        for (int r = j + 1 + tid; r < M; r += NT) {
-            if constexpr (COMPACT) refl[(size_t)r * NP + j] = w[r] * sh.scal[j];
-            else am[(size_t)r * lda + j] = w[r] * sh.scal[j];
-        }
+        // the tail of reflector j into am's lower part (v = scal x)
+        for (int r = j + 1 + tid; r < M; r += NT) am[(size_t)r * lda + j] = w[r] * sh.scal[j];
         if (c > j && c < N) {  // quad-uniform
             double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
@@ -391,9 +363,6 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
                 y3 = fma(w[ch + kOsQK * (i + 3)], a[i + 3], y3);
             }
             const double k = sh.tw[j] * os_quad_sum((y0 + y1) + (y2 + y3));
-            // tight register budget: read w again rather than keep 32 more
-            // values live across the sum
-            if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");
 #pragma unroll
             for (int i = 0; i < kOsQR; ++i) a[i] = fma(-k, w[ch + kOsQK * i], a[i]);
         }
@@ -411,50 +380,40 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
 // U = H_0 H_1 ... H_{N-1} [W; 0] for the M x N result: on entry am's first N
 // rows hold W (leading dimension lda); the reflector tails are read from
 // refl (rows > j of column j, leading dimension NP).  The columns of U are
-// independent, so after reflectors are staged in LDS (zero above row j, 1 at
-// row j; all of them in am, or kOsCK at a time for the compact sizes) each
-// column's quad applies them without a barrier.  The result stays in u (the
-// same layout as the QR's registers).
+// independent, so after the reflectors are staged in am (zero above row j,
+// 1 at row j) each column's quad applies all of them without a barrier.
+// The result stays in u (the same layout as the QR's registers).
 template <int NT, int NP>
 __device__ void os_apply_q_regs(int M, int N, double (&u)[kOsQR], double* am, int lda, const double* refl,
                                 const OsShared& sh) {
-    constexpr bool COMPACT = NP <= kOsCompactMaxNP;
-    constexpr bool TIGHT = COMPACT || NT >= 1024;  // at most 128 VGPRs
-    constexpr int CK = COMPACT ? kOsCK : NP;        // reflectors per staged chunk
-    constexpr int SL = COMPACT ? kOsCKL : NP + 1;   // its leading dimension (lda when not compact)
     const int tid = tix(), c = tid >> 2, ch = tid & 3;
 #pragma unroll
     for (int i = 0; i < kOsQR; ++i) {
         const int r = ch + kOsQK * i;
         u[i] = (r < N && c < N) ? am[(size_t)r * lda + c] : 0.0;
     }
-    for (int j0 = ((N - 1) / CK) * CK; j0 >= 0; j0 -= CK) {
-        __syncthreads();  // u loaded / the previous chunk applied
-        for (int e = tid; e < kOsRows * CK; e += NT) {
-            const int r = e / CK, jj = e % CK, j = j0 + jj;
-            if (COMPACT || j < N)
-                am[r * SL + jj] = (r > j && r < M && j < N) ? refl[(size_t)r * NP + j] : (r == j ? 1.0 : 0.0);
-        }
-        __syncthreads();
-        if (c < N) {  // quad-uniform
-            const int jn = N - j0 < CK ? N - j0 : CK;
-            for (int jj = jn - 1; jj >= 0; --jj) {
-                const double tau = sh.tau[j0 + jj];
-                if (tau == 0.0) continue;  // uniform
-                const double* vj = am + ch * SL + jj;  // row ch + 4 i: vj[4 i SL]
-                double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+    __syncthreads();
+    for (int e = tid; e < kOsRows * NP; e += NT) {
+        const int r = e / NP, j = e % NP;  // refl: leading dimension NP
+        if (j < N) am[(size_t)r * lda + j] = r > j ? (r < M ? refl[(size_t)r * NP + j] : 0.0) : (r == j ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (c < N) {  // quad-uniform
+        for (int j = N - 1; j >= 0; --j) {
+            const double tau = sh.tau[j];
+            if (tau == 0.0) continue;  // uniform
+            const double* vj = am + (size_t)ch * lda + j;  // row ch + 4 i: vj[4 i lda]
+            double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
-                for (int i = 0; i < kOsQR; i += 4) {
-                    y0 = fma(vj[kOsQK * i * SL], u[i], y0);
-                    y1 = fma(vj[kOsQK * (i + 1) * SL], u[i + 1], y1);
-                    y2 = fma(vj[kOsQK * (i + 2) * SL], u[i + 2], y2);
-                    y3 = fma(vj[kOsQK * (i + 3) * SL], u[i + 3], y3);
-                }
-                const double k = tau * os_quad_sum((y0 + y1) + (y2 + y3));
-                if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");  // reload v (register budget)
-#pragma unroll
-                for (int i = 0; i < kOsQR; ++i) u[i] = fma(-k, vj[kOsQK * i * SL], u[i]);
+            for (int i = 0; i < kOsQR; i += 4) {
+                y0 = fma(vj[(size_t)kOsQK * i * lda], u[i], y0);
+                y1 = fma(vj[(size_t)kOsQK * (i + 1) * lda], u[i + 1], y1);
+                y2 = fma(vj[(size_t)kOsQK * (i + 2) * lda], u[i + 2], y2);
+                y3 = fma(vj[(size_t)kOsQK * (i + 3) * lda], u[i + 3], y3);
             }
+            const double k = tau * os_quad_sum((y0 + y1) + (y2 + y3));
+#pragma unroll
+            for (int i = 0; i < kOsQR; ++i) u[i] = fma(-k, vj[(size_t)kOsQK * i * lda], u[i]);
         }
     }
 }
@@ -504,15 +463,14 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
         const double isc = 1.0 / sc;
 #pragma unroll
         for (int i = 0; i < kOsQR; ++i) a[i] *= isc;
-        os_qr_regs<NT, NP>(M, N, a, am, LDA, vb, refl, sh);
+        os_qr_regs<NT>(M, N, a, am, LDA, vb, sh);
     }
     __syncthreads();
     clk.mark(kPhQR);
-    if constexpr (NP > kOsCompactMaxNP) {  // reflector tails to the global slot (read back when U = Q W is formed)
-        for (int e = tid; e < M * NP; e += NT) {
-            const int i = e / NP, j = e % NP;
-            if (i > j && j < N) refl[(size_t)i * NP + j] = am[i * LDA + j];
-        }
+    // reflector tails to the global slot (read back when U = Q W is formed)
+    for (int e = tid; e < M * NP; e += NT) {
+        const int i = e / NP, j = e % NP;
+        if (i > j && j < N) refl[(size_t)i * NP + j] = am[i * LDA + j];
     }
 
     // ---- registers: X = R^T (row r, position g holds R[g][r]), W = I
@@ -922,7 +880,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
 // ---- kernels ---------------------------------------------------------------
 
 template <int NP, int TPR = OsDefault<NP>::TPR, int NW = OsDefault<NP>::NW, int CHH = OsDefault<NP>::CH>
-__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(os_min_waves(NP)))) void os_svd_kernel(const Item* __restrict__ items, const int* count_ptr,
+__global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict__ items, const int* count_ptr,
                                                          int* counter, Pools pools, double* gws, int64_t gws_stride) {
     using C = OsCfg<NP, TPR, NW, CHH>;
     extern __shared__ double dyn_lds[];
@@ -930,8 +888,7 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(os_min_
     const int count = *count_ptr;
     double* refl = gws + (size_t)blockIdx.x * gws_stride;
     // LDS: the QR's two reflector buffers (2 x 128 doubles), then the matrix
-    // area (os_area_doubles(NP): R, W, V, the sweeps' partial sums, staged
-    // reflectors)
+    // area (128 x LDA; the sweeps' partial sums use it too)
     double* vb = dyn_lds;
     double* am = dyn_lds + 2 * kOsRows;
     for (;;) {
@@ -947,7 +904,6 @@ __global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(os_min_
 
 template <int NP>
 using OsDefCfg = OsCfg<NP, OsDefault<NP>::TPR, OsDefault<NP>::NW, OsDefault<NP>::CH>;
-
 
 // One persistent launch over the padded sizes 80..128, which share the
 // launch shape (64 lanes per row, 8 waves, 2 positions per lane): the queue
@@ -1029,8 +985,8 @@ int os_threads(int np) {
 }
 
 size_t os_lds_bytes(int np, int M) {
-    (void)M;  // the register layout's row slots cover 128 rows whatever M is
-    return sizeof(double) * (2 * (size_t)kOsRows + (size_t)os_area_doubles(np));
+    (void)M;  // the matrix area always has 128 rows (the register layout's row slots)
+    return sizeof(double) * (2 * (size_t)kOsRows + (size_t)kOsRows * (np + 1));
 }
 
 int64_t os_slot_doubles(int np, int M) {

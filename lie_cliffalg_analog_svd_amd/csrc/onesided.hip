@@ -452,15 +452,28 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
         static_assert(NT / kOsQK >= NP, "a column slot per column");
         const int qc = tid >> 2, qch = tid & 3;
         double a[kOsQR];
-        double ss = 0.0;
+        double amx = 0.0;
 #pragma unroll
         for (int i = 0; i < kOsQR; ++i) {
             const int r = qch + kOsQK * i;
             a[i] = (r < M && qc < N) ? ain(r, qc) : 0.0;
+            amx = fmax(amx, fabs(a[i]));
+        }
+        // ||A||_F without overflow or underflow: the entries are first
+        // brought to [-1, 1] by the power of two of the largest (exact, so
+        // a matrix and its power-of-two multiples give the same factors);
+        // NaN and Inf entries propagate to the results
+        const int ex = __builtin_amdgcn_frexp_exp(block_max<NT>(amx, sh.red));
+        double ss = 0.0;
+#pragma unroll
+        for (int i = 0; i < kOsQR; ++i) {
+            a[i] = ldexp(a[i], -ex);
             ss = fma(a[i], a[i], ss);
         }
-        sc = fmax(sqrt(block_sum<NT>(ss, sh.red)), 1e-300);
-        const double isc = 1.0 / sc;
+        ss = block_sum<NT>(ss, sh.red);
+        const double scr = ss > 0.0 ? sqrt(ss) : 1.0;
+        sc = ss > 0.0 ? ldexp(scr, ex) : 0.0;
+        const double isc = 1.0 / scr;
 #pragma unroll
         for (int i = 0; i < kOsQR; ++i) a[i] *= isc;
         os_qr_regs<NT>(M, N, a, am, LDA, vb, sh);

@@ -223,3 +223,20 @@ def test_onesided_single_solve():
     h.set_solve_precision(LSVD_PREC_F64)
     u, s, vt = LieSvdSmall.solve(cases[0], handle=h)
     assert LieSvdSmall.last_info["path"] == 0, LieSvdSmall.last_info
+
+
+def test_onesided_power_of_two_scaling_is_exact():
+    """||A||_F is formed after bringing the entries to [-1, 1] by the largest
+    one's power of two, so A 2^k gives the same U and V bitwise and sigma
+    2^k exactly -- also where the squares of the entries overflow (k = 600)
+    or underflow (k = -900) in double precision."""
+    base = [uniform_matrix(1200, 40, 30), uniform_matrix(1201, 100, 100),
+            imposed_spectrum(1202, 64, 48, np.logspace(0.0, -8.0, 48)), uniform_matrix(1203, 17, 70)]
+    for k in (600, -900):
+        mats = [np.ldexp(a, k) for a in base]
+        _say("scaling 2^", k)
+        out = svd_batched(base + mats, precision=OS, handle=_handle())
+        for i in range(len(base)):
+            (u0, s0, vt0), (u1, s1, vt1) = out[i], out[i + len(base)]
+            assert np.array_equal(u0, u1) and np.array_equal(vt0, vt1), (k, base[i].shape)
+            assert np.array_equal(np.ldexp(s0, k), s1), (k, base[i].shape)

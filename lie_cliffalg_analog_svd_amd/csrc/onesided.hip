@@ -291,6 +291,17 @@ __device__ __forceinline__ double os_quad_sum(double v) {
     return v + os_dpp<0x4E>(v);
 }
 
+// Sum over the wave, valid in lane 0 (the other lanes hold partial sums):
+// quads by DPP quad_perm, rows of 16 by row_ror 4 and 8, then the swaps
+// across bits 4 and 5.
+__device__ __forceinline__ double os_wave_sum0(double v) {
+    v = os_quad_sum(v);
+    v = v + os_dpp<0x124>(v);
+    v = v + os_dpp<0x128>(v);
+    v = bfly16(v);
+    return bfly32(v);
+}
+
 // Householder QR of the M x N matrix held in registers (a[i] = row
 // t % 4 + 4 i of column t / 4), in the LAPACK dlarfg convention: H_j = I -
 // tau_j v_j v_j^T, v_j = 1 at row j.  On return `am` (leading dimension
@@ -300,9 +311,9 @@ __device__ __forceinline__ double os_quad_sum(double v) {
 // Step j needs the reflector of column j in every thread.  The quad of
 // column j writes the column into vb[j % 2] (128 rows), reads alpha = a_jj
 // back, zeroes the rows <= j (the whole wave, after the quad's stores, in
-// one wave's program order), forms ||x||^2 of the rows > j from the buffer,
-// then beta, tau and 1 / (alpha - beta) = scal, and stores alpha - beta at
-// row j.  With that buffer w (w_r = 0 for r < j, alpha - beta at j, x_r
+// one wave's program order), forms ||x||^2 of the rows > j from the buffer
+// (two rows a lane, a wave sum), then beta, tau and 1 / (alpha - beta) =
+// scal, and stores alpha - beta at row j.  With that buffer w (w_r = 0 for r < j, alpha - beta at j, x_r
 // below), y_c = sum_r w_r a_rc = (alpha - beta) v_j^T a_c, and column c's
 // update is a_c -= tau scal^2 y_c w: one pass over the column's registers
 // for the sum, one for the update, no masks.  The quad of column j + 1 builds
@@ -323,28 +334,21 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
         if (lane <= p) w[lane] = 0.0;
         if (lane + 64 <= p) w[lane + 64] = 0.0;
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        if (c == p) {
-            double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-            for (int i = 0; i < kOsQR; i += 2) {
-                const double x0 = w[ch + kOsQK * i], x1 = w[ch + kOsQK * (i + 1)];
-                s0 = fma(x0, x0, s0);
-                s1 = fma(x1, x1, s1);
-            }
-            const double xnorm2 = os_quad_sum(s0 + s1);
+        // ||x||^2 of the rows > p over the whole wave (two rows a lane)
+        const double x0 = w[lane], x1 = w[lane + 64];
+        const double xnorm2 = os_wave_sum0(fma(x0, x0, x1 * x1));
+        if (lane == 0) {
             double tau = 0.0, beta = alpha, scal = 0.0;
             if (xnorm2 > 0.0) {
                 beta = -copysign(sqrt(alpha * alpha + xnorm2), alpha);
                 tau = (beta - alpha) / beta;
                 scal = 1.0 / (alpha - beta);
             }
-            if (ch == 0) {
-                w[p] = alpha - beta;
-                sh.tau[p] = tau;
-                sh.tw[p] = tau * scal * scal;
-                sh.scal[p] = scal;
-                am[(size_t)p * lda + p] = beta;
-            }
+            w[p] = alpha - beta;
+            sh.tau[p] = tau;
+            sh.tw[p] = tau * scal * scal;
+            sh.scal[p] = scal;
+            am[(size_t)p * lda + p] = beta;
         }
     };
     if (N > 0) pivot(0);

@@ -71,7 +71,7 @@ struct lsvd_handle_s {
     // LSVD_PREC_F64_ONESIDED plans: SVD items of padded size <= os_max_np
     // take QR + one-sided Jacobi, the larger ones the scaled Newton-Schulz
     // kernels (autotuned; on the ragged batch every size is faster one-sided:
-    // whole plan 7.59 ms at 128 against 7.93 at 64 and 8.35-9.0 in between)
+    // whole plan 5.43 ms at 128 against 7.2-9.1 ms for the smaller splits)
     int os_max_np = 128;
     // precision of the single-matrix entry points' one-item plans
     // (lsvd_set_solve_precision): the reference iteration by default

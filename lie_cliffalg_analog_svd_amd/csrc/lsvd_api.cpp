@@ -3,10 +3,11 @@
 // A handle owns a HIP stream, its lanes and the autotuner's choices.  Every
 // solve is expressed as a batch plan: a list of Items sorted longest-first
 // and split into buckets by the tall-ified shape (M x N, M >= N).  Each
-// bucket is one persistent launch of a small_svd variant sized for that
-// bucket (the padded sizes 64..128 share one launch); square n <= 4 items go
-// to the micro kernel, whose escalations are drained by a last small_svd
-// launch.  A plan whose buckets fit one lane replays a captured hipGraph; a
+// bucket is one persistent launch sized for that bucket: the reference
+// iteration's kernels (padded sizes 64..128 share one launch) or, for
+// LSVD_PREC_F64_ONESIDED, the QR + one-sided Jacobi kernels (padded sizes
+// 80..128 share one launch); square n <= 4 items go to the micro kernel,
+// whose escalations are drained by a last small_svd launch.  A plan whose buckets fit one lane replays a captured hipGraph; a
 // plan spread over several lanes is enqueued on them directly (see
 // lsvd_batch_plan_execute for the measured reason).
 #include <hip/hip_runtime.h>

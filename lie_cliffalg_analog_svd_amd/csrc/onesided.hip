@@ -138,6 +138,7 @@ struct OsShared {
     double tw[128];      // tau scal^2 per column (the QR's update coefficient)
     double scal[128];    // 1 / (alpha - beta) per column (reflector tails = scal x)
     double tau[128];
+    double tau_a[128];   // A's tau while Y is factored (the W-free sizes)
     int item;
     int flag;
 };
@@ -278,6 +279,19 @@ constexpr int kOsQK = 4;               // threads per column
 constexpr int kOsQR = 128 / kOsQK;     // row slots per thread
 constexpr int kOsRows = 128;           // rows of the LDS matrix area
 
+// From padded size 80 on the sweeps rotate X only (no W): V = B / sigma as
+// before, and U comes from Y = R V diag(sigma)^-1 (= W up to rounding, with
+// column errors eps sigma_max / sigma_i), orthonormalised by a second
+// register QR in sigma order, U = Q_A [Q_Y D; 0], D the signs of R_Y's
+// diagonal.  The matrix then lives in registers and the global slot, and
+// the LDS area is only the sweeps' partial sums (or a staged chunk), so two
+// matrices of 8 waves share a CU (128 registers a lane).
+constexpr int kOsWFreeMinNP = 80;
+constexpr int kOsKC = 8;    // k per chunk of the product R V
+constexpr int kOsCKF = 16;  // reflectors per staged chunk when W-free
+constexpr int os_area_doubles(int np) { return np >= kOsWFreeMinNP ? 3 * 8 * 128 : kOsRows * (np + 1); }
+constexpr int os_min_waves(int np) { return np >= kOsWFreeMinNP ? 4 : 1; }
+
 template <int CTRL>
 __device__ __forceinline__ double os_dpp(double x) {
     return __hiloint2double(__builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true),
@@ -289,6 +303,15 @@ __device__ __forceinline__ double os_dpp(double x) {
 __device__ __forceinline__ double os_quad_sum(double v) {
     v = v + os_dpp<0xB1>(v);
     return v + os_dpp<0x4E>(v);
+}
+
+// A workgroup barrier that orders LDS only: global stores still in flight
+// (the W-free QR writes R and the reflector tails to the global slot, read
+// only after the factorisation) are not waited for.
+__device__ __forceinline__ void os_lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
 // Sum over the wave, valid in lane 0 (the other lanes hold partial sums):
@@ -318,8 +341,8 @@ __device__ __forceinline__ double os_wave_sum0(double v) {
 // update is a_c -= tau scal^2 y_c w: one pass over the column's registers
 // for the sum, one for the update, no masks.  The quad of column j + 1 builds
 // the next buffer right after its own update, so a column costs one barrier.
-template <int NT>
-__device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda, double* vb, OsShared& sh) {
+template <int NT, bool TIGHT = false>
+__device__ __forceinline__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda, double* vb, OsShared& sh) {
     const int tid = tix(), c = tid >> 2, ch = tid & 3, lane = tid & 63;
     // the reflector buffer of column p, built by the wave that holds it
     auto pivot = [&](int p) {
@@ -353,7 +376,7 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
     };
     if (N > 0) pivot(0);
     for (int j = 0; j < N; ++j) {
-        __syncthreads();
+        os_lds_barrier();
         const double* w = vb + (j & 1) * kOsRows;
         // the tail of reflector j into am's lower part (v = scal x)
         for (int r = j + 1 + tid; r < M; r += NT) am[(size_t)r * lda + j] = w[r] * sh.scal[j];
@@ -367,6 +390,9 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
                 y3 = fma(w[ch + kOsQK * (i + 3)], a[i + 3], y3);
             }
             const double k = sh.tw[j] * os_quad_sum((y0 + y1) + (y2 + y3));
+            // a 128-register budget: read w again rather than keep 32 more
+            // values live across the sum
+            if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");
 #pragma unroll
             for (int i = 0; i < kOsQR; ++i) a[i] = fma(-k, w[ch + kOsQK * i], a[i]);
         }
@@ -381,52 +407,74 @@ __device__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda
     }
 }
 
-// U = H_0 H_1 ... H_{N-1} [W; 0] for the M x N result: on entry am's first N
-// rows hold W (leading dimension lda); the reflector tails are read from
-// refl (rows > j of column j, leading dimension NP).  The columns of U are
-// independent, so after the reflectors are staged in am (zero above row j,
-// 1 at row j) each column's quad applies all of them without a barrier.
-// The result stays in u (the same layout as the QR's registers).
-template <int NT, int NP>
-__device__ void os_apply_q_regs(int M, int N, double (&u)[kOsQR], double* am, int lda, const double* refl,
-                                const OsShared& sh) {
+// u <- H_0 H_1 ... H_{N-1} u for the M-row columns in u (the QR's register
+// layout): the reflector tails are read from refl (rows > j of column j,
+// leading dimension ldr), tau from `tau`.  The columns are independent, so
+// the reflectors are staged in LDS CK at a time (`stg`: 128 x (CK + 1)
+// doubles, zero above row j, 1 at row j) and each column's quad applies a
+// chunk without a barrier.  TIGHT (a 128-register budget) reads the staged
+// values again for the update instead of keeping 32 of them live.
+template <int NT, int CK, bool TIGHT>
+__device__ __forceinline__ void os_apply_refl(int M, int N, double (&u)[kOsQR], const double* refl, int ldr, const double* tau,
+                              double* stg) {
+    constexpr int SL = CK + 1;
     const int tid = tix(), c = tid >> 2, ch = tid & 3;
+    for (int j0 = ((N - 1) / CK) * CK; j0 >= 0; j0 -= CK) {
+        __syncthreads();  // stg free (the previous chunk applied, or its earlier use done)
+        for (int e = tid; e < kOsRows * CK; e += NT) {
+            const int r = e / CK, jj = e % CK, j = j0 + jj;
+            stg[r * SL + jj] = (r > j && r < M && j < N) ? refl[(size_t)r * ldr + j] : (r == j ? 1.0 : 0.0);
+        }
+        __syncthreads();
+        if (c < N) {  // quad-uniform
+            const int jn = N - j0 < CK ? N - j0 : CK;
+            for (int jj = jn - 1; jj >= 0; --jj) {
+                const double t = tau[j0 + jj];
+                if (t == 0.0) continue;  // uniform
+                const double* vj = stg + ch * SL + jj;  // row ch + 4 i: vj[4 i SL]
+                double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
-    for (int i = 0; i < kOsQR; ++i) {
-        const int r = ch + kOsQK * i;
-        u[i] = (r < N && c < N) ? am[(size_t)r * lda + c] : 0.0;
-    }
-    __syncthreads();
-    for (int e = tid; e < kOsRows * NP; e += NT) {
-        const int r = e / NP, j = e % NP;  // refl: leading dimension NP
-        if (j < N) am[(size_t)r * lda + j] = r > j ? (r < M ? refl[(size_t)r * NP + j] : 0.0) : (r == j ? 1.0 : 0.0);
-    }
-    __syncthreads();
-    if (c < N) {  // quad-uniform
-        for (int j = N - 1; j >= 0; --j) {
-            const double tau = sh.tau[j];
-            if (tau == 0.0) continue;  // uniform
-            const double* vj = am + (size_t)ch * lda + j;  // row ch + 4 i: vj[4 i lda]
-            double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
+                for (int i = 0; i < kOsQR; i += 4) {
+                    y0 = fma(vj[kOsQK * i * SL], u[i], y0);
+                    y1 = fma(vj[kOsQK * (i + 1) * SL], u[i + 1], y1);
+                    y2 = fma(vj[kOsQK * (i + 2) * SL], u[i + 2], y2);
+                    y3 = fma(vj[kOsQK * (i + 3) * SL], u[i + 3], y3);
+                }
+                const double k = t * os_quad_sum((y0 + y1) + (y2 + y3));
+                if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");
 #pragma unroll
-            for (int i = 0; i < kOsQR; i += 4) {
-                y0 = fma(vj[(size_t)kOsQK * i * lda], u[i], y0);
-                y1 = fma(vj[(size_t)kOsQK * (i + 1) * lda], u[i + 1], y1);
-                y2 = fma(vj[(size_t)kOsQK * (i + 2) * lda], u[i + 2], y2);
-                y3 = fma(vj[(size_t)kOsQK * (i + 3) * lda], u[i + 3], y3);
+                for (int i = 0; i < kOsQR; ++i) u[i] = fma(-k, vj[kOsQK * i * SL], u[i]);
             }
-            const double k = tau * os_quad_sum((y0 + y1) + (y2 + y3));
-#pragma unroll
-            for (int i = 0; i < kOsQR; ++i) u[i] = fma(-k, vj[(size_t)kOsQK * i * lda], u[i]);
         }
     }
+}
+
+// U (M x N) from the column-resident registers to row-major output, or its
+// transpose (N x M) for a wide input.
+__device__ __forceinline__ void os_write_u(const double (&u)[kOsQR], int M, int N, double* ud, bool wide) {
+    const int qc = tix() >> 2, qch = tix() & 3;
+    if (qc < N)
+#pragma unroll
+        for (int i = 0; i < kOsQR; ++i) {
+            const int r = qch + kOsQK * i;
+            if (r < M) {
+                if (!wide) ud[(size_t)r * N + qc] = u[i];
+                else ud[(size_t)qc * M + r] = u[i];
+            }
+        }
 }
 
 // ---- the per-matrix solver -----------------------------------------------
 
 template <int NP, int TPR, int NW, int CHH>
-__device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb, double* refl, OsShared& sh) {
+__device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb, double* refl, OsShared& sh) {
     using C = OsCfg<NP, TPR, NW, CHH>;
+    constexpr bool WFREE = NP >= kOsWFreeMinNP;
+    static_assert(!WFREE || (NW == 8 && 3 * NW * C::NL <= os_area_doubles(NP) &&
+                             kOsRows * (kOsKC + 1) + kOsKC * (NP + 1) <= os_area_doubles(NP) &&
+                             kOsRows * (kOsCKF + 1) + kOsCKF * kOsCKF + 2 * kOsCKF * (kOsCKF - 1) <=
+                                 os_area_doubles(NP)),
+                  "the W-free LDS area holds the partial sums and the staged chunks");
     constexpr int NL = C::NL;
     constexpr int NT = C::NT, CH = C::CH, PC = C::P, PP = C::PP, XR = C::XR, LDA = C::LDA;
     constexpr int SC = C::SC, NA = C::NA, LB = C::LB, RGW = C::RGW;
@@ -450,6 +498,14 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
             }
         }
     } clk{P.timing ? P.timing + (size_t)itm.idx * kPhases : nullptr, P.timing && threadIdx.x == 0 ? clock64() : 0};
+
+    // global slot: A's R and reflector tails (kOsRows x NP); when W-free
+    // also Y's (NP x NP) and V, row-major (NP x NP)
+    double* slot_a = refl;
+    double* slot_y = refl + (size_t)kOsRows * NP;
+    double* slot_v = slot_y + (size_t)NP * NP;
+    (void)slot_y;
+    (void)slot_v;
 
     // ---- A / ||A||_F into the column-resident registers, QR
     {
@@ -480,26 +536,34 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
         const double isc = 1.0 / scr;
 #pragma unroll
         for (int i = 0; i < kOsQR; ++i) a[i] *= isc;
-        os_qr_regs<NT>(M, N, a, am, LDA, vb, sh);
+        if constexpr (WFREE) os_qr_regs<NT, true>(M, N, a, slot_a, NP, vb, sh);  // R and tails to the slot
+        else os_qr_regs<NT>(M, N, a, am, LDA, vb, sh);
     }
     __syncthreads();
     clk.mark(kPhQR);
-    // reflector tails to the global slot (read back when U = Q W is formed)
-    for (int e = tid; e < M * NP; e += NT) {
-        const int i = e / NP, j = e % NP;
-        if (i > j && j < N) refl[(size_t)i * NP + j] = am[i * LDA + j];
+    if constexpr (WFREE) {
+        for (int j = tid; j < N; j += NT) sh.tau_a[j] = sh.tau[j];
+    } else {  // reflector tails to the global slot (read back when U = Q W is formed)
+        for (int e = tid; e < M * NP; e += NT) {
+            const int i = e / NP, j = e % NP;
+            if (i > j && j < N) refl[(size_t)i * NP + j] = am[i * LDA + j];
+        }
     }
 
     // ---- registers: X = R^T (row r, position g holds R[g][r]), W = I
-    double x[XR][CH], wv[XR][CH];
+    double x[XR][CH], wv[WFREE ? 1 : XR][CH];
 #pragma unroll
     for (int k = 0; k < XR; ++k) {
         const int r = rg * XR + k;
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             const int g = vch * CH + c;
-            x[k][c] = (g <= r && r < N) ? am[g * LDA + r] : 0.0;
-            wv[k][c] = g == r ? 1.0 : 0.0;
+            if constexpr (WFREE) {
+                x[k][c] = (g <= r && r < N) ? slot_a[(size_t)g * NP + r] : 0.0;
+            } else {
+                x[k][c] = (g <= r && r < N) ? am[g * LDA + r] : 0.0;
+                wv[k][c] = g == r ? 1.0 : 0.0;
+            }
         }
     }
     __syncthreads();  // am is reused below
@@ -751,7 +815,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
             }
         };
         rot_rows(x, std::true_type{});
-        rot_rows(wv, std::false_type{});
+        if constexpr (!WFREE) rot_rows(wv, std::false_type{});
         any = __any(any);
         rots += nrot;
         return any;
@@ -815,6 +879,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
     if (tid == 0) sh.flag = 0;
     __syncthreads();
 
+    if constexpr (!WFREE) {
     // ---- V = B diag(sigma)^-1 in sorted column order (am: N x LDA)
 #pragma unroll
     for (int k = 0; k < XR; ++k) {
@@ -870,18 +935,100 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
     __syncthreads();
     {
         double u[kOsQR];
-        os_apply_q_regs<NT, NP>(M, N, u, am, LDA, refl, sh);
         const int qc = tid >> 2, qch = tid & 3;
-        double* ud = wide ? P.vt + itm.vt_off : P.u + itm.u_off;  // wide: (N x M) = U_t^T
-        if (qc < N)
 #pragma unroll
-            for (int i = 0; i < kOsQR; ++i) {
-                const int r = qch + kOsQK * i;
-                if (r < M) {
-                    if (!wide) ud[(size_t)r * N + qc] = u[i];
-                    else ud[(size_t)qc * M + r] = u[i];
-                }
+        for (int i = 0; i < kOsQR; ++i) {
+            const int r = qch + kOsQK * i;
+            u[i] = (r < N && qc < N) ? am[(size_t)r * LDA + qc] : 0.0;
+        }
+        os_apply_refl<NT, NP, false>(M, N, u, refl, NP, sh.tau, am);  // all reflectors staged at once (am: 128 x LDA)
+        os_write_u(u, M, N, wide ? P.vt + itm.vt_off : P.u + itm.u_off, wide);
+    }
+    } else {
+    // ---- V = B diag(sigma)^-1, sorted, row-major in the slot (then out)
+#pragma unroll
+    for (int k = 0; k < XR; ++k) {
+        const int r = rg * XR + k;
+#pragma unroll
+        for (int c = 0; c < CH; ++c) {
+            const int g = vch * CH + c;
+            const int idx = g < NP ? os_index_at(NP, G, g) : NP;
+            if (r < N && idx < N) {
+                const double sg = sh.sig[idx];  // below kOsTiny: completed as above
+                const bool tiny = !(sg > kOsTiny) && !isnan(sg);
+                slot_v[(size_t)r * NP + sh.rank[idx]] = tiny ? 0.0 : x[k][c] / sg;
+                if (tiny) sh.flag = 1;
             }
+        }
+    }
+    for (int i = tid; i < N; i += NT) sh.tw[sh.rank[i]] = sh.sig[i];  // sigma in sorted order
+    __syncthreads();
+    if (sh.flag) {
+        gram_schmidt_g<NT>(N, N, View{slot_v, NP, 0}, slot_v, NP, sh.red);
+        info.repaired = 1;
+        __syncthreads();
+    }
+    {
+        double* sdst = P.s + itm.s_off;
+        for (int i = tid; i < N; i += NT) sdst[sh.rank[i]] = sh.sig[i] * sc;
+        double* vdst = wide ? P.u + itm.u_off : P.vt + itm.vt_off;
+        for (int e = tid; e < NP * NP; e += NT) {
+            const int r = e / NP, j = e % NP;  // V[r][j]
+            if (r < N && j < N) {
+                const double v = slot_v[(size_t)r * NP + j];
+                if (!wide) vdst[(size_t)j * N + r] = v;
+                else vdst[(size_t)r * N + j] = v;
+            }
+        }
+    }
+    clk.mark(kPhU);
+
+    // ---- Y = R V diag(sigma)^-1 into the column-resident registers, k in
+    // chunks of kOsKC through LDS (R's columns k0.. and V's rows k0..)
+    const int qc = tid >> 2, qch = tid & 3;
+    double y[kOsQR];
+#pragma unroll
+    for (int i = 0; i < kOsQR; ++i) y[i] = 0.0;
+    double* rc = am;                           // 128 x (kOsKC + 1)
+    double* vc = am + kOsRows * (kOsKC + 1);   // kOsKC x (NP + 1)
+    for (int k0 = 0; k0 < N; k0 += kOsKC) {
+        __syncthreads();
+        for (int e = tid; e < kOsRows * kOsKC; e += NT) {
+            const int r = e / kOsKC, kk = e % kOsKC, k = k0 + kk;
+            rc[r * (kOsKC + 1) + kk] = (r <= k && k < N) ? slot_a[(size_t)r * NP + k] : 0.0;
+        }
+        for (int e = tid; e < kOsKC * NP; e += NT) {
+            const int kk = e / NP, cc = e % NP, k = k0 + kk;
+            vc[kk * (NP + 1) + cc] = (k < N && cc < N) ? slot_v[(size_t)k * NP + cc] : 0.0;
+        }
+        __syncthreads();
+        if (qc < N) {
+#pragma unroll
+            for (int kk = 0; kk < kOsKC; ++kk) {
+                const double v = vc[kk * (NP + 1) + qc];
+#pragma unroll
+                for (int i = 0; i < kOsQR; ++i) y[i] = fma(rc[(qch + kOsQK * i) * (kOsKC + 1) + kk], v, y[i]);
+            }
+        }
+    }
+    {
+        const double sg = qc < N ? sh.tw[qc] : 0.0;
+        const double is = (sg > kOsTiny) ? 1.0 / sg : 0.0;  // a completed V column: Y's column is zero
+#pragma unroll
+        for (int i = 0; i < kOsQR; ++i) y[i] *= is;
+    }
+    // ---- Y = Q_Y R_Y (Y's reflectors and R_Y to the slot), U = Q_A [Q_Y D; 0]
+    os_qr_regs<NT, true>(N, N, y, slot_y, NP, vb, sh);
+    __syncthreads();
+    {
+        double u[kOsQR];
+        const double d = qc < N ? (slot_y[(size_t)qc * NP + qc] < 0.0 ? -1.0 : 1.0) : 0.0;
+#pragma unroll
+        for (int i = 0; i < kOsQR; ++i) u[i] = (qch + kOsQK * i == qc) ? d : 0.0;
+        os_apply_refl<NT, kOsCKF, true>(N, N, u, slot_y, NP, sh.tau, am);
+        os_apply_refl<NT, kOsCKF, true>(M, N, u, slot_a, NP, sh.tau_a, am);
+        os_write_u(u, M, N, wide ? P.vt + itm.vt_off : P.u + itm.u_off, wide);
+    }
     }
     clk.mark(kPhOut);
     rots = (int)block_sum<NT>((double)rots, sh.red);
@@ -897,7 +1044,7 @@ __device__ void os_solve(const Item& itm, const Pools& P, double* am, double* vb
 // ---- kernels ---------------------------------------------------------------
 
 template <int NP, int TPR = OsDefault<NP>::TPR, int NW = OsDefault<NP>::NW, int CHH = OsDefault<NP>::CH>
-__global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict__ items, const int* count_ptr,
+__global__ __launch_bounds__(64 * NW) __attribute__((amdgpu_waves_per_eu(os_min_waves(NP)))) void os_svd_kernel(const Item* __restrict__ items, const int* count_ptr,
                                                          int* counter, Pools pools, double* gws, int64_t gws_stride) {
     using C = OsCfg<NP, TPR, NW, CHH>;
     extern __shared__ double dyn_lds[];
@@ -905,7 +1052,8 @@ __global__ __launch_bounds__(64 * NW) void os_svd_kernel(const Item* __restrict_
     const int count = *count_ptr;
     double* refl = gws + (size_t)blockIdx.x * gws_stride;
     // LDS: the QR's two reflector buffers (2 x 128 doubles), then the matrix
-    // area (128 x LDA; the sweeps' partial sums use it too)
+    // area (os_area_doubles: R, W and V below padded size 80; the sweeps'
+    // partial sums and staged chunks)
     double* vb = dyn_lds;
     double* am = dyn_lds + 2 * kOsRows;
     for (;;) {
@@ -953,7 +1101,7 @@ __device__ __forceinline__ void os_multi_drain(const Item* items, int end, int* 
     }
 }
 
-__global__ __launch_bounds__(kOsMultiNT) void os_svd_multi_kernel(const Item* __restrict__ items,
+__global__ __launch_bounds__(kOsMultiNT) __attribute__((amdgpu_waves_per_eu(os_min_waves(kOsMultiMinNP)))) void os_svd_multi_kernel(const Item* __restrict__ items,
                                                                   const int* count_ptr, int* counter, Pools pools,
                                                                   double* gws, int64_t gws_stride) {
     extern __shared__ double dyn_lds[];
@@ -1002,11 +1150,13 @@ int os_threads(int np) {
 }
 
 size_t os_lds_bytes(int np, int M) {
-    (void)M;  // the matrix area always has 128 rows (the register layout's row slots)
-    return sizeof(double) * (2 * (size_t)kOsRows + (size_t)kOsRows * (np + 1));
+    (void)M;  // the register layout's row slots cover 128 rows whatever M is
+    return sizeof(double) * (2 * (size_t)kOsRows + (size_t)os_area_doubles(np));
 }
 
 int64_t os_slot_doubles(int np, int M) {
+    if (np >= kOsWFreeMinNP)  // A's R and tails (128 rows), Y's, V
+        return ((int64_t)kOsRows * np + 2 * (int64_t)np * np + 31) / 32 * 32;
     const int mp = (M + 15) & ~15;
     const int rows = mp > np ? mp : np;
     return ((int64_t)rows * np + 31) / 32 * 32;

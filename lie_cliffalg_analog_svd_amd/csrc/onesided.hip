@@ -226,14 +226,14 @@ __device__ __forceinline__ int os_index_at(int NP, int G, int g) {
     return v0 < NP ? v0 : two - 1 - v0;
 }
 
-// 1/sqrt(x) for x within a few binades of 1: the hardware estimate
-// (v_rsq_f64, relative error ~2^-26) refined by two Newton steps.
+// 1/sqrt(x) for x within a few binades of 1: the hardware estimate y
+// (v_rsq_f64, relative error ~2^-26) refined by one third-order step,
+// 1/sqrt(x) = y (1 - e)^(-1/2) = y (1 + e/2 + 3 e^2/8 + O(e^3)) with
+// e = 1 - x y^2 (four dependent operations instead of two Newton steps' six).
 __device__ __forceinline__ double os_rsq(double x) {
-    double y = __builtin_amdgcn_rsq(x);
-    const double h = 0.5 * x;
-    y = y * fma(-h * y, y, 1.5);
-    y = y * fma(-h * y, y, 1.5);
-    return y;
+    const double y = __builtin_amdgcn_rsq(x);
+    const double e = fma(-x * y, y, 1.0);
+    return fma(y * e, fma(e, 0.375, 0.5), y);
 }
 
 // The Jacobi rotation of the 2 x 2 Gram matrix [[a, g], [g, b]]
@@ -241,8 +241,8 @@ __device__ __forceinline__ double os_rsq(double x) {
 // (x_p, x_q) -> (c x_p - s x_q, s x_p + c x_q) makes them orthogonal.  Its
 // inputs are scaled by a power of two so that max(|b - a|, |2 g|) is in
 // [0.5, 1); r and 1 / h from reciprocal square roots.  With WANT_T it also
-// returns t = s / c (from the hardware reciprocal refined by two Newton
-// steps): the rotated Gram diagonal is then a - t g (the column c x_p - s x_q)
+// returns t = s / c (from the hardware reciprocal refined by one third-order
+// step): the rotated Gram diagonal is then a - t g (the column c x_p - s x_q)
 // and b + t g (s x_p + c x_q).
 template <bool WANT_T = false>
 __device__ __forceinline__ void os_angle(double a, double b, double g, double& c, double& s, double* t = nullptr) {
@@ -257,10 +257,10 @@ __device__ __forceinline__ void os_angle(double a, double b, double g, double& c
     c = den * ih;
     s = sgn * as * ih;
     if constexpr (WANT_T) {
-        double y = __builtin_amdgcn_rcp(den);
-        y = y * fma(-den, y, 2.0);
-        y = y * fma(-den, y, 2.0);
-        *t = sgn * as * y;
+        // 1/den = y / (1 - e) = y (1 + e + e^2 + O(e^3)), e = 1 - den y
+        const double y = __builtin_amdgcn_rcp(den);
+        const double e = fma(-den, y, 1.0);
+        *t = sgn * as * fma(y, fma(e, e, e), y);
     }
 }
 
@@ -314,6 +314,17 @@ __device__ __forceinline__ void os_lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// p[0] + p[st] + ... + p[(n - 1) st] as a pairwise tree
+template <int n>
+__device__ __forceinline__ double os_tree_sum(const double* p, int st) {
+    if constexpr (n == 1) {
+        return p[0];
+    } else {
+        constexpr int h = n / 2;
+        return os_tree_sum<h>(p, st) + os_tree_sum<n - h>(p + (size_t)h * st, st);
+    }
+}
+
 // Sum over the wave, valid in lane 0 (the other lanes hold partial sums):
 // quads by DPP quad_perm, rows of 16 by row_ror 4 and 8, then the swaps
 // across bits 4 and 5.
@@ -341,8 +352,8 @@ __device__ __forceinline__ double os_wave_sum0(double v) {
 // update is a_c -= tau scal^2 y_c w: one pass over the column's registers
 // for the sum, one for the update, no masks.  The quad of column j + 1 builds
 // the next buffer right after its own update, so a column costs one barrier.
-template <int NT, bool TIGHT = false>
-__device__ __forceinline__ void os_qr_regs(int M, int N, double (&a)[kOsQR], double* am, int lda, double* vb, OsShared& sh) {
+template <int NT, bool TIGHT = false, int QS = kOsQR>
+__device__ __forceinline__ void os_qr_regs(int M, int N, double (&a)[QS], double* am, int lda, double* vb, OsShared& sh) {
     const int tid = tix(), c = tid >> 2, ch = tid & 3, lane = tid & 63;
     // the reflector buffer of column p, built by the wave that holds it
     auto pivot = [&](int p) {
@@ -350,7 +361,7 @@ __device__ __forceinline__ void os_qr_regs(int M, int N, double (&a)[kOsQR], dou
         double* w = vb + (p & 1) * kOsRows;
         if (c == p)
 #pragma unroll
-            for (int i = 0; i < kOsQR; ++i) w[ch + kOsQK * i] = a[i];
+            for (int i = 0; i < QS; ++i) w[ch + kOsQK * i] = a[i];
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         const double alpha = w[p];
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
@@ -383,7 +394,7 @@ __device__ __forceinline__ void os_qr_regs(int M, int N, double (&a)[kOsQR], dou
         if (c > j && c < N) {  // quad-uniform
             double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
-            for (int i = 0; i < kOsQR; i += 4) {
+            for (int i = 0; i < QS; i += 4) {
                 y0 = fma(w[ch + kOsQK * i], a[i], y0);
                 y1 = fma(w[ch + kOsQK * (i + 1)], a[i + 1], y1);
                 y2 = fma(w[ch + kOsQK * (i + 2)], a[i + 2], y2);
@@ -394,14 +405,14 @@ __device__ __forceinline__ void os_qr_regs(int M, int N, double (&a)[kOsQR], dou
             // values live across the sum
             if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");
 #pragma unroll
-            for (int i = 0; i < kOsQR; ++i) a[i] = fma(-k, w[ch + kOsQK * i], a[i]);
+            for (int i = 0; i < QS; ++i) a[i] = fma(-k, w[ch + kOsQK * i], a[i]);
         }
         if (j + 1 < N) pivot(j + 1);
     }
     __syncthreads();
     // R above the diagonal (the diagonal is in am already)
 #pragma unroll
-    for (int i = 0; i < kOsQR; ++i) {
+    for (int i = 0; i < QS; ++i) {
         const int r = ch + kOsQK * i;
         if (r < c && c < N) am[(size_t)r * lda + c] = a[i];
     }
@@ -414,14 +425,14 @@ __device__ __forceinline__ void os_qr_regs(int M, int N, double (&a)[kOsQR], dou
 // doubles, zero above row j, 1 at row j) and each column's quad applies a
 // chunk without a barrier.  TIGHT (a 128-register budget) reads the staged
 // values again for the update instead of keeping 32 of them live.
-template <int NT, int CK, bool TIGHT>
-__device__ __forceinline__ void os_apply_refl(int M, int N, double (&u)[kOsQR], const double* refl, int ldr, const double* tau,
+template <int NT, int CK, bool TIGHT, int QS = kOsQR>
+__device__ __forceinline__ void os_apply_refl(int M, int N, double (&u)[QS], const double* refl, int ldr, const double* tau,
                               double* stg) {
     constexpr int SL = CK + 1;
     const int tid = tix(), c = tid >> 2, ch = tid & 3;
     for (int j0 = ((N - 1) / CK) * CK; j0 >= 0; j0 -= CK) {
         __syncthreads();  // stg free (the previous chunk applied, or its earlier use done)
-        for (int e = tid; e < kOsRows * CK; e += NT) {
+        for (int e = tid; e < kOsQK * QS * CK; e += NT) {
             const int r = e / CK, jj = e % CK, j = j0 + jj;
             stg[r * SL + jj] = (r > j && r < M && j < N) ? refl[(size_t)r * ldr + j] : (r == j ? 1.0 : 0.0);
         }
@@ -434,7 +445,7 @@ __device__ __forceinline__ void os_apply_refl(int M, int N, double (&u)[kOsQR], 
                 const double* vj = stg + ch * SL + jj;  // row ch + 4 i: vj[4 i SL]
                 double y0 = 0.0, y1 = 0.0, y2 = 0.0, y3 = 0.0;
 #pragma unroll
-                for (int i = 0; i < kOsQR; i += 4) {
+                for (int i = 0; i < QS; i += 4) {
                     y0 = fma(vj[kOsQK * i * SL], u[i], y0);
                     y1 = fma(vj[kOsQK * (i + 1) * SL], u[i + 1], y1);
                     y2 = fma(vj[kOsQK * (i + 2) * SL], u[i + 2], y2);
@@ -443,7 +454,7 @@ __device__ __forceinline__ void os_apply_refl(int M, int N, double (&u)[kOsQR], 
                 const double k = t * os_quad_sum((y0 + y1) + (y2 + y3));
                 if constexpr (TIGHT) __asm__ __volatile__("" ::: "memory");
 #pragma unroll
-                for (int i = 0; i < kOsQR; ++i) u[i] = fma(-k, vj[kOsQK * i * SL], u[i]);
+                for (int i = 0; i < QS; ++i) u[i] = fma(-k, vj[kOsQK * i * SL], u[i]);
             }
         }
     }
@@ -620,7 +631,10 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
                     v[j][2] = fma(p, q, v[j][2]);
                 }
             }
-        } else {
+        } else {  // g only: two chains over the rows (even and odd k)
+            double g1[PP];
+#pragma unroll
+            for (int j = 0; j < PP; ++j) g1[j] = 0.0;
 #pragma unroll
             for (int k = 0; k < XR; ++k) {
                 const double xn = ODD ? xnx[k] : 0.0;
@@ -628,9 +642,12 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
                 for (int j = 0; j < PC; ++j) {
                     const double p = x[k][lo(j)];
                     const double q = (ODD && j == PC - 1) ? xn : x[k][lo(j) + 1];
-                    v[j][2] = fma(p, q, v[j][2]);
+                    if (k & 1) g1[j] = fma(p, q, g1[j]);
+                    else v[j][2] = fma(p, q, v[j][2]);
                 }
             }
+#pragma unroll
+            for (int j = 0; j < PC; ++j) v[j][2] += g1[j];
         }
         // reduce-scatter over lane bit 5, then bit 4: pair index owned
         int kid = 0;
@@ -688,19 +705,14 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 #pragma unroll
             for (int j = 0; j < NA; ++j) {
                 const int pr = vch * PC + (kid + j < PC ? kid + j : 0);
-                double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+                // the waves' partials summed as a tree (the same order in
+                // every wave)
+                double s0 = 0.0, s1 = 0.0;
                 if (full) {
-#pragma unroll
-                    for (int ww = 0; ww < NW; ++ww) {
-                        const double* d = pb + (size_t)ww * 3 * C::S + pr;
-                        s0 += d[0];
-                        s1 += d[C::S];
-                        s2 += d[2 * C::S];
-                    }
-                } else {
-#pragma unroll
-                    for (int ww = 0; ww < NW; ++ww) s2 += pb[(size_t)ww * 3 * C::S + 2 * C::S + pr];
+                    s0 = os_tree_sum<NW>(pb + pr, 3 * C::S);
+                    s1 = os_tree_sum<NW>(pb + C::S + pr, 3 * C::S);
                 }
+                const double s2 = os_tree_sum<NW>(pb + 2 * C::S + pr, 3 * C::S);
                 v[j][0] = s0;
                 v[j][1] = s1;
                 v[j][2] = s2;
@@ -985,15 +997,20 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 
     // ---- Y = R V diag(sigma)^-1 into the column-resident registers, k in
     // chunks of kOsKC through LDS (R's columns k0.. and V's rows k0..)
+    // Y has N <= NP rows: its QR and Q_Y take NP / 4 row slots a lane (the
+    // pivot buffers' rows from NP on stay zero)
+    constexpr int QY = NP / kOsQK;
+    static_assert(QY % 4 == 0 && QY <= kOsQR, "row slots of Y");
+    for (int e = tid; e < 2 * (kOsRows - NP); e += NT) vb[(e / (kOsRows - NP)) * kOsRows + NP + e % (kOsRows - NP)] = 0.0;
     const int qc = tid >> 2, qch = tid & 3;
-    double y[kOsQR];
+    double y[QY];
 #pragma unroll
-    for (int i = 0; i < kOsQR; ++i) y[i] = 0.0;
-    double* rc = am;                           // 128 x (kOsKC + 1)
+    for (int i = 0; i < QY; ++i) y[i] = 0.0;
+    double* rc = am;                           // NP x (kOsKC + 1)
     double* vc = am + kOsRows * (kOsKC + 1);   // kOsKC x (NP + 1)
     for (int k0 = 0; k0 < N; k0 += kOsKC) {
         __syncthreads();
-        for (int e = tid; e < kOsRows * kOsKC; e += NT) {
+        for (int e = tid; e < NP * kOsKC; e += NT) {
             const int r = e / kOsKC, kk = e % kOsKC, k = k0 + kk;
             rc[r * (kOsKC + 1) + kk] = (r <= k && k < N) ? slot_a[(size_t)r * NP + k] : 0.0;
         }
@@ -1007,7 +1024,7 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
             for (int kk = 0; kk < kOsKC; ++kk) {
                 const double v = vc[kk * (NP + 1) + qc];
 #pragma unroll
-                for (int i = 0; i < kOsQR; ++i) y[i] = fma(rc[(qch + kOsQK * i) * (kOsKC + 1) + kk], v, y[i]);
+                for (int i = 0; i < QY; ++i) y[i] = fma(rc[(qch + kOsQK * i) * (kOsKC + 1) + kk], v, y[i]);
             }
         }
     }
@@ -1015,17 +1032,22 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
         const double sg = qc < N ? sh.tw[qc] : 0.0;
         const double is = (sg > kOsTiny) ? 1.0 / sg : 0.0;  // a completed V column: Y's column is zero
 #pragma unroll
-        for (int i = 0; i < kOsQR; ++i) y[i] *= is;
+        for (int i = 0; i < QY; ++i) y[i] *= is;
     }
     // ---- Y = Q_Y R_Y (Y's reflectors and R_Y to the slot), U = Q_A [Q_Y D; 0]
-    os_qr_regs<NT, true>(N, N, y, slot_y, NP, vb, sh);
+    os_qr_regs<NT, true, QY>(N, N, y, slot_y, NP, vb, sh);
     __syncthreads();
     {
         double u[kOsQR];
-        const double d = qc < N ? (slot_y[(size_t)qc * NP + qc] < 0.0 ? -1.0 : 1.0) : 0.0;
+        {
+            double uy[QY];
+            const double d = qc < N ? (slot_y[(size_t)qc * NP + qc] < 0.0 ? -1.0 : 1.0) : 0.0;
 #pragma unroll
-        for (int i = 0; i < kOsQR; ++i) u[i] = (qch + kOsQK * i == qc) ? d : 0.0;
-        os_apply_refl<NT, kOsCKF, true>(N, N, u, slot_y, NP, sh.tau, am);
+            for (int i = 0; i < QY; ++i) uy[i] = (qch + kOsQK * i == qc) ? d : 0.0;
+            os_apply_refl<NT, kOsCKF, true, QY>(N, N, uy, slot_y, NP, sh.tau, am);
+#pragma unroll
+            for (int i = 0; i < kOsQR; ++i) u[i] = i < QY ? uy[i] : 0.0;
+        }
         os_apply_refl<NT, kOsCKF, true>(M, N, u, slot_a, NP, sh.tau_a, am);
         os_write_u(u, M, N, wide ? P.vt + itm.vt_off : P.u + itm.u_off, wide);
     }

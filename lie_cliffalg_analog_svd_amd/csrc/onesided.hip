@@ -291,6 +291,10 @@ constexpr int kOsKC = 8;    // k per chunk of the product R V
 constexpr int kOsCKF = 16;  // reflectors per staged chunk when W-free
 constexpr int os_area_doubles(int np) { return np >= kOsWFreeMinNP ? 3 * 8 * 128 : kOsRows * (np + 1); }
 constexpr int os_min_waves(int np) { return np >= kOsWFreeMinNP ? 4 : 1; }
+// Y's QR and Q_Y keep the pivot / reflector values of a step live in
+// registers up to 24 row slots (NP <= 96); above that they read them again
+template <int NP>
+constexpr bool kOsYTight = NP / kOsQK > 24;
 
 template <int CTRL>
 __device__ __forceinline__ double os_dpp(double x) {
@@ -462,11 +466,12 @@ __device__ __forceinline__ void os_apply_refl(int M, int N, double (&u)[QS], con
 
 // U (M x N) from the column-resident registers to row-major output, or its
 // transpose (N x M) for a wide input.
-__device__ __forceinline__ void os_write_u(const double (&u)[kOsQR], int M, int N, double* ud, bool wide) {
+template <int QS>
+__device__ __forceinline__ void os_write_u(const double (&u)[QS], int M, int N, double* ud, bool wide) {
     const int qc = tix() >> 2, qch = tix() & 3;
     if (qc < N)
 #pragma unroll
-        for (int i = 0; i < kOsQR; ++i) {
+        for (int i = 0; i < QS; ++i) {
             const int r = qch + kOsQK * i;
             if (r < M) {
                 if (!wide) ud[(size_t)r * N + qc] = u[i];
@@ -518,14 +523,23 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
     (void)slot_y;
     (void)slot_v;
 
-    // ---- A / ||A||_F into the column-resident registers, QR
-    {
+    // ---- A / ||A||_F into the column-resident registers, QR.  W-free
+    // sizes up to 96 with M <= 96 take 24 row slots a lane (the pivot
+    // buffers' rows from 96 on stay zero) and keep a step's pivot values
+    // live; the others 32, reading them again when W-free.
+    constexpr bool A24 = WFREE && NP <= 96;
+    const bool small_a = A24 && M <= kOsQK * 24;  // uniform
+    auto load_qr = [&](auto qs_tag) {
+        constexpr int QS = decltype(qs_tag)::value;
         static_assert(NT / kOsQK >= NP, "a column slot per column");
         const int qc = tid >> 2, qch = tid & 3;
-        double a[kOsQR];
+        if constexpr (QS < kOsQR)
+            for (int e = tid; e < 2 * (kOsRows - kOsQK * QS); e += NT)
+                vb[(e / (kOsRows - kOsQK * QS)) * kOsRows + kOsQK * QS + e % (kOsRows - kOsQK * QS)] = 0.0;
+        double a[QS];
         double amx = 0.0;
 #pragma unroll
-        for (int i = 0; i < kOsQR; ++i) {
+        for (int i = 0; i < QS; ++i) {
             const int r = qch + kOsQK * i;
             a[i] = (r < M && qc < N) ? ain(r, qc) : 0.0;
             amx = fmax(amx, fabs(a[i]));
@@ -537,7 +551,7 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
         const int ex = __builtin_amdgcn_frexp_exp(block_max<NT>(amx, sh.red));
         double ss = 0.0;
 #pragma unroll
-        for (int i = 0; i < kOsQR; ++i) {
+        for (int i = 0; i < QS; ++i) {
             a[i] = ldexp(a[i], -ex);
             ss = fma(a[i], a[i], ss);
         }
@@ -546,9 +560,15 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
         sc = ss > 0.0 ? ldexp(scr, ex) : 0.0;
         const double isc = 1.0 / scr;
 #pragma unroll
-        for (int i = 0; i < kOsQR; ++i) a[i] *= isc;
-        if constexpr (WFREE) os_qr_regs<NT, true>(M, N, a, slot_a, NP, vb, sh);  // R and tails to the slot
+        for (int i = 0; i < QS; ++i) a[i] *= isc;
+        if constexpr (WFREE) os_qr_regs<NT, (QS > 24), QS>(M, N, a, slot_a, NP, vb, sh);  // R and tails to the slot
         else os_qr_regs<NT>(M, N, a, am, LDA, vb, sh);
+    };
+    if constexpr (A24) {
+        if (small_a) load_qr(std::integral_constant<int, 24>{});
+        else load_qr(std::integral_constant<int, kOsQR>{});
+    } else {
+        load_qr(std::integral_constant<int, kOsQR>{});
     }
     __syncthreads();
     clk.mark(kPhQR);
@@ -1035,21 +1055,30 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
         for (int i = 0; i < QY; ++i) y[i] *= is;
     }
     // ---- Y = Q_Y R_Y (Y's reflectors and R_Y to the slot), U = Q_A [Q_Y D; 0]
-    os_qr_regs<NT, true, QY>(N, N, y, slot_y, NP, vb, sh);
+    os_qr_regs<NT, kOsYTight<NP>, QY>(N, N, y, slot_y, NP, vb, sh);
     __syncthreads();
     {
-        double u[kOsQR];
-        {
-            double uy[QY];
-            const double d = qc < N ? (slot_y[(size_t)qc * NP + qc] < 0.0 ? -1.0 : 1.0) : 0.0;
+        double uy[QY];
+        const double d = qc < N ? (slot_y[(size_t)qc * NP + qc] < 0.0 ? -1.0 : 1.0) : 0.0;
 #pragma unroll
-            for (int i = 0; i < QY; ++i) uy[i] = (qch + kOsQK * i == qc) ? d : 0.0;
-            os_apply_refl<NT, kOsCKF, true, QY>(N, N, uy, slot_y, NP, sh.tau, am);
+        for (int i = 0; i < QY; ++i) uy[i] = (qch + kOsQK * i == qc) ? d : 0.0;
+        os_apply_refl<NT, kOsCKF, kOsYTight<NP>, QY>(N, N, uy, slot_y, NP, sh.tau, am);
+        // Q_A on M rows: as many row slots as A's QR took
+        auto apply_a = [&](auto qs_tag) {
+            constexpr int QS = decltype(qs_tag)::value;
+            static_assert(QS >= QY, "Q_Y D fits the slots");
+            double u[QS];
 #pragma unroll
-            for (int i = 0; i < kOsQR; ++i) u[i] = i < QY ? uy[i] : 0.0;
+            for (int i = 0; i < QS; ++i) u[i] = i < QY ? uy[i] : 0.0;
+            os_apply_refl<NT, kOsCKF, (QS > 24), QS>(M, N, u, slot_a, NP, sh.tau_a, am);
+            os_write_u(u, M, N, wide ? P.vt + itm.vt_off : P.u + itm.u_off, wide);
+        };
+        if constexpr (A24) {
+            if (small_a) apply_a(std::integral_constant<int, 24>{});
+            else apply_a(std::integral_constant<int, kOsQR>{});
+        } else {
+            apply_a(std::integral_constant<int, kOsQR>{});
         }
-        os_apply_refl<NT, kOsCKF, true>(M, N, u, slot_a, NP, sh.tau_a, am);
-        os_write_u(u, M, N, wide ? P.vt + itm.vt_off : P.u + itm.u_off, wide);
     }
     }
     clk.mark(kPhOut);

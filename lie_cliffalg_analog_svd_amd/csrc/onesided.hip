@@ -488,8 +488,7 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
     constexpr bool WFREE = NP >= kOsWFreeMinNP;
     static_assert(!WFREE || (NW == 8 && 3 * NW * C::NL <= os_area_doubles(NP) &&
                              kOsRows * (kOsKC + 1) + kOsKC * (NP + 1) <= os_area_doubles(NP) &&
-                             kOsRows * (kOsCKF + 1) + kOsCKF * kOsCKF + 2 * kOsCKF * (kOsCKF - 1) <=
-                                 os_area_doubles(NP)),
+                             kOsRows * (kOsCKF + 1) <= os_area_doubles(NP)),
                   "the W-free LDS area holds the partial sums and the staged chunks");
     constexpr int NL = C::NL;
     constexpr int NT = C::NT, CH = C::CH, PC = C::P, PP = C::PP, XR = C::XR, LDA = C::LDA;
@@ -834,7 +833,11 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
                     const double c = cs[j][0], s = cs[j][1];
                     const double p = m[k][l];
                     if (ODD && j == PC - 1) {
-                        nx[l] = fma(s, p, c * mn);
+                        // the addend is this column scaled in place, so the
+                        // accumulating FMA leaves the result in the
+                        // column's own register (no copies at the loop's
+                        // back edge; as nx[0] below)
+                        nx[l] = fma(c, mn, s * p);
                     } else {
                         const double q = m[k][l + 1];
                         nx[l] = fma(s, p, c * q);

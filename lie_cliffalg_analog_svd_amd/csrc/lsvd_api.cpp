@@ -313,11 +313,14 @@ static int plan_build(lsvd_handle_t h, int count, const int32_t* m, const int32_
         bk.gws_offset = (int64_t)gws_doubles;
         gws_doubles += configure_bucket(h, bk, bk.variant);
     }
-    // fallback launch: micro escalations, n <= 4 square -> variant 0
+    // fallback launch: micro escalations, n <= 4 square -> variant 0.  It
+    // runs after the join and its queue is usually empty, so its grid is
+    // at most one workgroup per CU (a full-occupancy grid of workgroups
+    // that find nothing to do cost ~45 us per execution)
     {
         const size_t arena = small_arena_doubles(4, 4) * 8;
         const int per_cu = small_variant_blocks_per_cu(0, arena);
-        p->fallback_grid = std::max(1, std::min(std::max(p->n_micro, 1), per_cu * h->num_cus));
+        p->fallback_grid = std::max(1, std::min({std::max(p->n_micro, 1), per_cu * h->num_cus, h->num_cus}));
         p->fallback_lds = arena;
     }
 

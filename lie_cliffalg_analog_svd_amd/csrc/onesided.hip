@@ -62,7 +62,7 @@ namespace lsvd {
 #define LSVD_OS_CFG_32 8, 2, 4
 #endif
 #ifndef LSVD_OS_CFG_48
-#define LSVD_OS_CFG_48 8, 3, 6
+#define LSVD_OS_CFG_48 32, 4, 2
 #endif
 #ifndef LSVD_OS_CFG_64
 #define LSVD_OS_CFG_64 32, 4, 2

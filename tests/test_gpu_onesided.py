@@ -85,10 +85,12 @@ def _check_batch(mats, what, refs=None, resid=1e-11):
     return out, info
 
 
-# one shape per padded size 16..128: square, tall, and wide (solved transposed)
+# one shape per padded size 16..128: square, tall, and wide (solved transposed);
+# at padded sizes 80 and 96 both row-slot counts of A's QR (at most 96 rows
+# or more), also for wide inputs
 SHAPES = [(5, 5), (16, 16), (17, 9), (9, 17), (32, 32), (40, 33), (33, 100), (48, 48), (128, 47), (64, 64),
           (100, 63), (65, 65), (80, 80), (90, 77), (96, 96), (128, 95), (112, 112), (120, 101), (128, 128),
-          (128, 113), (113, 128), (128, 1), (1, 128), (128, 2)]
+          (128, 113), (113, 128), (128, 1), (1, 128), (128, 2), (81, 96), (70, 97), (97, 112), (97, 81)]
 
 
 def test_onesided_uniform_shapes_vs_oracle():

@@ -576,16 +576,17 @@ typedef struct {
     double default_ms;     /* the handle's settings before tuning */
     double tuned_ms;       /* the chosen settings */
     int32_t os_max_np;     /* LSVD_PREC_F64_ONESIDED: padded sizes <= this take QR + one-sided Jacobi */
-    int32_t reserved;
+    int32_t lead_np;       /* padded size of the per-size bucket launched first (0: longest estimate first) */
 } lsvd_tune_report_t;
 
 /* Runs the autotuner for a batch's shapes and stores its choices in the
  * handle (used by later plans): for the on-chip buckets (N, M <= 128) a
  * coordinate search over (LSVD_PREC_F64_ONESIDED) the largest padded size
  * that takes QR + one-sided Jacobi, the split point of the multi-size launch
- * (64, 80, 96 or none), the lane count (1..4) and the threads per workgroup
- * of each per-size bucket (default or twice), timing whole plans; for the buckets
- * above 128 the kernel variant of each bucket, timing that bucket. */
+ * (64, 80, 96 or none), the lane count (1..4), the threads per workgroup
+ * of each per-size bucket (default or twice) and which per-size bucket, if
+ * any, is launched first, timing whole plans; for the buckets above 128 the
+ * kernel variant of each bucket, timing that bucket. */
 int lsvd_autotune_batch(lsvd_handle_t h, int count, const int32_t* m, const int32_t* n, int precision);
 /* The plan-level part of lsvd_autotune_batch timed on the caller's own
  * device pools (the handle in LSVD_PTR_DEVICE mode, the layout of

@@ -69,7 +69,7 @@ class LsvdTuneReport(C.Structure):
         ("default_ms", C.c_double),
         ("tuned_ms", C.c_double),
         ("os_max_np", C.c_int32),
-        ("reserved", C.c_int32),
+        ("lead_np", C.c_int32),
     ]
 
     def as_dict(self):

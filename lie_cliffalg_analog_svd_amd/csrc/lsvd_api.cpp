@@ -400,13 +400,28 @@ hipError_t plan_ensure_streams(lsvd_plan_t p) {
     return hipSuccess;
 }
 
+// Padded size of a per-size bucket (0 for the multi-size launches and the
+// variants above 128).
+static int bucket_np(const Bucket& bk) {
+    const int v = bk.variant;
+    if (v >= kOsMulti) return 0;
+    if (v >= kOsBase) return v - kOsBase;
+    if (v >= kV2Wide) return v - kV2Wide;
+    if (v >= kV2Multi) return 0;
+    if (v >= kV2Base) return v - kV2Base;
+    return 0;
+}
+
 // Enqueues the whole plan on `st` with the given device pools.  The
 // buckets (and the micro launch) are spread over up to four concurrent
 // lanes (fork/join with events, which a hipGraph capture turns into
 // parallel branches): longest estimated bucket first, each onto the lane
 // with the least estimated work, so a lane never queues a long bucket
 // behind another one while the CUs a large-N bucket leaves idle take the
-// small-N buckets.  The micro escalations drain after the join.
+// small-N buckets.  The handle's lead_np (autotuned) moves one per-size
+// bucket to the front: launched first, its workgroups take CUs before the
+// persistent multi-size launch fills them.  The micro escalations drain
+// after the join.
 hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const hipEvent_t* tev) {
     hipError_t e = launch_reset_counters(p->d_counters, p->n_counters, st);
     if (e != hipSuccess) return e;
@@ -422,6 +437,11 @@ hipError_t plan_enqueue(lsvd_plan_t p, const Pools& pools, hipStream_t st, const
         return bk.cost_sum / std::max(bk.grid, 1) + bk.cost_max;
     };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return est(x) > est(y); });
+    if (p->h->lead_np > 0) {
+        auto it = std::find_if(order.begin(), order.end(),
+                               [&](int i) { return bucket_np(p->buckets[i]) == p->h->lead_np; });
+        if (it != order.end()) std::rotate(order.begin(), it, it + 1);
+    }
     std::vector<double> load(std::max(lanes, 1), 0.0);
     auto least = [&]() { return (int)(std::min_element(load.begin(), load.end()) - load.begin()); };
     e = hipEventRecord(p->ev[0], st);  // fork point
@@ -681,6 +701,7 @@ int lsvd_create(int device, lsvd_handle_t* out) {
     if (const char* env = std::getenv("LSVD_JADE_CHUNK_MB"))
         h->jade_chunk_bytes = (size_t)std::max(0L, std::strtol(env, nullptr, 10)) << 20;
     if (const char* env = std::getenv("LSVD_LANE0_CUS")) h->lane0_cus = (int)std::strtol(env, nullptr, 10);
+    if (const char* env = std::getenv("LSVD_LEAD_NP")) h->lead_np = (int)std::strtol(env, nullptr, 10);
     if (const char* env = std::getenv("LSVD_MICRO_MAX_SWEEPS")) {
         char* end = nullptr;
         const long v = std::strtol(env, &end, 10);
@@ -806,7 +827,7 @@ static int tune_plan_knobs(lsvd_handle_t h, int count, const int32_t* m, const i
     const int saved_ptr_mode = h->ptr_mode;
     std::vector<int32_t> pmodes(count);
     for (int b = 0; b < count; ++b) pmodes[b] = (m[b] == n[b] && n[b] <= 4) ? kModeMicro : kModeSvd;
-    const int saved_mm = h->multi_min_np, saved_lanes = h->max_lanes, saved_os = h->os_max_np;
+    const int saved_mm = h->multi_min_np, saved_lanes = h->max_lanes, saved_os = h->os_max_np, saved_lead = h->lead_np;
     const uint32_t saved_wide = h->wide_mask;
     int evals = 0;
     float t_ms = 0.f;
@@ -859,14 +880,16 @@ static int tune_plan_knobs(lsvd_handle_t h, int count, const int32_t* m, const i
     float t_best = t_ms;
     // coordinate search: a candidate replaces the best so far when it
     // saves more than 1.5 %
-    int best_mm = saved_mm, best_lanes = saved_lanes, best_os = saved_os;
+    int best_mm = saved_mm, best_lanes = saved_lanes, best_os = saved_os, best_lead = saved_lead;
     uint32_t best_wide = saved_wide;
-    auto try_knobs = [&](int mm, int lanes, uint32_t wide, int osn) -> int {
-        if (mm == best_mm && lanes == best_lanes && wide == best_wide && osn == best_os) return LSVD_OK;
+    auto try_knobs = [&](int mm, int lanes, uint32_t wide, int osn, int lead) -> int {
+        if (mm == best_mm && lanes == best_lanes && wide == best_wide && osn == best_os && lead == best_lead)
+            return LSVD_OK;
         h->multi_min_np = mm;
         h->max_lanes = lanes;
         h->wide_mask = wide;
         h->os_max_np = osn;
+        h->lead_np = lead;
         float t = 0.f;
         const int r = time_plan(&t);
         if (r == LSVD_OK && t < t_best * 0.985f) {
@@ -875,35 +898,53 @@ static int tune_plan_knobs(lsvd_handle_t h, int count, const int32_t* m, const i
             best_lanes = lanes;
             best_wide = wide;
             best_os = osn;
+            best_lead = lead;
         }
         h->multi_min_np = best_mm;
         h->max_lanes = best_lanes;
         h->wide_mask = best_wide;
         h->os_max_np = best_os;
+        h->lead_np = best_lead;
         return r;
     };
     if (precision == LSVD_PREC_F64_ONESIDED) {  // where QR + one-sided Jacobi hands over to Newton-Schulz
         for (int osn = 16; osn <= 128; osn += 16) {
-            const int r = try_knobs(best_mm, best_lanes, best_wide, osn);
+            const int r = try_knobs(best_mm, best_lanes, best_wide, osn, best_lead);
             if (r != LSVD_OK) return r;
         }
     }
     const int mms[] = {64, 80, 96, 129};
     for (int mm : mms) {
-        const int r = try_knobs(mm, best_lanes, best_wide, best_os);
+        const int r = try_knobs(mm, best_lanes, best_wide, best_os, best_lead);
         if (r != LSVD_OK) return r;
     }
     for (int lanes = 1; lanes <= kPlanLanes; ++lanes) {
-        const int r = try_knobs(best_mm, lanes, best_wide, best_os);
+        const int r = try_knobs(best_mm, lanes, best_wide, best_os, best_lead);
         if (r != LSVD_OK) return r;
     }
     for (int np = 16; np <= 64; np += 16) {
         if (np >= best_mm && h->multi_launch) continue;  // inside the multi-size launch
-        const int r = try_knobs(best_mm, best_lanes, best_wide ^ (1u << (np / 16 - 1)), best_os);
+        const int r = try_knobs(best_mm, best_lanes, best_wide ^ (1u << (np / 16 - 1)), best_os, best_lead);
         if (r != LSVD_OK) return r;
     }
+    {  // which per-size bucket, if any, goes first
+        h->ptr_mode = LSVD_PTR_DEVICE;
+        lsvd_plan_t q = nullptr;
+        int r = plan_build(h, count, m, n, a_off, u_off, s_off, vt_off, pmodes.data(), precision, &q);
+        h->ptr_mode = saved_ptr_mode;
+        if (r != LSVD_OK) return r;
+        std::vector<int> nps;
+        for (const Bucket& bk : q->buckets)
+            if (bucket_np(bk) > 0 && bucket_np(bk) != best_lead) nps.push_back(bucket_np(bk));
+        lsvd_batch_plan_destroy(q);
+        if (best_lead > 0) nps.push_back(0);
+        for (int lead : nps) {
+            r = try_knobs(best_mm, best_lanes, best_wide, best_os, lead);
+            if (r != LSVD_OK) return r;
+        }
+    }
     h->tune_report = lsvd_tune_report_t{best_mm, best_lanes, (int32_t)best_wide, evals, (double)t_default,
-                                        (double)t_best, best_os, 0};
+                                        (double)t_best, best_os, best_lead};
         return LSVD_OK;
 }
 

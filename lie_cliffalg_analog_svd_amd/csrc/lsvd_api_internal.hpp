@@ -77,6 +77,7 @@ struct lsvd_handle_s {
     // (lsvd_set_solve_precision): the reference iteration by default
     int solve_precision = LSVD_PREC_F64;
     int max_lanes = 4;
+    int lead_np = 0;  // padded size of the per-size bucket launched first (0: longest estimate first)
     uint32_t wide_mask = 0;
     lsvd_tune_report_t tune_report{};
     // Phase-JADE sweeps run this many bytes of families (working copies and

@@ -21,11 +21,12 @@
 // consecutive layers applies no rotation, or at the reference's sweep cap
 // 60 + 20 N.
 //
-// Parallel ordering: odd-even ("caterpillar") layers over NP positions --
-// even layers pair (2k, 2k+1), odd layers (2k+1, 2k+2) -- and the two
-// columns of every pair always trade positions (rotated or not), so the
-// movement of the columns is independent of the data: any NP consecutive
-// layers meet every pair exactly once, and the index held by a position
+// Parallel ordering: odd-even ("caterpillar") layers over NE positions (N
+// rounded up to whole lanes' positions) -- even layers pair (2k, 2k+1), odd
+// layers (2k+1, 2k+2) -- and the two columns of every pair always trade
+// positions (rotated or not), so the movement of the columns is independent
+// of the data: any NE consecutive layers meet every pair exactly once, and
+// the index held by a position
 // after G layers has a closed form (index_at).  In registers a trade is
 // free (the rotation writes its two outputs to swapped registers); only the
 // pair that straddles two lanes in an odd layer needs one neighbour-lane
@@ -600,11 +601,16 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 
     const double tol = sqrt((double)N) * 0x1p-52;
     const double tol2 = tol * tol;
-    const int maxG = (60 + 20 * N) * NP;
+    // The ordering runs over NE = N rounded up to whole lanes' positions
+    // rather than the padded size: the positions from NE on hold zero
+    // columns behind the dead pair of the lane that ends at NE - 1, so a
+    // sweep takes NE layers.
+    const int NE = (N + CH - 1) / CH * CH;
+    const int maxG = (60 + 20 * N) * NE;
     int G = 0, quiet = 0, rots = 0;
-    // the lane whose last position is NP - 1, and a row's last lane: their
+    // the lane whose last position is NE - 1, and a row's last lane: their
     // straddling pairs are dead; the lanes after them start a segment
-    const bool lastv = vch == NP / CH - 1 || vch == TPR - 1, firstv = vch == 0 || vch == NP / CH;
+    const bool lastv = vch == NE / CH - 1 || vch == TPR - 1, firstv = vch == 0 || vch == NE / CH;
     // lane bits that the reduce-scatter did not split (replicated sums)
     const int rep_mask = (RGW - 1) & ~((SC >= 1 ? (RGW >> 1) : 0) | (SC >= 2 ? (RGW >> 2) : 0));
     const bool rep_lane = (rgw & rep_mask) == 0 && w == 0;
@@ -858,24 +864,24 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 
     if (N > 1) {
         int since = 0, full_left = 0;  // layers since the sweep's first; full layers still owed
-        while (quiet < NP && G < maxG) {
+        while (quiet < NE && G < maxG) {
             const bool a0 = layer(std::integral_constant<bool, false>{}, since == 0 || full_left > 0);
             full_left = refresh ? 2 : (full_left > 0 ? full_left - 1 : 0);
             ++G;
             quiet = a0 ? 0 : quiet + 1;
-            if (quiet >= NP) break;
+            if (quiet >= NE) break;
             const bool a1 = layer(std::integral_constant<bool, true>{}, full_left > 0);
             full_left = refresh ? 2 : (full_left > 0 ? full_left - 1 : 0);
             ++G;
             quiet = a1 ? 0 : quiet + 1;
             since += 2;
-            if (since >= NP) since = 0;
+            if (since >= NE) since = 0;
         }
     }
     __syncthreads();  // every wave's last partial-sum reads are done before am is reused
     clk.mark(kPhJacobi);
     info.ns_iters = 0;
-    info.jacobi_sweeps = (G - quiet + NP - 1) / NP;
+    info.jacobi_sweeps = (G - quiet + NE - 1) / NE;
     info.polar_ok = 1;
 
     // ---- column norms -> sigma (normalised), rank
@@ -895,10 +901,10 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 #pragma unroll
             for (int c = 0; c < CH; ++c) am[w * NL + vch * CH + c] = cn[c];
         __syncthreads();
-        for (int g = tid; g < NP; g += NT) {
+        for (int g = tid; g < NE; g += NT) {
             double s = 0.0;
             for (int ww = 0; ww < NW; ++ww) s += am[ww * NL + g];
-            sh.sig[os_index_at(NP, G, g)] = sqrt(s);
+            sh.sig[os_index_at(NE, G, g)] = sqrt(s);
         }
         __syncthreads();
     }
@@ -922,7 +928,7 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             const int g = vch * CH + c;
-            const int idx = g < NP ? os_index_at(NP, G, g) : NP;
+            const int idx = g < NE ? os_index_at(NE, G, g) : NE;
             if (r < N && idx < N) {
                 // columns whose norm is below kOsTiny (the input is scaled to
                 // ||A||_F = 1) are zero to the rotation test (their Gram
@@ -963,7 +969,7 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             const int g = vch * CH + c;
-            const int idx = g < NP ? os_index_at(NP, G, g) : NP;
+            const int idx = g < NE ? os_index_at(NE, G, g) : NE;
             if (r < N && idx < N) am[r * LDA + sh.rank[idx]] = wv[k][c];
         }
     }
@@ -987,7 +993,7 @@ __device__ __forceinline__ void os_solve(const Item& itm, const Pools& P, double
 #pragma unroll
         for (int c = 0; c < CH; ++c) {
             const int g = vch * CH + c;
-            const int idx = g < NP ? os_index_at(NP, G, g) : NP;
+            const int idx = g < NE ? os_index_at(NE, G, g) : NE;
             if (r < N && idx < N) {
                 const double sg = sh.sig[idx];  // below kOsTiny: completed as above
                 const bool tiny = !(sg > kOsTiny) && !isnan(sg);

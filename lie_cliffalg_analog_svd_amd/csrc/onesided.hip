@@ -249,19 +249,22 @@ template <bool WANT_T = false>
 __device__ __forceinline__ void os_angle(double a, double b, double g, double& c, double& s, double* t = nullptr) {
     const double d = b - a, a2 = 2.0 * g;
     const int ex = __builtin_amdgcn_frexp_exp(fmax(fabs(d), fabs(a2)));
-    const double ds = ldexp(d, -ex), as = fabs(ldexp(a2, -ex));
-    const double x = fma(ds, ds, as * as);  // in [0.25, 2)
+    const double ds = ldexp(d, -ex), ga = ldexp(a2, -ex);
+    // s carries sign(d) sign(g) (+ for d = +0): g's scaled value with d's
+    // sign bit folded into its own
+    const double sas = __hiloint2double(__double2hiint(ga) ^ (__double2hiint(ds) & (int)0x80000000),
+                                        __double2loint(ga));
+    const double x = fma(ds, ds, ga * ga);  // in [0.25, 2)
     const double r = x * os_rsq(x);
     const double den = fabs(ds) + r;  // in [0.5, 2.5)
-    const double sgn = (d == 0.0 || ((d > 0.0) == (a2 > 0.0))) ? 1.0 : -1.0;
-    const double ih = os_rsq(fma(den, den, as * as));  // in [0.25, 6)
+    const double ih = os_rsq(fma(den, den, ga * ga));  // in [0.25, 6)
     c = den * ih;
-    s = sgn * as * ih;
+    s = sas * ih;
     if constexpr (WANT_T) {
         // 1/den = y / (1 - e) = y (1 + e + e^2 + O(e^3)), e = 1 - den y
         const double y = __builtin_amdgcn_rcp(den);
         const double e = fma(-den, y, 1.0);
-        *t = sgn * as * fma(y, fma(e, e, e), y);
+        *t = sas * fma(y, fma(e, e, e), y);
     }
 }
 

@@ -292,7 +292,7 @@ constexpr int kOsRows = 128;           // rows of the LDS matrix area
 // matrices of 8 waves share a CU (128 registers a lane).
 constexpr int kOsWFreeMinNP = 80;
 constexpr int kOsKC = 8;    // k per chunk of the product R V
-constexpr int kOsCKF = 16;  // reflectors per staged chunk when W-free
+constexpr int kOsCKF = 8;   // reflectors per staged chunk when W-free
 constexpr int os_area_doubles(int np) { return np >= kOsWFreeMinNP ? 3 * 8 * 128 : kOsRows * (np + 1); }
 constexpr int os_min_waves(int np) { return np >= kOsWFreeMinNP ? 4 : 1; }
 // Y's QR and Q_Y keep the pivot / reflector values of a step live in
